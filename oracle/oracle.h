@@ -1,0 +1,94 @@
+/* CPU oracle for the HTJ2K tile-decode path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
+ * CPU decoder (frames-sg/signinum, crates/signinum-j2k-native) used as the
+ * parity checker by the test suite, by the smoke check and by the
+ * CPU-baseline leg of the benchmark.  The product library never links,
+ * loads or calls it.
+ */
+#ifndef SGN_ORACLE_H
+#define SGN_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    ORACLE_OK = 0,
+    ORACLE_E_ARG = -1,
+    ORACLE_E_UNSUPPORTED = -2,
+    ORACLE_E_MALFORMED = -3,
+    ORACLE_E_BLOCK = -4,
+    ORACLE_E_BUFFER = -5,
+    ORACLE_E_NOMEM = -6
+};
+
+/* Pixel formats, numbered like the product's C header. */
+enum {
+    ORACLE_GRAY8 = 0,
+    ORACLE_RGB8 = 1,
+    ORACLE_RGBA8 = 2,
+    ORACLE_GRAY16 = 3,
+    ORACLE_RGB16 = 4
+};
+
+typedef struct {
+    uint32_t width, height;
+    uint32_t num_components;
+    uint32_t bit_depth;
+    uint32_t num_levels;
+    uint32_t reversible;
+    uint32_t mct;
+    uint32_t high_throughput;
+    uint32_t tiles_x, tiles_y;
+    uint32_t cb_width, cb_height;
+    uint32_t progression;
+} oracle_info_t;
+
+void oracle_build_tables(void);
+void oracle_get_tables(uint16_t *vlc0, uint16_t *vlc1, uint16_t *uvlc0, uint16_t *uvlc1);
+
+int oracle_inspect(const uint8_t *data, size_t len, oracle_info_t *info);
+
+/* Output dimensions of a decode at 1/denominator resolution. */
+int oracle_scaled_dims(const uint8_t *data, size_t len, uint32_t denominator,
+                       uint32_t *out_w, uint32_t *out_h);
+
+/* Full decode into an interleaved buffer.  denominator is 1, 2, 4 or 8. */
+int oracle_decode(const uint8_t *data, size_t len, int fmt, uint32_t denominator,
+                  uint8_t *out, size_t out_len, size_t stride);
+
+/* Decode to per-component planes before rounding (sign shift and MCT
+ * applied).  use_f64 selects the textbook double-precision restatement of
+ * the 9/7 and ICT steps instead of the reference's f32 arithmetic. */
+int oracle_decode_planes(const uint8_t *data, size_t len, uint32_t denominator, int use_f64,
+                         double *planes, size_t plane_len);
+
+/* Tile batch like the reference's decode_tiles_into: `workers` threads,
+ * contiguous chunks of ceil(n / workers), one decoder state per worker.
+ * Returns ORACLE_OK or the first failing status, with its index. */
+int oracle_decode_batch(const uint8_t *const *datas, const size_t *lens, size_t n, int fmt,
+                        uint32_t denominator, uint8_t *const *outs, const size_t *out_lens,
+                        const size_t *strides, int workers, size_t *first_fail_index);
+
+/* One HT code-block: cleanup segment of lcup bytes followed by lref bytes of
+ * refinement data.  Writes the u32 sign-magnitude samples (bit 31 sign,
+ * magnitude aligned below bit 30) with row stride `stride`.  phase_limit 0
+ * stops after cleanup, 1 after SigProp, 2 runs all passes. */
+int oracle_ht_decode_block(const uint8_t *data, uint32_t lcup, uint32_t lref, uint32_t width,
+                           uint32_t height, uint32_t missing_msbs, uint32_t num_passes,
+                           int stripe_causal, int phase_limit, uint32_t *out, uint32_t stride);
+
+int32_t oracle_coefficient_to_i32(uint32_t value, uint32_t k_max);
+
+/* Statistics of the last oracle_decode call on this thread: HT code-blocks
+ * decoded, code-blocks with refinement data, maximum coding passes. */
+void oracle_last_stats(uint32_t *blocks, uint32_t *refinement_blocks, uint32_t *max_passes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
