@@ -1,0 +1,238 @@
+/* signinum-j2k-hip: MI355X (gfx950) HTJ2K tile decoder, C ABI.
+ *
+ * This is the drop-in boundary for the HTJ2K tile-decode path of
+ * frames-sg/signinum.  Every entry point names the reference interface it
+ * replaces.  Signatures carry plain pointers and sizes only.  All functions
+ * return SGN_OK (0) or a negative status; sgn_last_error_message() returns
+ * a thread-local description of the last failure.
+ *
+ * Threading: a session owns one HIP stream and its device buffers and must
+ * be used from one host thread at a time; separate sessions may be used
+ * concurrently (crates/signinum-j2k-cuda/src/session.rs:11-49 has the same
+ * rule for CudaSession).
+ */
+#ifndef SIGNINUM_J2K_HIP_H
+#define SIGNINUM_J2K_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (mirrors J2kError, crates/signinum-j2k/src/error.rs:5-74,
+ * and the adapter Error enum, crates/signinum-j2k-cuda/src/error.rs:6-41) */
+#define SGN_OK 0
+#define SGN_E_ARG (-1)         /* invalid argument / buffer / stride */
+#define SGN_E_UNSUPPORTED (-2) /* J2kError::Unsupported */
+#define SGN_E_MALFORMED (-3)   /* codestream parse failure */
+#define SGN_E_BLOCK (-4)       /* DecodingError::CodeBlockDecodeFailure */
+#define SGN_E_BUFFER (-5)      /* output buffer too small */
+#define SGN_E_NOMEM (-6)
+#define SGN_E_DEVICE (-7)      /* HIP runtime failure / device unavailable */
+
+/* PixelFormat (crates/signinum-core/src/pixel.rs:13-58). */
+typedef enum {
+    SGN_GRAY8 = 0,
+    SGN_RGB8 = 1,
+    SGN_RGBA8 = 2,
+    SGN_GRAY16 = 3,
+    SGN_RGB16 = 4
+} sgn_pixel_format;
+
+/* Downscale denominators (crates/signinum-core/src/scale.rs:3-30):
+ * None = 1, Half = 2, Quarter = 4, Eighth = 8. */
+
+/* Image information (crates/signinum-j2k/src/view.rs:102-112, Info). */
+typedef struct {
+    uint32_t width, height;
+    uint32_t num_components;
+    uint32_t bit_depth;
+    uint32_t num_levels;      /* decomposition levels (minimum over components) */
+    uint32_t reversible;      /* 1: 5/3, 0: 9/7 */
+    uint32_t mct;             /* multiple-component transform signalled */
+    uint32_t high_throughput; /* HT block coder (Part 15) */
+    uint32_t tiles_x, tiles_y;
+    uint32_t cb_width, cb_height;
+    uint32_t progression;
+} sgn_j2k_info;
+
+const char *sgn_last_error_message(void);
+
+/* J2kDecoder::inspect (crates/signinum-j2k/src/view.rs:102). */
+int sgn_j2k_inspect(const uint8_t *bytes, size_t len, sgn_j2k_info *info);
+
+/* Output dimensions of decode_scaled_into at 1/denominator
+ * (crates/signinum-j2k/src/decode.rs:12-33). */
+int sgn_j2k_scaled_dims(const uint8_t *bytes, size_t len, uint32_t denominator,
+                        uint32_t *width, uint32_t *height);
+
+/* ---- sessions (CudaSession, crates/signinum-j2k-cuda/src/session.rs:11-49) */
+typedef struct sgn_hip_session sgn_hip_session;
+int sgn_hip_session_create(int device, sgn_hip_session **out);
+void sgn_hip_session_destroy(sgn_hip_session *session);
+/* HIP stream handle of the session (a hipStream_t), for callers that order
+ * their own work after a decode. */
+void *sgn_hip_session_stream(sgn_hip_session *session);
+int sgn_hip_session_synchronize(sgn_hip_session *session);
+/* Kernel launches issued by the session so far
+ * (CudaSurfaceStats::kernel_dispatches, surface.rs:18-27). */
+uint64_t sgn_hip_session_dispatches(sgn_hip_session *session);
+
+/* ---- single codestream decode ------------------------------------------
+ * J2kDecoder::decode_into / decode_scaled_into (crates/signinum-j2k/src/
+ * view.rs:168,267; adapter decoder.rs:156-200).  Decodes on the device and
+ * copies the result to host memory `out` with row stride `stride`. */
+int sgn_j2k_decode_into(sgn_hip_session *session, const uint8_t *bytes, size_t len, int fmt,
+                        uint32_t denominator, uint8_t *out, size_t out_len, size_t stride);
+
+/* ImageDecodeSubmit::submit_to_device / submit_scaled_to_device
+ * (crates/signinum-j2k-cuda/src/decoder.rs:225-266): decodes into device
+ * memory `dst` (pitch in bytes) on the session stream; asynchronous. */
+int sgn_j2k_decode_device(sgn_hip_session *session, const uint8_t *bytes, size_t len, int fmt,
+                          uint32_t denominator, void *dst, size_t dst_pitch, size_t dst_capacity);
+
+/* ---- tile batches --------------------------------------------------------
+ * TileDecodeJob (crates/signinum-j2k/src/batch.rs:24-33) with a device
+ * destination. */
+typedef struct {
+    const uint8_t *bytes;
+    size_t len;
+    void *dst;        /* device pointer */
+    size_t pitch;     /* bytes per output row */
+    size_t capacity;  /* bytes available at dst */
+} sgn_tile_job;
+
+/* Execution modes of a batch decode. */
+#define SGN_MODE_PLAIN 0    /* one launch sequence per tile */
+#define SGN_MODE_BATCHED 1  /* one launch per stage for the whole batch */
+#define SGN_MODE_GRAPH 2    /* batched, captured once in a hipGraph and replayed */
+
+/* decode_tiles_into / TileBatchDecodeManyDevice::decode_tiles_to_device
+ * (crates/signinum-j2k/src/batch.rs:110-147; crates/signinum-j2k-cuda/src/
+ * codec.rs:190-210).  On failure *first_fail holds the first failing tile
+ * index (TileBatchError, batch.rs:47-67).  Synchronous. */
+int sgn_j2k_decode_tiles_device(sgn_hip_session *session, const sgn_tile_job *jobs, size_t n,
+                                int fmt, uint32_t denominator, int mode, size_t *first_fail);
+
+/* ---- plans: parse once, decode many times -------------------------------
+ * The flattened device plan plays the role of J2kDirectColorPlan
+ * (crates/signinum-j2k-native/src/direct_plan.rs:12-205): code-block table,
+ * IDWT steps and store/colour steps for every tile of a batch. */
+typedef struct sgn_j2k_plan sgn_j2k_plan;
+
+/* Parses all codestreams on the host (worker threads), builds the tables
+ * and uploads codestream bytes and tables to device memory.  tile_status
+ * (optional, n entries) receives each tile's parse status. */
+int sgn_j2k_plan_create(sgn_hip_session *session, const sgn_tile_job *jobs, size_t n, int fmt,
+                        uint32_t denominator, sgn_j2k_plan **out, int32_t *tile_status);
+void sgn_j2k_plan_destroy(sgn_j2k_plan *plan);
+/* Re-targets tile i of the plan to a new device destination. */
+int sgn_j2k_plan_set_output(sgn_j2k_plan *plan, size_t i, void *dst, size_t pitch,
+                            size_t capacity);
+/* Enqueues the decode of the whole plan on the session stream
+ * (asynchronous); mode as above. */
+int sgn_j2k_plan_decode(sgn_hip_session *session, sgn_j2k_plan *plan, int mode);
+
+/* Enqueues one stage of the batched decode, for per-kernel timing:
+ * SGN_STAGE_BLOCKS (HT block decode), SGN_STAGE_LEVELS (intermediate
+ * synthesis levels), SGN_STAGE_FINAL (last level + colour + packing).
+ * Running the three in order equals sgn_j2k_plan_decode(SGN_MODE_BATCHED). */
+#define SGN_STAGE_BLOCKS 0
+#define SGN_STAGE_LEVELS 1
+#define SGN_STAGE_FINAL 2
+int sgn_j2k_plan_decode_stage(sgn_hip_session *session, sgn_j2k_plan *plan, int stage);
+
+typedef struct {
+    uint64_t tiles;
+    uint64_t code_blocks;
+    uint64_t coded_bytes;      /* codestream bytes uploaded */
+    uint64_t output_bytes;     /* W*H*C*bytes-per-sample over all tiles */
+    uint64_t output_pixels;
+    uint64_t coefficient_bytes;/* device coefficient arena */
+    uint32_t launches_batched; /* kernel launches per batched decode */
+} sgn_j2k_plan_stats;
+int sgn_j2k_plan_get_stats(const sgn_j2k_plan *plan, sgn_j2k_plan_stats *stats);
+/* Output geometry of tile i: width, height, bytes per row (unpadded). */
+int sgn_j2k_plan_tile_dims(const sgn_j2k_plan *plan, size_t i, uint32_t *width,
+                           uint32_t *height, size_t *row_bytes);
+
+/* ---- stage hooks (HtCodeBlockDecoder, crates/signinum-j2k-native/src/
+ * lib.rs:832-909).  Host buffers in, host buffers out; for parity tests. */
+
+/* HtCodeBlockDecodeJob (lib.rs:149-174). */
+typedef struct {
+    uint64_t data_offset;      /* cleanup bytes followed by refinement bytes */
+    uint32_t cleanup_length;
+    uint32_t refinement_length;
+    uint32_t width, height;
+    uint32_t output_offset;    /* element offset of the block's first sample */
+    uint32_t output_stride;    /* in samples */
+    uint8_t missing_bit_planes;
+    uint8_t number_of_coding_passes;
+    uint8_t num_bitplanes;
+    uint8_t stripe_causal;
+    float dequantization_step;
+} sgn_ht_code_block_job;
+
+/* The HT code-block jobs a full-resolution decode of `bytes` hands to the
+ * hook (decode.rs:1572-1669 collects them per sub-band): data_offset is
+ * relative to `bytes`, and each block gets its own packed output region
+ * (output_offset = sum of preceding w*h, output_stride = width).  Blocks
+ * that decode to zero are included with number_of_coding_passes = 0.  With
+ * jobs == NULL only *count is written. */
+int sgn_j2k_code_blocks(const uint8_t *bytes, size_t len, sgn_ht_code_block_job *jobs,
+                        size_t capacity, size_t *count);
+
+/* decode_code_block / decode_sub_band (lib.rs:864-909): decodes every job
+ * on the device; writes f32 = coefficient * step into out (host). */
+int sgn_ht_decode_code_blocks(sgn_hip_session *session, const uint8_t *data, size_t data_len,
+                              const sgn_ht_code_block_job *jobs, size_t n, float *out,
+                              size_t out_len);
+/* Same, returning the raw u32 sign-magnitude samples of the HT passes
+ * (ht_block_decode.rs:1563-1680) before conversion. */
+int sgn_ht_decode_code_blocks_raw(sgn_hip_session *session, const uint8_t *data,
+                                  size_t data_len, const sgn_ht_code_block_job *jobs, size_t n,
+                                  uint32_t *out, size_t out_len);
+
+/* decode_single_decomposition_idwt (lib.rs:877; J2kSingleDecompositionIdwtJob
+ * lib.rs:767-780): one 2D synthesis level over rect [x0,x1) x [y0,y1);
+ * band arrays are row-major with the Annex B band sizes. */
+int sgn_idwt_level(sgn_hip_session *session, int reversible, uint32_t x0, uint32_t y0,
+                   uint32_t x1, uint32_t y1, const float *ll, const float *hl, const float *lh,
+                   const float *hh, float *out);
+
+/* decode_inverse_mct (lib.rs:890; J2kInverseMctJob lib.rs:785-800):
+ * planes updated in place, addends applied after the transform. */
+int sgn_inverse_mct(sgn_hip_session *session, int reversible, float *plane0, float *plane1,
+                    float *plane2, size_t n, float addend0, float addend1, float addend2);
+
+/* ---- test-input encoder (encode_j2k_lossless, crates/signinum-j2k/src/
+ * encode.rs:166-182; J2kLosslessEncodeOptions encode.rs:71-94) */
+typedef struct {
+    uint32_t levels;          /* clamped to floor(log2(min tile side)) */
+    uint32_t cb_width_log2;   /* code-block width exponent, 2..10 */
+    uint32_t cb_height_log2;
+    uint32_t reversible;      /* 1: 5/3 lossless; 0: 9/7 quantised */
+    uint32_t use_mct;         /* RCT / ICT for 3+ components */
+    uint32_t progression;     /* 0 LRCP .. 4 CPRL */
+    uint32_t guard_bits;      /* 0: 1 (2 with RCT) reversible, 2 irreversible */
+    float base_step;          /* irreversible base step relative to 2^P (0: 1/64) */
+    uint32_t precinct_log2;   /* 0: maximal precincts; else PPx = PPy */
+    uint32_t sop, eph;        /* SOP / EPH markers */
+    uint32_t tile_width, tile_height; /* 0: one tile */
+} sgn_j2k_encode_options;
+
+void sgn_j2k_encode_default_options(sgn_j2k_encode_options *opts);
+/* pixels: interleaved u8 (bit_depth <= 8) or native-endian u16 samples.
+ * On SGN_E_BUFFER *out_len holds the required size. */
+int sgn_j2k_encode_ht(const void *pixels, uint32_t width, uint32_t height,
+                      uint32_t num_components, uint32_t bit_depth, size_t row_stride,
+                      const sgn_j2k_encode_options *opts, uint8_t *out, size_t out_cap,
+                      size_t *out_len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -42,6 +42,12 @@ static uint32_t cdiv(uint32_t a, uint32_t b) { return (uint32_t)(((uint64_t)a + 
 /* ------------------------------------------------------------------ */
 /* Thread-local statistics of the last decode.                         */
 static __thread uint32_t st_blocks, st_ref_blocks, st_max_passes;
+static __thread int zero_refinement;
+
+/* Test double of the reference's ZeroRefinementHtDecoder
+ * (crates/signinum-j2k-native/src/lib.rs:2777-2792): when set, refinement
+ * bytes are zeroed before each HT code-block is decoded. */
+void oracle_set_zero_refinement(int enable) { zero_refinement = enable; }
 
 void oracle_last_stats(uint32_t *blocks, uint32_t *refinement_blocks, uint32_t *max_passes)
 {
@@ -470,7 +476,7 @@ static int build_tcomp(const header_t *h, const comp_t *c, rect_t trect, tcomp_t
         uint32_t xcb = c->xcb, ycb = c->ycb;
         if (r > 0) {
             if (rs->ppx == 0 || rs->ppy == 0) {
-                if (rs->npx * rs->npy) return ORACLE_E_MALFORMED;
+                if (rs->npx && rs->npy) return ORACLE_E_MALFORMED;
             } else {
                 if (xcb > rs->ppx - 1) xcb = rs->ppx - 1;
                 if (ycb > rs->ppy - 1) ycb = rs->ppy - 1;
@@ -533,7 +539,7 @@ static int build_tcomp(const header_t *h, const comp_t *c, rect_t trect, tcomp_t
                     uint32_t ncy = (rh(sb) == 0 || cy1 <= cy0) ? 0 : (cy1 - cy0) / rs->cbh;
                     pb->ncbx = ncx;
                     pb->ncby = ncy;
-                    if (ncx * ncy) {
+                    if (ncx && ncy) {
                         pb->cbs = (cblk_t *)calloc((size_t)ncx * ncy, sizeof(cblk_t));
                         if (!pb->cbs) return ORACLE_E_NOMEM;
                     }
@@ -1068,8 +1074,21 @@ static int decode_band(const comp_t *c, res_t *rs, int bi, uint32_t r, int use_f
                 bufcap = need;
                 if (!buf) return ORACLE_E_NOMEM;
             }
-            int rc = oracle_ht_decode_block(cb->data, cb->lcup, cb->lref, cw, ch, cb->zblk,
+            const uint8_t *src = cb->data;
+            uint8_t *zcopy = NULL;
+            if (zero_refinement && cb->lref) {
+                zcopy = (uint8_t *)malloc((size_t)cb->lcup + cb->lref);
+                if (!zcopy) {
+                    free(buf);
+                    return ORACLE_E_NOMEM;
+                }
+                memcpy(zcopy, cb->data, cb->lcup);
+                memset(zcopy + cb->lcup, 0, cb->lref);
+                src = zcopy;
+            }
+            int rc = oracle_ht_decode_block(src, cb->lcup, cb->lref, cw, ch, cb->zblk,
                                             cb->passes, stripe_causal, 2, buf, cw);
+            free(zcopy);
             if (rc) {
                 free(buf);
                 return rc;
@@ -1178,7 +1197,7 @@ static void sr_f64(double *s, size_t n, size_t step, uint32_t x0, int rev)
     {                                                                                          \
         rect_t o = rs->rect;                                                                   \
         size_t w = rw(o), h = rh(o);                                                           \
-        T *out = (T *)calloc(w * h ? w * h : 1, sizeof(T));                                    \
+        T *out = (T *)calloc((w && h) ? w * h : 1, sizeof(T));                                    \
         if (!out) return NULL;                                                                 \
         uint32_t lx0 = (o.x0 + 1) / 2, hx0 = o.x0 / 2, ly0 = (o.y0 + 1) / 2, hy0 = o.y0 / 2;   \
         size_t wl = (o.x1 + 1) / 2 - lx0, wh = o.x1 / 2 - hx0;                                 \
@@ -1629,4 +1648,50 @@ int oracle_decode_batch(const uint8_t *const *datas, const size_t *lens, size_t 
     free(jobs);
     free(th);
     return rc;
+}
+
+/* ------------------------------------------------------------------ */
+/* Stage entry points for hook parity tests                             */
+
+int oracle_idwt_level(int reversible, uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1,
+                      const float *ll, const float *hl, const float *lh, const float *hh,
+                      float *out)
+{
+    res_t rs;
+    memset(&rs, 0, sizeof(rs));
+    rs.rect.x0 = x0;
+    rs.rect.y0 = y0;
+    rs.rect.x1 = x1;
+    rs.rect.y1 = y1;
+    rs.nbands = 3;
+    rs.bands[0].coef = (float *)hl;
+    rs.bands[1].coef = (float *)lh;
+    rs.bands[2].coef = (float *)hh;
+    float *r = idwt_level_f32(ll, &rs, reversible);
+    if (!r) return ORACLE_E_NOMEM;
+    memcpy(out, r, (size_t)rw(rs.rect) * rh(rs.rect) * sizeof(float));
+    free(r);
+    return ORACLE_OK;
+}
+
+int oracle_inverse_mct(int reversible, float *p0, float *p1, float *p2, size_t n, float a0,
+                       float a1, float a2)
+{
+    for (size_t i = 0; i < n; ++i) {
+        float y0 = p0[i], y1 = p1[i], y2 = p2[i];
+        if (reversible) {
+            float i1 = y0 - floorf((y2 + y1) * 0.25f);
+            p0[i] = y2 + i1;
+            p1[i] = i1;
+            p2[i] = y1 + i1;
+        } else {
+            p0[i] = fmaf(y2, 1.402f, y0);
+            p1[i] = fmaf(y2, -0.71414f, fmaf(y1, -0.34413f, y0));
+            p2[i] = fmaf(y1, 1.772f, y0);
+        }
+        p0[i] += a0;
+        p1[i] += a1;
+        p2[i] += a2;
+    }
+    return ORACLE_OK;
 }

@@ -84,6 +84,19 @@ int oracle_ht_decode_block(const uint8_t *data, uint32_t lcup, uint32_t lref, ui
 
 int32_t oracle_coefficient_to_i32(uint32_t value, uint32_t k_max);
 
+/* One 2D synthesis level over [x0,x1) x [y0,y1) from four f32 bands
+ * (idwt.rs:535-1143); out holds the level row-major. */
+int oracle_idwt_level(int reversible, uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1,
+                      const float *ll, const float *hl, const float *lh, const float *hh,
+                      float *out);
+
+/* Inverse RCT / ICT on f32 planes followed by the addends (mct.rs:71-156). */
+int oracle_inverse_mct(int reversible, float *p0, float *p1, float *p2, size_t n, float a0,
+                       float a1, float a2);
+
+/* Test double: zero every refinement segment before block decoding. */
+void oracle_set_zero_refinement(int enable);
+
 /* Statistics of the last oracle_decode call on this thread: HT code-blocks
  * decoded, code-blocks with refinement data, maximum coding passes. */
 void oracle_last_stats(uint32_t *blocks, uint32_t *refinement_blocks, uint32_t *max_passes);

@@ -1,0 +1,502 @@
+"""signinum-j2k-hip: MI355X-native HTJ2K tile decode.
+
+A drop-in for the HTJ2K decode surface of frames-sg/signinum
+(crates/signinum-j2k/src/view.rs ``J2kDecoder``, batch.rs
+``decode_tiles_into`` and the device adapter shape of
+crates/signinum-j2k-cuda).  Host code parses codestreams into flat
+code-block tables; hand-written gfx950 kernels do HT block decoding,
+dequantisation, inverse DWT, inverse colour transform and packing.
+
+Everything here is a thin layer over the C ABI of libsigninum_j2k_hip.so.
+"""
+import ctypes
+import enum
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import (MODE_BATCHED, MODE_GRAPH, MODE_PLAIN, EncodeOptions, HtCodeBlockJob, J2kInfo,
+                   PlanStats, TileJob)
+
+__all__ = [
+    "PixelFormat", "Downscale", "BackendRequest", "J2kError", "UnsupportedError",
+    "MalformedError", "CodeBlockDecodeError", "BufferError", "DeviceError", "TileBatchError",
+    "UnsupportedBackendError", "HipSession", "J2kDecoder", "DeviceSurface", "TilePlan",
+    "decode_tiles_into", "decode_tiles_to_device", "encode_j2k_lossless", "encode_j2k_ht",
+    "HtCodeBlockDecoder", "code_block_jobs", "MODE_PLAIN", "MODE_BATCHED", "MODE_GRAPH",
+]
+
+
+class PixelFormat(enum.IntEnum):
+    """PixelFormat (crates/signinum-core/src/pixel.rs:13-58)."""
+    GRAY8 = 0
+    RGB8 = 1
+    RGBA8 = 2
+    GRAY16 = 3
+    RGB16 = 4
+
+    @property
+    def channels(self):
+        return {0: 1, 1: 3, 2: 4, 3: 1, 4: 3}[int(self)]
+
+    @property
+    def bytes_per_sample(self):
+        return 2 if self in (PixelFormat.GRAY16, PixelFormat.RGB16) else 1
+
+    @property
+    def bytes_per_pixel(self):
+        return self.channels * self.bytes_per_sample
+
+
+class Downscale(enum.IntEnum):
+    """Downscale (crates/signinum-core/src/scale.rs:3-30); value = denominator."""
+    NONE = 1
+    HALF = 2
+    QUARTER = 4
+    EIGHTH = 8
+
+    @property
+    def denominator(self):
+        return int(self)
+
+
+class BackendRequest(enum.Enum):
+    """BackendRequest (crates/signinum-core/src/backend.rs:15-22) with a HIP variant."""
+    AUTO = "auto"
+    CPU = "cpu"
+    HIP = "hip"
+
+
+class J2kError(Exception):
+    """J2kError (crates/signinum-j2k/src/error.rs:5-74)."""
+
+    def __init__(self, status, message):
+        super().__init__(f"{message} (status {status})")
+        self.status = status
+        self.message = message
+
+
+class UnsupportedError(J2kError):
+    pass
+
+
+class MalformedError(J2kError):
+    pass
+
+
+class CodeBlockDecodeError(J2kError):
+    pass
+
+
+class BufferError(J2kError):
+    pass
+
+
+class DeviceError(J2kError):
+    pass
+
+
+class UnsupportedBackendError(J2kError):
+    """Error::UnsupportedBackend (crates/signinum-j2k-cuda/src/error.rs:6-41)."""
+
+
+class TileBatchError(Exception):
+    """TileBatchError (crates/signinum-j2k/src/batch.rs:47-67)."""
+
+    def __init__(self, index, source):
+        super().__init__(f"tile {index} decode failed: {source}")
+        self.index = index
+        self.source = source
+
+
+_ERRORS = {
+    _lib.SGN_E_UNSUPPORTED: UnsupportedError,
+    _lib.SGN_E_MALFORMED: MalformedError,
+    _lib.SGN_E_BLOCK: CodeBlockDecodeError,
+    _lib.SGN_E_BUFFER: BufferError,
+    _lib.SGN_E_DEVICE: DeviceError,
+    _lib.SGN_E_NOMEM: J2kError,
+    _lib.SGN_E_ARG: J2kError,
+}
+
+
+def _check(rc):
+    if rc != _lib.SGN_OK:
+        raise _ERRORS.get(rc, J2kError)(rc, _lib.last_error())
+
+
+def _buf(data):
+    """(pointer, length, keepalive) of a bytes-like object."""
+    if isinstance(data, (bytes, bytearray)):
+        arr = np.frombuffer(data, dtype=np.uint8)
+    else:
+        arr = np.ascontiguousarray(np.asarray(data).view(np.uint8).reshape(-1))
+    return arr.ctypes.data, arr.size, arr
+
+
+def _route(backend):
+    """Route rules of the device adapters (crates/signinum-j2k-cuda/src/runtime.rs:10-59)."""
+    if backend in (None, BackendRequest.AUTO, BackendRequest.HIP):
+        return
+    raise UnsupportedBackendError(_lib.SGN_E_UNSUPPORTED,
+                                  f"backend request {backend} is not supported by signinum-j2k-hip"
+                                  " (it decodes on the device)")
+
+
+class HipSession:
+    """One HIP stream and its device tables (CudaSession,
+    crates/signinum-j2k-cuda/src/session.rs:11-49).  Use from one thread at
+    a time."""
+
+    def __init__(self, device=0):
+        self._lib = _lib.load()
+        h = ctypes.c_void_p()
+        _check(self._lib.sgn_hip_session_create(int(device), ctypes.byref(h)))
+        self.handle = h
+        self.device = device
+
+    @property
+    def stream(self):
+        return self._lib.sgn_hip_session_stream(self.handle)
+
+    @property
+    def dispatches(self):
+        return int(self._lib.sgn_hip_session_dispatches(self.handle))
+
+    def synchronize(self):
+        _check(self._lib.sgn_hip_session_synchronize(self.handle))
+
+    def close(self):
+        if self.handle:
+            self._lib.sgn_hip_session_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class DeviceSurface:
+    """DeviceSurface of the device adapters (crates/signinum-j2k-cuda/src/
+    surface.rs:11-127): a pitched device image held by a torch tensor."""
+    tensor: object
+    width: int
+    height: int
+    pixel_format: PixelFormat
+    pitch_bytes: int
+
+    @property
+    def device_ptr(self):
+        return self.tensor.data_ptr()
+
+    @property
+    def byte_len(self):
+        return self.pitch_bytes * self.height
+
+    def download_into(self, out, stride):
+        rb = self.width * self.pixel_format.bytes_per_pixel
+        host = self.tensor.cpu().numpy().reshape(-1)
+        dst = np.asarray(out).view(np.uint8).reshape(-1)
+        for y in range(self.height):
+            dst[y * stride:y * stride + rb] = host[y * self.pitch_bytes:y * self.pitch_bytes + rb]
+
+    def to_numpy(self):
+        """Tightly packed (H, W, C) array (uint8 or little-endian uint16)."""
+        rb = self.width * self.pixel_format.bytes_per_pixel
+        host = self.tensor.cpu().numpy().reshape(self.height, self.pitch_bytes)[:, :rb]
+        arr = np.ascontiguousarray(host)
+        if self.pixel_format.bytes_per_sample == 2:
+            arr = arr.view("<u2")
+        return arr.reshape(self.height, self.width, self.pixel_format.channels)
+
+
+def _alloc_surface(session, width, height, fmt):
+    import torch
+    pitch = width * fmt.bytes_per_pixel
+    t = torch.empty(max(1, pitch * height), dtype=torch.uint8, device=f"cuda:{session.device}")
+    return DeviceSurface(t, width, height, fmt, pitch)
+
+
+def scaled_dims(data, scale=Downscale.NONE):
+    ptr, n, _keep = _buf(data)
+    w = ctypes.c_uint32()
+    h = ctypes.c_uint32()
+    _check(_lib.load().sgn_j2k_scaled_dims(ptr, n, int(scale), ctypes.byref(w), ctypes.byref(h)))
+    return w.value, h.value
+
+
+class J2kDecoder:
+    """J2kDecoder (crates/signinum-j2k/src/view.rs:84-311; device adapter
+    crates/signinum-j2k-cuda/src/decoder.rs:17-280)."""
+
+    def __init__(self, data, session=None):
+        self._data = bytes(data)
+        self._info = J2kDecoder.inspect(self._data)
+        self._session = session
+        self._own_session = None
+
+    @staticmethod
+    def inspect(data):
+        ptr, n, _keep = _buf(data)
+        info = J2kInfo()
+        _check(_lib.load().sgn_j2k_inspect(ptr, n, ctypes.byref(info)))
+        return {f: getattr(info, f) for f, _ in J2kInfo._fields_}
+
+    @property
+    def info(self):
+        return dict(self._info)
+
+    @property
+    def dimensions(self):
+        return self._info["width"], self._info["height"]
+
+    def _sess(self):
+        if self._session is None:
+            if self._own_session is None:
+                self._own_session = HipSession()
+            return self._own_session
+        return self._session
+
+    def decode_into(self, out, stride, fmt):
+        return self.decode_scaled_into(None, out, stride, fmt, Downscale.NONE)
+
+    def decode_into_with_scratch(self, pool, out, stride, fmt):
+        return self.decode_scaled_into(pool, out, stride, fmt, Downscale.NONE)
+
+    def decode_scaled_into(self, pool, out, stride, fmt, scale):
+        """Decodes on the device and copies to host `out` (row stride in bytes)."""
+        fmt = PixelFormat(fmt)
+        scale = Downscale(scale)
+        dst = np.asarray(out)
+        if not dst.flags["C_CONTIGUOUS"] or not dst.flags["WRITEABLE"]:
+            raise BufferError(_lib.SGN_E_BUFFER, "output must be a writable contiguous array")
+        s = self._sess()
+        ptr, n, _keep = _buf(self._data)
+        _check(_lib.load().sgn_j2k_decode_into(s.handle, ptr, n, int(fmt), int(scale),
+                                               dst.ctypes.data, dst.nbytes, int(stride)))
+        w, h = scaled_dims(self._data, scale)
+        return {"decoded": (0, 0, w, h)}
+
+    def submit_to_device(self, session, fmt, backend=BackendRequest.HIP, scale=Downscale.NONE):
+        """ImageDecodeSubmit::submit_to_device / submit_scaled_to_device."""
+        _route(backend)
+        fmt = PixelFormat(fmt)
+        w, h = scaled_dims(self._data, scale)
+        surf = _alloc_surface(session, w, h, fmt)
+        ptr, n, _keep = _buf(self._data)
+        _check(_lib.load().sgn_j2k_decode_device(session.handle, ptr, n, int(fmt), int(scale),
+                                                 surf.device_ptr, surf.pitch_bytes,
+                                                 surf.byte_len))
+        return surf
+
+    def decode_to_numpy(self, fmt, scale=Downscale.NONE):
+        fmt = PixelFormat(fmt)
+        w, h = scaled_dims(self._data, scale)
+        out = np.zeros(h * w * fmt.bytes_per_pixel, np.uint8)
+        self.decode_scaled_into(None, out, w * fmt.bytes_per_pixel, fmt, scale)
+        arr = out.view("<u2") if fmt.bytes_per_sample == 2 else out
+        return arr.reshape(h, w, fmt.channels)
+
+
+class TilePlan:
+    """A parsed batch resident in device memory: codestream bytes, the
+    code-block table and the synthesis/packing jobs (the device form of
+    J2kDirectColorPlan, crates/signinum-j2k-native/src/direct_plan.rs:12-205).
+    decode() re-runs the whole batch without host work."""
+
+    def __init__(self, session, inputs, fmt, scale=Downscale.NONE, surfaces=None):
+        self.session = session
+        self.fmt = PixelFormat(fmt)
+        self.scale = Downscale(scale)
+        self._inputs = [bytes(b) for b in inputs]
+        self._keep = [np.frombuffer(b, np.uint8) for b in self._inputs]
+        if surfaces is None:
+            surfaces = []
+            for b in self._inputs:
+                w, h = scaled_dims(b, self.scale)
+                surfaces.append(_alloc_surface(session, w, h, self.fmt))
+        self.surfaces = surfaces
+        n = len(self._inputs)
+        jobs = (TileJob * max(1, n))()
+        for i, (k, s) in enumerate(zip(self._keep, surfaces)):
+            jobs[i] = TileJob(k.ctypes.data, k.size, s.device_ptr, s.pitch_bytes, s.byte_len)
+        status = (ctypes.c_int32 * max(1, n))()
+        h = ctypes.c_void_p()
+        rc = _lib.load().sgn_j2k_plan_create(session.handle, jobs, n, int(self.fmt),
+                                             int(self.scale), ctypes.byref(h), status)
+        if rc != _lib.SGN_OK:
+            bad = next((i for i in range(n) if status[i] != 0), 0)
+            raise TileBatchError(bad, _ERRORS.get(rc, J2kError)(rc, _lib.last_error()))
+        self.handle = h
+
+    def decode(self, mode=MODE_GRAPH):
+        _check(_lib.load().sgn_j2k_plan_decode(self.session.handle, self.handle, int(mode)))
+
+    def decode_stage(self, stage):
+        """One stage of the batched decode: 0 blocks, 1 levels, 2 final."""
+        _check(_lib.load().sgn_j2k_plan_decode_stage(self.session.handle, self.handle, int(stage)))
+
+    def stats(self):
+        st = PlanStats()
+        _check(_lib.load().sgn_j2k_plan_get_stats(self.handle, ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in PlanStats._fields_}
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.load().sgn_j2k_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def decode_tiles_to_device(session, inputs, fmt, scale=Downscale.NONE, mode=MODE_BATCHED,
+                           backend=BackendRequest.HIP):
+    """TileBatchDecodeManyDevice::decode_tiles_to_device
+    (crates/signinum-j2k-cuda/src/codec.rs:190-210): decodes every input in
+    one batch and returns one DeviceSurface per input, in input order."""
+    _route(backend)
+    fmt = PixelFormat(fmt)
+    scale = Downscale(scale)
+    inputs = [bytes(b) for b in inputs]
+    keep = [np.frombuffer(b, np.uint8) for b in inputs]
+    surfaces = []
+    for b in inputs:
+        try:
+            w, h = scaled_dims(b, scale)
+        except J2kError as e:
+            raise TileBatchError(len(surfaces), e)
+        surfaces.append(_alloc_surface(session, w, h, fmt))
+    n = len(inputs)
+    jobs = (TileJob * max(1, n))()
+    for i, (k, s) in enumerate(zip(keep, surfaces)):
+        jobs[i] = TileJob(k.ctypes.data, k.size, s.device_ptr, s.pitch_bytes, s.byte_len)
+    ff = ctypes.c_size_t(0)
+    rc = _lib.load().sgn_j2k_decode_tiles_device(session.handle, jobs, n, int(fmt), int(scale),
+                                                 int(mode), ctypes.byref(ff))
+    if rc != _lib.SGN_OK:
+        raise TileBatchError(ff.value, _ERRORS.get(rc, J2kError)(rc, _lib.last_error()))
+    return surfaces
+
+
+def decode_tiles_into(jobs, fmt, session=None, scale=Downscale.NONE, mode=MODE_BATCHED):
+    """decode_tiles_into (crates/signinum-j2k/src/batch.rs:110-147): jobs are
+    (input_bytes, out_array, stride) triples; outputs are written in place.
+    Raises TileBatchError with the first failing index."""
+    own = session is None
+    if own:
+        session = HipSession()
+    try:
+        surfs = decode_tiles_to_device(session, [j[0] for j in jobs], fmt, scale, mode)
+        for (inp, out, stride), s in zip(jobs, surfs):
+            s.download_into(out, stride)
+    finally:
+        if own:
+            session.close()
+
+
+def default_encode_options():
+    o = EncodeOptions()
+    _lib.load().sgn_j2k_encode_default_options(ctypes.byref(o))
+    return o
+
+
+def encode_j2k_ht(samples, bit_depth=None, **options):
+    """HTJ2K encode of an (H, W) or (H, W, C) uint8/uint16 array.  Options
+    mirror sgn_j2k_encode_options: levels, cb_width_log2, cb_height_log2,
+    reversible, use_mct, progression, guard_bits, base_step, precinct_log2,
+    sop, eph, tile_width, tile_height."""
+    arr = np.ascontiguousarray(samples)
+    if arr.ndim == 2:
+        arr = arr[:, :, None]
+    h, w, c = arr.shape
+    if bit_depth is None:
+        bit_depth = 8 if arr.dtype == np.uint8 else 16
+    if arr.dtype not in (np.uint8, np.uint16) or (bit_depth > 8) != (arr.dtype == np.uint16):
+        raise J2kError(_lib.SGN_E_ARG, "samples must be uint8 (<= 8 bits) or uint16")
+    o = default_encode_options()
+    for k, v in options.items():
+        if not hasattr(o, k):
+            raise TypeError(f"unknown encode option {k}")
+        setattr(o, k, v)
+    cap = arr.nbytes * 2 + 65536
+    out = np.empty(cap, np.uint8)
+    n = ctypes.c_size_t(0)
+    lib = _lib.load()
+    rc = lib.sgn_j2k_encode_ht(arr.ctypes.data, w, h, c, bit_depth, w * c * arr.itemsize,
+                               ctypes.byref(o), out.ctypes.data, cap, ctypes.byref(n))
+    if rc == _lib.SGN_E_BUFFER and n.value > cap:
+        out = np.empty(n.value, np.uint8)
+        rc = lib.sgn_j2k_encode_ht(arr.ctypes.data, w, h, c, bit_depth, w * c * arr.itemsize,
+                                   ctypes.byref(o), out.ctypes.data, n.value, ctypes.byref(n))
+    _check(rc)
+    return out[:n.value].tobytes()
+
+
+def encode_j2k_lossless(samples, bit_depth=None, levels=5, cb_log2=6, progression=0):
+    """encode_j2k_lossless with HTJ2K block coding, reversible 5/3 and RCT
+    (crates/signinum-j2k/src/encode.rs:166-182)."""
+    return encode_j2k_ht(samples, bit_depth=bit_depth, levels=levels, cb_width_log2=cb_log2,
+                         cb_height_log2=cb_log2, reversible=1, use_mct=1,
+                         progression=progression)
+
+
+def code_block_jobs(data):
+    """HT code-block jobs of a full-resolution decode (sgn_j2k_code_blocks)."""
+    ptr, n, _keep = _buf(data)
+    lib = _lib.load()
+    cnt = ctypes.c_size_t(0)
+    _check(lib.sgn_j2k_code_blocks(ptr, n, None, 0, ctypes.byref(cnt)))
+    arr = (HtCodeBlockJob * max(1, cnt.value))()
+    _check(lib.sgn_j2k_code_blocks(ptr, n, arr, cnt.value, ctypes.byref(cnt)))
+    return list(arr[:cnt.value])
+
+
+class HtCodeBlockDecoder:
+    """Stage hooks of HtCodeBlockDecoder (crates/signinum-j2k-native/src/
+    lib.rs:832-909), executed on the device with host buffers."""
+
+    def __init__(self, session):
+        self.session = session
+
+    def decode_code_blocks(self, data, jobs, out_len, raw=False):
+        arr = (HtCodeBlockJob * max(1, len(jobs)))(*jobs)
+        ptr, n, _keep = _buf(data)
+        lib = _lib.load()
+        if raw:
+            out = np.zeros(out_len, np.uint32)
+            _check(lib.sgn_ht_decode_code_blocks_raw(self.session.handle, ptr, n, arr, len(jobs),
+                                                     out.ctypes.data, out_len))
+        else:
+            out = np.zeros(out_len, np.float32)
+            _check(lib.sgn_ht_decode_code_blocks(self.session.handle, ptr, n, arr, len(jobs),
+                                                 out.ctypes.data, out_len))
+        return out
+
+    def decode_single_decomposition_idwt(self, reversible, rect, ll, hl, lh, hh):
+        x0, y0, x1, y1 = rect
+        out = np.zeros((y1 - y0) * (x1 - x0), np.float32)
+        bands = [np.ascontiguousarray(b, dtype=np.float32) for b in (ll, hl, lh, hh)]
+        _check(_lib.load().sgn_idwt_level(self.session.handle, int(bool(reversible)), x0, y0, x1,
+                                          y1, *[b.ctypes.data for b in bands], out.ctypes.data))
+        return out.reshape(y1 - y0, x1 - x0)
+
+    def decode_inverse_mct(self, reversible, p0, p1, p2, addends):
+        planes = [np.ascontiguousarray(p, dtype=np.float32).copy() for p in (p0, p1, p2)]
+        _check(_lib.load().sgn_inverse_mct(self.session.handle, int(bool(reversible)),
+                                           *[p.ctypes.data for p in planes], planes[0].size,
+                                           *[float(a) for a in addends]))
+        return planes

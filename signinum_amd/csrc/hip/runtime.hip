@@ -1,0 +1,863 @@
+// HIP runtime of signinum-j2k-hip: sessions, batch plans, launch sequences
+// and the C ABI entry points that need the device.
+//
+// The role matches the reference's CUDA runtime helper
+// (crates/signinum-cuda-runtime/src/lib.rs:281-775: context, buffers,
+// launches) and the adapter's decode entry points
+// (crates/signinum-j2k-cuda/src/decoder.rs:17-280, codec.rs:18-210), but the
+// decode itself runs on the device instead of uploading CPU-decoded pixels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "../host/parse.hpp"
+#include "../host/sgn_api.hpp"
+#include "ht_decode.hip"
+#include "idwt.hip"
+
+using namespace sgn;
+
+#define HIP_TRY(expr)                                                              \
+    do {                                                                           \
+        hipError_t e_ = (expr);                                                    \
+        if (e_ != hipSuccess) {                                                    \
+            set_last_error(std::string("HIP: ") + hipGetErrorString(e_) + " at " #expr); \
+            return sgn::kErrDevice;                                                \
+        }                                                                          \
+    } while (0)
+
+struct sgn_hip_session {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint16_t *d_tables = nullptr;
+    uint64_t dispatches = 0;
+};
+
+namespace {
+
+constexpr uint32_t kLaneMaxDim = 64;
+
+bool lane_eligible(const CbJob &j) {
+    return j.w <= kLaneMaxDim && j.h <= kLaneMaxDim && ((j.flags & kCbZero) || j.passes == 1);
+}
+
+struct LevelRange {
+    uint8_t rev, depth;
+    uint32_t start, count;
+};
+
+template <typename T>
+Status dev_upload(T *&dptr, const std::vector<T> &v) {
+    dptr = nullptr;
+    if (v.empty()) return kOk;
+    if (hipMalloc(&dptr, v.size() * sizeof(T)) != hipSuccess) {
+        set_last_error("hipMalloc failed for plan tables");
+        return kErrDevice;
+    }
+    if (hipMemcpy(dptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) {
+        set_last_error("plan table upload failed");
+        return kErrDevice;
+    }
+    return kOk;
+}
+
+}  // namespace
+
+struct sgn_j2k_plan {
+    int fmt = 0;
+    uint32_t den = 1;
+    size_t ntiles = 0;
+    // host tables
+    std::vector<CbJob> cb_lane, cb_lane_plain, cb_wave;
+    std::vector<uint32_t> lane_plain_begin, wave_begin;  // per tile, size ntiles+1
+    std::vector<LevelJob> lv[2];                          // [rev] sorted by (depth, tile)
+    std::vector<uint32_t> lv_group[2];                    // start index of each depth, size D+1
+    std::vector<std::vector<LevelRange>> tile_levels;     // per tile
+    std::vector<LevelJob> final_lv;
+    std::vector<FinalJob> fin[2];
+    std::vector<std::pair<uint8_t, uint32_t>> tile_final_first;  // (rev, first index)
+    std::vector<uint32_t> tile_final_count;
+    std::vector<uint32_t> tile_width, tile_height;
+    std::vector<size_t> tile_row_bytes;
+    uint32_t depth_max = 0;
+    uint64_t coef_elems = 0, arena_len = 0, out_bytes = 0, out_pixels = 0;
+    // device
+    uint8_t *d_arena = nullptr;
+    uint32_t *d_coef = nullptr;
+    CbJob *d_cb_lane = nullptr, *d_cb_lane_plain = nullptr, *d_cb_wave = nullptr;
+    LevelJob *d_lv[2] = {nullptr, nullptr};
+    LevelJob *d_final_lv = nullptr;
+    FinalJob *d_fin[2] = {nullptr, nullptr};
+    uint8_t *d_status_lane = nullptr, *d_status_wave = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    uint32_t launches_batched = 0;
+
+    ~sgn_j2k_plan() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        void *ptrs[] = {d_arena, d_coef, d_cb_lane, d_cb_lane_plain, d_cb_wave, d_lv[0],
+                        d_lv[1], d_final_lv, d_fin[0], d_fin[1], d_status_lane, d_status_wave};
+        for (void *p : ptrs)
+            if (p) (void)hipFree(p);
+    }
+};
+
+namespace {
+
+Status build_plan(const sgn_tile_job *jobs, size_t n, int fmt, uint32_t den, sgn_j2k_plan &P,
+                  int32_t *tile_status) {
+    P.fmt = fmt;
+    P.den = den;
+    P.ntiles = n;
+    std::vector<LocalPlan> lps(n);
+    std::vector<int32_t> st(n, 0);
+    std::vector<std::string> errs(n);
+    const size_t nthreads =
+        std::max<size_t>(1, std::min<size_t>(n, std::min(16u, std::max(1u, std::thread::hardware_concurrency()))));
+    auto work = [&](size_t t0) {
+        for (size_t i = t0; i < n; i += nthreads) {
+            st[i] = build_local_plan(jobs[i].bytes, jobs[i].len, fmt, den,
+                                     reinterpret_cast<uint64_t>(jobs[i].dst), jobs[i].pitch,
+                                     jobs[i].capacity, lps[i]);
+            if (st[i]) errs[i] = last_error();
+        }
+    };
+    if (nthreads == 1) {
+        work(0);
+    } else {
+        std::vector<std::thread> th;
+        for (size_t t = 0; t < nthreads; ++t) th.emplace_back(work, t);
+        for (auto &t : th) t.join();
+    }
+    int32_t first_bad = 0;
+    size_t bad_index = 0;
+    for (size_t i = 0; i < n; ++i) {
+        if (tile_status) tile_status[i] = st[i];
+        if (st[i] && !first_bad) {
+            first_bad = st[i];
+            bad_index = i;
+        }
+    }
+    if (first_bad) {
+        set_last_error("tile " + std::to_string(bad_index) + ": " + errs[bad_index]);
+        return static_cast<Status>(first_bad);
+    }
+    // ---- merge with rebasing ----
+    uint64_t arena = 0, coef = 0;
+    std::vector<uint64_t> arena_base(n), coef_base(n);
+    for (size_t i = 0; i < n; ++i) {
+        arena_base[i] = arena;
+        coef_base[i] = coef;
+        arena += (jobs[i].len + 15) & ~size_t(15);
+        coef += lps[i].coef_elems;
+    }
+    if (arena >= (1ull << 32) || coef >= (1ull << 32)) {
+        set_last_error("batch too large for 32-bit plan offsets; split it");
+        return kErrArg;
+    }
+    P.arena_len = arena;
+    P.coef_elems = coef;
+    P.lane_plain_begin.assign(1, 0);
+    P.wave_begin.assign(1, 0);
+    P.tile_levels.assign(n, {});
+    P.tile_final_first.assign(n, {0, 0});
+    P.tile_final_count.assign(n, 0);
+    P.tile_width.resize(n);
+    P.tile_height.resize(n);
+    P.tile_row_bytes.resize(n);
+    for (size_t i = 0; i < n; ++i) {
+        LocalPlan &lp = lps[i];
+        P.tile_width[i] = lp.width;
+        P.tile_height[i] = lp.height;
+        P.tile_row_bytes[i] = lp.row_bytes;
+        P.out_bytes += lp.out_bytes;
+        P.out_pixels += uint64_t(lp.width) * lp.height;
+        for (CbJob j : lp.cbs) {
+            j.src += static_cast<uint32_t>(arena_base[i]);
+            j.dst += static_cast<uint32_t>(coef_base[i]);
+            j.tile = static_cast<uint32_t>(i);
+            if (lane_eligible(j))
+                P.cb_lane_plain.push_back(j);
+            else
+                P.cb_wave.push_back(j);
+        }
+        P.lane_plain_begin.push_back(static_cast<uint32_t>(P.cb_lane_plain.size()));
+        P.wave_begin.push_back(static_cast<uint32_t>(P.cb_wave.size()));
+        for (LevelJob &l : lp.levels) {
+            for (int b = 0; b < 4; ++b) l.band[b] += static_cast<uint32_t>(coef_base[i]);
+            l.out += static_cast<uint32_t>(coef_base[i]);
+        }
+        for (LevelJob &l : lp.finals_lv)
+            for (int b = 0; b < 4; ++b) l.band[b] += static_cast<uint32_t>(coef_base[i]);
+        for (size_t d = 0; d < lp.level_depth.size(); ++d)
+            P.depth_max = std::max<uint32_t>(P.depth_max, lp.level_depth[d]);
+    }
+    // code-blocks for the batched lane kernel, ordered by predicted cost so
+    // that a wave's lanes finish together
+    P.cb_lane = P.cb_lane_plain;
+    std::stable_sort(P.cb_lane.begin(), P.cb_lane.end(),
+                     [](const CbJob &a, const CbJob &b) { return a.cost > b.cost; });
+    // synthesis levels grouped by (transform, depth), tiles in order
+    for (int rev = 0; rev < 2; ++rev) {
+        P.lv_group[rev].assign(1, 0);
+        uint32_t blocks = 0;
+        for (uint32_t d = 1; d <= P.depth_max; ++d) {
+            for (size_t i = 0; i < n; ++i) {
+                const LocalPlan &lp = lps[i];
+                LevelRange rg{static_cast<uint8_t>(rev), static_cast<uint8_t>(d),
+                              static_cast<uint32_t>(P.lv[rev].size()), 0};
+                for (size_t k = 0; k < lp.levels.size(); ++k) {
+                    if (lp.level_depth[k] != d || lp.levels[k].reversible != rev) continue;
+                    LevelJob l = lp.levels[k];
+                    l.block_base = blocks;
+                    blocks += uint32_t(l.nbx) * l.nby;
+                    P.lv[rev].push_back(l);
+                    rg.count++;
+                }
+                if (rg.count) P.tile_levels[i].push_back(rg);
+            }
+            P.lv_group[rev].push_back(static_cast<uint32_t>(P.lv[rev].size()));
+        }
+    }
+    // finals
+    uint32_t fblocks[2] = {0, 0};
+    for (size_t i = 0; i < n; ++i) {
+        LocalPlan &lp = lps[i];
+        const uint32_t lv_base = static_cast<uint32_t>(P.final_lv.size());
+        P.final_lv.insert(P.final_lv.end(), lp.finals_lv.begin(), lp.finals_lv.end());
+        for (size_t f = 0; f < lp.finals.size(); ++f) {
+            FinalJob fj = lp.finals[f];
+            for (int c = 0; c < 4; ++c) fj.level[c] += lv_base;
+            const int rev = fj.reversible ? 1 : 0;
+            for (int c = 0; c < fj.ncomp && c < 3; ++c)
+                if (P.final_lv[fj.level[c]].reversible != rev) {
+                    set_last_error("tile " + std::to_string(i) +
+                                   ": components with different wavelet transforms");
+                    return kErrUnsupported;
+                }
+            fj.block_base = fblocks[rev];
+            fblocks[rev] += uint32_t(fj.nbx) * fj.nby;
+            if (f == 0) P.tile_final_first[i] = {static_cast<uint8_t>(rev),
+                                                 static_cast<uint32_t>(P.fin[rev].size())};
+            P.fin[rev].push_back(fj);
+            P.tile_final_count[i]++;
+        }
+    }
+    return kOk;
+}
+
+Status upload_plan(const sgn_tile_job *jobs, sgn_j2k_plan &P) {
+    if (P.arena_len) {
+        HIP_TRY(hipMalloc(&P.d_arena, P.arena_len));
+        std::vector<uint8_t> staging(P.arena_len, 0);
+        uint64_t off = 0;
+        for (size_t i = 0; i < P.ntiles; ++i) {
+            std::memcpy(staging.data() + off, jobs[i].bytes, jobs[i].len);
+            off += (jobs[i].len + 15) & ~size_t(15);
+        }
+        HIP_TRY(hipMemcpy(P.d_arena, staging.data(), P.arena_len, hipMemcpyHostToDevice));
+    }
+    if (P.coef_elems) HIP_TRY(hipMalloc(&P.d_coef, P.coef_elems * sizeof(uint32_t)));
+    Status st;
+    if ((st = dev_upload(P.d_cb_lane, P.cb_lane))) return st;
+    if ((st = dev_upload(P.d_cb_lane_plain, P.cb_lane_plain))) return st;
+    if ((st = dev_upload(P.d_cb_wave, P.cb_wave))) return st;
+    for (int r = 0; r < 2; ++r) {
+        if ((st = dev_upload(P.d_lv[r], P.lv[r]))) return st;
+        if ((st = dev_upload(P.d_fin[r], P.fin[r]))) return st;
+    }
+    if ((st = dev_upload(P.d_final_lv, P.final_lv))) return st;
+    if (!P.cb_lane.empty()) HIP_TRY(hipMalloc(&P.d_status_lane, P.cb_lane.size()));
+    if (!P.cb_wave.empty()) HIP_TRY(hipMalloc(&P.d_status_wave, P.cb_wave.size()));
+    return kOk;
+}
+
+uint32_t group_blocks(const std::vector<LevelJob> &v, uint32_t a, uint32_t b) {
+    if (a >= b) return 0;
+    return v[b - 1].block_base + uint32_t(v[b - 1].nbx) * v[b - 1].nby - v[a].block_base;
+}
+
+uint32_t final_blocks(const std::vector<FinalJob> &v, uint32_t a, uint32_t b) {
+    if (a >= b) return 0;
+    return v[b - 1].block_base + uint32_t(v[b - 1].nbx) * v[b - 1].nby - v[a].block_base;
+}
+
+// Enqueue helpers; each returns the number of kernel launches issued.
+uint32_t launch_lane(sgn_hip_session *s, sgn_j2k_plan &P, const CbJob *d_jobs, uint32_t count,
+                     uint8_t *status) {
+    if (!count) return 0;
+    const uint32_t grid = (count + kLaneWg - 1) / kLaneWg;
+    hipLaunchKernelGGL(sgn_ht_cleanup_lanes, dim3(grid), dim3(kLaneWg), 0, s->stream, d_jobs,
+                       count, P.d_arena, reinterpret_cast<int32_t *>(P.d_coef), s->d_tables, status);
+    return 1;
+}
+
+uint32_t launch_wave(sgn_hip_session *s, sgn_j2k_plan &P, const CbJob *d_jobs, uint32_t count,
+                     uint8_t *status) {
+    if (!count) return 0;
+    hipLaunchKernelGGL(sgn_ht_block_wave, dim3(count), dim3(64), 0, s->stream, d_jobs, count,
+                       P.d_arena, reinterpret_cast<int32_t *>(P.d_coef), s->d_tables, status, 0u);
+    return 1;
+}
+
+uint32_t launch_levels(sgn_hip_session *s, sgn_j2k_plan &P, int rev, uint32_t a, uint32_t b) {
+    const uint32_t blocks = group_blocks(P.lv[rev], a, b);
+    if (!blocks) return 0;
+    const uint32_t off = P.lv[rev][a].block_base;
+    if (rev)
+        hipLaunchKernelGGL(sgn_idwt_level_rev, dim3(blocks), dim3(kIdwtThreads), 0, s->stream,
+                           P.d_lv[1] + a, b - a, off, P.d_coef);
+    else
+        hipLaunchKernelGGL(sgn_idwt_level_irr, dim3(blocks), dim3(kIdwtThreads), 0, s->stream,
+                           P.d_lv[0] + a, b - a, off, P.d_coef);
+    return 1;
+}
+
+uint32_t launch_finals(sgn_hip_session *s, sgn_j2k_plan &P, int rev, uint32_t a, uint32_t b) {
+    const uint32_t blocks = final_blocks(P.fin[rev], a, b);
+    if (!blocks) return 0;
+    const uint32_t off = P.fin[rev][a].block_base;
+    if (rev)
+        hipLaunchKernelGGL(sgn_final_rev, dim3(blocks), dim3(kIdwtThreads), 0, s->stream,
+                           P.d_fin[1] + a, b - a, off, P.d_final_lv, P.d_coef);
+    else
+        hipLaunchKernelGGL(sgn_final_irr, dim3(blocks), dim3(kIdwtThreads), 0, s->stream,
+                           P.d_fin[0] + a, b - a, off, P.d_final_lv, P.d_coef);
+    return 1;
+}
+
+uint32_t enqueue_batched(sgn_hip_session *s, sgn_j2k_plan &P) {
+    uint32_t k = 0;
+    k += launch_lane(s, P, P.d_cb_lane, static_cast<uint32_t>(P.cb_lane.size()), P.d_status_lane);
+    k += launch_wave(s, P, P.d_cb_wave, static_cast<uint32_t>(P.cb_wave.size()), P.d_status_wave);
+    for (uint32_t d = 1; d <= P.depth_max; ++d)
+        for (int rev = 0; rev < 2; ++rev)
+            k += launch_levels(s, P, rev, P.lv_group[rev][d - 1], P.lv_group[rev][d]);
+    for (int rev = 0; rev < 2; ++rev)
+        k += launch_finals(s, P, rev, 0, static_cast<uint32_t>(P.fin[rev].size()));
+    return k;
+}
+
+uint32_t enqueue_plain(sgn_hip_session *s, sgn_j2k_plan &P) {
+    uint32_t k = 0;
+    for (size_t t = 0; t < P.ntiles; ++t) {
+        const uint32_t a0 = P.lane_plain_begin[t], a1 = P.lane_plain_begin[t + 1];
+        k += launch_lane(s, P, P.d_cb_lane_plain + a0, a1 - a0, P.d_status_lane + a0);
+        const uint32_t b0 = P.wave_begin[t], b1 = P.wave_begin[t + 1];
+        k += launch_wave(s, P, P.d_cb_wave + b0, b1 - b0, P.d_status_wave + b0);
+        for (const LevelRange &rg : P.tile_levels[t])
+            k += launch_levels(s, P, rg.rev, rg.start, rg.start + rg.count);
+        if (P.tile_final_count[t]) {
+            const auto ff = P.tile_final_first[t];
+            k += launch_finals(s, P, ff.first, ff.second, ff.second + P.tile_final_count[t]);
+        }
+    }
+    return k;
+}
+
+Status decode_plan(sgn_hip_session *s, sgn_j2k_plan &P, int mode) {
+    if (mode == SGN_MODE_PLAIN) {
+        s->dispatches += enqueue_plain(s, P);
+    } else if (mode == SGN_MODE_BATCHED) {
+        P.launches_batched = enqueue_batched(s, P);
+        s->dispatches += P.launches_batched;
+    } else if (mode == SGN_MODE_GRAPH) {
+        if (!P.graph_exec) {
+            HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+            P.launches_batched = enqueue_batched(s, P);
+            HIP_TRY(hipStreamEndCapture(s->stream, &P.graph));
+            HIP_TRY(hipGraphInstantiate(&P.graph_exec, P.graph, nullptr, nullptr, 0));
+        }
+        HIP_TRY(hipGraphLaunch(P.graph_exec, s->stream));
+        s->dispatches += P.launches_batched;
+    } else {
+        set_last_error("unknown decode mode");
+        return kErrArg;
+    }
+    HIP_TRY(hipGetLastError());
+    return kOk;
+}
+
+// First failing tile from the per-block status words (after a sync).
+Status check_status(sgn_j2k_plan &P, int mode, size_t *first_fail) {
+    size_t bad = SIZE_MAX;
+    if (!P.cb_lane.empty()) {
+        std::vector<uint8_t> h(P.cb_lane.size());
+        HIP_TRY(hipMemcpy(h.data(), P.d_status_lane, h.size(), hipMemcpyDeviceToHost));
+        const std::vector<CbJob> &jobs = mode == SGN_MODE_PLAIN ? P.cb_lane_plain : P.cb_lane;
+        for (size_t k = 0; k < h.size(); ++k)
+            if (h[k]) bad = std::min<size_t>(bad, jobs[k].tile);
+    }
+    if (!P.cb_wave.empty()) {
+        std::vector<uint8_t> h(P.cb_wave.size());
+        HIP_TRY(hipMemcpy(h.data(), P.d_status_wave, h.size(), hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < h.size(); ++k)
+            if (h[k]) bad = std::min<size_t>(bad, P.cb_wave[k].tile);
+    }
+    if (bad != SIZE_MAX) {
+        if (first_fail) *first_fail = bad;
+        set_last_error("code-block decode failure in tile " + std::to_string(bad));
+        return kErrBlock;
+    }
+    return kOk;
+}
+
+Status make_plan(sgn_hip_session *s, const sgn_tile_job *jobs, size_t n, int fmt, uint32_t den,
+                 std::unique_ptr<sgn_j2k_plan> &out, int32_t *tile_status) {
+    if (!s || (!jobs && n)) {
+        set_last_error("null session or jobs");
+        return kErrArg;
+    }
+    HIP_TRY(hipSetDevice(s->device));
+    auto P = std::make_unique<sgn_j2k_plan>();
+    Status st = build_plan(jobs, n, fmt, den, *P, tile_status);
+    if (st) return st;
+    st = upload_plan(jobs, *P);
+    if (st) return st;
+    out = std::move(P);
+    return kOk;
+}
+
+}  // namespace
+
+#define API_GUARD_BEGIN try {
+#define API_GUARD_END                                \
+    }                                                \
+    catch (const std::bad_alloc &) {                 \
+        set_last_error("out of host memory");        \
+        return SGN_E_NOMEM;                          \
+    }                                                \
+    catch (const std::exception &e) {                \
+        set_last_error(e.what());                    \
+        return SGN_E_ARG;                            \
+    }
+
+extern "C" {
+
+int sgn_hip_session_create(int device, sgn_hip_session **out) {
+    API_GUARD_BEGIN
+    if (!out) return SGN_E_ARG;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+        set_last_error("no HIP device available");
+        return SGN_E_DEVICE;
+    }
+    if (device < 0 || device >= count) {
+        set_last_error("device index out of range");
+        return SGN_E_ARG;
+    }
+    auto s = std::make_unique<sgn_hip_session>();
+    s->device = device;
+    HIP_TRY(hipSetDevice(device));
+    HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    const HtTables &T = ht_tables();
+    std::vector<uint16_t> tab(2624);
+    std::memcpy(tab.data(), T.vlc0, 2048);
+    std::memcpy(tab.data() + 1024, T.vlc1, 2048);
+    std::memcpy(tab.data() + 2048, T.uvlc0, 640);
+    std::memcpy(tab.data() + 2368, T.uvlc1, 512);
+    HIP_TRY(hipMalloc(&s->d_tables, tab.size() * 2));
+    HIP_TRY(hipMemcpy(s->d_tables, tab.data(), tab.size() * 2, hipMemcpyHostToDevice));
+    *out = s.release();
+    return SGN_OK;
+    API_GUARD_END
+}
+
+void sgn_hip_session_destroy(sgn_hip_session *s) {
+    if (!s) return;
+    (void)hipSetDevice(s->device);
+    if (s->stream) {
+        (void)hipStreamSynchronize(s->stream);
+        (void)hipStreamDestroy(s->stream);
+    }
+    if (s->d_tables) (void)hipFree(s->d_tables);
+    delete s;
+}
+
+void *sgn_hip_session_stream(sgn_hip_session *s) { return s ? s->stream : nullptr; }
+
+int sgn_hip_session_synchronize(sgn_hip_session *s) {
+    if (!s) return SGN_E_ARG;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return SGN_OK;
+}
+
+uint64_t sgn_hip_session_dispatches(sgn_hip_session *s) { return s ? s->dispatches : 0; }
+
+int sgn_j2k_plan_create(sgn_hip_session *s, const sgn_tile_job *jobs, size_t n, int fmt,
+                        uint32_t den, sgn_j2k_plan **out, int32_t *tile_status) {
+    API_GUARD_BEGIN
+    if (!out) return SGN_E_ARG;
+    *out = nullptr;
+    std::unique_ptr<sgn_j2k_plan> P;
+    const Status st = make_plan(s, jobs, n, fmt, den, P, tile_status);
+    if (st) return st;
+    *out = P.release();
+    return SGN_OK;
+    API_GUARD_END
+}
+
+void sgn_j2k_plan_destroy(sgn_j2k_plan *p) { delete p; }
+
+int sgn_j2k_plan_set_output(sgn_j2k_plan *P, size_t i, void *dst, size_t pitch, size_t capacity) {
+    API_GUARD_BEGIN
+    if (!P || i >= P->ntiles) return SGN_E_ARG;
+    if (pitch < P->tile_row_bytes[i] ||
+        (P->tile_height[i] &&
+         capacity < pitch * (P->tile_height[i] - 1) + P->tile_row_bytes[i])) {
+        set_last_error("output surface too small for the decoded image");
+        return SGN_E_BUFFER;
+    }
+    const auto ff = P->tile_final_first[i];
+    std::vector<FinalJob> &v = P->fin[ff.first];
+    const uint64_t bpp = P->tile_width[i] ? P->tile_row_bytes[i] / P->tile_width[i] : 1;
+    for (uint32_t k = 0; k < P->tile_final_count[i]; ++k) {
+        FinalJob &fj = v[ff.second + k];
+        fj.pitch = pitch;
+        fj.dst = reinterpret_cast<uint64_t>(dst) + uint64_t(fj.out_y) * pitch + fj.out_x * bpp;
+    }
+    HIP_TRY(hipMemcpy(P->d_fin[ff.first] + ff.second, v.data() + ff.second,
+                      P->tile_final_count[i] * sizeof(FinalJob), hipMemcpyHostToDevice));
+    if (P->graph_exec) {
+        (void)hipGraphExecDestroy(P->graph_exec);
+        (void)hipGraphDestroy(P->graph);
+        P->graph_exec = nullptr;
+        P->graph = nullptr;
+    }
+    return SGN_OK;
+    API_GUARD_END
+}
+
+int sgn_j2k_plan_decode(sgn_hip_session *s, sgn_j2k_plan *P, int mode) {
+    API_GUARD_BEGIN
+    if (!s || !P) return SGN_E_ARG;
+    HIP_TRY(hipSetDevice(s->device));
+    return decode_plan(s, *P, mode);
+    API_GUARD_END
+}
+
+int sgn_j2k_plan_decode_stage(sgn_hip_session *s, sgn_j2k_plan *P, int stage) {
+    API_GUARD_BEGIN
+    if (!s || !P) return SGN_E_ARG;
+    HIP_TRY(hipSetDevice(s->device));
+    uint32_t k = 0;
+    if (stage == SGN_STAGE_BLOCKS) {
+        k += launch_lane(s, *P, P->d_cb_lane, static_cast<uint32_t>(P->cb_lane.size()),
+                         P->d_status_lane);
+        k += launch_wave(s, *P, P->d_cb_wave, static_cast<uint32_t>(P->cb_wave.size()),
+                         P->d_status_wave);
+    } else if (stage == SGN_STAGE_LEVELS) {
+        for (uint32_t d = 1; d <= P->depth_max; ++d)
+            for (int rev = 0; rev < 2; ++rev)
+                k += launch_levels(s, *P, rev, P->lv_group[rev][d - 1], P->lv_group[rev][d]);
+    } else if (stage == SGN_STAGE_FINAL) {
+        for (int rev = 0; rev < 2; ++rev)
+            k += launch_finals(s, *P, rev, 0, static_cast<uint32_t>(P->fin[rev].size()));
+    } else {
+        set_last_error("unknown stage");
+        return SGN_E_ARG;
+    }
+    s->dispatches += k;
+    HIP_TRY(hipGetLastError());
+    return SGN_OK;
+    API_GUARD_END
+}
+
+int sgn_j2k_plan_get_stats(const sgn_j2k_plan *P, sgn_j2k_plan_stats *st) {
+    if (!P || !st) return SGN_E_ARG;
+    st->tiles = P->ntiles;
+    st->code_blocks = P->cb_lane.size() + P->cb_wave.size();
+    st->coded_bytes = P->arena_len;
+    st->output_bytes = P->out_bytes;
+    st->output_pixels = P->out_pixels;
+    st->coefficient_bytes = P->coef_elems * 4;
+    st->launches_batched = P->launches_batched;
+    return SGN_OK;
+}
+
+int sgn_j2k_plan_tile_dims(const sgn_j2k_plan *P, size_t i, uint32_t *w, uint32_t *h,
+                           size_t *row_bytes) {
+    if (!P || i >= P->ntiles) return SGN_E_ARG;
+    if (w) *w = P->tile_width[i];
+    if (h) *h = P->tile_height[i];
+    if (row_bytes) *row_bytes = P->tile_row_bytes[i];
+    return SGN_OK;
+}
+
+int sgn_j2k_decode_tiles_device(sgn_hip_session *s, const sgn_tile_job *jobs, size_t n, int fmt,
+                                uint32_t den, int mode, size_t *first_fail) {
+    API_GUARD_BEGIN
+    std::vector<int32_t> ts(n, 0);
+    std::unique_ptr<sgn_j2k_plan> P;
+    Status st = make_plan(s, jobs, n, fmt, den, P, ts.data());
+    if (st) {
+        if (first_fail)
+            for (size_t i = 0; i < n; ++i)
+                if (ts[i]) {
+                    *first_fail = i;
+                    break;
+                }
+        return st;
+    }
+    st = decode_plan(s, *P, mode);
+    if (st) return st;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return check_status(*P, mode, first_fail);
+    API_GUARD_END
+}
+
+int sgn_j2k_decode_device(sgn_hip_session *s, const uint8_t *bytes, size_t len, int fmt,
+                          uint32_t den, void *dst, size_t pitch, size_t cap) {
+    sgn_tile_job j{bytes, len, dst, pitch, cap};
+    size_t ff = 0;
+    return sgn_j2k_decode_tiles_device(s, &j, 1, fmt, den, SGN_MODE_BATCHED, &ff);
+}
+
+int sgn_j2k_decode_into(sgn_hip_session *s, const uint8_t *bytes, size_t len, int fmt,
+                        uint32_t den, uint8_t *out, size_t out_len, size_t stride) {
+    API_GUARD_BEGIN
+    if (!s || !bytes || !out) return SGN_E_ARG;
+    MainHeader h;
+    Status st = read_main_header(bytes, len, h);
+    if (st) return st;
+    OutputGeom og;
+    st = output_geometry(h, fmt, den, og);
+    if (st) return st;
+    if (stride < og.row_bytes || (og.height && out_len < stride * (og.height - 1) + og.row_bytes)) {
+        set_last_error("output buffer too small");
+        return SGN_E_BUFFER;
+    }
+    HIP_TRY(hipSetDevice(s->device));
+    void *d = nullptr;
+    const size_t bytes_dev = std::max<size_t>(1, og.row_bytes * og.height);
+    HIP_TRY(hipMalloc(&d, bytes_dev));
+    int rc = sgn_j2k_decode_device(s, bytes, len, fmt, den, d, og.row_bytes, bytes_dev);
+    if (rc == SGN_OK && og.height) {
+        if (hipMemcpy2D(out, stride, d, og.row_bytes, og.row_bytes, og.height,
+                        hipMemcpyDeviceToHost) != hipSuccess) {
+            set_last_error("download failed");
+            rc = SGN_E_DEVICE;
+        }
+    }
+    (void)hipFree(d);
+    return rc;
+    API_GUARD_END
+}
+
+// ---- stage hooks ----------------------------------------------------------
+
+static int hook_blocks(sgn_hip_session *s, const uint8_t *data, size_t data_len,
+                       const sgn_ht_code_block_job *jobs, size_t n, int32_t *out, size_t out_len,
+                       bool raw) {
+    if (!s || (!data && data_len) || (!jobs && n) || (!out && out_len)) return SGN_E_ARG;
+    HIP_TRY(hipSetDevice(s->device));
+    std::vector<CbJob> lane, wave;
+    for (size_t i = 0; i < n; ++i) {
+        const sgn_ht_code_block_job &q = jobs[i];
+        CbJob j{};
+        if (q.width == 0 || q.height == 0) continue;
+        if (q.data_offset + q.cleanup_length + q.refinement_length > data_len ||
+            q.output_offset + uint64_t(q.output_stride) * (q.height - 1) + q.width > out_len ||
+            q.output_stride < q.width || q.width * q.height > 4096 || q.cleanup_length > 65535 ||
+            q.refinement_length > 65535 || data_len >= (1ull << 32)) {
+            set_last_error("code-block job outside its buffers");
+            return SGN_E_ARG;
+        }
+        j.src = static_cast<uint32_t>(q.data_offset);
+        j.dst = q.output_offset;
+        j.stride = q.output_stride;
+        j.lcup = static_cast<uint16_t>(q.cleanup_length);
+        j.lref = static_cast<uint16_t>(q.refinement_length);
+        j.w = static_cast<uint16_t>(q.width);
+        j.h = static_cast<uint16_t>(q.height);
+        j.zblk = q.missing_bit_planes;
+        j.passes = q.number_of_coding_passes;
+        j.kmax = q.num_bitplanes;
+        j.flags = q.stripe_causal ? kCbStripeCausal : 0;
+        const uint32_t actual = q.num_bitplanes > q.missing_bit_planes
+                                    ? q.num_bitplanes - q.missing_bit_planes : 0;
+        if (q.num_bitplanes == 0 || q.number_of_coding_passes == 0 || actual == 0)
+            j.flags |= kCbZero;
+        j.tile = static_cast<uint32_t>(i);
+        if (!raw && lane_eligible(j))
+            lane.push_back(j);
+        else
+            wave.push_back(j);
+    }
+    uint8_t *d_data = nullptr, *d_st = nullptr;
+    uint32_t *d_out = nullptr;
+    CbJob *d_lane = nullptr, *d_wave = nullptr;
+    int rc = SGN_OK;
+    std::vector<uint8_t> stl(lane.size()), stw(wave.size());
+    auto cleanup = [&]() {
+        void *p[] = {d_data, d_st, d_out, d_lane, d_wave};
+        for (void *q : p)
+            if (q) (void)hipFree(q);
+    };
+    if (hipMalloc(&d_data, std::max<size_t>(16, data_len)) != hipSuccess ||
+        hipMalloc(&d_out, std::max<size_t>(4, out_len * 4)) != hipSuccess ||
+        hipMalloc(&d_st, std::max<size_t>(1, n)) != hipSuccess) {
+        cleanup();
+        set_last_error("hipMalloc failed");
+        return SGN_E_DEVICE;
+    }
+    if (data_len) (void)hipMemcpy(d_data, data, data_len, hipMemcpyHostToDevice);
+    (void)hipMemset(d_out, 0, out_len * 4);
+    if (dev_upload(d_lane, lane) || dev_upload(d_wave, wave)) {
+        cleanup();
+        return SGN_E_DEVICE;
+    }
+    if (!lane.empty())
+        hipLaunchKernelGGL(sgn_ht_cleanup_lanes, dim3((lane.size() + kLaneWg - 1) / kLaneWg),
+                           dim3(kLaneWg), 0, s->stream, d_lane, uint32_t(lane.size()), d_data,
+                           reinterpret_cast<int32_t *>(d_out), s->d_tables, d_st);
+    if (!wave.empty())
+        hipLaunchKernelGGL(sgn_ht_block_wave, dim3(wave.size()), dim3(64), 0, s->stream, d_wave,
+                           uint32_t(wave.size()), d_data, reinterpret_cast<int32_t *>(d_out),
+                           s->d_tables, d_st + lane.size(), raw ? 1u : 0u);
+    s->dispatches += (lane.empty() ? 0 : 1) + (wave.empty() ? 0 : 1);
+    if (hipStreamSynchronize(s->stream) != hipSuccess ||
+        hipMemcpy(out, d_out, out_len * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+        cleanup();
+        set_last_error("block decode launch failed");
+        return SGN_E_DEVICE;
+    }
+    std::vector<uint8_t> sts(n);
+    (void)hipMemcpy(sts.data(), d_st, lane.size() + wave.size(), hipMemcpyDeviceToHost);
+    for (size_t k = 0; k < lane.size() + wave.size(); ++k)
+        if (sts[k]) {
+            set_last_error("code-block decode failure");
+            rc = SGN_E_BLOCK;
+            break;
+        }
+    cleanup();
+    return rc;
+}
+
+int sgn_ht_decode_code_blocks(sgn_hip_session *s, const uint8_t *data, size_t data_len,
+                              const sgn_ht_code_block_job *jobs, size_t n, float *out,
+                              size_t out_len) {
+    API_GUARD_BEGIN
+    std::vector<int32_t> tmp(out_len);
+    const int rc = hook_blocks(s, data, data_len, jobs, n, tmp.data(), out_len, false);
+    if (rc) return rc;
+    std::vector<float> step(out_len, 1.0f);
+    for (size_t i = 0; i < n; ++i) {
+        const sgn_ht_code_block_job &q = jobs[i];
+        for (uint32_t y = 0; y < q.height; ++y)
+            for (uint32_t x = 0; x < q.width; ++x)
+                step[q.output_offset + size_t(y) * q.output_stride + x] = q.dequantization_step;
+    }
+    for (size_t i = 0; i < out_len; ++i) out[i] = static_cast<float>(tmp[i]) * step[i];
+    return SGN_OK;
+    API_GUARD_END
+}
+
+int sgn_ht_decode_code_blocks_raw(sgn_hip_session *s, const uint8_t *data, size_t data_len,
+                                  const sgn_ht_code_block_job *jobs, size_t n, uint32_t *out,
+                                  size_t out_len) {
+    API_GUARD_BEGIN
+    return hook_blocks(s, data, data_len, jobs, n, reinterpret_cast<int32_t *>(out), out_len, true);
+    API_GUARD_END
+}
+
+int sgn_idwt_level(sgn_hip_session *s, int reversible, uint32_t x0, uint32_t y0, uint32_t x1,
+                   uint32_t y1, const float *ll, const float *hl, const float *lh,
+                   const float *hh, float *out) {
+    API_GUARD_BEGIN
+    if (!s || x1 < x0 || y1 < y0) return SGN_E_ARG;
+    if (reversible && ((x1 - x0 == 1 && (x0 & 1)) || (y1 - y0 == 1 && (y0 & 1)))) {
+        set_last_error("odd-origin single-sample level in a reversible transform");
+        return SGN_E_UNSUPPORTED;
+    }
+    const uint32_t w = x1 - x0, h = y1 - y0;
+    if (!w || !h) return SGN_OK;
+    HIP_TRY(hipSetDevice(s->device));
+    const uint32_t wl = (x1 + 1) / 2 - (x0 + 1) / 2, wh = x1 / 2 - x0 / 2;
+    const uint32_t hl_ = (y1 + 1) / 2 - (y0 + 1) / 2, hh_ = y1 / 2 - y0 / 2;
+    const size_t n[4] = {size_t(wl) * hl_, size_t(wh) * hl_, size_t(wl) * hh_, size_t(wh) * hh_};
+    const float *src[4] = {ll, hl, lh, hh};
+    std::vector<uint32_t> host;
+    LevelJob lj{};
+    for (int b = 0; b < 4; ++b) {
+        lj.band[b] = static_cast<uint32_t>(host.size());
+        lj.bw[b] = (b & 1) ? wh : wl;
+        lj.delta[b] = 1.0f;
+        for (size_t i = 0; i < n[b]; ++i) {
+            uint32_t bits;
+            std::memcpy(&bits, &src[b][i], 4);
+            host.push_back(bits);
+        }
+    }
+    lj.float_in = 0xF;
+    lj.out = static_cast<uint32_t>(host.size());
+    host.resize(host.size() + size_t(w) * h, 0);
+    lj.x0 = x0;
+    lj.y0 = y0;
+    lj.x1 = x1;
+    lj.y1 = y1;
+    lj.nbx = static_cast<uint16_t>(ceil_div(w, 64));
+    lj.nby = static_cast<uint16_t>(ceil_div(h, 64));
+    lj.reversible = reversible ? 1 : 0;
+    lj.levels = 1;
+    uint32_t *d_coef = nullptr;
+    LevelJob *d_job = nullptr;
+    HIP_TRY(hipMalloc(&d_coef, host.size() * 4));
+    HIP_TRY(hipMemcpy(d_coef, host.data(), host.size() * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&d_job, sizeof(LevelJob)));
+    HIP_TRY(hipMemcpy(d_job, &lj, sizeof(LevelJob), hipMemcpyHostToDevice));
+    const uint32_t blocks = uint32_t(lj.nbx) * lj.nby;
+    if (reversible)
+        hipLaunchKernelGGL(sgn_idwt_level_rev, dim3(blocks), dim3(kIdwtThreads), 0, s->stream,
+                           d_job, 1u, 0u, d_coef);
+    else
+        hipLaunchKernelGGL(sgn_idwt_level_irr, dim3(blocks), dim3(kIdwtThreads), 0, s->stream,
+                           d_job, 1u, 0u, d_coef);
+    s->dispatches++;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    std::vector<uint32_t> res(size_t(w) * h);
+    HIP_TRY(hipMemcpy(res.data(), d_coef + lj.out, res.size() * 4, hipMemcpyDeviceToHost));
+    (void)hipFree(d_coef);
+    (void)hipFree(d_job);
+    for (size_t i = 0; i < res.size(); ++i) {
+        if (reversible)
+            out[i] = static_cast<float>(static_cast<int32_t>(res[i]));
+        else
+            std::memcpy(&out[i], &res[i], 4);
+    }
+    return SGN_OK;
+    API_GUARD_END
+}
+
+int sgn_inverse_mct(sgn_hip_session *s, int reversible, float *p0, float *p1, float *p2, size_t n,
+                    float a0, float a1, float a2) {
+    API_GUARD_BEGIN
+    if (!s || (n && (!p0 || !p1 || !p2))) return SGN_E_ARG;
+    if (!n) return SGN_OK;
+    HIP_TRY(hipSetDevice(s->device));
+    float *d = nullptr;
+    HIP_TRY(hipMalloc(&d, n * 12));
+    HIP_TRY(hipMemcpy(d, p0, n * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d + n, p1, n * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d + 2 * n, p2, n * 4, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(sgn_inverse_mct_planes, dim3((n + 255) / 256), dim3(256), 0, s->stream, d,
+                       d + n, d + 2 * n, uint32_t(n), reversible, a0, a1, a2);
+    s->dispatches++;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemcpy(p0, d, n * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(p1, d + n, n * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(p2, d + 2 * n, n * 4, hipMemcpyDeviceToHost));
+    (void)hipFree(d);
+    return SGN_OK;
+    API_GUARD_END
+}
+
+}  // extern "C"

@@ -1,0 +1,119 @@
+// C ABI entry points that need no device: inspection and scaled geometry.
+#include <algorithm>
+#include <exception>
+
+#include "parse.hpp"
+#include "sgn_api.hpp"
+
+extern "C" {
+
+int sgn_j2k_inspect(const uint8_t *bytes, size_t len, sgn_j2k_info *info) {
+    try {
+        if (!bytes || !info) return SGN_E_ARG;
+        sgn::MainHeader h;
+        const sgn::Status st = sgn::read_main_header(bytes, len, h);
+        if (st) return st;
+        *info = sgn_j2k_info{};
+        info->width = h.xsiz - h.xo;
+        info->height = h.ysiz - h.yo;
+        info->num_components = h.ncomp;
+        info->bit_depth = h.comp[0].precision;
+        uint32_t nl = 255;
+        for (int c = 0; c < h.ncomp; ++c) nl = h.comp[c].levels < nl ? h.comp[c].levels : nl;
+        info->num_levels = nl;
+        info->reversible = h.comp[0].transform == 1;
+        info->mct = h.mct;
+        info->high_throughput = (h.comp[0].cbstyle & 0x40) != 0;
+        info->tiles_x = h.tiles_x();
+        info->tiles_y = h.tiles_y();
+        info->cb_width = 1u << h.comp[0].xcb;
+        info->cb_height = 1u << h.comp[0].ycb;
+        info->progression = h.prog;
+        return SGN_OK;
+    } catch (const std::exception &e) {
+        sgn::set_last_error(e.what());
+        return SGN_E_ARG;
+    }
+}
+
+int sgn_j2k_scaled_dims(const uint8_t *bytes, size_t len, uint32_t denominator, uint32_t *width,
+                        uint32_t *height) {
+    if (!bytes) return SGN_E_ARG;
+    sgn::MainHeader h;
+    sgn::Status st = sgn::read_main_header(bytes, len, h);
+    if (st) return st;
+    if (denominator != 1 && denominator != 2 && denominator != 4 && denominator != 8) {
+        sgn::set_last_error("downscale denominator must be 1, 2, 4 or 8");
+        return SGN_E_ARG;
+    }
+    const uint32_t sk = sgn::skipped_levels(h, denominator);
+    if (width) *width = sgn::ceil_div(h.xsiz, 1u << sk);
+    if (height) *height = sgn::ceil_div(h.ysiz, 1u << sk);
+    return SGN_OK;
+}
+
+}  // extern "C"
+
+extern "C" int sgn_j2k_code_blocks(const uint8_t *bytes, size_t len, sgn_ht_code_block_job *jobs,
+                                   size_t capacity, size_t *count) {
+    try {
+        if (!bytes || !count) return SGN_E_ARG;
+        sgn::MainHeader h;
+        sgn::Status st = sgn::read_main_header(bytes, len, h);
+        if (st) return st;
+        std::vector<sgn::TileBody> tiles;
+        st = sgn::parse_tiles(bytes, len, h, tiles);
+        if (st) return st;
+        std::vector<sgn_ht_code_block_job> out;
+        std::vector<sgn::TileCompGeom> tcs(h.ncomp);
+        std::vector<std::vector<sgn::CbSeg>> segs;
+        uint64_t packed = 0;
+        for (const sgn::TileBody &t : tiles) {
+            for (int c = 0; c < h.ncomp; ++c) {
+                st = sgn::build_tile_comp(t.comp[c], t.rect, tcs[c]);
+                if (st) return st;
+            }
+            st = sgn::parse_packets(bytes, t, h.ncomp, tcs.data(), segs);
+            if (st) return st;
+            for (int c = 0; c < h.ncomp; ++c) {
+                const sgn::TileCompGeom &g = tcs[c];
+                for (size_t k = 0; k < g.cbs.size(); ++k) {
+                    const sgn::CbGeom &cg = g.cbs[k];
+                    const sgn::BandGeom &bg = g.bands[cg.band];
+                    const sgn::CbSeg &sg = segs[c][k];
+                    if (cg.rect.w() == 0 || cg.rect.h() == 0) continue;
+                    uint32_t K;
+                    float delta;
+                    st = sgn::band_quant(t.comp[c], bg.type, bg.res, K, delta);
+                    if (st) return st;
+                    sgn_ht_code_block_job j{};
+                    j.data_offset = sg.offset >= 0 ? uint64_t(sg.offset) : 0;
+                    j.cleanup_length = sg.offset >= 0 ? sg.lcup : 0;
+                    j.refinement_length = sg.offset >= 0 ? sg.lref : 0;
+                    j.width = cg.rect.w();
+                    j.height = cg.rect.h();
+                    j.output_offset = static_cast<uint32_t>(packed);
+                    j.output_stride = j.width;
+                    j.missing_bit_planes = static_cast<uint8_t>(std::min<uint32_t>(sg.zblk, 255));
+                    j.number_of_coding_passes = static_cast<uint8_t>(sg.offset >= 0 ? sg.passes : 0);
+                    j.num_bitplanes = static_cast<uint8_t>(K);
+                    j.stripe_causal = (t.comp[c].cbstyle & 0x08) ? 1 : 0;
+                    j.dequantization_step = delta;
+                    packed += uint64_t(j.width) * j.height;
+                    out.push_back(j);
+                }
+            }
+        }
+        *count = out.size();
+        if (!jobs) return SGN_OK;
+        if (capacity < out.size()) {
+            sgn::set_last_error("code-block job buffer too small");
+            return SGN_E_BUFFER;
+        }
+        std::copy(out.begin(), out.end(), jobs);
+        return SGN_OK;
+    } catch (const std::exception &e) {
+        sgn::set_last_error(e.what());
+        return SGN_E_ARG;
+    }
+}

@@ -1,0 +1,58 @@
+// Codestream parsing and plan building (host side).
+#pragma once
+#include <utility>
+#include <vector>
+
+#include "j2k_host.hpp"
+#include "plan_types.h"
+
+namespace sgn {
+
+struct TileBody {
+    Rect rect;
+    CompParams comp[kMaxComps];
+    uint8_t prog = 0;
+    bool mct = false;
+    uint16_t layers = 1;
+    std::vector<std::pair<size_t, size_t>> parts;  // (offset, length) of tile-part bodies
+};
+
+struct CbSeg {
+    int64_t offset = -1;  // byte offset of cleanup||refinement in the codestream
+    uint32_t lcup = 0, lref = 0, zblk = 0, passes = 0;
+};
+
+Status read_main_header(const uint8_t *data, size_t len, MainHeader &h);
+uint32_t skipped_levels(const MainHeader &h, uint32_t denominator);
+Status parse_tiles(const uint8_t *data, size_t len, const MainHeader &h,
+                   std::vector<TileBody> &tiles);
+Status parse_packets(const uint8_t *cs, const TileBody &t, int ncomp, const TileCompGeom *tcs,
+                     std::vector<std::vector<CbSeg>> &segs);
+
+// Output geometry of a decode at 1/denominator.
+struct OutputGeom {
+    uint32_t width = 0, height = 0, ncomp = 0, channels = 0, bytes_per_sample = 1;
+    size_t row_bytes = 0;
+};
+Status output_geometry(const MainHeader &h, int fmt, uint32_t denominator, OutputGeom &g);
+
+// Plan records of one codestream with offsets local to it; merged into a
+// batch plan by rebasing (arena, coefficient and job offsets).
+struct LocalPlan {
+    std::vector<CbJob> cbs;          // every code-block of the retained resolutions
+    std::vector<LevelJob> levels;    // intermediate synthesis levels
+    std::vector<uint8_t> level_depth;// r of each intermediate level job (1-based)
+    std::vector<LevelJob> finals_lv; // last level per tile and component
+    std::vector<FinalJob> finals;    // one per codestream tile
+    uint64_t coef_elems = 0;
+    uint32_t width = 0, height = 0;
+    size_t row_bytes = 0;
+    uint64_t out_bytes = 0;
+};
+
+// Parses one codestream into plan records; the output of tile t starts at
+// dst + y0 * pitch + x0 * bytes_per_pixel.
+Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denominator,
+                        uint64_t dst, uint64_t pitch, uint64_t capacity, LocalPlan &lp);
+
+}  // namespace sgn

@@ -1,0 +1,265 @@
+// Plan building: one codestream -> code-block table, synthesis-level jobs
+// and fused final jobs.  Mirrors the data flow of the reference's
+// decode_tile (crates/signinum-j2k-native/src/j2c/decode.rs:796-870):
+// sub-band decode of the retained resolutions, per-level synthesis
+// (idwt.rs:54-193), store with sign shift and inverse MCT (decode.rs:
+// 1720-1902, mct.rs:11-156), then output conversion (lib.rs:1772-1835,
+// 2202-2295).
+#include <algorithm>
+
+#include "parse.hpp"
+#include "sgn_api.hpp"
+
+namespace sgn {
+
+static int fmt_channels(int fmt) {
+    switch (fmt) {
+    case SGN_GRAY8:
+    case SGN_GRAY16:
+        return 1;
+    case SGN_RGB8:
+    case SGN_RGB16:
+        return 3;
+    case SGN_RGBA8:
+        return 4;
+    default:
+        return 0;
+    }
+}
+
+Status output_geometry(const MainHeader &h, int fmt, uint32_t denominator, OutputGeom &g) {
+    const int ch = fmt_channels(fmt);
+    if (!ch) {
+        set_last_error("unknown pixel format");
+        return kErrArg;
+    }
+    if (denominator != 1 && denominator != 2 && denominator != 4 && denominator != 8) {
+        set_last_error("downscale denominator must be 1, 2, 4 or 8");
+        return kErrArg;
+    }
+    if ((ch == 1 && h.ncomp != 1) || (ch >= 3 && h.ncomp != 3)) {
+        set_last_error("colour space cannot be mapped to the requested pixel format");
+        return kErrUnsupported;
+    }
+    const uint32_t sk = skipped_levels(h, denominator);
+    g.width = ceil_div(h.xsiz, 1u << sk);
+    g.height = ceil_div(h.ysiz, 1u << sk);
+    g.ncomp = h.ncomp;
+    g.channels = uint32_t(ch);
+    g.bytes_per_sample = (fmt == SGN_GRAY16 || fmt == SGN_RGB16) ? 2 : 1;
+    g.row_bytes = size_t(g.width) * ch * g.bytes_per_sample;
+    return kOk;
+}
+
+Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denominator,
+                        uint64_t dst, uint64_t pitch, uint64_t capacity, LocalPlan &lp) {
+    lp = LocalPlan();
+    MainHeader h;
+    Status st = read_main_header(cs, len, h);
+    if (st) return st;
+    OutputGeom og;
+    st = output_geometry(h, fmt, denominator, og);
+    if (st) return st;
+    lp.width = og.width;
+    lp.height = og.height;
+    lp.row_bytes = og.row_bytes;
+    lp.out_bytes = uint64_t(og.row_bytes) * og.height;
+    if (pitch < og.row_bytes ||
+        (og.height && capacity < pitch * (og.height - 1) + og.row_bytes)) {
+        set_last_error("output surface too small for the decoded image");
+        return kErrBuffer;
+    }
+    const uint32_t sk = skipped_levels(h, denominator);
+    std::vector<TileBody> tiles;
+    st = parse_tiles(cs, len, h, tiles);
+    if (st) return st;
+    const int nc = h.ncomp;
+    const uint32_t bpp = og.channels * og.bytes_per_sample;
+    bool same8 = true;
+    for (int c = 0; c < nc; ++c) same8 &= h.comp[c].precision == 8;
+
+    std::vector<TileCompGeom> tcs(static_cast<size_t>(nc));
+    std::vector<std::vector<CbSeg>> segs;
+    for (const TileBody &t : tiles) {
+        if (t.layers > 1) {
+            set_last_error("multi-layer HTJ2K packet assembly");
+            return kErrUnsupported;
+        }
+        for (int c = 0; c < nc; ++c) {
+            if (!(t.comp[c].cbstyle & 0x40)) {
+                set_last_error("classic (EBCOT) code-blocks are not on the HT path");
+                return kErrUnsupported;
+            }
+            st = build_tile_comp(t.comp[c], t.rect, tcs[c]);
+            if (st) return st;
+            if (tcs[c].nres <= sk) {
+                set_last_error("resolution levels below the requested scale");
+                return kErrMalformed;
+            }
+        }
+        st = parse_packets(cs, t, nc, tcs.data(), segs);
+        if (st) return st;
+        const bool mct = t.mct && nc >= 3;
+        if (mct && (t.comp[0].transform != t.comp[1].transform ||
+                    t.comp[1].transform != t.comp[2].transform)) {
+            set_last_error("inverse MCT over components with different transforms");
+            return kErrMalformed;
+        }
+        FinalJob fj{};
+        Rect out_rect;
+        for (int c = 0; c < nc; ++c) {
+            const TileCompGeom &g = tcs[c];
+            const CompParams &cp = t.comp[c];
+            const uint32_t R = g.nres - sk;
+            const bool rev = cp.transform == 1;
+            // coefficient layout: bands of the retained resolutions
+            std::vector<uint32_t> band_off(g.bands.size(), 0);
+            std::vector<float> band_delta(g.bands.size(), 1.0f);
+            std::vector<uint32_t> band_k(g.bands.size(), 0);
+            for (size_t b = 0; b < g.bands.size(); ++b) {
+                const BandGeom &bg = g.bands[b];
+                if (bg.res >= R) continue;
+                band_off[b] = static_cast<uint32_t>(lp.coef_elems);
+                lp.coef_elems += uint64_t(bg.rect.w()) * bg.rect.h();
+                uint32_t K;
+                float d;
+                st = band_quant(cp, bg.type, bg.res, K, d);
+                if (st) {
+                    set_last_error("invalid quantisation parameters");
+                    return st;
+                }
+                if (rev && d != 1.0f) {
+                    set_last_error("quantised reversible transform");
+                    return kErrUnsupported;
+                }
+                band_delta[b] = d;
+                band_k[b] = K;
+            }
+            // code-blocks (decode.rs:1117-1187,1554-1718)
+            for (size_t k = 0; k < g.cbs.size(); ++k) {
+                const CbGeom &cg = g.cbs[k];
+                const BandGeom &bg = g.bands[cg.band];
+                if (bg.res >= R) continue;
+                const CbSeg &sg = segs[c][k];
+                const uint32_t K = band_k[cg.band];
+                CbJob j{};
+                j.stride = bg.rect.w();
+                j.dst = band_off[cg.band] + (cg.rect.y0 - bg.rect.y0) * bg.rect.w() +
+                        (cg.rect.x0 - bg.rect.x0);
+                j.w = static_cast<uint16_t>(cg.rect.w());
+                j.h = static_cast<uint16_t>(cg.rect.h());
+                j.kmax = static_cast<uint8_t>(K);
+                j.flags = (cp.cbstyle & 0x08) ? kCbStripeCausal : 0;
+                if (j.w == 0 || j.h == 0) continue;
+                const uint32_t actual = K > sg.zblk ? K - sg.zblk : 0;
+                if (K > 31 && sg.passes > 0) {
+                    set_last_error("too many bit-planes");
+                    return kErrMalformed;
+                }
+                if (K == 0 || sg.passes == 0 || actual == 0 || sg.offset < 0) {
+                    j.flags |= kCbZero;
+                } else {
+                    j.src = static_cast<uint32_t>(sg.offset);
+                    j.lcup = static_cast<uint16_t>(sg.lcup);
+                    j.lref = static_cast<uint16_t>(sg.lref);
+                    j.zblk = static_cast<uint8_t>(std::min<uint32_t>(sg.zblk, 255));
+                    j.passes = static_cast<uint8_t>(sg.passes);
+                }
+                j.cost = (j.flags & kCbZero) ? 0 : sg.lcup + sg.lref + (uint32_t(j.w) * j.h) / 4;
+                lp.cbs.push_back(j);
+            }
+            // synthesis levels (idwt.rs:54-193)
+            uint32_t prev_out = band_off[g.res[0].band[0]];
+            uint32_t prev_w = g.bands[g.res[0].band[0]].rect.w();
+            float prev_delta = band_delta[g.res[0].band[0]];
+            bool prev_float = false;
+            if (R == 1) {
+                LevelJob lj{};
+                lj.band[0] = prev_out;
+                lj.bw[0] = prev_w;
+                lj.delta[0] = prev_delta;
+                const Rect rr = g.res[0].rect;
+                lj.x0 = rr.x0;
+                lj.y0 = rr.y0;
+                lj.x1 = rr.x1;
+                lj.y1 = rr.y1;
+                lj.reversible = rev;
+                lj.levels = 0;
+                fj.level[c] = static_cast<uint32_t>(lp.finals_lv.size());
+                lp.finals_lv.push_back(lj);
+            }
+            for (uint32_t r = 1; r < R; ++r) {
+                const ResGeom &rs = g.res[r];
+                // a single-sample row or column with odd origin halves the
+                // sample (idwt.rs:726-731); not integer-exact for 5/3
+                if (rev && ((rs.rect.w() == 1 && (rs.rect.x0 & 1)) ||
+                            (rs.rect.h() == 1 && (rs.rect.y0 & 1)))) {
+                    set_last_error("odd-origin single-sample level in a reversible tile");
+                    return kErrUnsupported;
+                }
+                LevelJob lj{};
+                lj.band[0] = prev_out;
+                lj.bw[0] = prev_w;
+                lj.delta[0] = prev_delta;
+                lj.float_in = prev_float ? 1 : 0;
+                for (int b = 0; b < 3; ++b) {
+                    const uint32_t bi = rs.band[b];
+                    lj.band[b + 1] = band_off[bi];
+                    lj.bw[b + 1] = g.bands[bi].rect.w();
+                    lj.delta[b + 1] = band_delta[bi];
+                }
+                lj.x0 = rs.rect.x0;
+                lj.y0 = rs.rect.y0;
+                lj.x1 = rs.rect.x1;
+                lj.y1 = rs.rect.y1;
+                lj.reversible = rev;
+                lj.levels = 1;
+                lj.nbx = static_cast<uint16_t>(ceil_div(rs.rect.w(), 64));
+                lj.nby = static_cast<uint16_t>(ceil_div(rs.rect.h(), 64));
+                if (r + 1 < R) {
+                    lj.out = static_cast<uint32_t>(lp.coef_elems);
+                    lp.coef_elems += uint64_t(rs.rect.w()) * rs.rect.h();
+                    lp.levels.push_back(lj);
+                    lp.level_depth.push_back(static_cast<uint8_t>(r));
+                    prev_out = lj.out;
+                    prev_w = rs.rect.w();
+                    prev_delta = 1.0f;
+                    prev_float = !rev;
+                } else {
+                    fj.level[c] = static_cast<uint32_t>(lp.finals_lv.size());
+                    lp.finals_lv.push_back(lj);
+                }
+            }
+            const Rect fr = g.res[R - 1].rect;
+            if (c == 0)
+                out_rect = fr;
+            else if (fr.x0 != out_rect.x0 || fr.y0 != out_rect.y0 || fr.x1 != out_rect.x1 ||
+                     fr.y1 != out_rect.y1) {
+                set_last_error("components decode to different rectangles");
+                return kErrUnsupported;
+            }
+            fj.precision[c < 4 ? c : 3] = cp.precision;
+        }
+        if (out_rect.x1 > og.width || out_rect.y1 > og.height) {
+            set_last_error("tile outside the output image");
+            return kErrMalformed;
+        }
+        fj.dst = dst + uint64_t(out_rect.y0) * pitch + uint64_t(out_rect.x0) * bpp;
+        fj.pitch = pitch;
+        fj.out_x = out_rect.x0;
+        fj.out_y = out_rect.y0;
+        fj.width = out_rect.w();
+        fj.height = out_rect.h();
+        fj.nbx = static_cast<uint16_t>(ceil_div(fj.width, 64));
+        fj.nby = static_cast<uint16_t>(ceil_div(fj.height, 64));
+        fj.ncomp = static_cast<uint8_t>(nc);
+        fj.fmt = static_cast<uint8_t>(fmt);
+        fj.mct = mct ? 1 : 0;
+        fj.reversible = t.comp[0].transform == 1;
+        fj.same8 = same8 ? 1 : 0;
+        if (fj.width && fj.height) lp.finals.push_back(fj);
+    }
+    return kOk;
+}
+
+}  // namespace sgn

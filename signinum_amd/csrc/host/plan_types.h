@@ -1,0 +1,56 @@
+// Device-visible plan records shared by the host plan builder and the HIP
+// kernels.  Plain structs of fixed size, uploaded once per plan.
+#pragma once
+#include <stdint.h>
+
+namespace sgn {
+
+// One HT code-block (the device form of HtCodeBlockDecodeJob,
+// crates/signinum-j2k-native/src/lib.rs:149-174).
+struct CbJob {
+    uint32_t src;     // byte offset of cleanup||refinement data in the arena
+    uint32_t dst;     // element offset of the block's first sample (coefficient arena)
+    uint32_t stride;  // band row width in elements
+    uint16_t lcup, lref;
+    uint16_t w, h;
+    uint8_t zblk, passes, kmax, flags;  // flags: kCbStripeCausal | kCbZero
+    uint32_t cost;    // predicted cost (ordering key)
+    uint32_t tile;    // index of the owning tile job in the plan
+};
+static_assert(sizeof(CbJob) == 32, "CbJob layout");
+
+enum : uint8_t { kCbStripeCausal = 1, kCbZero = 2 };
+
+// One 2D synthesis level of one tile-component (IDWT step,
+// J2kDirectIdwtStep in crates/signinum-j2k-native/src/direct_plan.rs).
+struct LevelJob {
+    uint32_t band[4];   // element offsets: LL, HL, LH, HH
+    uint32_t bw[4];     // band row widths
+    float delta[4];     // dequantisation steps (irreversible), 1 for float inputs
+    uint32_t out;       // element offset of the output (intermediate levels)
+    uint32_t x0, y0, x1, y1;  // level rectangle (absolute coordinates)
+    uint32_t block_base;      // first work-group of this job
+    uint16_t nbx, nby;        // 64x64 output blocks
+    uint8_t reversible;
+    uint8_t float_in;         // bit b set: band b holds float samples, not integers
+    uint8_t levels;           // 1: one synthesis level, 0: copy LL only
+    uint8_t pad;
+};
+
+// Final stage of one tile: last synthesis level of every component fused
+// with inverse MCT, sign shift, rounding and packing into the output.
+struct FinalJob {
+    uint32_t level[4];        // LevelJob index per component
+    uint64_t dst;             // device address of the tile's first output pixel
+    uint64_t pitch;           // bytes per output row
+    uint32_t width, height;   // output rectangle
+    uint32_t block_base;
+    uint16_t nbx, nby;
+    uint8_t ncomp, fmt, mct, reversible;
+    uint8_t precision[4];
+    uint8_t same8;            // all components 8-bit
+    uint8_t pad[3];
+    uint32_t out_x, out_y;    // position of this J2K tile inside the output image
+};
+
+}  // namespace sgn

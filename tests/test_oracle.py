@@ -1,0 +1,153 @@
+"""The CPU oracle against golden vectors, closed forms and OpenJPEG.
+
+Pins (each one fails for a plausible mistake in the restatement):
+- OpenHTJ2K conformance codestreams with OpenJPH-decoded pixels
+  (crates/signinum-j2k-native/fixtures/htj2k, reference test
+  crates/signinum-j2k-native/src/lib.rs:3463-3545): exact pixels, block
+  counts, refinement-job counts, maximum passes, and the number of samples
+  that change when refinement bytes are zeroed;
+- lossless round trips: the decode of a reversible codestream equals the
+  source pixels (crates/signinum-j2k/tests/encode_lossless.rs:55-427);
+- OpenJPEG 2.5.4 (bundled with Pillow) decoding the same codestreams;
+- structure of the VLC tables (complete prefix codes per context).
+"""
+import io
+
+import numpy as np
+import pytest
+
+import signinum_amd as S
+from signinum_amd import synth
+from tests import oracle_ref as O
+
+GOLDEN = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden")
+
+
+def _fixture(name):
+    cs = open(f"{GOLDEN}/{name}.j2k", "rb").read()
+    exp = np.frombuffer(open(f"{GOLDEN}/{name}.gray", "rb").read(), np.uint8)
+    return cs, exp
+
+
+@pytest.mark.parametrize("name,w,h,blocks,ref_blocks,max_passes,zero_diffs", [
+    ("openhtj2k_ds0_ht_12_b11", 3, 5, 8, 2, 3, 4),
+    ("openhtj2k_ds0_ht_09_b11", 17, 37, 14, 14, 3, 629),
+])
+def test_openhtj2k_fixtures(oracle, name, w, h, blocks, ref_blocks, max_passes, zero_diffs):
+    cs, exp = _fixture(name)
+    out = O.decode(cs, 0)
+    assert out.shape == (h, w, 1)
+    assert np.array_equal(out.reshape(-1), exp)
+    assert O.last_stats() == (blocks, ref_blocks, max_passes)
+    planes = O.decode_planes(cs)
+    O.set_zero_refinement(True)
+    try:
+        zeroed = O.decode_planes(cs)
+    finally:
+        O.set_zero_refinement(False)
+    assert int((np.abs(planes - zeroed) > np.finfo(np.float32).eps).sum()) == zero_diffs
+
+
+def test_vlc_tables_are_complete_prefix_codes(oracle):
+    v0, v1, u0, u1 = O.tables()
+    for table in (v0, v1):
+        for c_q in range(8):
+            entries = table[c_q * 128:(c_q + 1) * 128]
+            lens = entries & 7
+            assert (lens > 0).all()
+            # every 7-bit window decodes; codewords cover the space exactly once
+            codes = {}
+            for i, e in enumerate(entries):
+                ln = int(e & 7)
+                codes.setdefault((i & ((1 << ln) - 1), ln), e)
+            assert sum(2.0 ** -ln for (_c, ln) in codes) == pytest.approx(1.0)
+    # U-VLC: a lone residual with prefix "1" is u=1 (no suffix), "000" needs 5 bits
+    assert (u1[64 + 1] >> 10) & 7 == 1 and (u1[64 + 1] & 7) == 1
+    assert (u1[64 + 0] >> 10) & 7 == 5 and ((u1[64 + 0] >> 3) & 0xF) == 5
+
+
+def test_coefficient_to_i32_alignment(oracle):
+    # ht_block_decode.rs:1695-1700
+    aligned = 3 << (31 - 5)
+    assert O.coefficient_to_i32(aligned, 5) == 3
+    assert O.coefficient_to_i32(0x80000000 | aligned, 5) == -3
+
+
+def _pillow(cs):
+    from PIL import Image
+    return np.array(Image.open(io.BytesIO(cs)))
+
+
+@pytest.mark.parametrize("shape,levels,prog,cb", [
+    ((64, 64), 5, 0, 6), ((37, 53), 3, 1, 5), ((129, 200), 5, 2, 6), ((5, 3), 1, 3, 6),
+    ((256, 256), 5, 4, 5), ((1, 7), 2, 0, 4), ((100, 64), 0, 0, 6), ((80, 96), 4, 0, 4),
+])
+def test_lossless_roundtrip_rgb(oracle, shape, levels, prog, cb):
+    rng = np.random.default_rng(sum(shape) + levels)
+    img = synth.gradient_noise_rgb(rng, shape[1], shape[0])
+    cs = S.encode_j2k_ht(img, levels=levels, progression=prog, cb_width_log2=cb,
+                         cb_height_log2=cb)
+    assert np.array_equal(O.decode(cs, 1), img)
+    assert np.array_equal(_pillow(cs), img)
+
+
+@pytest.mark.parametrize("kw", [dict(precinct_log2=5), dict(sop=1, eph=1),
+                                dict(tile_width=96, tile_height=80),
+                                dict(cb_width_log2=7, cb_height_log2=5)])
+def test_lossless_roundtrip_options(oracle, kw):
+    rng = np.random.default_rng(7)
+    img = synth.tissue_rgb(rng, 200, 180)
+    cs = S.encode_j2k_ht(img, **kw)
+    assert np.array_equal(O.decode(cs, 1), img)
+    assert np.array_equal(_pillow(cs), img)
+
+
+def test_gray12_roundtrip(oracle):
+    rng = np.random.default_rng(3)
+    img = synth.spots_gray12(rng, 90, 70)
+    cs = S.encode_j2k_ht(img, bit_depth=12, use_mct=0, cb_width_log2=5, cb_height_log2=5)
+    out = O.decode(cs, 3)
+    assert np.array_equal(out[..., 0], img)
+    assert np.array_equal(_pillow(cs).astype(np.int64) >> 4, img.astype(np.int64)) or \
+        np.array_equal(_pillow(cs), img)
+
+
+@pytest.mark.parametrize("den", [2, 4, 8])
+def test_reduced_resolution_matches_openjpeg(oracle, den):
+    from PIL import Image
+    rng = np.random.default_rng(den)
+    img = synth.gradient_noise_rgb(rng, 160, 120)
+    cs = S.encode_j2k_lossless(img)
+    out = O.decode(cs, 1, den)
+    pil = Image.open(io.BytesIO(cs))
+    pil.reduce = {2: 1, 4: 2, 8: 3}[den]
+    ref = np.array(pil)
+    assert out.shape == ref.shape
+    assert np.array_equal(out, ref)
+
+
+def test_irreversible_close_to_openjpeg_and_f64(oracle):
+    rng = np.random.default_rng(11)
+    img = synth.tissue_rgb(rng, 128, 96)
+    cs = S.encode_j2k_ht(img, reversible=0, base_step=1.0 / 64)
+    out = O.decode(cs, 1).astype(np.int64)
+    # OpenJPEG reconstructs quantised values at the bin midpoint; the
+    # reference truncates (ht_block_decode.rs:225-234), so the two differ by
+    # a fraction of a step, not more.
+    diff = np.abs(out - _pillow(cs).astype(np.int64))
+    assert diff.max() <= 6 and diff.mean() < 1.5
+    p32 = O.decode_planes(cs)
+    p64 = O.decode_planes(cs, use_f64=True)
+    assert np.abs(p32 - p64).max() < 0.05
+    # lossy but close to the source at this step size
+    assert np.abs(out - img.astype(np.int64)).max() < 40
+
+
+def test_batch_first_failing_index(oracle):
+    rng = np.random.default_rng(5)
+    streams = [S.encode_j2k_lossless(synth.texture_rgb(rng, 32, 32)) for _ in range(5)]
+    bad = bytearray(streams[3])
+    bad[:] = bad[:60]  # truncated main header
+    streams[3] = bytes(bad)
+    rc, idx, _outs = O.decode_batch(streams, 1, workers=2)
+    assert rc != 0 and idx == 3
