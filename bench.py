@@ -42,8 +42,6 @@ def main():
               "collectives)", file=sys.stderr)
         sys.exit(2)
 
-    import torch
-
     import signinum_amd as S
     from signinum_amd import synth
 
@@ -53,8 +51,6 @@ def main():
     log(f"generated {args.tiles} tiles in {time.time() - t0:.1f}s, {coded / 1e6:.1f} MB coded")
 
     sess = S.HipSession(0)
-    ext = torch.cuda.ExternalStream(sess.stream)
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
     plan = S.TilePlan(sess, streams, S.PixelFormat.RGB8)
     sess.synchronize()
@@ -64,20 +60,18 @@ def main():
     log(f"plan: {plan_ms:.1f} ms host parse + upload; {st['code_blocks']} code-blocks")
 
     def timed(fn, reps):
-        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-               for _ in range(reps)]
+        # HIP events on the session stream, where the kernels are launched
+        evs = [(S.HipEvent(), S.HipEvent()) for _ in range(reps)]
         for a, b in evs:
-            a.record(ext)
+            a.record(sess)
             fn()
-            b.record(ext)
-        torch.cuda.synchronize()
+            b.record(sess)
         sess.synchronize()
-        return [a.elapsed_time(b) for a, b in evs]
+        return [a.elapsed_ms(b) for a, b in evs]
 
     # ---- batched, graph-replayed ----
     for _ in range(args.warmup):
         plan.decode(S.MODE_GRAPH)
-    torch.cuda.synchronize()
     sess.synchronize()
     t0 = time.perf_counter()
     step_ms = timed(lambda: plan.decode(S.MODE_GRAPH), args.steps)

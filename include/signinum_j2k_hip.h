@@ -80,6 +80,26 @@ int sgn_hip_session_synchronize(sgn_hip_session *session);
  * (CudaSurfaceStats::kernel_dispatches, surface.rs:18-27). */
 uint64_t sgn_hip_session_dispatches(sgn_hip_session *session);
 
+/* ---- device buffers and events (the role of CudaDeviceBuffer and the
+ * launch/copy helpers, crates/signinum-cuda-runtime/src/lib.rs:325-376,
+ * 778-950).  Copies and fills are ordered on the session stream; the
+ * download/upload helpers return after the copy completed. */
+int sgn_device_alloc(sgn_hip_session *session, size_t bytes, void **out);
+void sgn_device_free(sgn_hip_session *session, void *ptr);
+/* Fills bytes with `value` using a kernel (no memset node), asynchronous. */
+int sgn_device_fill(sgn_hip_session *session, void *ptr, size_t bytes, uint8_t value);
+int sgn_device_download(sgn_hip_session *session, void *host, const void *dev, size_t bytes);
+int sgn_device_upload(sgn_hip_session *session, void *dev, const void *host, size_t bytes);
+/* Pitched download of `rows` rows of `row_bytes` each. */
+int sgn_device_download_2d(sgn_hip_session *session, void *host, size_t host_stride,
+                           const void *dev, size_t dev_pitch, size_t row_bytes, size_t rows);
+int sgn_hip_event_create(void **event);
+void sgn_hip_event_destroy(void *event);
+/* Records the event on the session stream. */
+int sgn_hip_event_record(sgn_hip_session *session, void *event);
+/* Milliseconds between two recorded events (waits for `end`). */
+int sgn_hip_event_elapsed_ms(void *start, void *end, float *ms);
+
 /* ---- single codestream decode ------------------------------------------
  * J2kDecoder::decode_into / decode_scaled_into (crates/signinum-j2k/src/
  * view.rs:168,267; adapter decoder.rs:156-200).  Decodes on the device and

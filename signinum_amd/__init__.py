@@ -11,7 +11,6 @@ Everything here is a thin layer over the C ABI of libsigninum_j2k_hip.so.
 """
 import ctypes
 import enum
-from dataclasses import dataclass
 
 import numpy as np
 
@@ -22,7 +21,7 @@ from ._lib import (MODE_BATCHED, MODE_GRAPH, MODE_PLAIN, EncodeOptions, HtCodeBl
 __all__ = [
     "PixelFormat", "Downscale", "BackendRequest", "J2kError", "UnsupportedError",
     "MalformedError", "CodeBlockDecodeError", "BufferError", "DeviceError", "TileBatchError",
-    "UnsupportedBackendError", "HipSession", "J2kDecoder", "DeviceSurface", "TilePlan",
+    "UnsupportedBackendError", "HipSession", "HipEvent", "J2kDecoder", "DeviceSurface", "TilePlan",
     "decode_tiles_into", "decode_tiles_to_device", "encode_j2k_lossless", "encode_j2k_ht",
     "HtCodeBlockDecoder", "code_block_jobs", "MODE_PLAIN", "MODE_BATCHED", "MODE_GRAPH",
 ]
@@ -185,46 +184,92 @@ class HipSession:
             pass
 
 
-@dataclass
 class DeviceSurface:
     """DeviceSurface of the device adapters (crates/signinum-j2k-cuda/src/
-    surface.rs:11-127): a pitched device image held by a torch tensor."""
-    tensor: object
-    width: int
-    height: int
-    pixel_format: PixelFormat
-    pitch_bytes: int
+    surface.rs:11-127): a pitched device image in memory owned by the
+    library (sgn_device_alloc), released with the surface."""
+
+    def __init__(self, session, width, height, pixel_format, pitch_bytes=None):
+        self.session = session
+        self.width = int(width)
+        self.height = int(height)
+        self.pixel_format = PixelFormat(pixel_format)
+        self.pitch_bytes = int(pitch_bytes or self.width * self.pixel_format.bytes_per_pixel)
+        p = ctypes.c_void_p()
+        _check(_lib.load().sgn_device_alloc(session.handle, max(1, self.byte_len), ctypes.byref(p)))
+        self._ptr = p.value
 
     @property
     def device_ptr(self):
-        return self.tensor.data_ptr()
+        return self._ptr
 
     @property
     def byte_len(self):
         return self.pitch_bytes * self.height
 
+    @property
+    def row_bytes(self):
+        return self.width * self.pixel_format.bytes_per_pixel
+
+    def fill(self, value=0):
+        _check(_lib.load().sgn_device_fill(self.session.handle, self._ptr, self.byte_len, value))
+
     def download_into(self, out, stride):
-        rb = self.width * self.pixel_format.bytes_per_pixel
-        host = self.tensor.cpu().numpy().reshape(-1)
-        dst = np.asarray(out).view(np.uint8).reshape(-1)
-        for y in range(self.height):
-            dst[y * stride:y * stride + rb] = host[y * self.pitch_bytes:y * self.pitch_bytes + rb]
+        dst = np.asarray(out)
+        if not dst.flags["C_CONTIGUOUS"] or not dst.flags["WRITEABLE"]:
+            raise BufferError(_lib.SGN_E_BUFFER, "output must be a writable contiguous array")
+        if self.height and dst.nbytes < stride * (self.height - 1) + self.row_bytes:
+            raise BufferError(_lib.SGN_E_BUFFER, "output buffer too small")
+        _check(_lib.load().sgn_device_download_2d(self.session.handle, dst.ctypes.data, stride,
+                                                  self._ptr, self.pitch_bytes, self.row_bytes,
+                                                  self.height))
 
     def to_numpy(self):
         """Tightly packed (H, W, C) array (uint8 or little-endian uint16)."""
-        rb = self.width * self.pixel_format.bytes_per_pixel
-        host = self.tensor.cpu().numpy().reshape(self.height, self.pitch_bytes)[:, :rb]
-        arr = np.ascontiguousarray(host)
+        out = np.zeros(max(1, self.row_bytes * self.height), np.uint8)
+        self.download_into(out, self.row_bytes)
+        out = out[:self.row_bytes * self.height]
         if self.pixel_format.bytes_per_sample == 2:
-            arr = arr.view("<u2")
-        return arr.reshape(self.height, self.width, self.pixel_format.channels)
+            out = out.view("<u2")
+        return out.reshape(self.height, self.width, self.pixel_format.channels)
+
+    def close(self):
+        if getattr(self, "_ptr", None):
+            _lib.load().sgn_device_free(self.session.handle, self._ptr)
+            self._ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _alloc_surface(session, width, height, fmt):
-    import torch
-    pitch = width * fmt.bytes_per_pixel
-    t = torch.empty(max(1, pitch * height), dtype=torch.uint8, device=f"cuda:{session.device}")
-    return DeviceSurface(t, width, height, fmt, pitch)
+    return DeviceSurface(session, width, height, fmt)
+
+
+class HipEvent:
+    """A HIP event recorded on a session's stream."""
+
+    def __init__(self):
+        e = ctypes.c_void_p()
+        _check(_lib.load().sgn_hip_event_create(ctypes.byref(e)))
+        self.handle = e.value
+
+    def record(self, session):
+        _check(_lib.load().sgn_hip_event_record(session.handle, self.handle))
+
+    def elapsed_ms(self, end):
+        ms = ctypes.c_float()
+        _check(_lib.load().sgn_hip_event_elapsed_ms(self.handle, end.handle, ctypes.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        try:
+            _lib.load().sgn_hip_event_destroy(self.handle)
+        except Exception:
+            pass
 
 
 def scaled_dims(data, scale=Downscale.NONE):

@@ -77,6 +77,20 @@ SIGNATURES = {
     "sgn_hip_session_stream": (ctypes.c_void_p, [ctypes.c_void_p]),
     "sgn_hip_session_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
     "sgn_hip_session_dispatches": (ctypes.c_uint64, [ctypes.c_void_p]),
+    "sgn_device_alloc": (ctypes.c_int, [ctypes.c_void_p, c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
+    "sgn_device_free": (None, [ctypes.c_void_p, ctypes.c_void_p]),
+    "sgn_device_fill": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, c_size_t, ctypes.c_uint8]),
+    "sgn_device_download": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                           c_size_t]),
+    "sgn_device_upload": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                         c_size_t]),
+    "sgn_device_download_2d": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, c_size_t,
+                                              ctypes.c_void_p, c_size_t, c_size_t, c_size_t]),
+    "sgn_hip_event_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p)]),
+    "sgn_hip_event_destroy": (None, [ctypes.c_void_p]),
+    "sgn_hip_event_record": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "sgn_hip_event_elapsed_ms": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
+                                                ctypes.POINTER(ctypes.c_float)]),
     "sgn_j2k_decode_into": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, c_size_t, ctypes.c_int,
                                            ctypes.c_uint32, ctypes.c_void_p, c_size_t, c_size_t]),
     "sgn_j2k_decode_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, c_size_t,

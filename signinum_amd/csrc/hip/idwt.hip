@@ -303,6 +303,23 @@ sgn_final_irr(const FinalJob *jobs, uint32_t njobs, uint32_t block_offset, const
     final_body<false>(jobs, njobs, block_offset, lv, coef);
 }
 
+// Byte fill used instead of memset nodes (safe under graph capture).
+extern "C" __global__ void sgn_fill_bytes(uint8_t *p, size_t bytes, size_t words, uint32_t v4) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    uint32_t *w = reinterpret_cast<uint32_t *>(p);
+    const bool aligned = (reinterpret_cast<uintptr_t>(p) & 3) == 0;
+    if (aligned) {
+        for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += stride)
+            w[i] = v4;
+        for (size_t i = words * 4 + (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < bytes;
+             i += stride)
+            p[i] = (uint8_t)v4;
+    } else {
+        for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < bytes; i += stride)
+            p[i] = (uint8_t)v4;
+    }
+}
+
 // Inverse MCT on f32 planes (J2kInverseMctJob hook, lib.rs:785-800).
 extern "C" __global__ void sgn_inverse_mct_planes(float *p0, float *p1, float *p2, uint32_t n,
                                                   int reversible, float a0, float a1, float a2) {

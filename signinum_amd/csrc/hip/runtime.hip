@@ -492,6 +492,89 @@ int sgn_hip_session_synchronize(sgn_hip_session *s) {
 
 uint64_t sgn_hip_session_dispatches(sgn_hip_session *s) { return s ? s->dispatches : 0; }
 
+int sgn_device_alloc(sgn_hip_session *s, size_t bytes, void **out) {
+    if (!s || !out) return SGN_E_ARG;
+    *out = nullptr;
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipMalloc(out, bytes ? bytes : 1));
+    return SGN_OK;
+}
+
+void sgn_device_free(sgn_hip_session *s, void *p) {
+    if (!p) return;
+    if (s) {
+        (void)hipSetDevice(s->device);
+        (void)hipStreamSynchronize(s->stream);
+    }
+    (void)hipFree(p);
+}
+
+int sgn_device_fill(sgn_hip_session *s, void *p, size_t bytes, uint8_t value) {
+    if (!s || (!p && bytes)) return SGN_E_ARG;
+    if (!bytes) return SGN_OK;
+    HIP_TRY(hipSetDevice(s->device));
+    const size_t words = bytes / 4;
+    const uint32_t v4 = 0x01010101u * value;
+    const uint32_t grid = static_cast<uint32_t>(std::min<size_t>((bytes + 1023) / 1024, 65535));
+    hipLaunchKernelGGL(sgn_fill_bytes, dim3(grid), dim3(256), 0, s->stream,
+                       static_cast<uint8_t *>(p), bytes, words, v4);
+    s->dispatches++;
+    HIP_TRY(hipGetLastError());
+    return SGN_OK;
+}
+
+int sgn_device_download(sgn_hip_session *s, void *host, const void *dev, size_t bytes) {
+    if (!s || (!bytes)) return bytes ? SGN_E_ARG : SGN_OK;
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return SGN_OK;
+}
+
+int sgn_device_upload(sgn_hip_session *s, void *dev, const void *host, size_t bytes) {
+    if (!s || (!bytes)) return bytes ? SGN_E_ARG : SGN_OK;
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return SGN_OK;
+}
+
+int sgn_device_download_2d(sgn_hip_session *s, void *host, size_t host_stride, const void *dev,
+                           size_t dev_pitch, size_t row_bytes, size_t rows) {
+    if (!s || !host || !dev) return SGN_E_ARG;
+    if (!rows || !row_bytes) return SGN_OK;
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipMemcpy2DAsync(host, host_stride, dev, dev_pitch, row_bytes, rows,
+                             hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return SGN_OK;
+}
+
+int sgn_hip_event_create(void **ev) {
+    if (!ev) return SGN_E_ARG;
+    hipEvent_t e = nullptr;
+    HIP_TRY(hipEventCreate(&e));
+    *ev = e;
+    return SGN_OK;
+}
+
+void sgn_hip_event_destroy(void *ev) {
+    if (ev) (void)hipEventDestroy(static_cast<hipEvent_t>(ev));
+}
+
+int sgn_hip_event_record(sgn_hip_session *s, void *ev) {
+    if (!s || !ev) return SGN_E_ARG;
+    HIP_TRY(hipEventRecord(static_cast<hipEvent_t>(ev), s->stream));
+    return SGN_OK;
+}
+
+int sgn_hip_event_elapsed_ms(void *a, void *b, float *ms) {
+    if (!a || !b || !ms) return SGN_E_ARG;
+    HIP_TRY(hipEventSynchronize(static_cast<hipEvent_t>(b)));
+    HIP_TRY(hipEventElapsedTime(ms, static_cast<hipEvent_t>(a), static_cast<hipEvent_t>(b)));
+    return SGN_OK;
+}
+
 int sgn_j2k_plan_create(sgn_hip_session *s, const sgn_tile_job *jobs, size_t n, int fmt,
                         uint32_t den, sgn_j2k_plan **out, int32_t *tile_status) {
     API_GUARD_BEGIN

@@ -139,10 +139,8 @@ def test_plan_replay_graph(session, oracle):
     streams, pix, _cls = synth.lossless_batch_with_pixels(12, size=96)
     plan = S.TilePlan(session, streams, S.PixelFormat.RGB8)
     for _ in range(3):
-        import torch
         for s in plan.surfaces:
-            s.tensor.zero_()
-        torch.cuda.synchronize()
+            s.fill(0)
         plan.decode(S.MODE_GRAPH)
         session.synchronize()
         for s, img in zip(plan.surfaces, pix):
