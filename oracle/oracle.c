@@ -1695,3 +1695,68 @@ int oracle_inverse_mct(int reversible, float *p0, float *p1, float *p2, size_t n
     }
     return ORACLE_OK;
 }
+
+/* Code-block listing of a full-resolution decode, in geometry order
+ * (component, resolution, precinct, band, block): rect in band coordinates,
+ * data offset within the codestream (or -1), lengths, Zblk and passes. */
+int oracle_code_blocks(const uint8_t *data, size_t len, oracle_cb_t *out, size_t cap,
+                       size_t *count)
+{
+    oracle_build_tables();
+    header_t *h = (header_t *)calloc(1, sizeof(header_t));
+    if (!h) return ORACLE_E_NOMEM;
+    int rc = read_main_header(data, len, h, 0, 0);
+    if (rc) {
+        free(h);
+        return rc;
+    }
+    uint32_t ntx = cdiv(h->xsiz - h->xto, h->xt), nty = cdiv(h->ysiz - h->yto, h->yt);
+    tile_t *tiles = (tile_t *)calloc(ntx * nty, sizeof(tile_t));
+    tcomp_t *tcs = (tcomp_t *)calloc(h->ncomp, sizeof(tcomp_t));
+    size_t n = 0;
+    rc = tiles && tcs ? parse_tiles(data, len, h, tiles, ntx * nty) : ORACLE_E_NOMEM;
+    for (uint32_t ti = 0; ti < ntx * nty && !rc; ++ti) {
+        tile_t *t = &tiles[ti];
+        for (int c = 0; c < h->ncomp && !rc; ++c) rc = build_tcomp(h, &t->comp[c], t->rect, &tcs[c]);
+        if (rc) break;
+        prog_list_t pl = {0};
+        rc = build_progression(t, h->ncomp, tcs, &pl);
+        if (!rc) parse_packets(t, h->ncomp, tcs, &pl);
+        free(pl.v);
+        for (int c = 0; c < h->ncomp; ++c) {
+            for (int r = 0; r < tcs[c].nres; ++r) {
+                res_t *rs = &tcs[c].res[r];
+                uint32_t np = rs->npx * rs->npy;
+                for (uint32_t p = 0; p < np; ++p)
+                    for (int b = 0; b < rs->nbands; ++b) {
+                        prec_band_t *pb = &rs->prec[(size_t)p * rs->nbands + b];
+                        for (uint32_t k = 0; k < pb->ncbx * pb->ncby; ++k, ++n) {
+                            cblk_t *cb = &pb->cbs[k];
+                            if (!out || n >= cap) continue;
+                            oracle_cb_t *o = &out[n];
+                            o->tile = ti;
+                            o->comp = (uint32_t)c;
+                            o->res = (uint32_t)r;
+                            o->band = (uint32_t)rs->bands[b].type;
+                            o->x0 = cb->rect.x0;
+                            o->y0 = cb->rect.y0;
+                            o->x1 = cb->rect.x1;
+                            o->y1 = cb->rect.y1;
+                            o->offset = cb->data ? (int64_t)(cb->data - t->body) : -1;
+                            o->lcup = cb->lcup;
+                            o->lref = cb->lref;
+                            o->zblk = cb->zblk;
+                            o->passes = cb->data ? cb->passes : 0;
+                        }
+                    }
+            }
+            free_tcomp(&tcs[c]);
+        }
+    }
+    for (uint32_t ti = 0; tiles && ti < ntx * nty; ++ti) free(tiles[ti].body);
+    free(tiles);
+    free(tcs);
+    free(h);
+    if (count) *count = n;
+    return rc;
+}

@@ -94,6 +94,17 @@ int oracle_idwt_level(int reversible, uint32_t x0, uint32_t y0, uint32_t x1, uin
 int oracle_inverse_mct(int reversible, float *p0, float *p1, float *p2, size_t n, float a0,
                        float a1, float a2);
 
+typedef struct {
+    uint32_t tile, comp, res, band;
+    uint32_t x0, y0, x1, y1;
+    int64_t offset; /* within the tile's concatenated tile-part bodies */
+    uint32_t lcup, lref, zblk, passes;
+} oracle_cb_t;
+
+/* Code-blocks of a full-resolution decode in geometry order. */
+int oracle_code_blocks(const uint8_t *data, size_t len, oracle_cb_t *out, size_t cap,
+                       size_t *count);
+
 /* Test double: zero every refinement segment before block decoding. */
 void oracle_set_zero_refinement(int enable);
 

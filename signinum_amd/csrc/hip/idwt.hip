@@ -53,11 +53,19 @@ template <bool REV>
 __device__ __forceinline__ typename Sample<REV>::T load_level(const LevelJob &lj,
                                                               const uint32_t *__restrict__ coef,
                                                               int lx, int ly, int w, int h) {
-    const uint32_t ax = lj.x0 + (uint32_t)reflect(lx, w);
-    const uint32_t ay = lj.y0 + (uint32_t)reflect(ly, h);
-    const uint32_t b = (ax & 1u) | ((ay & 1u) << 1);
-    const uint32_t col = (ax & 1u) ? (ax >> 1) - (lj.x0 >> 1) : (ax >> 1) - ((lj.x0 + 1) >> 1);
-    const uint32_t row = (ay & 1u) ? (ay >> 1) - (lj.y0 >> 1) : (ay >> 1) - ((lj.y0 + 1) >> 1);
+    const uint32_t mx = (uint32_t)reflect(lx, w), my = (uint32_t)reflect(ly, h);
+    uint32_t b, col, row;
+    if (lj.levels == 0) {
+        // no synthesis: the low-pass band is the level itself
+        b = 0;
+        col = mx;
+        row = my;
+    } else {
+        const uint32_t ax = lj.x0 + mx, ay = lj.y0 + my;
+        b = (ax & 1u) | ((ay & 1u) << 1);
+        col = (ax & 1u) ? (ax >> 1) - (lj.x0 >> 1) : (ax >> 1) - ((lj.x0 + 1) >> 1);
+        row = (ay & 1u) ? (ay >> 1) - (lj.y0 >> 1) : (ay >> 1) - ((lj.y0 + 1) >> 1);
+    }
     const uint32_t v = coef[lj.band[b] + row * lj.bw[b] + col];
     const bool isf = (lj.float_in >> b) & 1u;
     if constexpr (REV) {

@@ -174,3 +174,23 @@ def tables():
     u1 = np.zeros(256, np.uint16)
     lib().oracle_get_tables(v0.ctypes.data, v1.ctypes.data, u0.ctypes.data, u1.ctypes.data)
     return v0, v1, u0, u1
+
+
+class OracleCb(ctypes.Structure):
+    _fields_ = ([(n, ctypes.c_uint32) for n in ("tile", "comp", "res", "band", "x0", "y0", "x1",
+                                                 "y1")] + [("offset", ctypes.c_int64)] +
+                [(n, ctypes.c_uint32) for n in ("lcup", "lref", "zblk", "passes")])
+
+
+def code_blocks(cs):
+    L = lib()
+    L.oracle_code_blocks.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.POINTER(OracleCb),
+                                     ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
+    p, n, _k = _ptr(cs)
+    cnt = ctypes.c_size_t()
+    rc = L.oracle_code_blocks(p, n, None, 0, ctypes.byref(cnt))
+    assert rc == 0, rc
+    arr = (OracleCb * max(1, cnt.value))()
+    rc = L.oracle_code_blocks(p, n, arr, cnt.value, ctypes.byref(cnt))
+    assert rc == 0, rc
+    return list(arr[:cnt.value])

@@ -94,3 +94,34 @@ def test_cpu_backend_request_is_rejected_before_any_work():
     cs = S.encode_j2k_lossless(synth.texture_rgb(rng, 16, 16))
     with pytest.raises(S.UnsupportedBackendError):
         S.decode_tiles_to_device(None, [cs], S.PixelFormat.RGB8, backend=S.BackendRequest.CPU)
+
+
+@pytest.mark.parametrize("kw", [
+    dict(), dict(progression=1), dict(progression=2, precinct_log2=5),
+    dict(progression=3, precinct_log2=4), dict(progression=4, precinct_log2=6),
+    dict(tile_width=100, tile_height=60, progression=2), dict(sop=1, eph=1),
+    dict(levels=0), dict(cb_width_log2=4, cb_height_log2=7, levels=3),
+])
+def test_packet_parse_matches_oracle(oracle, kw):
+    """The host packet-header parser yields the oracle's code-block table:
+    same blocks in geometry order with equal lengths, Zblk and passes."""
+    rng = np.random.default_rng(len(kw) * 13 + sum(map(int, kw.values())))
+    img = synth.tissue_rgb(rng, 150, 90)
+    cs = S.encode_j2k_ht(img, **kw)
+    ours = [(j.width, j.height, j.cleanup_length, j.refinement_length, j.number_of_coding_passes,
+             j.missing_bit_planes if j.number_of_coding_passes else None)
+            for j in S.code_block_jobs(cs)]
+    ref = [(c.x1 - c.x0, c.y1 - c.y0, c.lcup if c.passes else 0, c.lref if c.passes else 0,
+            c.passes, c.zblk if c.passes else None)
+           for c in O.code_blocks(cs) if c.x1 > c.x0 and c.y1 > c.y0]
+    assert ours == ref
+
+
+def test_packet_parse_matches_oracle_on_fixtures(oracle):
+    for name in ("openhtj2k_ds0_ht_12_b11", "openhtj2k_ds0_ht_09_b11"):
+        cs = open(os.path.join(ROOT, "tests", "golden", f"{name}.j2k"), "rb").read()
+        ours = [(j.width, j.height, j.cleanup_length, j.refinement_length,
+                 j.number_of_coding_passes) for j in S.code_block_jobs(cs)]
+        ref = [(c.x1 - c.x0, c.y1 - c.y0, c.lcup if c.passes else 0, c.lref if c.passes else 0,
+                c.passes) for c in O.code_blocks(cs) if c.x1 > c.x0 and c.y1 > c.y0]
+        assert ours == ref
