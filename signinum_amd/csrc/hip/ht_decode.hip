@@ -38,15 +38,21 @@ __device__ __forceinline__ void load_tables(const uint16_t *g, uint16_t *s, int 
 
 // Magnitude/sign of one sample (ht_block_decode.rs:1010-1032); returns the
 // u32 sign-magnitude sample and the decoded v_n.
-__device__ __forceinline__ uint32_t ms_sample(FwdReader &ms, uint32_t inf, uint32_t bit,
+__device__ __forceinline__ uint32_t ms_fetch(FwdReader &r) { return fwd_fetch(r); }
+__device__ __forceinline__ uint32_t ms_fetch(FwdChunked &r) { return fwdc_fetch(r); }
+__device__ __forceinline__ void ms_advance(FwdReader &r, uint32_t n) { fwd_advance(r, n); }
+__device__ __forceinline__ void ms_advance(FwdChunked &r, uint32_t n) { fwdc_advance(r, n); }
+
+template <typename Reader>
+__device__ __forceinline__ uint32_t ms_sample(Reader &ms, uint32_t inf, uint32_t bit,
                                               uint32_t uq, uint32_t p, uint32_t &vn) {
     if ((inf & (1u << (4 + bit))) == 0) {
         vn = 0;
         return 0;
     }
-    const uint32_t ms_val = fwd_fetch(ms);
+    const uint32_t ms_val = ms_fetch(ms);
     const uint32_t m_n = uq - ((inf >> (12 + bit)) & 1u);
-    fwd_advance(ms, m_n);
+    ms_advance(ms, m_n);
     uint32_t value = ms_val << 31;
     const uint32_t mask = m_n == 0 ? 0u : (m_n >= 32 ? 0xFFFFFFFFu : ((1u << m_n) - 1));
     uint32_t v = ms_val & mask;
@@ -99,21 +105,22 @@ sgn_ht_cleanup_lanes(const CbJob *__restrict__ jobs, uint32_t n, const uint8_t *
         s_rho[q][lane] = 0;
         s_e[q][lane] = 0;
     }
-    MelState mel;
-    mel.d = d;
-    mel.pos = lcup - scup;
-    mel.remaining = scup - 1;
+    MelChunked mel;
+    fwdb_init(mel.src, d + (lcup - scup), scup - 1, 0xFF);
     mel.unstuff = 0;
     mel.cur = 0;
     mel.bits_left = 0;
     mel.k = 0;
     mel.num_runs = 0;
     mel.runs = 0;
-    RevReader vlc;
-    rev_init_vlc(vlc, d, lcup, scup);
-    FwdReader ms;
-    fwd_init(ms, d, lcup - scup, 0xFF);
-    int run = mel_run(mel);
+    RevChunked vlc;
+    revc_init_vlc(vlc, d, lcup, scup);
+    FwdChunked ms;
+    fwdb_init(ms.src, d, lcup - scup, 0xFF);
+    ms.tmp = 0;
+    ms.bits = 0;
+    ms.unstuff = 0;
+    int run = melc_run(mel);
 
     for (uint32_t y = 0; y < h && st == kCbOk; y += 2) {
         const bool first_row = (y == 0);
@@ -127,36 +134,36 @@ sgn_ht_cleanup_lanes(const CbJob *__restrict__ jobs, uint32_t n, const uint8_t *
             const uint32_t pt2 = (uint32_t)s_rho[qj + 2][lane] << 4;
             uint32_t t0, t1, u0, u1;
             if (first_row) {
-                uint32_t vv = rev_fetch(vlc);
+                uint32_t vv = revc_fetch(vlc);
                 t0 = vlc0[c_q + (vv & 0x7F)];
                 if (c_q == 0) {
                     run -= 2;
                     t0 = (run == -1) ? t0 : 0;
-                    if (run < 0) run = mel_run(mel);
+                    if (run < 0) run = melc_run(mel);
                 }
                 c_q = ((t0 & 0x10) << 3) | ((t0 & 0xE0) << 2);
-                vv = rev_advance(vlc, t0 & 7);
+                vv = revc_advance(vlc, t0 & 7);
                 t1 = vlc0[c_q + (vv & 0x7F)];
                 if (c_q == 0 && x + 2 < w) {
                     run -= 2;
                     t1 = (run == -1) ? t1 : 0;
-                    if (run < 0) run = mel_run(mel);
+                    if (run < 0) run = melc_run(mel);
                 }
                 if (x + 2 >= w) t1 = 0;
                 c_q = ((t1 & 0x10) << 3) | ((t1 & 0xE0) << 2);
-                vv = rev_advance(vlc, t1 & 7);
+                vv = revc_advance(vlc, t1 & 7);
                 uint32_t mode = ((t0 & 8) << 3) | ((t1 & 8) << 4);
                 if (mode == 0xC0) {
                     run -= 2;
                     if (run == -1) mode += 0x40;
-                    if (run < 0) run = mel_run(mel);
+                    if (run < 0) run = melc_run(mel);
                 }
                 uint32_t e = uvlc0[mode + (vv & 0x3F)];
-                vv = rev_advance(vlc, e & 7);
+                vv = revc_advance(vlc, e & 7);
                 e >>= 3;
                 uint32_t len = e & 0xF;
                 const uint32_t tmp = vv & ((1u << len) - 1);
-                rev_advance(vlc, len);
+                revc_advance(vlc, len);
                 e >>= 4;
                 len = e & 7;
                 e >>= 3;
@@ -165,35 +172,35 @@ sgn_ht_cleanup_lanes(const CbJob *__restrict__ jobs, uint32_t n, const uint8_t *
             } else {
                 c_q |= (pt0 & 0xA0) << 2;
                 c_q |= (pt1 & 0x20) << 4;
-                uint32_t vv = rev_fetch(vlc);
+                uint32_t vv = revc_fetch(vlc);
                 t0 = vlc1[c_q + (vv & 0x7F)];
                 if (c_q == 0) {
                     run -= 2;
                     t0 = (run == -1) ? t0 : 0;
-                    if (run < 0) run = mel_run(mel);
+                    if (run < 0) run = melc_run(mel);
                 }
                 c_q = ((t0 & 0x40) << 2) | ((t0 & 0x80) << 1);
                 c_q |= pt0 & 0x80;
                 c_q |= (pt1 & 0xA0) << 2;
                 c_q |= (pt2 & 0x20) << 4;
-                vv = rev_advance(vlc, t0 & 7);
+                vv = revc_advance(vlc, t0 & 7);
                 t1 = vlc1[c_q + (vv & 0x7F)];
                 if (c_q == 0 && x + 2 < w) {
                     run -= 2;
                     t1 = (run == -1) ? t1 : 0;
-                    if (run < 0) run = mel_run(mel);
+                    if (run < 0) run = melc_run(mel);
                 }
                 if (x + 2 >= w) t1 = 0;
                 c_q = ((t1 & 0x40) << 2) | ((t1 & 0x80) << 1);
                 c_q |= pt1 & 0x80;
-                vv = rev_advance(vlc, t1 & 7);
+                vv = revc_advance(vlc, t1 & 7);
                 const uint32_t mode = ((t0 & 8) << 3) | ((t1 & 8) << 4);
                 uint32_t e = uvlc1[mode + (vv & 0x3F)];
-                vv = rev_advance(vlc, e & 7);
+                vv = revc_advance(vlc, e & 7);
                 e >>= 3;
                 uint32_t len = e & 0xF;
                 const uint32_t tmp = vv & ((1u << len) - 1);
-                rev_advance(vlc, len);
+                revc_advance(vlc, len);
                 e >>= 4;
                 len = e & 7;
                 e >>= 3;

@@ -785,7 +785,8 @@ static int hook_blocks(sgn_hip_session *s, const uint8_t *data, size_t data_len,
         for (void *q : p)
             if (q) (void)hipFree(q);
     };
-    if (hipMalloc(&d_data, std::max<size_t>(16, data_len)) != hipSuccess ||
+    // chunked stream readers load whole aligned 8-byte words: pad to 16
+    if (hipMalloc(&d_data, (std::max<size_t>(16, data_len) + 15) & ~size_t(15)) != hipSuccess ||
         hipMalloc(&d_out, std::max<size_t>(4, out_len * 4)) != hipSuccess ||
         hipMalloc(&d_st, std::max<size_t>(1, n)) != hipSuccess) {
         cleanup();
