@@ -88,6 +88,8 @@ int sgn_device_alloc(sgn_hip_session *session, size_t bytes, void **out);
 void sgn_device_free(sgn_hip_session *session, void *ptr);
 /* Fills bytes with `value` using a kernel (no memset node), asynchronous. */
 int sgn_device_fill(sgn_hip_session *session, void *ptr, size_t bytes, uint8_t value);
+/* Device-to-device copy on the session stream (asynchronous). */
+int sgn_device_copy(sgn_hip_session *session, void *dst, const void *src, size_t bytes);
 int sgn_device_download(sgn_hip_session *session, void *host, const void *dev, size_t bytes);
 int sgn_device_upload(sgn_hip_session *session, void *dev, const void *host, size_t bytes);
 /* Pitched download of `rows` rows of `row_bytes` each. */

@@ -214,6 +214,14 @@ class DeviceSurface:
     def fill(self, value=0):
         _check(_lib.load().sgn_device_fill(self.session.handle, self._ptr, self.byte_len, value))
 
+    def copy_from(self, other):
+        """Device-to-device copy of another surface of the same byte length
+        (asynchronous on the session stream)."""
+        if other.byte_len != self.byte_len:
+            raise BufferError(_lib.SGN_E_BUFFER, "surfaces differ in size")
+        _check(_lib.load().sgn_device_copy(self.session.handle, self._ptr, other._ptr,
+                                           self.byte_len))
+
     def download_into(self, out, stride):
         dst = np.asarray(out)
         if not dst.flags["C_CONTIGUOUS"] or not dst.flags["WRITEABLE"]:

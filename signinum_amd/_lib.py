@@ -80,6 +80,7 @@ SIGNATURES = {
     "sgn_device_alloc": (ctypes.c_int, [ctypes.c_void_p, c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
     "sgn_device_free": (None, [ctypes.c_void_p, ctypes.c_void_p]),
     "sgn_device_fill": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, c_size_t, ctypes.c_uint8]),
+    "sgn_device_copy": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, c_size_t]),
     "sgn_device_download": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                            c_size_t]),
     "sgn_device_upload": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,

@@ -158,35 +158,37 @@ __device__ __forceinline__ int32_t to_i32(uint32_t v, uint32_t kmax) {
 // The lane kernel reads its three streams through 8-byte aligned loads with
 // one chunk prefetched ahead, so the serial bit-unstuffing logic runs on
 // registers and a global load is only waited for once every eight bytes.
-// Loads never touch an aligned word that holds no byte of the segment; the
-// rest of such a word lies inside the (16-byte padded) allocation.
+// Every load is unconditional with its address clamped to the last aligned
+// word that holds a byte of the segment: the loaded register is the
+// prefetch register itself, so the wait for it falls at its use one chunk
+// later, not at the load.  Bytes past the end of a segment are replaced by
+// the stream's pad value.  The arena holding the segments is 8-byte aligned
+// and padded to a multiple of 16 bytes.
 
 struct FwdBytes {
-    const uint64_t *p;  // next aligned chunk to prefetch
+    const uint64_t *p;     // next aligned chunk to prefetch
+    const uint64_t *last;  // last chunk holding a byte of the segment
     uint64_t cur, nxt;
-    uint32_t cur_n;     // bytes left in cur
-    int32_t left;       // segment bytes not yet delivered
-    int32_t covered;    // segment bytes inside the chunks loaded so far
-    int32_t len;
-    uint32_t pad;       // value delivered past the end of the segment
+    uint32_t cur_n;        // bytes left in cur
+    int32_t left;          // segment bytes not yet delivered
+    uint32_t pad;          // value delivered past the end of the segment
 };
 
-__device__ __forceinline__ void fwdb_init(FwdBytes &s, const uint8_t *d, uint32_t len,
-                                          uint32_t pad) {
-    const uintptr_t a = reinterpret_cast<uintptr_t>(d);
-    const uint64_t *p = reinterpret_cast<const uint64_t *>(a & ~uintptr_t(7));
-    const uint32_t sh = (uint32_t)(a & 7);
+// The stream starts at byte `off` of `base`.  Addresses are formed from
+// `base` by pointer arithmetic only, so the compiler keeps them in the
+// global address space (global loads wait on vmcnt alone; a generic pointer
+// would make every LDS wait also wait for the prefetch).
+__device__ __forceinline__ void fwdb_init(FwdBytes &s, const uint8_t *base, uint32_t off,
+                                          uint32_t len, uint32_t pad) {
+    const uint64_t *w = reinterpret_cast<const uint64_t *>(base);
+    const uint64_t *p = w + (off >> 3);
+    const uint32_t sh = off & 7u;
+    s.last = w + ((off + (len ? len - 1 : 0)) >> 3);
     s.pad = pad;
-    s.len = (int32_t)len;
     s.left = (int32_t)len;
-    s.cur = len ? (p[0] >> (8 * sh)) : 0;
+    s.cur = p[0] >> (8 * sh);
     s.cur_n = 8 - sh;
-    s.covered = (int32_t)(8 - sh);
-    s.nxt = 0;
-    if (s.covered < s.len) {
-        s.nxt = p[1];
-        s.covered += 8;
-    }
+    s.nxt = *(p + 1 <= s.last ? p + 1 : s.last);
     s.p = p + 2;
 }
 
@@ -194,11 +196,7 @@ __device__ __forceinline__ uint32_t fwdb_next(FwdBytes &s) {
     if (s.cur_n == 0) {
         s.cur = s.nxt;
         s.cur_n = 8;
-        s.nxt = 0;
-        if (s.covered < s.len) {
-            s.nxt = *s.p;
-            s.covered += 8;
-        }
+        s.nxt = *(s.p <= s.last ? s.p : s.last);
         ++s.p;
     }
     uint32_t b = (uint32_t)(s.cur & 0xFF);
@@ -209,36 +207,35 @@ __device__ __forceinline__ uint32_t fwdb_next(FwdBytes &s) {
     return b;
 }
 
-// Bytes at d[start], d[start-1], ... for `len` bytes, then zeros.
+// Bytes base[start], base[start-1], ... for `len` bytes, then zeros.
 struct RevBytes {
-    const uint64_t *p;
+    const uint64_t *p;      // next aligned chunk to prefetch
+    const uint64_t *first;  // lowest chunk holding a byte of the segment
     uint64_t cur, nxt;
     uint32_t cur_n;
-    int32_t left, covered, len;
+    int32_t left;
 };
 
-__device__ __forceinline__ void revb_init(RevBytes &s, const uint8_t *d, int32_t start,
+// Requires start + 1 >= len.
+__device__ __forceinline__ void revb_init(RevBytes &s, const uint8_t *base, uint32_t start,
                                           uint32_t len) {
-    s.len = (int32_t)len;
+    const uint64_t *w = reinterpret_cast<const uint64_t *>(base);
     s.left = (int32_t)len;
-    s.nxt = 0;
     if (len == 0) {
-        s.p = nullptr;
+        // nothing to deliver: every fetch reads the arena's first word
+        s.p = w;
+        s.first = w;
         s.cur = 0;
         s.cur_n = 8;
-        s.covered = 0;
+        s.nxt = 0;
         return;
     }
-    const uintptr_t a = reinterpret_cast<uintptr_t>(d + start);
-    const uint64_t *p = reinterpret_cast<const uint64_t *>(a & ~uintptr_t(7));
-    const uint32_t sh = (uint32_t)(a & 7);
+    const uint64_t *p = w + (start >> 3);
+    const uint32_t sh = start & 7u;
+    s.first = w + ((start + 1 - len) >> 3);
     s.cur = p[0] << (8 * (7 - sh));
     s.cur_n = sh + 1;
-    s.covered = (int32_t)(sh + 1);
-    if (s.covered < s.len) {
-        s.nxt = p[-1];
-        s.covered += 8;
-    }
+    s.nxt = *(p - 1 >= s.first ? p - 1 : s.first);
     s.p = p - 2;
 }
 
@@ -246,11 +243,7 @@ __device__ __forceinline__ uint32_t revb_next(RevBytes &s) {
     if (s.cur_n == 0) {
         s.cur = s.nxt;
         s.cur_n = 8;
-        s.nxt = 0;
-        if (s.covered < s.len) {
-            s.nxt = *s.p;
-            s.covered += 8;
-        }
+        s.nxt = *(s.p >= s.first ? s.p : s.first);
         --s.p;
     }
     uint32_t b = (uint32_t)(s.cur >> 56);
@@ -339,13 +332,14 @@ struct RevChunked {
     uint32_t bits, unstuff;
 };
 
-__device__ __forceinline__ void revc_init_vlc(RevChunked &r, const uint8_t *d, uint32_t lcup,
-                                              uint32_t scup) {
-    const uint32_t b = d[lcup - 2];
+// The block's cleanup segment is base[src .. src + lcup).
+__device__ __forceinline__ void revc_init_vlc(RevChunked &r, const uint8_t *base, uint32_t src,
+                                              uint32_t lcup, uint32_t scup) {
+    const uint32_t b = base[src + lcup - 2];
     r.tmp = b >> 4;
     r.bits = 4 - (((r.tmp & 7) == 7) ? 1u : 0u);
     r.unstuff = ((b | 0x0F) > 0x8F) ? 1u : 0u;
-    revb_init(r.src, d, (int32_t)lcup - 3, scup - 2);
+    revb_init(r.src, base, src + lcup - 3, scup - 2);
 }
 
 __device__ __forceinline__ uint32_t revc_fetch(RevChunked &r) {

@@ -106,7 +106,7 @@ sgn_ht_cleanup_lanes(const CbJob *__restrict__ jobs, uint32_t n, const uint8_t *
         s_e[q][lane] = 0;
     }
     MelChunked mel;
-    fwdb_init(mel.src, d + (lcup - scup), scup - 1, 0xFF);
+    fwdb_init(mel.src, arena, j.src + (lcup - scup), scup - 1, 0xFF);
     mel.unstuff = 0;
     mel.cur = 0;
     mel.bits_left = 0;
@@ -114,9 +114,9 @@ sgn_ht_cleanup_lanes(const CbJob *__restrict__ jobs, uint32_t n, const uint8_t *
     mel.num_runs = 0;
     mel.runs = 0;
     RevChunked vlc;
-    revc_init_vlc(vlc, d, lcup, scup);
+    revc_init_vlc(vlc, arena, j.src, lcup, scup);
     FwdChunked ms;
-    fwdb_init(ms.src, d, lcup - scup, 0xFF);
+    fwdb_init(ms.src, arena, j.src, lcup - scup, 0xFF);
     ms.tmp = 0;
     ms.bits = 0;
     ms.unstuff = 0;

@@ -30,6 +30,7 @@ constexpr float kNegGamma = -0.8829111f;
 constexpr float kNegDelta = -0.44350687f;
 
 __device__ __forceinline__ int reflect(int i, int n) {
+    if ((unsigned)i < (unsigned)n) return i;  // interior: no extension
     if (n == 1) return 0;
     const int period = 2 * (n - 1);
     int m = i % period;
@@ -48,61 +49,108 @@ struct Sample<false> {
     using T = float;
 };
 
+// Per-level constants.  The per-band tables live in LDS (indexed by the
+// band of each sample); the rest are wave-uniform scalars.  Nothing is
+// copied into a by-value struct that per-lane indexing would spill.
+struct BandLds {
+    uint32_t base[4];
+    uint32_t pitch[4];
+    float delta[4];
+};
+
+struct LevelView {
+    const BandLds *bands;
+    uint32_t x0, y0, fin;
+    int w, h, levels;
+};
+
+// Stages the band table of `lj` into `sb`; ends with a barrier.
+__device__ __forceinline__ LevelView level_view(const LevelJob *__restrict__ lj, BandLds *sb) {
+    const int t = threadIdx.x;
+    if (t < 4) {
+        sb->base[t] = lj->band[t];
+        sb->pitch[t] = lj->bw[t];
+        sb->delta[t] = lj->delta[t];
+    }
+    LevelView v;
+    v.bands = sb;
+    v.x0 = lj->x0;
+    v.y0 = lj->y0;
+    v.fin = lj->float_in;
+    v.w = (int)(lj->x1 - lj->x0);
+    v.h = (int)(lj->y1 - lj->y0);
+    v.levels = lj->levels;
+    __syncthreads();
+    return v;
+}
+
 // Value of level-local position (lx, ly) after extension.
 template <bool REV>
-__device__ __forceinline__ typename Sample<REV>::T load_level(const LevelJob &lj,
+__device__ __forceinline__ typename Sample<REV>::T load_level(const LevelView &L,
                                                               const uint32_t *__restrict__ coef,
-                                                              int lx, int ly, int w, int h) {
-    const uint32_t mx = (uint32_t)reflect(lx, w), my = (uint32_t)reflect(ly, h);
+                                                              int lx, int ly) {
+    const uint32_t mx = (uint32_t)reflect(lx, L.w), my = (uint32_t)reflect(ly, L.h);
     uint32_t b, col, row;
-    if (lj.levels == 0) {
+    if (L.levels == 0) {
         // no synthesis: the low-pass band is the level itself
         b = 0;
         col = mx;
         row = my;
     } else {
-        const uint32_t ax = lj.x0 + mx, ay = lj.y0 + my;
+        const uint32_t ax = L.x0 + mx, ay = L.y0 + my;
         b = (ax & 1u) | ((ay & 1u) << 1);
-        col = (ax & 1u) ? (ax >> 1) - (lj.x0 >> 1) : (ax >> 1) - ((lj.x0 + 1) >> 1);
-        row = (ay & 1u) ? (ay >> 1) - (lj.y0 >> 1) : (ay >> 1) - ((lj.y0 + 1) >> 1);
+        col = (ax & 1u) ? (ax >> 1) - (L.x0 >> 1) : (ax >> 1) - ((L.x0 + 1) >> 1);
+        row = (ay & 1u) ? (ay >> 1) - (L.y0 >> 1) : (ay >> 1) - ((L.y0 + 1) >> 1);
     }
-    const uint32_t v = coef[lj.band[b] + row * lj.bw[b] + col];
-    const bool isf = (lj.float_in >> b) & 1u;
+    const uint32_t v = coef[L.bands->base[b] + row * L.bands->pitch[b] + col];
+    const bool isf = (L.fin >> b) & 1u;
     if constexpr (REV) {
         return isf ? (int32_t)__uint_as_float(v) : (int32_t)v;
     } else {
-        return isf ? __uint_as_float(v) : (float)(int32_t)v * lj.delta[b];
+        return isf ? __uint_as_float(v) : (float)(int32_t)v * L.bands->delta[b];
     }
 }
 
-// One-dimensional lifting over `count` lines of a window.  Element e of
-// line l is at base[l * lstride + e * estride]; absolute parity of e is
-// (origin + e) & 1.  Only elements 1..n-2 are updated.
-template <bool REV, typename T>
-__device__ __forceinline__ void lift_lines(T *base, int count, int lstride, int n, int estride,
-                                           uint32_t origin, bool single, bool odd_single) {
+// Lifting over the window.  HORIZ: lines are window rows (all nh of them),
+// elements run along x; otherwise lines are the 64 block columns and
+// elements run along y.  Window pitch WN and the pair count are
+// compile-time constants, so index arithmetic needs no runtime division.
+template <bool REV, int WN, bool HORIZ, typename T>
+__device__ __forceinline__ void lift(T *win, int nlines, int n, uint32_t origin, bool single,
+                                     bool odd_single) {
+    constexpr int H = REV ? 2 : 4;
+    constexpr int PAIRS = WN / 2;
     const int tid = threadIdx.x;
+    auto at = [&](int line, int e) -> T & {
+        return HORIZ ? win[line * WN + e] : win[e * WN + H + line];
+    };
     if (single) {
-        // length-1 signal: identity, or halved when it sits at an odd
-        // coordinate (idwt.rs:726-731,942-957)
+        // length-1 signal: identity, or halved at an odd coordinate
+        // (idwt.rs:726-731,942-957)
         if (!REV && odd_single)
-            for (int k = tid; k < count * n; k += kIdwtThreads) {
-                const int l = k / n, e = k - l * n;
-                base[l * lstride + e * estride] = base[l * lstride + e * estride] * (T)0.5f;
+            for (int k = tid; k < nlines * WN; k += kIdwtThreads) {
+                const int l = HORIZ ? k / WN : k % kBlk, e = HORIZ ? k % WN : k / kBlk;
+                if (l < nlines && e < n) at(l, e) = at(l, e) * (T)0.5f;
             }
         __syncthreads();
         return;
     }
-    const int half = n / 2 + 1;
-    const int items = count * half;
+    const int items = HORIZ ? nlines * PAIRS : PAIRS * kBlk;
     auto step = [&](uint32_t parity, auto f) {
         for (int k = tid; k < items; k += kIdwtThreads) {
-            const int l = k / half;
-            int e = 2 * (k - l * half);
+            int l, e;
+            if (HORIZ) {
+                l = k / PAIRS;
+                e = 2 * (k - l * PAIRS);
+            } else {
+                l = k % kBlk;
+                e = 2 * (k / kBlk);
+                if (l >= nlines) continue;
+            }
             if (((origin + (uint32_t)e) & 1u) != parity) e += 1;
             if (e < 1 || e > n - 2) continue;
-            T *p = base + l * lstride + e * estride;
-            p[0] = f(p[0], p[-estride], p[estride]);
+            T &c = at(l, e);
+            c = f(c, at(l, e - 1), at(l, e + 1));
         }
         __syncthreads();
     };
@@ -111,10 +159,13 @@ __device__ __forceinline__ void lift_lines(T *base, int count, int lstride, int 
         step(1u, [](int32_t s, int32_t a, int32_t b) { return s + ((a + b) >> 1); });
     } else {
         const float inv_k = 1.0f / kKappa;
-        for (int k = tid; k < count * n; k += kIdwtThreads) {
-            const int l = k / n, e = k - l * n;
-            T *p = base + l * lstride + e * estride;
-            p[0] = p[0] * (((origin + (uint32_t)e) & 1u) ? inv_k : kKappa);
+        // scaling (steps 1-2 of F.3.8.2) on every element of the lines
+        const int all = HORIZ ? nlines * WN : WN * kBlk;
+        for (int k = tid; k < all; k += kIdwtThreads) {
+            const int l = HORIZ ? k / WN : k % kBlk, e = HORIZ ? k % WN : k / kBlk;
+            if (l >= nlines || e >= n) continue;
+            T &c = at(l, e);
+            c = c * (((origin + (uint32_t)e) & 1u) ? inv_k : kKappa);
         }
         __syncthreads();
         step(0u, [](float s, float a, float b) { return fmaf(a + b, kNegDelta, s); });
@@ -124,29 +175,24 @@ __device__ __forceinline__ void lift_lines(T *base, int count, int lstride, int 
     }
 }
 
-// Synthesises block (bx, by) of one level into the window; returns the
-// block's geometry.  Result element (x, y) of the block is at
-// win[(y + H) * WN + x + H].
+// Synthesises block (bx, by) of one level into the window.  Result element
+// (x, y) of the block is at win[(y + H) * WN + x + H].
 template <bool REV>
-__device__ void synth_block(const LevelJob &lj, const uint32_t *__restrict__ coef, int bx, int by,
+__device__ __forceinline__ void synth_block(const LevelView &L, const uint32_t *__restrict__ coef, int bx, int by,
                             typename Sample<REV>::T *win, int &bw_out, int &bh_out) {
     constexpr int H = REV ? 2 : 4;
     constexpr int WN = kBlk + 2 * H;
-    const int w = (int)(lj.x1 - lj.x0), h = (int)(lj.y1 - lj.y0);
     const int ox = bx * kBlk, oy = by * kBlk;
-    const int bw = min(kBlk, w - ox), bh = min(kBlk, h - oy);
+    const int bw = min(kBlk, L.w - ox), bh = min(kBlk, L.h - oy);
     const int nw = bw + 2 * H, nh = bh + 2 * H;
-    for (int k = threadIdx.x; k < nw * nh; k += kIdwtThreads) {
-        const int yy = k / nw, xx = k - yy * nw;
-        win[yy * WN + xx] = load_level<REV>(lj, coef, ox - H + xx, oy - H + yy, w, h);
+    for (int k = threadIdx.x; k < nh * WN; k += kIdwtThreads) {
+        const int yy = k / WN, xx = k - yy * WN;
+        if (xx < nw) win[k] = load_level<REV>(L, coef, ox - H + xx, oy - H + yy);
     }
     __syncthreads();
-    if (lj.levels) {
-        // rows (all window rows), origin = absolute x of window column 0
-        lift_lines<REV>(win, nh, WN, nw, 1, lj.x0 + (uint32_t)(ox - H), w == 1, (lj.x0 & 1u) != 0);
-        // columns of the block interior
-        lift_lines<REV>(win + H, bw, 1, nh, WN, lj.y0 + (uint32_t)(oy - H), h == 1,
-                        (lj.y0 & 1u) != 0);
+    if (L.levels) {
+        lift<REV, WN, true>(win, nh, nw, L.x0 + (uint32_t)(ox - H), L.w == 1, (L.x0 & 1u) != 0);
+        lift<REV, WN, false>(win, bw, nh, L.y0 + (uint32_t)(oy - H), L.h == 1, (L.y0 & 1u) != 0);
     }
     bw_out = bw;
     bh_out = bh;
@@ -166,27 +212,30 @@ __device__ __forceinline__ uint32_t find_job(const J *jobs, uint32_t njobs, uint
 }
 
 template <bool REV>
-__device__ void level_body(const LevelJob *jobs, uint32_t njobs, uint32_t block_offset,
+__device__ __forceinline__ void level_body(const LevelJob *jobs, uint32_t njobs, uint32_t block_offset,
                            uint32_t *coef) {
     constexpr int H = REV ? 2 : 4;
     constexpr int WN = kBlk + 2 * H;
     __shared__ typename Sample<REV>::T win[WN * WN];
     const uint32_t blk = blockIdx.x + block_offset;
-    const LevelJob lj = jobs[find_job(jobs, njobs, blk)];
-    const uint32_t local = blk - lj.block_base;
-    const int bx = (int)(local % lj.nbx), by = (int)(local / lj.nbx);
+    const LevelJob *lj = jobs + find_job(jobs, njobs, blk);
+    __shared__ BandLds sb;
+    const LevelView L = level_view(lj, &sb);
+    const uint32_t local = blk - lj->block_base;
+    const uint32_t nbx = lj->nbx, out = lj->out;
+    const int bx = (int)(local % nbx), by = (int)(local / nbx);
     int bw, bh;
-    synth_block<REV>(lj, coef, bx, by, win, bw, bh);
-    const int w = (int)(lj.x1 - lj.x0);
-    for (int k = threadIdx.x; k < bw * bh; k += kIdwtThreads) {
-        const int y = k / bw, x = k - y * bw;
+    synth_block<REV>(L, coef, bx, by, win, bw, bh);
+    for (int k = threadIdx.x; k < kBlk * bh; k += kIdwtThreads) {
+        const int y = k / kBlk, x = k % kBlk;
+        if (x >= bw) continue;
         const auto v = win[(y + H) * WN + x + H];
         uint32_t bits;
         if constexpr (REV)
             bits = (uint32_t)v;
         else
             bits = __float_as_uint(v);
-        coef[lj.out + (uint32_t)(by * kBlk + y) * (uint32_t)w + (uint32_t)(bx * kBlk + x)] = bits;
+        coef[out + (uint32_t)(by * kBlk + y) * (uint32_t)L.w + (uint32_t)(bx * kBlk + x)] = bits;
     }
 }
 
@@ -209,8 +258,35 @@ __device__ __forceinline__ uint8_t sat_u8(float v) {
     return (uint8_t)v;
 }
 
+struct FinalView {
+    uint8_t *dst;
+    uint64_t pitch;
+    int width, height, nc, fmt, ch, bps, mct;
+    uint32_t prec0, prec1, prec2, same8;
+};
+
+// Converts one pixel's component values (after MCT and sign shift) into
+// output sample k of component c.
+__device__ __forceinline__ uint32_t out_u8(float v, bool same8, uint32_t prec) {
+    if (same8) return sat_u8(roundf(v));
+    const float mx = (float)((1u << prec) - 1);
+    return sat_u8(roundf((v / mx) * 255.0f));
+}
+
+__device__ __forceinline__ uint32_t out_u16(float v, uint32_t bd) {
+    const float mx = (float)((1ull << bd) - 1);
+    const float r = roundf(v);
+    const float cl = r < 0.0f ? 0.0f : (r > mx ? mx : r);
+    if (bd <= 8) {
+        uint32_t m8 = (1u << bd) - 1;
+        if (m8 == 0) m8 = 1;
+        return ((uint32_t)cl * 65535u + m8 / 2) / m8;
+    }
+    return (uint32_t)(cl > 65535.0f ? 65535.0f : cl);
+}
+
 template <bool REV>
-__device__ void final_body(const FinalJob *jobs, uint32_t njobs, uint32_t block_offset,
+__device__ __forceinline__ void final_body(const FinalJob *jobs, uint32_t njobs, uint32_t block_offset,
                            const LevelJob *lv, const uint32_t *coef) {
     constexpr int H = REV ? 2 : 4;
     constexpr int WN = kBlk + 2 * H;
@@ -218,83 +294,116 @@ __device__ void final_body(const FinalJob *jobs, uint32_t njobs, uint32_t block_
     __shared__ T win[WN * WN];
     __shared__ T res[2][kBlk * kBlk];  // all but the last component; the last stays in win
     const uint32_t blk = blockIdx.x + block_offset;
-    const FinalJob fj = jobs[find_job(jobs, njobs, blk)];
-    const uint32_t local = blk - fj.block_base;
-    const int bx = (int)(local % fj.nbx), by = (int)(local / fj.nbx);
+    const FinalJob *fj = jobs + find_job(jobs, njobs, blk);
+    const uint32_t local = blk - fj->block_base;
+    const uint32_t nbx = fj->nbx;
+    const int bx = (int)(local % nbx), by = (int)(local / nbx);
+    FinalView F;
+    F.nc = fj->ncomp > 3 ? 3 : fj->ncomp;
+    F.fmt = fj->fmt;
+    F.ch = (F.fmt == 0 || F.fmt == 3) ? 1 : (F.fmt == 2 ? 4 : 3);
+    F.bps = (F.fmt == 3 || F.fmt == 4) ? 2 : 1;
+    F.mct = fj->mct;
+    F.dst = reinterpret_cast<uint8_t *>(fj->dst);
+    F.pitch = fj->pitch;
+    F.prec0 = fj->precision[0];
+    F.prec1 = fj->precision[1];
+    F.prec2 = fj->precision[2];
+    F.same8 = fj->same8;
+    const uint32_t lvl0 = fj->level[0], lvl1 = fj->level[1], lvl2 = fj->level[2];
     int bw = 0, bh = 0;
-    const int nc = fj.ncomp > 3 ? 3 : fj.ncomp;
-    for (int c = 0; c < nc; ++c) {
-        const LevelJob lj = lv[fj.level[c]];
-        synth_block<REV>(lj, coef, bx, by, win, bw, bh);
-        if (c + 1 < nc) {
-            for (int k = threadIdx.x; k < bw * bh; k += kIdwtThreads) {
-                const int y = k / bw, x = k - y * bw;
-                res[c][k] = win[(y + H) * WN + x + H];
+    __shared__ BandLds sb;
+    for (int c = 0; c < F.nc; ++c) {
+        // the previous component's last reads of sb ended at a barrier
+        const LevelView L = level_view(lv + (c == 0 ? lvl0 : c == 1 ? lvl1 : lvl2), &sb);
+        synth_block<REV>(L, coef, bx, by, win, bw, bh);
+        if (c + 1 < F.nc) {
+            for (int k = threadIdx.x; k < kBlk * bh; k += kIdwtThreads) {
+                const int y = k / kBlk, x = k % kBlk;
+                if (x < bw) res[c][k] = win[(y + H) * WN + x + H];
             }
             __syncthreads();
         }
     }
-    const int fmt = fj.fmt;
-    const int ch = (fmt == 0 || fmt == 3) ? 1 : (fmt == 2 ? 4 : 3);
-    const int bps = (fmt == 3 || fmt == 4) ? 2 : 1;
-    uint8_t *dst = reinterpret_cast<uint8_t *>(fj.dst);
-    for (int k = threadIdx.x; k < bw * bh; k += kIdwtThreads) {
-        const int y = k / bw, x = k - y * bw;
-        float v[4];
-        if constexpr (REV) {
-            int32_t s[4];
-            for (int c = 0; c < nc; ++c)
-                s[c] = c + 1 < nc ? res[c][k] : win[(y + H) * WN + x + H];
-            if (fj.mct) {
-                const int32_t i1 = s[0] - ((s[2] + s[1]) >> 2);
-                const int32_t r = s[2] + i1, b = s[1] + i1;
-                s[0] = r;
-                s[1] = i1;
-                s[2] = b;
+    // Each thread converts and stores four horizontally adjacent pixels.
+    const bool words = F.bps == 1 && ((reinterpret_cast<uintptr_t>(F.dst) | F.pitch) & 3) == 0;
+    for (int k = threadIdx.x; k < (kBlk / 4) * bh; k += kIdwtThreads) {
+        const int y = k / (kBlk / 4), x4 = (k % (kBlk / 4)) * 4;
+        if (x4 >= bw) continue;
+        const int np = min(4, bw - x4);
+        uint32_t o[4][4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int x = x4 + q, ki = y * kBlk + x;
+            float v[3];
+            if constexpr (REV) {
+                int32_t s[3] = {0, 0, 0};
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    if (c < F.nc) s[c] = c + 1 < F.nc ? res[c < 2 ? c : 1][ki] : win[(y + H) * WN + x + H];
+                if (F.mct) {
+                    const int32_t i1 = s[0] - ((s[2] + s[1]) >> 2);
+                    const int32_t r = s[2] + i1, b = s[1] + i1;
+                    s[0] = r;
+                    s[1] = i1;
+                    s[2] = b;
+                }
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const uint32_t pc = c == 0 ? F.prec0 : c == 1 ? F.prec1 : F.prec2;
+                    v[c] = (float)(s[c] + (1 << (pc - 1)));
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    v[c] = c < F.nc ? (c + 1 < F.nc ? res[c < 2 ? c : 1][ki] : win[(y + H) * WN + x + H]) : 0.0f;
+                if (F.mct) {
+                    const float y0 = v[0], y1 = v[1], y2 = v[2];
+                    v[0] = fmaf(y2, 1.402f, y0);
+                    v[1] = fmaf(y2, -0.71414f, fmaf(y1, -0.34413f, y0));
+                    v[2] = fmaf(y1, 1.772f, y0);
+                }
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const uint32_t pc = c == 0 ? F.prec0 : c == 1 ? F.prec1 : F.prec2;
+                    v[c] = v[c] + (float)(1u << (pc - 1));
+                }
             }
-            for (int c = 0; c < nc; ++c) v[c] = (float)(s[c] + (1 << (fj.precision[c] - 1)));
-        } else {
-            for (int c = 0; c < nc; ++c)
-                v[c] = c + 1 < nc ? res[c][k] : win[(y + H) * WN + x + H];
-            if (fj.mct) {
-                const float y0 = v[0], y1 = v[1], y2 = v[2];
-                v[0] = fmaf(y2, 1.402f, y0);
-                v[1] = fmaf(y2, -0.71414f, fmaf(y1, -0.34413f, y0));
-                v[2] = fmaf(y1, 1.772f, y0);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const uint32_t pc = c == 0 ? F.prec0 : c == 1 ? F.prec1 : F.prec2;
+                o[q][c] = F.bps == 1 ? out_u8(v[c], F.same8, pc) : out_u16(v[c], F.prec0);
             }
-            for (int c = 0; c < nc; ++c) v[c] = v[c] + (float)(1u << (fj.precision[c] - 1));
+            o[q][3] = 255;
         }
-        const int gx = bx * kBlk + x, gy = by * kBlk + y;
-        uint8_t *px = dst + (size_t)gy * fj.pitch + (size_t)gx * ch * bps;
-        if (bps == 1) {
-            for (int c = 0; c < nc && c < ch; ++c) {
-                uint8_t o;
-                if (fj.same8) {
-                    o = sat_u8(roundf(v[c]));
-                } else {
-                    const float mx = (float)((1u << fj.precision[c]) - 1);
-                    o = sat_u8(roundf((v[c] / mx) * 255.0f));
-                }
-                px[c] = o;
+        const int gx = bx * kBlk + x4, gy = by * kBlk + y;
+        uint8_t *px = F.dst + (size_t)gy * F.pitch + (size_t)gx * F.ch * F.bps;
+        if (words && np == 4) {
+            // 4 pixels x ch bytes = ch 32-bit words
+            uint32_t *pw = reinterpret_cast<uint32_t *>(px);
+            if (F.ch == 1) {
+                pw[0] = o[0][0] | (o[1][0] << 8) | (o[2][0] << 16) | (o[3][0] << 24);
+            } else if (F.ch == 3) {
+                pw[0] = o[0][0] | (o[0][1] << 8) | (o[0][2] << 16) | (o[1][0] << 24);
+                pw[1] = o[1][1] | (o[1][2] << 8) | (o[2][0] << 16) | (o[2][1] << 24);
+                pw[2] = o[2][2] | (o[3][0] << 8) | (o[3][1] << 16) | (o[3][2] << 24);
+            } else {
+                for (int q = 0; q < 4; ++q)
+                    pw[q] = o[q][0] | (o[q][1] << 8) | (o[q][2] << 16) | (o[q][3] << 24);
             }
-            if (ch == 4) px[3] = 255;
         } else {
-            const uint32_t bd = fj.precision[0];
-            const float mx = (float)((1ull << bd) - 1);
-            for (int c = 0; c < nc && c < ch; ++c) {
-                const float r = roundf(v[c]);
-                const float cl = r < 0.0f ? 0.0f : (r > mx ? mx : r);
-                uint32_t s;
-                if (bd <= 8) {
-                    uint32_t m8 = (1u << bd) - 1;
-                    if (m8 == 0) m8 = 1;
-                    s = ((uint32_t)cl * 65535u + m8 / 2) / m8;
-                } else {
-                    s = (uint32_t)(cl > 65535.0f ? 65535.0f : cl);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    if (q >= np || c >= F.ch) continue;
+                    if (F.bps == 1) {
+                        px[q * F.ch + c] = (uint8_t)o[q][c];
+                    } else {
+                        px[2 * (q * F.ch + c)] = (uint8_t)(o[q][c] & 0xFF);
+                        px[2 * (q * F.ch + c) + 1] = (uint8_t)(o[q][c] >> 8);
+                    }
                 }
-                px[2 * c] = (uint8_t)(s & 0xFF);
-                px[2 * c + 1] = (uint8_t)(s >> 8);
-            }
         }
     }
 }
@@ -326,6 +435,22 @@ extern "C" __global__ void sgn_fill_bytes(uint8_t *p, size_t bytes, size_t words
         for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < bytes; i += stride)
             p[i] = (uint8_t)v4;
     }
+}
+
+// Device-to-device byte copy: 16-byte words when both ends are aligned.
+extern "C" __global__ void sgn_copy_bytes(uint8_t *__restrict__ dst, const uint8_t *__restrict__ src,
+                                          size_t bytes) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t head = 0;
+    if (((reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src)) & 15) == 0) {
+        const size_t n16 = bytes / 16;
+        const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
+        uint4 *d4 = reinterpret_cast<uint4 *>(dst);
+        for (size_t i = t; i < n16; i += stride) d4[i] = s4[i];
+        head = n16 * 16;
+    }
+    for (size_t i = head + t; i < bytes; i += stride) dst[i] = src[i];
 }
 
 // Inverse MCT on f32 planes (J2kInverseMctJob hook, lib.rs:785-800).

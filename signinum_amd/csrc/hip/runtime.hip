@@ -523,6 +523,18 @@ int sgn_device_fill(sgn_hip_session *s, void *p, size_t bytes, uint8_t value) {
     return SGN_OK;
 }
 
+int sgn_device_copy(sgn_hip_session *s, void *dst, const void *src, size_t bytes) {
+    if (!s || ((!dst || !src) && bytes)) return SGN_E_ARG;
+    if (!bytes) return SGN_OK;
+    HIP_TRY(hipSetDevice(s->device));
+    const uint32_t grid = static_cast<uint32_t>(std::min<size_t>((bytes + 4095) / 4096, 65535));
+    hipLaunchKernelGGL(sgn_copy_bytes, dim3(grid), dim3(256), 0, s->stream,
+                       static_cast<uint8_t *>(dst), static_cast<const uint8_t *>(src), bytes);
+    s->dispatches++;
+    HIP_TRY(hipGetLastError());
+    return SGN_OK;
+}
+
 int sgn_device_download(sgn_hip_session *s, void *host, const void *dev, size_t bytes) {
     if (!s || (!bytes)) return bytes ? SGN_E_ARG : SGN_OK;
     HIP_TRY(hipSetDevice(s->device));

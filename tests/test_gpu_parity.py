@@ -149,6 +149,20 @@ def test_plan_replay_graph(session, oracle):
     assert st["tiles"] == 12 and st["output_pixels"] == 12 * 96 * 96
 
 
+@pytest.mark.parametrize("width", [96, 95])
+def test_surface_copy(session, width):
+    # odd widths exercise the unaligned byte tail of the copy kernel
+    a = S.DeviceSurface(session, width, 7, S.PixelFormat.RGB8)
+    b = S.DeviceSurface(session, width, 7, S.PixelFormat.RGB8)
+    a.fill(0x5A)
+    b.fill(0)
+    b.copy_from(a)
+    session.synchronize()
+    assert (b.to_numpy() == 0x5A).all()
+    a.close()
+    b.close()
+
+
 def test_first_failing_tile_index(session, oracle):
     rng = np.random.default_rng(44)
     streams = [S.encode_j2k_lossless(synth.texture_rgb(rng, 64, 64)) for _ in range(6)]
