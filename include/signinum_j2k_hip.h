@@ -149,6 +149,21 @@ typedef struct sgn_j2k_plan sgn_j2k_plan;
  * (optional, n entries) receives each tile's parse status. */
 int sgn_j2k_plan_create(sgn_hip_session *session, const sgn_tile_job *jobs, size_t n, int fmt,
                         uint32_t denominator, sgn_j2k_plan **out, int32_t *tile_status);
+/* One request of a heterogeneous batch: per-tile pixel format and scale
+ * (TileRegionScaledDecodeJob, crates/signinum-j2k/src/batch.rs:35-47, over
+ * the full tile; the request record of the device plan). */
+typedef struct {
+    const uint8_t *bytes;
+    size_t len;
+    void *dst;            /* device pointer */
+    size_t pitch;         /* bytes per output row */
+    size_t capacity;      /* bytes available at dst */
+    int32_t fmt;          /* sgn_pixel_format */
+    uint32_t denominator; /* 1, 2, 4 or 8 */
+} sgn_tile_req;
+/* As sgn_j2k_plan_create, with format and scale per tile. */
+int sgn_j2k_plan_create_mixed(sgn_hip_session *session, const sgn_tile_req *reqs, size_t n,
+                              sgn_j2k_plan **out, int32_t *tile_status);
 void sgn_j2k_plan_destroy(sgn_j2k_plan *plan);
 /* Re-targets tile i of the plan to a new device destination. */
 int sgn_j2k_plan_set_output(sgn_j2k_plan *plan, size_t i, void *dst, size_t pitch,

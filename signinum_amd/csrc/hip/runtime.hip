@@ -70,8 +70,6 @@ Status dev_upload(T *&dptr, const std::vector<T> &v) {
 }  // namespace
 
 struct sgn_j2k_plan {
-    int fmt = 0;
-    uint32_t den = 1;
     size_t ntiles = 0;
     // host tables
     std::vector<CbJob> cb_lane, cb_lane_plain, cb_wave;
@@ -111,10 +109,7 @@ struct sgn_j2k_plan {
 
 namespace {
 
-Status build_plan(const sgn_tile_job *jobs, size_t n, int fmt, uint32_t den, sgn_j2k_plan &P,
-                  int32_t *tile_status) {
-    P.fmt = fmt;
-    P.den = den;
+Status build_plan(const sgn_tile_req *jobs, size_t n, sgn_j2k_plan &P, int32_t *tile_status) {
     P.ntiles = n;
     std::vector<LocalPlan> lps(n);
     std::vector<int32_t> st(n, 0);
@@ -123,7 +118,8 @@ Status build_plan(const sgn_tile_job *jobs, size_t n, int fmt, uint32_t den, sgn
         std::max<size_t>(1, std::min<size_t>(n, std::min(16u, std::max(1u, std::thread::hardware_concurrency()))));
     auto work = [&](size_t t0) {
         for (size_t i = t0; i < n; i += nthreads) {
-            st[i] = build_local_plan(jobs[i].bytes, jobs[i].len, fmt, den,
+            st[i] = build_local_plan(jobs[i].bytes, jobs[i].len, jobs[i].fmt,
+                                     jobs[i].denominator,
                                      reinterpret_cast<uint64_t>(jobs[i].dst), jobs[i].pitch,
                                      jobs[i].capacity, lps[i]);
             if (st[i]) errs[i] = last_error();
@@ -253,7 +249,7 @@ Status build_plan(const sgn_tile_job *jobs, size_t n, int fmt, uint32_t den, sgn
     return kOk;
 }
 
-Status upload_plan(const sgn_tile_job *jobs, sgn_j2k_plan &P) {
+Status upload_plan(const sgn_tile_req *jobs, sgn_j2k_plan &P) {
     if (P.arena_len) {
         HIP_TRY(hipMalloc(&P.d_arena, P.arena_len));
         std::vector<uint8_t> staging(P.arena_len, 0);
@@ -409,7 +405,7 @@ Status check_status(sgn_j2k_plan &P, int mode, size_t *first_fail) {
     return kOk;
 }
 
-Status make_plan(sgn_hip_session *s, const sgn_tile_job *jobs, size_t n, int fmt, uint32_t den,
+Status make_plan(sgn_hip_session *s, const sgn_tile_req *jobs, size_t n,
                  std::unique_ptr<sgn_j2k_plan> &out, int32_t *tile_status) {
     if (!s || (!jobs && n)) {
         set_last_error("null session or jobs");
@@ -417,12 +413,21 @@ Status make_plan(sgn_hip_session *s, const sgn_tile_job *jobs, size_t n, int fmt
     }
     HIP_TRY(hipSetDevice(s->device));
     auto P = std::make_unique<sgn_j2k_plan>();
-    Status st = build_plan(jobs, n, fmt, den, *P, tile_status);
+    Status st = build_plan(jobs, n, *P, tile_status);
     if (st) return st;
     st = upload_plan(jobs, *P);
     if (st) return st;
     out = std::move(P);
     return kOk;
+}
+
+// Uniform-format batch as per-tile requests.
+std::vector<sgn_tile_req> as_requests(const sgn_tile_job *jobs, size_t n, int fmt, uint32_t den) {
+    std::vector<sgn_tile_req> r(n);
+    for (size_t i = 0; i < n; ++i)
+        r[i] = sgn_tile_req{jobs[i].bytes, jobs[i].len, jobs[i].dst, jobs[i].pitch,
+                            jobs[i].capacity, fmt, den};
+    return r;
 }
 
 }  // namespace
@@ -592,8 +597,23 @@ int sgn_j2k_plan_create(sgn_hip_session *s, const sgn_tile_job *jobs, size_t n, 
     API_GUARD_BEGIN
     if (!out) return SGN_E_ARG;
     *out = nullptr;
+    if (!jobs && n) return SGN_E_ARG;
     std::unique_ptr<sgn_j2k_plan> P;
-    const Status st = make_plan(s, jobs, n, fmt, den, P, tile_status);
+    const std::vector<sgn_tile_req> reqs = as_requests(jobs, n, fmt, den);
+    const Status st = make_plan(s, reqs.data(), n, P, tile_status);
+    if (st) return st;
+    *out = P.release();
+    return SGN_OK;
+    API_GUARD_END
+}
+
+int sgn_j2k_plan_create_mixed(sgn_hip_session *s, const sgn_tile_req *reqs, size_t n,
+                              sgn_j2k_plan **out, int32_t *tile_status) {
+    API_GUARD_BEGIN
+    if (!out) return SGN_E_ARG;
+    *out = nullptr;
+    std::unique_ptr<sgn_j2k_plan> P;
+    const Status st = make_plan(s, reqs, n, P, tile_status);
     if (st) return st;
     *out = P.release();
     return SGN_OK;
@@ -690,9 +710,11 @@ int sgn_j2k_plan_tile_dims(const sgn_j2k_plan *P, size_t i, uint32_t *w, uint32_
 int sgn_j2k_decode_tiles_device(sgn_hip_session *s, const sgn_tile_job *jobs, size_t n, int fmt,
                                 uint32_t den, int mode, size_t *first_fail) {
     API_GUARD_BEGIN
+    if (!jobs && n) return SGN_E_ARG;
     std::vector<int32_t> ts(n, 0);
     std::unique_ptr<sgn_j2k_plan> P;
-    Status st = make_plan(s, jobs, n, fmt, den, P, ts.data());
+    const std::vector<sgn_tile_req> reqs = as_requests(jobs, n, fmt, den);
+    Status st = make_plan(s, reqs.data(), n, P, ts.data());
     if (st) {
         if (first_fail)
             for (size_t i = 0; i < n; ++i)
