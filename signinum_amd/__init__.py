@@ -365,28 +365,40 @@ class TilePlan:
     """A parsed batch resident in device memory: codestream bytes, the
     code-block table and the synthesis/packing jobs (the device form of
     J2kDirectColorPlan, crates/signinum-j2k-native/src/direct_plan.rs:12-205).
-    decode() re-runs the whole batch without host work."""
+    decode() re-runs the whole batch without host work.
+
+    `fmt` and `scale` are one value for the whole batch or one per tile
+    (a heterogeneous batch of formats and reduced resolutions)."""
 
     def __init__(self, session, inputs, fmt, scale=Downscale.NONE, surfaces=None):
         self.session = session
-        self.fmt = PixelFormat(fmt)
-        self.scale = Downscale(scale)
         self._inputs = [bytes(b) for b in inputs]
+        n = len(self._inputs)
+        fmts = list(fmt) if isinstance(fmt, (list, tuple)) else [fmt] * n
+        scales = list(scale) if isinstance(scale, (list, tuple)) else [scale] * n
+        if len(fmts) != n or len(scales) != n:
+            raise ValueError("one format and one scale per tile")
+        self.formats = [PixelFormat(f) for f in fmts]
+        self.scales = [Downscale(s) for s in scales]
+        self.fmt = self.formats[0] if n else PixelFormat(fmt if not isinstance(fmt, (list, tuple)) else 0)
+        self.scale = self.scales[0] if n else Downscale.NONE
         self._keep = [np.frombuffer(b, np.uint8) for b in self._inputs]
         if surfaces is None:
             surfaces = []
-            for b in self._inputs:
-                w, h = scaled_dims(b, self.scale)
-                surfaces.append(_alloc_surface(session, w, h, self.fmt))
+            for i, b in enumerate(self._inputs):
+                try:
+                    w, h = scaled_dims(b, self.scales[i])
+                except J2kError as e:
+                    raise TileBatchError(i, e)
+                surfaces.append(_alloc_surface(session, w, h, self.formats[i]))
         self.surfaces = surfaces
-        n = len(self._inputs)
-        jobs = (TileJob * max(1, n))()
+        reqs = (_lib.TileReq * max(1, n))()
         for i, (k, s) in enumerate(zip(self._keep, surfaces)):
-            jobs[i] = TileJob(k.ctypes.data, k.size, s.device_ptr, s.pitch_bytes, s.byte_len)
+            reqs[i] = _lib.TileReq(k.ctypes.data, k.size, s.device_ptr, s.pitch_bytes, s.byte_len,
+                                   int(self.formats[i]), int(self.scales[i]))
         status = (ctypes.c_int32 * max(1, n))()
         h = ctypes.c_void_p()
-        rc = _lib.load().sgn_j2k_plan_create(session.handle, jobs, n, int(self.fmt),
-                                             int(self.scale), ctypes.byref(h), status)
+        rc = _lib.load().sgn_j2k_plan_create_mixed(session.handle, reqs, n, ctypes.byref(h), status)
         if rc != _lib.SGN_OK:
             bad = next((i for i in range(n) if status[i] != 0), 0)
             raise TileBatchError(bad, _ERRORS.get(rc, J2kError)(rc, _lib.last_error()))

@@ -41,6 +41,12 @@ class TileJob(ctypes.Structure):
                 ("pitch", c_size_t), ("capacity", c_size_t)]
 
 
+class TileReq(ctypes.Structure):
+    _fields_ = [("bytes", ctypes.c_void_p), ("len", c_size_t), ("dst", ctypes.c_void_p),
+                ("pitch", c_size_t), ("capacity", c_size_t), ("fmt", ctypes.c_int32),
+                ("denominator", ctypes.c_uint32)]
+
+
 class PlanStats(ctypes.Structure):
     _fields_ = [("tiles", ctypes.c_uint64), ("code_blocks", ctypes.c_uint64),
                 ("coded_bytes", ctypes.c_uint64), ("output_bytes", ctypes.c_uint64),
@@ -100,6 +106,9 @@ SIGNATURES = {
     "sgn_j2k_decode_tiles_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(TileJob),
                                                    c_size_t, ctypes.c_int, ctypes.c_uint32,
                                                    ctypes.c_int, ctypes.POINTER(c_size_t)]),
+    "sgn_j2k_plan_create_mixed": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(TileReq), c_size_t,
+                                                  ctypes.POINTER(ctypes.c_void_p),
+                                                  ctypes.POINTER(ctypes.c_int32)]),
     "sgn_j2k_plan_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(TileJob), c_size_t,
                                            ctypes.c_int, ctypes.c_uint32,
                                            ctypes.POINTER(ctypes.c_void_p),

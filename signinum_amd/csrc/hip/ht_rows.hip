@@ -1,0 +1,395 @@
+// HT cleanup decode, one code-block per lane, with the compressed streams
+// staged through LDS.
+//
+// Each wave decodes 64 cleanup-only code-blocks (at most 64x64; the host
+// sorts blocks by predicted cost so the lanes of a wave carry similar work).
+// The decode walks every block in segments of kSegPairs quad pairs of one
+// quad row.  Between segments the wave stops at a uniform point and refills,
+// for all 64 lanes at once, three per-lane LDS rings (MagSgn, VLC, MEL) with
+// LDS-DMA loads of whole 16-byte units: every HBM line of a stream is read
+// once, coalesced, and the serial bit-unstuffing chains inside a segment read
+// LDS only.  The only global memory operations inside a segment are the
+// coefficient stores (four samples of a row per 16-byte store) and, when a
+// stream outruns its ring, a synchronous fallback load.
+//
+// MEL, VLC, UVLC and the magnitude/sign phase run fused per quad pair as in
+// crates/signinum-j2k-native/src/j2c/ht_block_decode.rs (decode_cleanup_
+// symbols :1056-1204, decode_magnitude_sign_phase :1435-1560); the bit
+// readers take four bytes at a time when no stuffing can occur in them.
+#include <hip/hip_runtime.h>
+
+#include "../host/plan_types.h"
+#include "ht_common.hpp"
+#include "ht_readers4.hpp"
+
+namespace sgn {
+
+constexpr int kRowsWg = 256;     // four independent waves share the VLC tables
+constexpr int kSegPairs = 8;     // quad pairs between two refills
+constexpr int kMsSlots = 8;      // 16-byte units per lane: MagSgn ring (128 B)
+constexpr int kVlcSlots = 4;     // VLC ring (64 B)
+constexpr int kMelSlots = 2;     // MEL ring (32 B)
+constexpr int kRowsMaxQ = 34;    // quads per row of a 64-wide block, plus 2
+
+// Synchronous 8-byte load for a stream that ran past its staged range.  The
+// wait sits inside the rarely taken branch, so the common path never waits
+// on vector memory (which would include the in-flight coefficient stores).
+__device__ __forceinline__ uint64_t load_now(const uint64_t *p) {
+    uint64_t v;
+    asm volatile("global_load_dwordx2 %0, %1, off\n\ts_waitcnt vmcnt(0)"
+                 : "=v"(v)
+                 : "v"(p)
+                 : "memory");
+    return v;
+}
+
+// Units [lo, hi) (16-byte units of the arena) held by a lane's ring of
+// SLOTS units; unit u lives in slot u % SLOTS.
+struct Staged {
+    uint32_t lo, hi;
+};
+
+// Word at arena byte address a (8-aligned) from the lane's ring, or from
+// memory when it is not staged.
+template <int SLOTS>
+__device__ __forceinline__ uint64_t ring_word(const uint8_t *ring, Staged s, uint32_t a,
+                                              const uint8_t *arena) {
+    const uint32_t u = a >> 4;
+    if (u >= s.lo && u < s.hi)
+        return *reinterpret_cast<const uint64_t *>(ring + (a & (SLOTS * 16 - 1)));
+    return load_now(reinterpret_cast<const uint64_t *>(arena) + (a >> 3));
+}
+
+// Loads, for every lane of the wave, the units of [want.lo, want.hi) that
+// its ring does not hold.  Must be called with all 64 lanes active.  The
+// loads complete asynchronously into LDS; the caller waits on vmcnt.
+// Word source of the bit readers: the lane's ring, memory past it.
+template <int SLOTS>
+struct RingSrc {
+    const uint8_t *ring;
+    Staged s;
+    const uint8_t *arena;
+    __device__ __forceinline__ uint64_t word(uint32_t a) const {
+        return ring_word<SLOTS>(ring, s, a, arena);
+    }
+};
+
+template <int SLOTS>
+__device__ __forceinline__ void stage_ring(uint4 *wave_ring, const uint8_t *arena, Staged want,
+                                           Staged have) {
+    constexpr int kOwners = 64 / SLOTS;  // lanes whose rings one load instruction fills
+    const int lane = threadIdx.x & 63;
+    const int sub = lane % SLOTS, grp = lane / SLOTS;
+    const uint4 *src = reinterpret_cast<const uint4 *>(arena);
+#pragma unroll
+    for (int base = 0; base < 64; base += kOwners) {
+        const int owner = base + grp;
+        const uint32_t wl = __shfl(want.lo, owner), wh = __shfl(want.hi, owner);
+        const uint32_t hl = __shfl(have.lo, owner), hh = __shfl(have.hi, owner);
+        // the unit of [wl, wl + SLOTS) that maps to this lane's slot
+        const uint32_t u = wl + ((uint32_t)(sub - (int)(wl % SLOTS)) & (SLOTS - 1));
+        if (u < wh && !(u >= hl && u < hh))
+            __builtin_amdgcn_global_load_lds(src + u, wave_ring + base * SLOTS, 16, 0, 0);
+    }
+}
+
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor(v, o));
+    return v;
+}
+
+extern "C" __global__ void __launch_bounds__(kRowsWg)
+sgn_ht_cleanup_rows(const CbJob *__restrict__ jobs, uint32_t n, const uint8_t *__restrict__ arena,
+                    int32_t *__restrict__ coef, const uint16_t *__restrict__ tables,
+                    uint8_t *__restrict__ status) {
+    __shared__ uint16_t s_tab[1024 + 1024 + 320 + 256];
+    __shared__ uint8_t s_rho[kRowsMaxQ][kRowsWg];  // significance nibble of the quad above
+    __shared__ uint8_t s_e[kRowsMaxQ][kRowsWg];    // floor(log2(v_n)) per column pair
+    __shared__ uint4 s_ms[kRowsWg * kMsSlots];
+    __shared__ uint4 s_vlc[kRowsWg * kVlcSlots];
+    __shared__ uint4 s_mel[kRowsWg * kMelSlots];
+    for (int t = threadIdx.x; t < 2624; t += kRowsWg) s_tab[t] = tables[t];
+    __syncthreads();
+    const uint16_t *vlc0 = s_tab, *vlc1 = s_tab + 1024, *uvlc0 = s_tab + 2048,
+                   *uvlc1 = s_tab + 2368;
+    const int tid = threadIdx.x, wv = tid >> 6;
+    const uint32_t i = blockIdx.x * kRowsWg + tid;
+    const bool valid = i < n;
+    CbJob j{};
+    if (valid) j = jobs[i];
+    int32_t *out = coef + j.dst;
+    const uint32_t w = j.w, h = j.h, stride = j.stride;
+    const bool vec_store = ((j.dst | stride) & 3u) == 0;
+    uint8_t st = kCbOk;
+    bool run_block = valid;
+    uint32_t scup = 0;
+    const uint32_t lcup = j.lcup, zblk = j.zblk, kmax = j.kmax;
+    if (valid && (j.flags & kCbZero)) {
+        for (uint32_t y = 0; y < h; ++y)
+            for (uint32_t x = 0; x < w; ++x) out[y * stride + x] = 0;
+        run_block = false;
+    } else if (valid) {
+        if (lcup >= 2) scup = ((uint32_t)arena[j.src + lcup - 1] << 4) + (arena[j.src + lcup - 2] & 0x0F);
+        if (lcup < 2 || zblk >= 30 || scup < 2 || scup > lcup || scup > 4079) {
+            for (uint32_t y = 0; y < h; ++y)
+                for (uint32_t x = 0; x < w; ++x) out[y * stride + x] = 0;
+            st = kCbBad;
+            run_block = false;
+        }
+    }
+    // stream geometry (arena byte addresses)
+    const uint32_t ms_off = j.src, ms_len = run_block ? lcup - scup : 0;
+    const uint32_t mel_off = j.src + lcup - scup, mel_len = run_block ? scup - 1 : 0;
+    const uint32_t vlc_start = j.src + lcup - 3, vlc_len = run_block ? scup - 2 : 0;
+    const uint8_t *ring_ms = reinterpret_cast<const uint8_t *>(s_ms + tid * kMsSlots);
+    const uint8_t *ring_vlc = reinterpret_cast<const uint8_t *>(s_vlc + tid * kVlcSlots);
+    const uint8_t *ring_mel = reinterpret_cast<const uint8_t *>(s_mel + tid * kMelSlots);
+    uint4 *wave_ms = s_ms + wv * 64 * kMsSlots;
+    uint4 *wave_vlc = s_vlc + wv * 64 * kVlcSlots;
+    uint4 *wave_mel = s_mel + wv * 64 * kMelSlots;
+    // words of each segment, counted from its first fetch address
+    auto fwd_words = [](uint32_t off, uint32_t len) {
+        return len ? (int32_t)((((off + len - 1) & ~7u) - (off & ~7u)) >> 3) + 1 : 0;
+    };
+    const int32_t rev_words0 =
+        vlc_len ? (int32_t)(((vlc_start & ~7u) - ((vlc_start + 1 - vlc_len) & ~7u)) >> 3) + 1 : 0;
+
+    // Units wanted in a ring: from the unit holding the next word to fetch,
+    // as many as the ring holds, limited to the segment.
+    auto want_fwd = [](uint32_t next, int32_t words, uint32_t slots) {
+        Staged r;
+        r.lo = next >> 4;
+        r.hi = r.lo;
+        if (words > 0) r.hi = min(r.lo + slots, ((next + 8u * (uint32_t)(words - 1)) >> 4) + 1);
+        return r;
+    };
+    auto want_rev = [](uint32_t next, int32_t words, uint32_t slots) {
+        Staged r;
+        r.lo = r.hi = 0;
+        if (words > 0) {
+            r.hi = (next >> 4) + 1;
+            const uint32_t first_u = (next - 8u * (uint32_t)(words - 1)) >> 4;
+            r.lo = r.hi - min(r.hi - first_u, slots);
+        }
+        return r;
+    };
+
+    Staged have_ms{0, 0}, have_vlc{0, 0}, have_mel{0, 0};
+    {
+        const Staged wm = want_fwd(ms_off & ~7u, fwd_words(ms_off, ms_len), kMsSlots);
+        const Staged wl = want_fwd(mel_off & ~7u, fwd_words(mel_off, mel_len), kMelSlots);
+        const Staged wr = want_rev(vlc_start & ~7u, rev_words0, kVlcSlots);
+        stage_ring<kMsSlots>(wave_ms, arena, wm, have_ms);
+        stage_ring<kVlcSlots>(wave_vlc, arena, wr, have_vlc);
+        stage_ring<kMelSlots>(wave_mel, arena, wl, have_mel);
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        have_ms = wm;
+        have_vlc = wr;
+        have_mel = wl;
+    }
+
+    Fwd4 fms, fmel;
+    Rev4 rvlc;
+    fwd4_init(fms, ms_off, ms_len, 0xFF, RingSrc<kMsSlots>{ring_ms, have_ms, arena});
+    fwd4_init(fmel, mel_off, mel_len, 0xFF, RingSrc<kMelSlots>{ring_mel, have_mel, arena});
+    rev4_init(rvlc, vlc_start, vlc_len, RingSrc<kVlcSlots>{ring_vlc, have_vlc, arena});
+    MsBits ms{0, 0, 0};
+    VlcBits vlc;
+    Mel mel{0, 0, (int32_t)mel_len, 0, 0, 0, 0, 0, 0};
+    {
+        const uint32_t b0 = run_block ? (uint32_t)arena[j.src + lcup - 2] : 0;
+        vlc.tmp = b0 >> 4;
+        vlc.bits = 4 - (((vlc.tmp & 7) == 7) ? 1u : 0u);
+        vlc.unstuff = ((b0 | 0x0F) > 0x8F) ? 1u : 0u;
+    }
+    const uint32_t p = 30 - zblk, mmsbp2 = zblk + 2;
+    const uint32_t nq = (w + 1) >> 1;
+    const uint32_t npairs = (w + 3) >> 2, nrows = (h + 1) >> 1;
+    const uint32_t nseg = (npairs + kSegPairs - 1) / kSegPairs;
+    const uint32_t total = run_block ? nrows * nseg : 0;
+    if (run_block)
+        for (uint32_t q = 0; q < nq + 2; ++q) {
+            s_rho[q][tid] = 0;
+            s_e[q][tid] = 0;
+        }
+    int run = 0;
+    if (run_block) run = mel_run4(mel, fmel, RingSrc<kMelSlots>{ring_mel, have_mel, arena});
+    uint32_t c_q = 0, prev_e3 = 0;
+    const uint32_t segs = wave_max(total);
+
+    for (uint32_t g = 0; g < segs; ++g) {
+        if (g) {
+            // uniform point: top up all rings of the wave
+            const Staged wm = want_fwd(fms.p, fms.words, kMsSlots);
+            const Staged wl = want_fwd(fmel.p, fmel.words, kMelSlots);
+            const Staged wr = want_rev(rvlc.p, rvlc.words, kVlcSlots);
+            stage_ring<kMsSlots>(wave_ms, arena, wm, have_ms);
+            stage_ring<kVlcSlots>(wave_vlc, arena, wr, have_vlc);
+            stage_ring<kMelSlots>(wave_mel, arena, wl, have_mel);
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            have_ms = wm;
+            have_vlc = wr;
+            have_mel = wl;
+        }
+        if (g >= total || st != kCbOk) continue;
+        const uint32_t row = g / nseg, seg = g - row * nseg;
+        const uint32_t y = 2 * row;
+        const bool first_row = y == 0;
+        const bool second = y + 1 < h;
+        if (seg == 0) {
+            c_q = 0;
+            prev_e3 = 0;
+        }
+        const uint32_t pa = seg * kSegPairs, pb = min(pa + kSegPairs, npairs);
+        for (uint32_t pr = pa; pr < pb; ++pr) {
+            const uint32_t x = 4 * pr, qj = 2 * pr;
+            // ---- cleanup symbols of the quad pair ----
+            const uint32_t pt0 = (uint32_t)s_rho[qj][tid] << 4;
+            const uint32_t pt1 = (uint32_t)s_rho[qj + 1][tid] << 4;
+            const uint32_t pt2 = (uint32_t)s_rho[qj + 2][tid] << 4;
+            uint32_t t0, t1, u0, u1;
+            // one fill per pair: a pair takes at most 7 + 7 + 3 + 10 bits
+            vlc_fill(vlc, rvlc, RingSrc<kVlcSlots>{ring_vlc, have_vlc, arena});
+            if (first_row) {
+                t0 = vlc0[c_q + (vlc.tmp & 0x7F)];
+                if (c_q == 0) {
+                    run -= 2;
+                    t0 = (run == -1) ? t0 : 0;
+                    if (run < 0) run = mel_run4(mel, fmel, RingSrc<kMelSlots>{ring_mel, have_mel, arena});
+                }
+                c_q = ((t0 & 0x10) << 3) | ((t0 & 0xE0) << 2);
+                vlc.tmp >>= t0 & 7;
+                vlc.bits -= t0 & 7;
+                t1 = vlc0[c_q + (vlc.tmp & 0x7F)];
+                if (c_q == 0 && x + 2 < w) {
+                    run -= 2;
+                    t1 = (run == -1) ? t1 : 0;
+                    if (run < 0) run = mel_run4(mel, fmel, RingSrc<kMelSlots>{ring_mel, have_mel, arena});
+                }
+                if (x + 2 >= w) t1 = 0;
+                c_q = ((t1 & 0x10) << 3) | ((t1 & 0xE0) << 2);
+                vlc.tmp >>= t1 & 7;
+                vlc.bits -= t1 & 7;
+                uint32_t mode = ((t0 & 8) << 3) | ((t1 & 8) << 4);
+                if (mode == 0xC0) {
+                    run -= 2;
+                    if (run == -1) mode += 0x40;
+                    if (run < 0) run = mel_run4(mel, fmel, RingSrc<kMelSlots>{ring_mel, have_mel, arena});
+                }
+                uint32_t e = uvlc0[mode + (vlc.tmp & 0x3F)];
+                vlc.tmp >>= e & 7;
+                vlc.bits -= e & 7;
+                e >>= 3;
+                uint32_t len = e & 0xF;
+                const uint32_t tmp = (uint32_t)vlc.tmp & ((1u << len) - 1);
+                vlc.tmp >>= len;
+                vlc.bits -= len;
+                e >>= 4;
+                len = e & 7;
+                e >>= 3;
+                u0 = 1 + (e & 7) + (tmp & ~(0xFFu << len));
+                u1 = 1 + (e >> 3) + (tmp >> len);
+            } else {
+                c_q |= (pt0 & 0xA0) << 2;
+                c_q |= (pt1 & 0x20) << 4;
+                t0 = vlc1[c_q + (vlc.tmp & 0x7F)];
+                if (c_q == 0) {
+                    run -= 2;
+                    t0 = (run == -1) ? t0 : 0;
+                    if (run < 0) run = mel_run4(mel, fmel, RingSrc<kMelSlots>{ring_mel, have_mel, arena});
+                }
+                c_q = ((t0 & 0x40) << 2) | ((t0 & 0x80) << 1);
+                c_q |= pt0 & 0x80;
+                c_q |= (pt1 & 0xA0) << 2;
+                c_q |= (pt2 & 0x20) << 4;
+                vlc.tmp >>= t0 & 7;
+                vlc.bits -= t0 & 7;
+                t1 = vlc1[c_q + (vlc.tmp & 0x7F)];
+                if (c_q == 0 && x + 2 < w) {
+                    run -= 2;
+                    t1 = (run == -1) ? t1 : 0;
+                    if (run < 0) run = mel_run4(mel, fmel, RingSrc<kMelSlots>{ring_mel, have_mel, arena});
+                }
+                if (x + 2 >= w) t1 = 0;
+                c_q = ((t1 & 0x40) << 2) | ((t1 & 0x80) << 1);
+                c_q |= pt1 & 0x80;
+                vlc.tmp >>= t1 & 7;
+                vlc.bits -= t1 & 7;
+                const uint32_t mode = ((t0 & 8) << 3) | ((t1 & 8) << 4);
+                uint32_t e = uvlc1[mode + (vlc.tmp & 0x3F)];
+                vlc.tmp >>= e & 7;
+                vlc.bits -= e & 7;
+                e >>= 3;
+                uint32_t len = e & 0xF;
+                const uint32_t tmp = (uint32_t)vlc.tmp & ((1u << len) - 1);
+                vlc.tmp >>= len;
+                vlc.bits -= len;
+                e >>= 4;
+                len = e & 7;
+                e >>= 3;
+                u0 = (e & 7) + (tmp & ~(0xFFu << len));
+                u1 = (e >> 3) + (tmp >> len);
+            }
+            s_rho[qj][tid] = (uint8_t)((t0 >> 4) & 0xF);
+            s_rho[qj + 1][tid] = (uint8_t)((t1 >> 4) & 0xF);
+
+            // ---- magnitude / sign of both quads ----
+            uint32_t r0[4] = {0, 0, 0, 0}, r1[4] = {0, 0, 0, 0};  // rows y, y+1 at x..x+3
+#pragma unroll
+            for (uint32_t k = 0; k < 2; ++k) {
+                const uint32_t xq = x + 2 * k;
+                if (xq >= w) break;
+                const uint32_t q = qj + k;
+                const uint32_t inf = k ? t1 : t0;
+                uint32_t uq = k ? u1 : u0;
+                if (!first_row) {
+                    uint32_t gamma = inf & 0xF0;
+                    gamma &= gamma - 0x10;
+                    uint32_t emax = max((uint32_t)s_e[q][tid], (uint32_t)s_e[q + 1][tid]);
+                    emax = max(emax, 1u);
+                    uq += gamma ? emax : 1u;
+                }
+                if (uq > mmsbp2) {
+                    st = kCbBad;
+                    break;
+                }
+                uint32_t v0, v1, v2, v3;
+                ms_fill(ms, fms, RingSrc<kMsSlots>{ring_ms, have_ms, arena});
+                const uint32_t a0 = ms_value(ms, inf, 0, uq, p, v0);
+                ms_fill(ms, fms, RingSrc<kMsSlots>{ring_ms, have_ms, arena});
+                const uint32_t a1 = ms_value(ms, inf, 1, uq, p, v1);
+                r0[2 * k] = (uint32_t)to_i32(a0, kmax);
+                r1[2 * k] = (uint32_t)to_i32(a1, kmax);
+                s_e[q][tid] = (uint8_t)max(prev_e3, flog2r(v1));
+                prev_e3 = 0;
+                if (xq + 1 >= w) break;
+                ms_fill(ms, fms, RingSrc<kMsSlots>{ring_ms, have_ms, arena});
+                const uint32_t a2 = ms_value(ms, inf, 2, uq, p, v2);
+                ms_fill(ms, fms, RingSrc<kMsSlots>{ring_ms, have_ms, arena});
+                const uint32_t a3 = ms_value(ms, inf, 3, uq, p, v3);
+                r0[2 * k + 1] = (uint32_t)to_i32(a2, kmax);
+                r1[2 * k + 1] = (uint32_t)to_i32(a3, kmax);
+                prev_e3 = flog2r(v3);
+            }
+            int32_t *o = out + y * stride + x;
+            if (vec_store && x + 4 <= w) {
+                *reinterpret_cast<uint4 *>(o) = make_uint4(r0[0], r0[1], r0[2], r0[3]);
+                if (second)
+                    *reinterpret_cast<uint4 *>(o + stride) = make_uint4(r1[0], r1[1], r1[2], r1[3]);
+            } else {
+#pragma unroll
+                for (uint32_t c = 0; c < 4; ++c) {
+                    if (x + c >= w) break;
+                    o[c] = (int32_t)r0[c];
+                    if (second) o[stride + c] = (int32_t)r1[c];
+                }
+            }
+            if (st != kCbOk) break;
+        }
+        if (pb == npairs) s_e[nq][tid] = (uint8_t)prev_e3;
+    }
+    if (valid) status[i] = st;
+}
+
+}  // namespace sgn

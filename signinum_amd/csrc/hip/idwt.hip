@@ -30,10 +30,14 @@ constexpr float kNegGamma = -0.8829111f;
 constexpr float kNegDelta = -0.44350687f;
 
 __device__ __forceinline__ int reflect(int i, int n) {
-    if ((unsigned)i < (unsigned)n) return i;  // interior: no extension
+    // one mirror at each end covers every halo of a level at least H wide
+    int m = i < 0 ? -i : i;
+    m = m >= n ? 2 * (n - 1) - m : m;
+    if ((unsigned)m < (unsigned)n) return m;
+    // narrower levels: periodic reflection
     if (n == 1) return 0;
     const int period = 2 * (n - 1);
-    int m = i % period;
+    m = i % period;
     if (m < 0) m += period;
     return m < n ? m : period - m;
 }
@@ -198,6 +202,17 @@ __device__ __forceinline__ void synth_block(const LevelView &L, const uint32_t *
     bh_out = bh;
 }
 
+// Work-groups of a launch go to the 8 XCDs round-robin (work-group w runs on
+// XCD w % 8, each XCD with its own L2).  The remap hands each XCD one
+// contiguous range of logical blocks, so neighbouring 64x64 blocks of a tile,
+// which share halo lines, are read through the same L2.
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t w, uint32_t n) {
+    constexpr uint32_t kXcds = 8;
+    const uint32_t x = w % kXcds, k = w / kXcds;
+    const uint32_t m = n / kXcds, r = n % kXcds;
+    return x * m + min(x, r) + k;
+}
+
 template <typename J>
 __device__ __forceinline__ uint32_t find_job(const J *jobs, uint32_t njobs, uint32_t block) {
     uint32_t lo = 0, hi = njobs;
@@ -217,7 +232,7 @@ __device__ __forceinline__ void level_body(const LevelJob *jobs, uint32_t njobs,
     constexpr int H = REV ? 2 : 4;
     constexpr int WN = kBlk + 2 * H;
     __shared__ typename Sample<REV>::T win[WN * WN];
-    const uint32_t blk = blockIdx.x + block_offset;
+    const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x) + block_offset;
     const LevelJob *lj = jobs + find_job(jobs, njobs, blk);
     __shared__ BandLds sb;
     const LevelView L = level_view(lj, &sb);
@@ -235,7 +250,8 @@ __device__ __forceinline__ void level_body(const LevelJob *jobs, uint32_t njobs,
             bits = (uint32_t)v;
         else
             bits = __float_as_uint(v);
-        coef[out + (uint32_t)(by * kBlk + y) * (uint32_t)L.w + (uint32_t)(bx * kBlk + x)] = bits;
+        coef[out + (uint32_t)(by * kBlk + y) * coef_pitch((uint32_t)L.w) +
+             (uint32_t)(bx * kBlk + x)] = bits;
     }
 }
 
@@ -293,7 +309,7 @@ __device__ __forceinline__ void final_body(const FinalJob *jobs, uint32_t njobs,
     using T = typename Sample<REV>::T;
     __shared__ T win[WN * WN];
     __shared__ T res[2][kBlk * kBlk];  // all but the last component; the last stays in win
-    const uint32_t blk = blockIdx.x + block_offset;
+    const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x) + block_offset;
     const FinalJob *fj = jobs + find_job(jobs, njobs, blk);
     const uint32_t local = blk - fj->block_base;
     const uint32_t nbx = fj->nbx;

@@ -19,6 +19,7 @@
 #include "../host/parse.hpp"
 #include "../host/sgn_api.hpp"
 #include "ht_decode.hip"
+#include "ht_rows.hip"
 #include "idwt.hip"
 
 using namespace sgn;
@@ -289,8 +290,8 @@ uint32_t final_blocks(const std::vector<FinalJob> &v, uint32_t a, uint32_t b) {
 uint32_t launch_lane(sgn_hip_session *s, sgn_j2k_plan &P, const CbJob *d_jobs, uint32_t count,
                      uint8_t *status) {
     if (!count) return 0;
-    const uint32_t grid = (count + kLaneWg - 1) / kLaneWg;
-    hipLaunchKernelGGL(sgn_ht_cleanup_lanes, dim3(grid), dim3(kLaneWg), 0, s->stream, d_jobs,
+    const uint32_t grid = (count + kRowsWg - 1) / kRowsWg;
+    hipLaunchKernelGGL(sgn_ht_cleanup_rows, dim3(grid), dim3(kRowsWg), 0, s->stream, d_jobs,
                        count, P.d_arena, reinterpret_cast<int32_t *>(P.d_coef), s->d_tables, status);
     return 1;
 }
@@ -834,8 +835,8 @@ static int hook_blocks(sgn_hip_session *s, const uint8_t *data, size_t data_len,
         return SGN_E_DEVICE;
     }
     if (!lane.empty())
-        hipLaunchKernelGGL(sgn_ht_cleanup_lanes, dim3((lane.size() + kLaneWg - 1) / kLaneWg),
-                           dim3(kLaneWg), 0, s->stream, d_lane, uint32_t(lane.size()), d_data,
+        hipLaunchKernelGGL(sgn_ht_cleanup_rows, dim3((lane.size() + kRowsWg - 1) / kRowsWg),
+                           dim3(kRowsWg), 0, s->stream, d_lane, uint32_t(lane.size()), d_data,
                            reinterpret_cast<int32_t *>(d_out), s->d_tables, d_st);
     if (!wave.empty())
         hipLaunchKernelGGL(sgn_ht_block_wave, dim3(wave.size()), dim3(64), 0, s->stream, d_wave,
@@ -917,7 +918,8 @@ int sgn_idwt_level(sgn_hip_session *s, int reversible, uint32_t x0, uint32_t y0,
     }
     lj.float_in = 0xF;
     lj.out = static_cast<uint32_t>(host.size());
-    host.resize(host.size() + size_t(w) * h, 0);
+    const uint32_t opitch = coef_pitch(w);  // the kernel writes rows at this pitch
+    host.resize(host.size() + size_t(opitch) * h, 0);
     lj.x0 = x0;
     lj.y0 = y0;
     lj.x1 = x1;
@@ -942,7 +944,8 @@ int sgn_idwt_level(sgn_hip_session *s, int reversible, uint32_t x0, uint32_t y0,
     s->dispatches++;
     HIP_TRY(hipStreamSynchronize(s->stream));
     std::vector<uint32_t> res(size_t(w) * h);
-    HIP_TRY(hipMemcpy(res.data(), d_coef + lj.out, res.size() * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy2D(res.data(), size_t(w) * 4, d_coef + lj.out, size_t(opitch) * 4,
+                        size_t(w) * 4, h, hipMemcpyDeviceToHost));
     (void)hipFree(d_coef);
     (void)hipFree(d_job);
     for (size_t i = 0; i < res.size(); ++i) {

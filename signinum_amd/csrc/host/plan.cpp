@@ -120,7 +120,7 @@ Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denomin
                 const BandGeom &bg = g.bands[b];
                 if (bg.res >= R) continue;
                 band_off[b] = static_cast<uint32_t>(lp.coef_elems);
-                lp.coef_elems += uint64_t(bg.rect.w()) * bg.rect.h();
+                lp.coef_elems += uint64_t(coef_pitch(bg.rect.w())) * bg.rect.h();
                 uint32_t K;
                 float d;
                 st = band_quant(cp, bg.type, bg.res, K, d);
@@ -143,8 +143,8 @@ Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denomin
                 const CbSeg &sg = segs[c][k];
                 const uint32_t K = band_k[cg.band];
                 CbJob j{};
-                j.stride = bg.rect.w();
-                j.dst = band_off[cg.band] + (cg.rect.y0 - bg.rect.y0) * bg.rect.w() +
+                j.stride = coef_pitch(bg.rect.w());
+                j.dst = band_off[cg.band] + (cg.rect.y0 - bg.rect.y0) * j.stride +
                         (cg.rect.x0 - bg.rect.x0);
                 j.w = static_cast<uint16_t>(cg.rect.w());
                 j.h = static_cast<uint16_t>(cg.rect.h());
@@ -170,7 +170,7 @@ Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denomin
             }
             // synthesis levels (idwt.rs:54-193)
             uint32_t prev_out = band_off[g.res[0].band[0]];
-            uint32_t prev_w = g.bands[g.res[0].band[0]].rect.w();
+            uint32_t prev_w = coef_pitch(g.bands[g.res[0].band[0]].rect.w());
             float prev_delta = band_delta[g.res[0].band[0]];
             bool prev_float = false;
             if (R == 1) {
@@ -205,7 +205,7 @@ Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denomin
                 for (int b = 0; b < 3; ++b) {
                     const uint32_t bi = rs.band[b];
                     lj.band[b + 1] = band_off[bi];
-                    lj.bw[b + 1] = g.bands[bi].rect.w();
+                    lj.bw[b + 1] = coef_pitch(g.bands[bi].rect.w());
                     lj.delta[b + 1] = band_delta[bi];
                 }
                 lj.x0 = rs.rect.x0;
@@ -218,11 +218,11 @@ Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denomin
                 lj.nby = static_cast<uint16_t>(ceil_div(rs.rect.h(), 64));
                 if (r + 1 < R) {
                     lj.out = static_cast<uint32_t>(lp.coef_elems);
-                    lp.coef_elems += uint64_t(rs.rect.w()) * rs.rect.h();
+                    lp.coef_elems += uint64_t(coef_pitch(rs.rect.w())) * rs.rect.h();
                     lp.levels.push_back(lj);
                     lp.level_depth.push_back(static_cast<uint8_t>(r));
                     prev_out = lj.out;
-                    prev_w = rs.rect.w();
+                    prev_w = coef_pitch(rs.rect.w());
                     prev_delta = 1.0f;
                     prev_float = !rev;
                 } else {

@@ -21,11 +21,16 @@ static_assert(sizeof(CbJob) == 32, "CbJob layout");
 
 enum : uint8_t { kCbStripeCausal = 1, kCbZero = 2 };
 
+// Row pitch (elements) of every band and intermediate level in the
+// coefficient arena: a multiple of four, so rows start 16-byte aligned and
+// four samples of a row move as one 16-byte access.
+constexpr inline uint32_t coef_pitch(uint32_t w) { return (w + 3u) & ~3u; }
+
 // One 2D synthesis level of one tile-component (IDWT step,
 // J2kDirectIdwtStep in crates/signinum-j2k-native/src/direct_plan.rs).
 struct LevelJob {
     uint32_t band[4];   // element offsets: LL, HL, LH, HH
-    uint32_t bw[4];     // band row widths
+    uint32_t bw[4];     // band row pitches (coef_pitch of the band width)
     float delta[4];     // dequantisation steps (irreversible), 1 for float inputs
     uint32_t out;       // element offset of the output (intermediate levels)
     uint32_t x0, y0, x1, y1;  // level rectangle (absolute coordinates)

@@ -149,6 +149,30 @@ def test_plan_replay_graph(session, oracle):
     assert st["tiles"] == 12 and st["output_pixels"] == 12 * 96 * 96
 
 
+def test_mixed_plan_formats_and_scales(session, oracle):
+    # one plan over tiles of different sizes, bit depths, formats and scales
+    rng = np.random.default_rng(77)
+    cases = []
+    for k in range(9):
+        if k % 3 == 2:
+            img = synth.spots_gray12(rng, 80 + 16 * k, 72)
+            cs = S.encode_j2k_ht(img, bit_depth=12, use_mct=0, cb_width_log2=5,
+                                 cb_height_log2=5, levels=3 + k % 4)
+            fmt = S.PixelFormat.GRAY16 if k % 2 else S.PixelFormat.GRAY8
+        else:
+            img = synth.tissue_rgb(rng, 64 + 24 * k, 40 + 20 * k)
+            cs = S.encode_j2k_lossless(img, levels=3 + k % 4)
+            fmt = S.PixelFormat.RGB8 if k % 2 else S.PixelFormat.RGBA8
+        cases.append((cs, fmt, S.Downscale((1, 2, 4)[k % 3])))
+    plan = S.TilePlan(session, [c[0] for c in cases], [c[1] for c in cases],
+                      [c[2] for c in cases])
+    plan.decode(S.MODE_GRAPH)
+    session.synchronize()
+    for (cs, fmt, den), surf in zip(cases, plan.surfaces):
+        assert np.array_equal(surf.to_numpy(), O.decode(cs, int(fmt), int(den)))
+    plan.close()
+
+
 @pytest.mark.parametrize("width", [96, 95])
 def test_surface_copy(session, width):
     # odd widths exercise the unaligned byte tail of the copy kernel
