@@ -36,6 +36,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=4096)
     ap.add_argument("--cpu-sample-1t", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build decodes on a single MI355X; "
@@ -59,6 +60,13 @@ def main():
     st = plan.stats()
     pixels = st["output_pixels"]
     log(f"plan: {plan_ms:.1f} ms host parse + upload; {st['code_blocks']} code-blocks")
+
+    tune = None
+    if not args.no_autotune:
+        tune = plan.autotune(reps=3)
+        log("autotune (HT lane kernel, ms): " +
+            ", ".join(f"{k} {v:.3f}" for k, v in tune.items()) +
+            f" -> {S.TilePlan.lane_variants()[plan.lane_variant]}")
 
     def timed(fn, reps):
         # HIP events on the session stream, where the kernels are launched
@@ -118,8 +126,22 @@ def main():
         "final": 4 * samples + samples,
     }
     achieved = alg_bytes[dom] / (stage_ms[dom] * 1e-3) / 1e9
+    kernel_name = {"blocks": "sgn_ht_rows", "levels": "sgn_idwt_level", "final": "sgn_final"}[dom]
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "hbm_traffic_4096x256_rgb8.json")
+    if args.tiles == 4096 and os.path.exists(tpath):
+        # HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
+        # this workload (scaled by a copy-kernel calibration)
+        with open(tpath) as fh:
+            tj = json.load(fh)
+        lane = S.TilePlan.lane_variants()[plan.lane_variant]
+        cands = [k for k in tj["kernels"].values() if k["kernel"].startswith(kernel_name) and
+                 (dom != "blocks" or k.get("variant") == lane)]
+        if cands:
+            k = max(cands, key=lambda k: k["read_bytes"] + k["write_bytes"])
+            traffic = int(k["read_bytes"] + k["write_bytes"])
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": dom, "algorithmic_bytes": int(alg_bytes[dom])}
 
     # ---- CPU baseline: the oracle on host cores, bounded sample ----
@@ -174,6 +196,8 @@ def main():
         "speedup_vs_plain": None if plain_ms is None else round(plain_ms / ms_per_step, 2),
         "batched_no_graph_ms": round(batched_ms, 4),
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "lane_kernel": S.TilePlan.lane_variants()[plan.lane_variant],
+        "autotune_ms": None if tune is None else {k: round(v, 4) for k, v in tune.items()},
         "plan_ms": round(plan_ms, 1),
         "end_to_end_mpix_s": round(pixels / ((plan_ms + ms_per_step) * 1e-3) / 1e6, 1),
         "roofline": roofline,

@@ -181,6 +181,22 @@ int sgn_j2k_plan_decode(sgn_hip_session *session, sgn_j2k_plan *plan, int mode);
 #define SGN_STAGE_FINAL 2
 int sgn_j2k_plan_decode_stage(sgn_hip_session *session, sgn_j2k_plan *plan, int stage);
 
+/* ---- autotuning of the HT lane kernel ------------------------------------
+ * No counterpart in the reference (its CPU decoder has no launch shapes):
+ * the lane kernel comes in variants of work-group size, quad pairs decoded
+ * between LDS refills, and one-shot grid versus persistent waves pulling
+ * 64-block chunks from a device work counter. */
+int sgn_ht_lane_variant_count(void);
+const char *sgn_ht_lane_variant_name(int variant);
+int sgn_j2k_plan_set_lane_variant(sgn_j2k_plan *plan, int variant);
+int sgn_j2k_plan_get_lane_variant(const sgn_j2k_plan *plan);
+/* Times every variant `reps` times on the plan's code-blocks (HIP events on
+ * the session stream), keeps the fastest for later decodes of the plan and
+ * writes it to *chosen; ms (optional, `count` entries) receives each
+ * variant's mean time in milliseconds.  Synchronous. */
+int sgn_j2k_plan_autotune(sgn_hip_session *session, sgn_j2k_plan *plan, int reps, float *ms,
+                          size_t count, int *chosen);
+
 typedef struct {
     uint64_t tiles;
     uint64_t code_blocks;
@@ -244,6 +260,97 @@ int sgn_idwt_level(sgn_hip_session *session, int reversible, uint32_t x0, uint32
  * planes updated in place, addends applied after the transform. */
 int sgn_inverse_mct(sgn_hip_session *session, int reversible, float *plane0, float *plane1,
                     float *plane2, size_t n, float addend0, float addend1, float addend2);
+
+/* ---- decode-cost model ---------------------------------------------------
+ * No counterpart in the reference: a 5-32-32-1 ReLU MLP over code-block
+ * header features (coded bytes, coding passes, Zblk, sub-band, sample
+ * count; 5 floats per block) predicting decode cycles, used to order the
+ * code-blocks of a plan so that the lanes of a wave carry similar work.  It
+ * trains on cycles that the lane kernel itself records, with the library's
+ * own HIP forward, backward and Adam kernels.  1281 float parameters:
+ * W1[32][5] | b1[32] | W2[32][32] | b2[32] | W3[32] | b3. */
+typedef struct sgn_cost_model sgn_cost_model;
+int sgn_cost_model_create(sgn_hip_session *session, uint64_t seed, sgn_cost_model **out);
+void sgn_cost_model_destroy(sgn_cost_model *model);
+int sgn_cost_model_param_count(void);
+int sgn_cost_model_get_params(const sgn_cost_model *model, float *out, size_t count);
+int sgn_cost_model_set_params(sgn_cost_model *model, const float *params, size_t count);
+/* Gradient of the mean loss 0.5 * (f(x) - y)^2 over n samples (X: 5 floats
+ * per sample) into grad (1281 floats), computed on the device. */
+int sgn_cost_model_gradient(sgn_hip_session *session, const sgn_cost_model *model,
+                            const float *X, const float *Y, size_t n, float *grad);
+/* `epochs` full-batch Adam steps on the device; *loss (optional) receives the
+ * final mean loss. */
+int sgn_cost_model_train(sgn_hip_session *session, sgn_cost_model *model, const float *X,
+                         const float *Y, size_t n, uint32_t epochs, float lr, float *loss);
+int sgn_cost_model_predict(sgn_hip_session *session, const sgn_cost_model *model,
+                           const float *X, size_t n, float *out);
+/* Decode-time samples: every `stride`-th non-empty code-block of the plan's
+ * lane queue (at most max_samples) is decoded alone by one wave, which
+ * records its shader-clock cycles.  features: 5 floats per sample. */
+int sgn_j2k_plan_time_code_blocks(sgn_hip_session *session, sgn_j2k_plan *plan, size_t stride,
+                                  size_t max_samples, float *features, float *cycles,
+                                  size_t *count);
+/* Order of the plan's lane code-blocks: 0 by the compressed-length
+ * heuristic (coded bytes + samples / 4, the default), 1 input order, 2 by the
+ * model's predicted cost (largest first). */
+int sgn_j2k_plan_order_code_blocks(sgn_hip_session *session, sgn_j2k_plan *plan, int order,
+                                   const sgn_cost_model *model);
+
+/* ---- serving ---------------------------------------------------------------
+ * No counterpart in the reference (its entry points decode per call): a
+ * continuous-batching server.  Requests of 1..n tiles (any size, bit depth,
+ * format, scale) are admitted as they arrive; a batch forms when the oldest
+ * waiting request has waited window_us or max_batch_tiles tiles wait, and
+ * takes waiting requests in arrival order while their device footprint fits
+ * hbm_budget.  Up to max_inflight batches run at once, each on its own
+ * stream, so new batches overlap running ones.  Outputs stay in device
+ * memory until the request is released.  The caller drives the server with
+ * sgn_server_pump(); one server is used from one thread at a time. */
+typedef struct sgn_server sgn_server;
+typedef struct {
+    uint32_t window_us;    /* batching window (default 2000) */
+    uint64_t hbm_budget;   /* device bytes for outputs and in-flight plans (default 64 GiB) */
+    uint32_t max_batch_tiles;  /* default 4096 */
+    uint32_t max_inflight;     /* concurrent batches / streams, 1..16 (default 4) */
+} sgn_server_config;
+typedef struct {
+    const uint8_t *bytes;  /* copied at submission */
+    size_t len;
+    int32_t fmt;           /* sgn_pixel_format */
+    uint32_t denominator;  /* 1, 2, 4 or 8 */
+} sgn_serve_tile;
+typedef struct {
+    uint64_t id;
+    int32_t status;        /* SGN_OK or the error of the first failing tile */
+    uint32_t first_fail;   /* index of that tile within the request */
+    uint32_t tiles;
+    uint64_t pixels;       /* decoded output pixels */
+    double latency_ms;     /* submission to completion observed by a pump */
+    double queue_ms;       /* submission to batch launch */
+} sgn_request_info;
+typedef struct {
+    uint64_t requests_submitted, requests_done, requests_failed;
+    uint64_t batches, tiles;
+    uint64_t pending, running_batches;
+    uint64_t hbm_in_use;
+} sgn_server_stats;
+void sgn_server_default_config(sgn_server_config *config);
+int sgn_server_create(int device, const sgn_server_config *config, sgn_server **out);
+/* Waits for running batches, frees every output. */
+void sgn_server_destroy(sgn_server *server);
+/* Validates the tiles' main headers and queues the request. */
+int sgn_server_submit(sgn_server *server, const sgn_serve_tile *tiles, size_t n, uint64_t *id);
+/* Completes finished batches, then launches due batches on free streams
+ * (flush != 0: launch regardless of the window).  Non-blocking. */
+int sgn_server_pump(sgn_server *server, int flush, uint32_t *completed);
+/* Moves up to cap finished-request records to out. */
+int sgn_server_poll(sgn_server *server, sgn_request_info *out, size_t cap, size_t *n);
+/* Device surface of tile `tile` of a finished request. */
+int sgn_server_output(sgn_server *server, uint64_t id, size_t tile, void **dev, size_t *pitch,
+                      uint32_t *width, uint32_t *height);
+int sgn_server_release(sgn_server *server, uint64_t id);
+int sgn_server_get_stats(const sgn_server *server, sgn_server_stats *out);
 
 /* ---- test-input encoder (encode_j2k_lossless, crates/signinum-j2k/src/
  * encode.rs:166-182; J2kLosslessEncodeOptions encode.rs:71-94) */

@@ -112,6 +112,14 @@ void oracle_set_zero_refinement(int enable);
  * decoded, code-blocks with refinement data, maximum coding passes. */
 void oracle_last_stats(uint32_t *blocks, uint32_t *refinement_blocks, uint32_t *max_passes);
 
+/* Decode-cost model (mlp.c): 5-32-32-1 ReLU MLP, 1281 parameters. */
+#define ORACLE_MLP_PARAMS 1281
+double oracle_mlp_predict(const double *params, const float *x);
+double oracle_mlp_loss(const double *params, const float *X, const float *Y, size_t n);
+/* Mean loss; grad (1281 doubles) receives its gradient. */
+double oracle_mlp_loss_grad(const double *params, const float *X, const float *Y, size_t n,
+                            double *grad);
+
 #ifdef __cplusplus
 }
 #endif

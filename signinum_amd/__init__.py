@@ -23,7 +23,8 @@ __all__ = [
     "MalformedError", "CodeBlockDecodeError", "BufferError", "DeviceError", "TileBatchError",
     "UnsupportedBackendError", "HipSession", "HipEvent", "J2kDecoder", "DeviceSurface", "TilePlan",
     "decode_tiles_into", "decode_tiles_to_device", "encode_j2k_lossless", "encode_j2k_ht",
-    "HtCodeBlockDecoder", "code_block_jobs", "MODE_PLAIN", "MODE_BATCHED", "MODE_GRAPH",
+    "HtCodeBlockDecoder", "code_block_jobs", "CostModel", "MODE_PLAIN", "MODE_BATCHED",
+    "MODE_GRAPH",
 ]
 
 
@@ -411,6 +412,53 @@ class TilePlan:
         """One stage of the batched decode: 0 blocks, 1 levels, 2 final."""
         _check(_lib.load().sgn_j2k_plan_decode_stage(self.session.handle, self.handle, int(stage)))
 
+    @staticmethod
+    def lane_variants():
+        """Names of the HT lane-kernel variants the autotuner chooses from."""
+        L = _lib.load()
+        return [L.sgn_ht_lane_variant_name(i).decode() for i in range(L.sgn_ht_lane_variant_count())]
+
+    @property
+    def lane_variant(self):
+        return int(_lib.load().sgn_j2k_plan_get_lane_variant(self.handle))
+
+    @lane_variant.setter
+    def lane_variant(self, v):
+        _check(_lib.load().sgn_j2k_plan_set_lane_variant(self.handle, int(v)))
+
+    def autotune(self, reps=3):
+        """Times every lane-kernel variant on this plan, keeps the fastest and
+        returns {variant name: mean ms}."""
+        L = _lib.load()
+        n = L.sgn_ht_lane_variant_count()
+        ms = (ctypes.c_float * n)()
+        chosen = ctypes.c_int()
+        _check(L.sgn_j2k_plan_autotune(self.session.handle, self.handle, int(reps), ms, n,
+                                       ctypes.byref(chosen)))
+        names = self.lane_variants()
+        return {names[i]: float(ms[i]) for i in range(n)}
+
+    ORDER_LENGTH, ORDER_INPUT, ORDER_MODEL = 0, 1, 2
+
+    def order_code_blocks(self, order, model=None):
+        """Orders the lane code-blocks: ORDER_LENGTH (compressed-length
+        heuristic), ORDER_INPUT, or ORDER_MODEL (predicted decode cost)."""
+        _check(_lib.load().sgn_j2k_plan_order_code_blocks(
+            self.session.handle, self.handle, int(order), model.handle if model else None))
+
+    def time_code_blocks(self, stride=1, max_samples=1 << 20):
+        """(features [n, 5] float32, cycles [n] float32) of code-blocks decoded
+        alone, one per wave, by the lane kernel."""
+        st = self.stats()
+        cap = min(max_samples, int(st["code_blocks"]) // max(1, stride) + 1)
+        feats = np.zeros((cap, 5), np.float32)
+        cyc = np.zeros(cap, np.float32)
+        n = ctypes.c_size_t()
+        _check(_lib.load().sgn_j2k_plan_time_code_blocks(
+            self.session.handle, self.handle, int(stride), cap, feats.ctypes.data,
+            cyc.ctypes.data, ctypes.byref(n)))
+        return feats[:n.value].copy(), cyc[:n.value].copy()
+
     def stats(self):
         st = PlanStats()
         _check(_lib.load().sgn_j2k_plan_get_stats(self.handle, ctypes.byref(st)))
@@ -419,6 +467,77 @@ class TilePlan:
     def close(self):
         if getattr(self, "handle", None):
             _lib.load().sgn_j2k_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class CostModel:
+    """Code-block decode-cost model (5-32-32-1 ReLU MLP) trained and run on
+    the device by the library's own kernels."""
+
+    PARAMS = 1281
+
+    def __init__(self, session, seed=1):
+        self.session = session
+        h = ctypes.c_void_p()
+        _check(_lib.load().sgn_cost_model_create(session.handle, int(seed), ctypes.byref(h)))
+        self.handle = h
+
+    @property
+    def params(self):
+        out = np.zeros(self.PARAMS, np.float32)
+        _check(_lib.load().sgn_cost_model_get_params(self.handle, out.ctypes.data, out.size))
+        return out
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        if p.size != self.PARAMS:
+            raise ValueError("1281 parameters expected")
+        _check(_lib.load().sgn_cost_model_set_params(self.handle, p.ctypes.data, p.size))
+
+    @staticmethod
+    def _xy(X, Y=None):
+        X = np.ascontiguousarray(X, np.float32).reshape(-1, 5)
+        if Y is None:
+            return X, None
+        Y = np.ascontiguousarray(Y, np.float32).reshape(-1)
+        if Y.size != X.shape[0]:
+            raise ValueError("one target per sample")
+        return X, Y
+
+    def gradient(self, X, Y):
+        X, Y = self._xy(X, Y)
+        g = np.zeros(self.PARAMS, np.float32)
+        _check(_lib.load().sgn_cost_model_gradient(self.session.handle, self.handle,
+                                                   X.ctypes.data, Y.ctypes.data, X.shape[0],
+                                                   g.ctypes.data))
+        return g
+
+    def train(self, X, Y, epochs=500, lr=1e-2):
+        """Full-batch Adam on the device; returns the final mean loss."""
+        X, Y = self._xy(X, Y)
+        loss = ctypes.c_float()
+        _check(_lib.load().sgn_cost_model_train(self.session.handle, self.handle, X.ctypes.data,
+                                                Y.ctypes.data, X.shape[0], int(epochs),
+                                                float(lr), ctypes.byref(loss)))
+        return float(loss.value)
+
+    def predict(self, X):
+        X, _ = self._xy(X)
+        out = np.zeros(X.shape[0], np.float32)
+        _check(_lib.load().sgn_cost_model_predict(self.session.handle, self.handle,
+                                                  X.ctypes.data, X.shape[0], out.ctypes.data))
+        return out
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.load().sgn_cost_model_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -47,6 +47,28 @@ class TileReq(ctypes.Structure):
                 ("denominator", ctypes.c_uint32)]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("window_us", ctypes.c_uint32), ("hbm_budget", ctypes.c_uint64),
+                ("max_batch_tiles", ctypes.c_uint32), ("max_inflight", ctypes.c_uint32)]
+
+
+class ServeTile(ctypes.Structure):
+    _fields_ = [("bytes", ctypes.c_void_p), ("len", c_size_t), ("fmt", ctypes.c_int32),
+                ("denominator", ctypes.c_uint32)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint64), ("status", ctypes.c_int32), ("first_fail", ctypes.c_uint32),
+                ("tiles", ctypes.c_uint32), ("pixels", ctypes.c_uint64),
+                ("latency_ms", ctypes.c_double), ("queue_ms", ctypes.c_double)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in (
+        "requests_submitted", "requests_done", "requests_failed", "batches", "tiles", "pending",
+        "running_batches", "hbm_in_use")]
+
+
 class PlanStats(ctypes.Structure):
     _fields_ = [("tiles", ctypes.c_uint64), ("code_blocks", ctypes.c_uint64),
                 ("coded_bytes", ctypes.c_uint64), ("output_bytes", ctypes.c_uint64),
@@ -109,6 +131,47 @@ SIGNATURES = {
     "sgn_j2k_plan_create_mixed": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(TileReq), c_size_t,
                                                   ctypes.POINTER(ctypes.c_void_p),
                                                   ctypes.POINTER(ctypes.c_int32)]),
+    "sgn_ht_lane_variant_count": (ctypes.c_int, []),
+    "sgn_ht_lane_variant_name": (ctypes.c_char_p, [ctypes.c_int]),
+    "sgn_j2k_plan_set_lane_variant": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "sgn_j2k_plan_get_lane_variant": (ctypes.c_int, [ctypes.c_void_p]),
+    "sgn_j2k_plan_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                             ctypes.POINTER(ctypes.c_float), c_size_t,
+                                             ctypes.POINTER(ctypes.c_int)]),
+    "sgn_cost_model_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64,
+                                             ctypes.POINTER(ctypes.c_void_p)]),
+    "sgn_cost_model_destroy": (None, [ctypes.c_void_p]),
+    "sgn_cost_model_param_count": (ctypes.c_int, []),
+    "sgn_cost_model_get_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, c_size_t]),
+    "sgn_cost_model_set_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, c_size_t]),
+    "sgn_cost_model_gradient": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                               ctypes.c_void_p, c_size_t, ctypes.c_void_p]),
+    "sgn_cost_model_train": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.c_void_p, c_size_t, ctypes.c_uint32,
+                                            ctypes.c_float, ctypes.POINTER(ctypes.c_float)]),
+    "sgn_cost_model_predict": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                              c_size_t, ctypes.c_void_p]),
+    "sgn_j2k_plan_time_code_blocks": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, c_size_t,
+                                                     c_size_t, ctypes.c_void_p, ctypes.c_void_p,
+                                                     ctypes.POINTER(c_size_t)]),
+    "sgn_j2k_plan_order_code_blocks": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
+                                                      ctypes.c_int, ctypes.c_void_p]),
+    "sgn_server_default_config": (None, [ctypes.POINTER(ServerConfig)]),
+    "sgn_server_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ServerConfig),
+                                         ctypes.POINTER(ctypes.c_void_p)]),
+    "sgn_server_destroy": (None, [ctypes.c_void_p]),
+    "sgn_server_submit": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ServeTile), c_size_t,
+                                         ctypes.POINTER(ctypes.c_uint64)]),
+    "sgn_server_pump": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_uint32)]),
+    "sgn_server_poll": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(RequestInfo), c_size_t,
+                                       ctypes.POINTER(c_size_t)]),
+    "sgn_server_output": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, c_size_t,
+                                         ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(c_size_t),
+                                         ctypes.POINTER(ctypes.c_uint32),
+                                         ctypes.POINTER(ctypes.c_uint32)]),
+    "sgn_server_release": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64]),
+    "sgn_server_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ServerStats)]),
     "sgn_j2k_plan_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(TileJob), c_size_t,
                                            ctypes.c_int, ctypes.c_uint32,
                                            ctypes.POINTER(ctypes.c_void_p),

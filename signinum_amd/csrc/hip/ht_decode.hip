@@ -1,35 +1,21 @@
-// HT code-block decode kernels for gfx950.
-//
-// ht_cleanup_lanes: one code-block per lane.  Cleanup-only blocks (one HT
-//   pass, the form every HTJ2K lossless encoder emits) of at most 64x64
-//   samples.  MEL/VLC/UVLC decoding and the magnitude/sign phase run fused,
-//   quad pair by quad pair, so no per-block symbol table is materialised:
-//   the only carried state is the significance nibble of each quad and the
-//   exponent bound of each column pair of the previous quad row, kept in LDS
-//   (lane-interleaved, conflict free).  Lanes of a wave run blocks of similar
-//   predicted cost (the host sorts them), which keeps divergence low.
+// HT code-block decode kernel for the general case, gfx950.
 //
 // ht_block_wave: one code-block per wavefront, any size and all three HT
 //   passes (cleanup, SigProp, MagRef).  The symbol table, sigma map and the
 //   block itself live in LDS; lane 0 walks the serial bit streams, then the
-//   whole wave converts and stores the block with coalesced writes.
+//   whole wave converts and stores the block with coalesced writes.  The
+//   cleanup-only blocks of at most 64x64 samples (every block of the lossless
+//   batches) take the lane kernel in ht_rows.hip instead.
 //
-// Both restate crates/signinum-j2k-native/src/j2c/ht_block_decode.rs
-// (decode_impl :1563-1680 and the phases it calls) and produce
-// coefficient_to_i32 integers in the sub-band arena.
+// Restates crates/signinum-j2k-native/src/j2c/ht_block_decode.rs (decode_impl
+// :1563-1680 and the phases it calls) and produces coefficient_to_i32
+// integers in the sub-band arena.
 #include <hip/hip_runtime.h>
 
 #include "../host/plan_types.h"
 #include "ht_common.hpp"
 
 namespace sgn {
-
-constexpr int kLaneWg = 256;
-constexpr int kMaxQ = 34;  // quads per row of a 64-wide block plus 2
-
-struct HtTableView {
-    const uint16_t *vlc0, *vlc1, *uvlc0, *uvlc1;
-};
 
 __device__ __forceinline__ void load_tables(const uint16_t *g, uint16_t *s, int n, int tid,
                                             int nthreads) {
@@ -38,21 +24,15 @@ __device__ __forceinline__ void load_tables(const uint16_t *g, uint16_t *s, int 
 
 // Magnitude/sign of one sample (ht_block_decode.rs:1010-1032); returns the
 // u32 sign-magnitude sample and the decoded v_n.
-__device__ __forceinline__ uint32_t ms_fetch(FwdReader &r) { return fwd_fetch(r); }
-__device__ __forceinline__ uint32_t ms_fetch(FwdChunked &r) { return fwdc_fetch(r); }
-__device__ __forceinline__ void ms_advance(FwdReader &r, uint32_t n) { fwd_advance(r, n); }
-__device__ __forceinline__ void ms_advance(FwdChunked &r, uint32_t n) { fwdc_advance(r, n); }
-
-template <typename Reader>
-__device__ __forceinline__ uint32_t ms_sample(Reader &ms, uint32_t inf, uint32_t bit,
+__device__ __forceinline__ uint32_t ms_sample(FwdReader &ms, uint32_t inf, uint32_t bit,
                                               uint32_t uq, uint32_t p, uint32_t &vn) {
     if ((inf & (1u << (4 + bit))) == 0) {
         vn = 0;
         return 0;
     }
-    const uint32_t ms_val = ms_fetch(ms);
+    const uint32_t ms_val = fwd_fetch(ms);
     const uint32_t m_n = uq - ((inf >> (12 + bit)) & 1u);
-    ms_advance(ms, m_n);
+    fwd_advance(ms, m_n);
     uint32_t value = ms_val << 31;
     const uint32_t mask = m_n == 0 ? 0u : (m_n >= 32 ? 0xFFFFFFFFu : ((1u << m_n) - 1));
     uint32_t v = ms_val & mask;
@@ -61,192 +41,6 @@ __device__ __forceinline__ uint32_t ms_sample(Reader &ms, uint32_t inf, uint32_t
     value |= (v + 2) << (p - 1);
     vn = v;
     return value;
-}
-
-__device__ __forceinline__ uint32_t flog2(uint32_t v) { return v ? 31u - __clz(v) : 0u; }
-
-extern "C" __global__ void __launch_bounds__(kLaneWg)
-sgn_ht_cleanup_lanes(const CbJob *__restrict__ jobs, uint32_t n, const uint8_t *__restrict__ arena,
-                     int32_t *__restrict__ coef, const uint16_t *__restrict__ tables,
-                     uint8_t *__restrict__ status) {
-    __shared__ uint16_t s_tab[1024 + 1024 + 320 + 256];
-    __shared__ uint8_t s_rho[kMaxQ][kLaneWg];  // significance nibble of the quad above
-    __shared__ uint8_t s_e[kMaxQ][kLaneWg];    // floor(log2(v_n)) per column pair
-    load_tables(tables, s_tab, 2624, threadIdx.x, kLaneWg);
-    __syncthreads();
-    const uint32_t i = blockIdx.x * kLaneWg + threadIdx.x;
-    if (i >= n) return;
-    const uint16_t *vlc0 = s_tab, *vlc1 = s_tab + 1024, *uvlc0 = s_tab + 2048,
-                   *uvlc1 = s_tab + 2368;
-    const int lane = threadIdx.x;
-    const CbJob j = jobs[i];
-    int32_t *out = coef + j.dst;
-    const uint32_t w = j.w, h = j.h, stride = j.stride;
-    uint8_t st = kCbOk;
-    if (j.flags & kCbZero) {
-        for (uint32_t y = 0; y < h; ++y)
-            for (uint32_t x = 0; x < w; ++x) out[y * stride + x] = 0;
-        status[i] = kCbOk;
-        return;
-    }
-    const uint8_t *d = arena + j.src;
-    const uint32_t lcup = j.lcup, zblk = j.zblk, kmax = j.kmax;
-    uint32_t scup = 0;
-    if (lcup >= 2) scup = ((uint32_t)d[lcup - 1] << 4) + (d[lcup - 2] & 0x0F);
-    if (lcup < 2 || zblk >= 30 || scup < 2 || scup > lcup || scup > 4079) {
-        for (uint32_t y = 0; y < h; ++y)
-            for (uint32_t x = 0; x < w; ++x) out[y * stride + x] = 0;
-        status[i] = kCbBad;
-        return;
-    }
-    const uint32_t p = 30 - zblk, mmsbp2 = zblk + 2;
-    const uint32_t nq = (w + 1) >> 1;
-    for (uint32_t q = 0; q < nq + 2; ++q) {
-        s_rho[q][lane] = 0;
-        s_e[q][lane] = 0;
-    }
-    MelChunked mel;
-    fwdb_init(mel.src, arena, j.src + (lcup - scup), scup - 1, 0xFF);
-    mel.unstuff = 0;
-    mel.cur = 0;
-    mel.bits_left = 0;
-    mel.k = 0;
-    mel.num_runs = 0;
-    mel.runs = 0;
-    RevChunked vlc;
-    revc_init_vlc(vlc, arena, j.src, lcup, scup);
-    FwdChunked ms;
-    fwdb_init(ms.src, arena, j.src, lcup - scup, 0xFF);
-    ms.tmp = 0;
-    ms.bits = 0;
-    ms.unstuff = 0;
-    int run = melc_run(mel);
-
-    for (uint32_t y = 0; y < h && st == kCbOk; y += 2) {
-        const bool first_row = (y == 0);
-        const bool second = (y + 1 < h);
-        uint32_t c_q = 0;
-        uint32_t prev_e3 = 0;  // floor(log2 v_n) of the lower-right sample of the previous quad
-        for (uint32_t x = 0, qj = 0; x < w; x += 4, qj += 2) {
-            // ---- cleanup symbols of the quad pair ----
-            const uint32_t pt0 = (uint32_t)s_rho[qj][lane] << 4;
-            const uint32_t pt1 = (uint32_t)s_rho[qj + 1][lane] << 4;
-            const uint32_t pt2 = (uint32_t)s_rho[qj + 2][lane] << 4;
-            uint32_t t0, t1, u0, u1;
-            if (first_row) {
-                uint32_t vv = revc_fetch(vlc);
-                t0 = vlc0[c_q + (vv & 0x7F)];
-                if (c_q == 0) {
-                    run -= 2;
-                    t0 = (run == -1) ? t0 : 0;
-                    if (run < 0) run = melc_run(mel);
-                }
-                c_q = ((t0 & 0x10) << 3) | ((t0 & 0xE0) << 2);
-                vv = revc_advance(vlc, t0 & 7);
-                t1 = vlc0[c_q + (vv & 0x7F)];
-                if (c_q == 0 && x + 2 < w) {
-                    run -= 2;
-                    t1 = (run == -1) ? t1 : 0;
-                    if (run < 0) run = melc_run(mel);
-                }
-                if (x + 2 >= w) t1 = 0;
-                c_q = ((t1 & 0x10) << 3) | ((t1 & 0xE0) << 2);
-                vv = revc_advance(vlc, t1 & 7);
-                uint32_t mode = ((t0 & 8) << 3) | ((t1 & 8) << 4);
-                if (mode == 0xC0) {
-                    run -= 2;
-                    if (run == -1) mode += 0x40;
-                    if (run < 0) run = melc_run(mel);
-                }
-                uint32_t e = uvlc0[mode + (vv & 0x3F)];
-                vv = revc_advance(vlc, e & 7);
-                e >>= 3;
-                uint32_t len = e & 0xF;
-                const uint32_t tmp = vv & ((1u << len) - 1);
-                revc_advance(vlc, len);
-                e >>= 4;
-                len = e & 7;
-                e >>= 3;
-                u0 = 1 + (e & 7) + (tmp & ~(0xFFu << len));
-                u1 = 1 + (e >> 3) + (tmp >> len);
-            } else {
-                c_q |= (pt0 & 0xA0) << 2;
-                c_q |= (pt1 & 0x20) << 4;
-                uint32_t vv = revc_fetch(vlc);
-                t0 = vlc1[c_q + (vv & 0x7F)];
-                if (c_q == 0) {
-                    run -= 2;
-                    t0 = (run == -1) ? t0 : 0;
-                    if (run < 0) run = melc_run(mel);
-                }
-                c_q = ((t0 & 0x40) << 2) | ((t0 & 0x80) << 1);
-                c_q |= pt0 & 0x80;
-                c_q |= (pt1 & 0xA0) << 2;
-                c_q |= (pt2 & 0x20) << 4;
-                vv = revc_advance(vlc, t0 & 7);
-                t1 = vlc1[c_q + (vv & 0x7F)];
-                if (c_q == 0 && x + 2 < w) {
-                    run -= 2;
-                    t1 = (run == -1) ? t1 : 0;
-                    if (run < 0) run = melc_run(mel);
-                }
-                if (x + 2 >= w) t1 = 0;
-                c_q = ((t1 & 0x40) << 2) | ((t1 & 0x80) << 1);
-                c_q |= pt1 & 0x80;
-                vv = revc_advance(vlc, t1 & 7);
-                const uint32_t mode = ((t0 & 8) << 3) | ((t1 & 8) << 4);
-                uint32_t e = uvlc1[mode + (vv & 0x3F)];
-                vv = revc_advance(vlc, e & 7);
-                e >>= 3;
-                uint32_t len = e & 0xF;
-                const uint32_t tmp = vv & ((1u << len) - 1);
-                revc_advance(vlc, len);
-                e >>= 4;
-                len = e & 7;
-                e >>= 3;
-                u0 = (e & 7) + (tmp & ~(0xFFu << len));
-                u1 = (e >> 3) + (tmp >> len);
-            }
-            s_rho[qj][lane] = (uint8_t)((t0 >> 4) & 0xF);
-            s_rho[qj + 1][lane] = (uint8_t)((t1 >> 4) & 0xF);
-
-            // ---- magnitude / sign of both quads (ht_block_decode.rs:1435-1560) ----
-            for (uint32_t k = 0; k < 2; ++k) {
-                const uint32_t xq = x + 2 * k;
-                if (xq >= w) break;
-                const uint32_t q = qj + k;
-                const uint32_t inf = k ? t1 : t0;
-                uint32_t uq = k ? u1 : u0;
-                if (!first_row) {
-                    uint32_t gamma = inf & 0xF0;
-                    gamma &= gamma - 0x10;
-                    uint32_t emax = max((uint32_t)s_e[q][lane], (uint32_t)s_e[q + 1][lane]);
-                    emax = max(emax, 1u);
-                    uq += gamma ? emax : 1u;
-                }
-                if (uq > mmsbp2) {
-                    st = kCbBad;
-                    break;
-                }
-                uint32_t v0, v1, v2, v3;
-                const uint32_t a0 = ms_sample(ms, inf, 0, uq, p, v0);
-                const uint32_t a1 = ms_sample(ms, inf, 1, uq, p, v1);
-                int32_t *o = out + y * stride + xq;
-                o[0] = to_i32(a0, kmax);
-                if (second) o[stride] = to_i32(a1, kmax);
-                s_e[q][lane] = (uint8_t)max(prev_e3, flog2(v1));
-                prev_e3 = 0;
-                if (xq + 1 >= w) break;
-                const uint32_t a2 = ms_sample(ms, inf, 2, uq, p, v2);
-                const uint32_t a3 = ms_sample(ms, inf, 3, uq, p, v3);
-                o[1] = to_i32(a2, kmax);
-                if (second) o[stride + 1] = to_i32(a3, kmax);
-                prev_e3 = flog2(v3);
-            }
-        }
-        s_e[nq][lane] = (uint8_t)prev_e3;
-    }
-    status[i] = st;
 }
 
 // ---------------------------------------------------------------------------

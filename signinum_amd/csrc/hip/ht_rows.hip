@@ -24,9 +24,6 @@
 
 namespace sgn {
 
-constexpr int kRowsWg = 256;     // four independent waves share the VLC tables
-constexpr int kSegPairs = 8;     // quad pairs between two refills
-constexpr int kMsSlots = 8;      // 16-byte units per lane: MagSgn ring (128 B)
 constexpr int kVlcSlots = 4;     // VLC ring (64 B)
 constexpr int kMelSlots = 2;     // MEL ring (32 B)
 constexpr int kRowsMaxQ = 34;    // quads per row of a 64-wide block, plus 2
@@ -99,22 +96,23 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
     return v;
 }
 
-extern "C" __global__ void __launch_bounds__(kRowsWg)
-sgn_ht_cleanup_rows(const CbJob *__restrict__ jobs, uint32_t n, const uint8_t *__restrict__ arena,
-                    int32_t *__restrict__ coef, const uint16_t *__restrict__ tables,
-                    uint8_t *__restrict__ status) {
-    __shared__ uint16_t s_tab[1024 + 1024 + 320 + 256];
-    __shared__ uint8_t s_rho[kRowsMaxQ][kRowsWg];  // significance nibble of the quad above
-    __shared__ uint8_t s_e[kRowsMaxQ][kRowsWg];    // floor(log2(v_n)) per column pair
-    __shared__ uint4 s_ms[kRowsWg * kMsSlots];
-    __shared__ uint4 s_vlc[kRowsWg * kVlcSlots];
-    __shared__ uint4 s_mel[kRowsWg * kMelSlots];
-    for (int t = threadIdx.x; t < 2624; t += kRowsWg) s_tab[t] = tables[t];
-    __syncthreads();
+// Decodes the 64 code-blocks jobs[first .. first + 64) (lane l takes
+// first + l) with the calling wave.  WAVES waves share the work-group's
+// tables; SEG quad pairs are decoded between refills of the MagSgn ring of
+// MS 16-byte slots.  Must be entered by all 64 lanes of the wave.
+template <int WAVES, int SEG, int MS>
+__device__ __forceinline__ void rows_wave(const CbJob *__restrict__ jobs, uint32_t n,
+                                          uint32_t first, const uint8_t *__restrict__ arena,
+                                          int32_t *__restrict__ coef,
+                                          uint8_t *__restrict__ status, const uint16_t *s_tab,
+                                          uint8_t (*s_rho)[WAVES * 64],
+                                          uint8_t (*s_e)[WAVES * 64], uint4 *s_ms, uint4 *s_vlc,
+                                          uint4 *s_mel) {
+    constexpr int kSegPairs = SEG, kMsSlots = MS;
     const uint16_t *vlc0 = s_tab, *vlc1 = s_tab + 1024, *uvlc0 = s_tab + 2048,
                    *uvlc1 = s_tab + 2368;
     const int tid = threadIdx.x, wv = tid >> 6;
-    const uint32_t i = blockIdx.x * kRowsWg + tid;
+    const uint32_t i = first + (uint32_t)(tid & 63);
     const bool valid = i < n;
     CbJob j{};
     if (valid) j = jobs[i];
@@ -143,6 +141,7 @@ sgn_ht_cleanup_rows(const CbJob *__restrict__ jobs, uint32_t n, const uint8_t *_
     const uint32_t mel_off = j.src + lcup - scup, mel_len = run_block ? scup - 1 : 0;
     const uint32_t vlc_start = j.src + lcup - 3, vlc_len = run_block ? scup - 2 : 0;
     const uint8_t *ring_ms = reinterpret_cast<const uint8_t *>(s_ms + tid * kMsSlots);
+    static_assert(kMsSlots >= kSegPairs, "MagSgn ring smaller than a segment's typical need");
     const uint8_t *ring_vlc = reinterpret_cast<const uint8_t *>(s_vlc + tid * kVlcSlots);
     const uint8_t *ring_mel = reinterpret_cast<const uint8_t *>(s_mel + tid * kMelSlots);
     uint4 *wave_ms = s_ms + wv * 64 * kMsSlots;
@@ -390,6 +389,63 @@ sgn_ht_cleanup_rows(const CbJob *__restrict__ jobs, uint32_t n, const uint8_t *_
         if (pb == npairs) s_e[nq][tid] = (uint8_t)prev_e3;
     }
     if (valid) status[i] = st;
+}
+
+// The lane kernel.  PERSIST: a grid sized to the resident capacity whose
+// waves take the next 64 code-blocks (in the host's cost order) from the
+// device work counter *queue until none are left; otherwise one wave per 64
+// code-blocks.  *queue must be zero at launch.
+template <int WAVES, int SEG, int MS, bool PERSIST>
+__global__ void __launch_bounds__(WAVES * 64)
+sgn_ht_rows(const CbJob *__restrict__ jobs, uint32_t n, const uint8_t *__restrict__ arena,
+            int32_t *__restrict__ coef, const uint16_t *__restrict__ tables,
+            uint8_t *__restrict__ status, uint32_t *__restrict__ queue) {
+    __shared__ uint16_t s_tab[1024 + 1024 + 320 + 256];
+    __shared__ uint8_t s_rho[kRowsMaxQ][WAVES * 64];  // significance nibble of the quad above
+    __shared__ uint8_t s_e[kRowsMaxQ][WAVES * 64];    // floor(log2(v_n)) per column pair
+    __shared__ uint4 s_ms[WAVES * 64 * MS];
+    __shared__ uint4 s_vlc[WAVES * 64 * kVlcSlots];
+    __shared__ uint4 s_mel[WAVES * 64 * kMelSlots];
+    for (int t = threadIdx.x; t < 2624; t += WAVES * 64) s_tab[t] = tables[t];
+    __syncthreads();
+    if constexpr (!PERSIST) {
+        const uint32_t first = (blockIdx.x * WAVES + (threadIdx.x >> 6)) * 64;
+        rows_wave<WAVES, SEG, MS>(jobs, n, first, arena, coef, status, s_tab, s_rho, s_e, s_ms,
+                                  s_vlc, s_mel);
+    } else {
+        for (;;) {
+            uint32_t first = 0;
+            if ((threadIdx.x & 63) == 0) first = atomicAdd(queue, 64u);
+            first = __shfl(first, 0);
+            if (first >= n) break;
+            rows_wave<WAVES, SEG, MS>(jobs, n, first, arena, coef, status, s_tab, s_rho, s_e,
+                                      s_ms, s_vlc, s_mel);
+        }
+    }
+}
+
+// Timing run for the decode-cost model: of each wave's 64 jobs only lane
+// 0's is a code-block (the others are empty), and the wave records the
+// shader-clock cycles it spent decoding it.
+extern "C" __global__ void __launch_bounds__(256)
+sgn_ht_rows_timed(const CbJob *__restrict__ jobs, uint32_t n, const uint8_t *__restrict__ arena,
+                  int32_t *__restrict__ coef, const uint16_t *__restrict__ tables,
+                  uint8_t *__restrict__ status, uint64_t *__restrict__ cycles) {
+    constexpr int W = 4, S = 8, M = 8;
+    __shared__ uint16_t s_tab[1024 + 1024 + 320 + 256];
+    __shared__ uint8_t s_rho[kRowsMaxQ][W * 64];
+    __shared__ uint8_t s_e[kRowsMaxQ][W * 64];
+    __shared__ uint4 s_ms[W * 64 * M];
+    __shared__ uint4 s_vlc[W * 64 * kVlcSlots];
+    __shared__ uint4 s_mel[W * 64 * kMelSlots];
+    for (int t = threadIdx.x; t < 2624; t += W * 64) s_tab[t] = tables[t];
+    __syncthreads();
+    const uint32_t wave = blockIdx.x * W + (threadIdx.x >> 6);
+    const uint64_t t0 = clock64();
+    rows_wave<W, S, M>(jobs, n, wave * 64, arena, coef, status, s_tab, s_rho, s_e, s_ms, s_vlc,
+                       s_mel);
+    const uint64_t t1 = clock64();
+    if ((threadIdx.x & 63) == 0 && wave * 64 < n) cycles[wave] = t1 - t0;
 }
 
 }  // namespace sgn

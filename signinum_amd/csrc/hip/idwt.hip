@@ -179,24 +179,243 @@ __device__ __forceinline__ void lift(T *win, int nlines, int n, uint32_t origin,
     }
 }
 
-// Synthesises block (bx, by) of one level into the window.  Result element
-// (x, y) of the block is at win[(y + H) * WN + x + H].
+// Window geometry: halo H (2 for 5/3, 4 for 9/7), N = 64 + 2H samples per
+// side, row pitch P = N + 1 (odd, so a wave walking down 64 rows of one
+// column, or along 64 columns of one row, touches 64 distinct LDS banks).
 template <bool REV>
-__device__ __forceinline__ void synth_block(const LevelView &L, const uint32_t *__restrict__ coef, int bx, int by,
-                            typename Sample<REV>::T *win, int &bw_out, int &bh_out) {
-    constexpr int H = REV ? 2 : 4;
-    constexpr int WN = kBlk + 2 * H;
+struct Win {
+    static constexpr int H = REV ? 2 : 4;
+    static constexpr int N = kBlk + 2 * H;
+    static constexpr int P = N + 1;
+    static constexpr int R = 8 + 2 * H;  // inputs of one 8-sample lifting segment
+};
+
+// Both lifting passes of an 8-sample segment in registers: v[0..R) holds the
+// signal at positions e0 - H .. e0 + 8 + H, v[0] at a position of parity Q
+// (0: low-pass / even absolute coordinate).  Afterwards v[H .. H + 8) are
+// the synthesised samples.  Each output depends only on inputs within H of
+// it, so segment results equal whole-line lifting (idwt.rs:741-764 for 5/3,
+// :767-835 for 9/7, same operation order).
+template <bool REV, int Q, typename T>
+__device__ __forceinline__ void lift_segment(T (&v)[Win<REV>::R]) {
+    // keep every rounding of the line-wise lifting: no contraction of the
+    // scaling products into the neighbour sums
+#pragma clang fp contract(off)
+    constexpr int R = Win<REV>::R;
+    if constexpr (REV) {
+#pragma unroll
+        for (int i = 1; i < R - 1; ++i)
+            if (((Q + i) & 1) == 0) v[i] = v[i] - ((v[i - 1] + v[i + 1] + 2) >> 2);
+#pragma unroll
+        for (int i = 2; i < R - 2; ++i)
+            if (((Q + i) & 1) == 1) v[i] = v[i] + ((v[i - 1] + v[i + 1]) >> 1);
+    } else {
+        const float inv_k = 1.0f / kKappa;
+#pragma unroll
+        for (int i = 0; i < R; ++i) v[i] = v[i] * (((Q + i) & 1) ? inv_k : kKappa);
+#pragma unroll
+        for (int i = 1; i < R - 1; ++i)
+            if (((Q + i) & 1) == 0) v[i] = fmaf(v[i - 1] + v[i + 1], kNegDelta, v[i]);
+#pragma unroll
+        for (int i = 2; i < R - 2; ++i)
+            if (((Q + i) & 1) == 1) v[i] = fmaf(v[i - 1] + v[i + 1], kNegGamma, v[i]);
+#pragma unroll
+        for (int i = 3; i < R - 3; ++i)
+            if (((Q + i) & 1) == 0) v[i] = fmaf(v[i - 1] + v[i + 1], kNegBeta, v[i]);
+#pragma unroll
+        for (int i = 4; i < R - 4; ++i)
+            if (((Q + i) & 1) == 1) v[i] = fmaf(v[i - 1] + v[i + 1], kNegAlpha, v[i]);
+    }
+}
+
+// Destinations of the synthesised block (block coordinates x, y < 64).
+// In-place sinks write the window itself; the others let the vertical pass
+// write its results straight to their consumer.
+template <bool REV>
+struct SinkWin {
+    typename Sample<REV>::T *win;
+    static constexpr bool kInPlace = true;
+    __device__ __forceinline__ void operator()(int x, int y, typename Sample<REV>::T v) const {
+        win[(y + Win<REV>::H) * Win<REV>::P + x + Win<REV>::H] = v;
+    }
+};
+
+template <bool REV>
+struct SinkPlane {  // a 64x64 LDS plane
+    typename Sample<REV>::T *plane;
+    static constexpr bool kInPlace = false;
+    __device__ __forceinline__ void operator()(int x, int y, typename Sample<REV>::T v) const {
+        plane[y * kBlk + x] = v;
+    }
+};
+
+template <bool REV>
+struct SinkGlobal {  // the level's output in the coefficient arena
+    uint32_t *coef;
+    uint32_t base, pitch;
+    int bh;
+    static constexpr bool kInPlace = false;
+    __device__ __forceinline__ void operator()(int x, int y, typename Sample<REV>::T v) const {
+        if (y >= bh) return;
+        uint32_t bits;
+        if constexpr (REV)
+            bits = (uint32_t)v;
+        else
+            bits = __float_as_uint(v);
+        coef[base + (uint32_t)y * pitch + (uint32_t)x] = bits;
+    }
+};
+
+// One lifting direction over the window with 8-sample register segments.
+// HORIZ: every window row in [0, nh), output columns [H, H + bw), written
+// back into the window; else every block column in [0, bw), output rows
+// [H, H + bh), handed to `sink`.  q: parity of window position 0 along the
+// direction.  All segments are read before any in-place write.
+template <bool REV, bool HORIZ, class Sink>
+__device__ __forceinline__ void lift_window(typename Sample<REV>::T *win, int nh, int bw, int bh,
+                                            uint32_t q, const Sink &sink) {
+    using T = typename Sample<REV>::T;
+    constexpr int H = Win<REV>::H, N = Win<REV>::N, P = Win<REV>::P, R = Win<REV>::R;
+    // items: HORIZ (row, segment) with the row fastest; else (column, segment)
+    constexpr int kLines = HORIZ ? N : kBlk;
+    constexpr int kItems = kLines * (kBlk / 8);
+    constexpr int kPer = (kItems + kIdwtThreads - 1) / kIdwtThreads;
+    constexpr bool kInPlace = HORIZ || Sink::kInPlace;
+    const int nlines = HORIZ ? nh : bw;
+    const int nseg = ((HORIZ ? bw : bh) + 7) >> 3;
+    T out[kPer][8];
+#pragma unroll
+    for (int it = 0; it < kPer; ++it) {
+        const int i = threadIdx.x + it * kIdwtThreads;
+        const int line = i % kLines, seg = i / kLines;
+        if (i >= kItems || line >= nlines || seg >= nseg) continue;
+        T v[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j)
+            v[j] = HORIZ ? win[line * P + 8 * seg + j] : win[(8 * seg + j) * P + H + line];
+        if (q) lift_segment<REV, 1>(v);
+        else lift_segment<REV, 0>(v);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) out[it][j] = v[H + j];
+    }
+    if (kInPlace) __syncthreads();
+#pragma unroll
+    for (int it = 0; it < kPer; ++it) {
+        const int i = threadIdx.x + it * kIdwtThreads;
+        const int line = i % kLines, seg = i / kLines;
+        if (i >= kItems || line >= nlines || seg >= nseg) continue;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            if (HORIZ)
+                win[line * P + H + 8 * seg + j] = out[it][j];
+            else
+                sink(line, 8 * seg + j, out[it][j]);
+        }
+    }
+    if (kInPlace) __syncthreads();
+}
+
+// Synthesises block (bx, by) of one level into the window.  Result element
+// (x, y) of the block is at win[(y + H) * P + x + H].  rows: LDS scratch of
+// N words.
+template <bool REV, class Sink>
+__device__ __forceinline__ void synth_block(const LevelView &L, const uint32_t *__restrict__ coef,
+                                            int bx, int by, typename Sample<REV>::T *win,
+                                            uint32_t *rows, const Sink &sink, int &bw_out,
+                                            int &bh_out) {
+    using T = typename Sample<REV>::T;
+    constexpr int H = Win<REV>::H, N = Win<REV>::N, P = Win<REV>::P;
     const int ox = bx * kBlk, oy = by * kBlk;
     const int bw = min(kBlk, L.w - ox), bh = min(kBlk, L.h - oy);
     const int nw = bw + 2 * H, nh = bh + 2 * H;
-    for (int k = threadIdx.x; k < nh * WN; k += kIdwtThreads) {
-        const int yy = k / WN, xx = k - yy * WN;
-        if (xx < nw) win[k] = load_level<REV>(L, coef, ox - H + xx, oy - H + yy);
+    const int tid = threadIdx.x;
+    // window row -> (band row, row parity), after extension
+    for (int yy = tid; yy < nh; yy += kIdwtThreads) {
+        const int my = reflect(oy - H + yy, L.h);
+        uint32_t row = (uint32_t)my, par = 0;
+        if (L.levels) {
+            const uint32_t ay = L.y0 + (uint32_t)my;
+            par = ay & 1u;
+            row = par ? (ay >> 1) - (L.y0 >> 1) : (ay >> 1) - ((L.y0 + 1) >> 1);
+        }
+        rows[yy] = row | (par << 31);
     }
     __syncthreads();
+    // Each thread owns one window column and walks down it: the column's
+    // band, band column and the two band tables it can hit are set up once,
+    // and all its loads are issued before the first one is used.  Pass 0:
+    // columns 0..63, a quarter of the rows per wave; pass 1: the 2H halo
+    // columns, 256 / 2H threads per column.
+    constexpr int kRowsPer = (N + 3) / 4;  // rows per thread in pass 0
+#pragma unroll
+    for (int pass = 0; pass < 2; ++pass) {
+        constexpr int kHalo = 2 * H;
+        const int xx = pass == 0 ? (tid & 63) : 64 + (tid % kHalo);
+        const int y0 = pass == 0 ? (tid >> 6) : tid / kHalo;
+        const int ystep = pass == 0 ? 4 : kIdwtThreads / kHalo;
+        if (xx >= nw || y0 >= nh) continue;
+        const int mx = reflect(ox - H + xx, L.w);
+        uint32_t col = (uint32_t)mx, xpar = 0;
+        if (L.levels) {
+            const uint32_t ax = L.x0 + (uint32_t)mx;
+            xpar = ax & 1u;
+            col = xpar ? (ax >> 1) - (L.x0 >> 1) : (ax >> 1) - ((L.x0 + 1) >> 1);
+        }
+        const uint32_t base0 = L.bands->base[xpar] + col, pitch0 = L.bands->pitch[xpar];
+        const uint32_t base1 = L.bands->base[xpar | 2] + col, pitch1 = L.bands->pitch[xpar | 2];
+        const bool f0 = (L.fin >> xpar) & 1u, f1 = (L.fin >> (xpar | 2)) & 1u;
+        float d0 = 1.0f, d1 = 1.0f;
+        if constexpr (!REV) {
+            d0 = L.bands->delta[xpar];
+            d1 = L.bands->delta[xpar | 2];
+        }
+        uint32_t raw[kRowsPer];
+#pragma unroll
+        for (int k = 0; k < kRowsPer; ++k) {
+            const int yy = y0 + k * ystep;
+            raw[k] = 0;
+            if (yy < nh) {
+                const uint32_t r = rows[yy];
+                const bool y1 = (r >> 31) != 0;
+                const uint32_t row = r & 0x7FFFFFFFu;
+                raw[k] = coef[(y1 ? base1 : base0) + row * (y1 ? pitch1 : pitch0)];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kRowsPer; ++k) {
+            const int yy = y0 + k * ystep;
+            if (yy >= nh) continue;
+            const bool y1 = (rows[yy] >> 31) != 0;
+            const bool isf = y1 ? f1 : f0;
+            const uint32_t v = raw[k];
+            T val;
+            if constexpr (REV)
+                val = isf ? (int32_t)__uint_as_float(v) : (int32_t)v;
+            else
+                val = isf ? __uint_as_float(v) : (float)(int32_t)v * (y1 ? d1 : d0);
+            win[yy * P + xx] = val;
+        }
+    }
+    __syncthreads();
+    bool in_window = true;  // result still in the window, not yet in the sink
     if (L.levels) {
-        lift<REV, WN, true>(win, nh, nw, L.x0 + (uint32_t)(ox - H), L.w == 1, (L.x0 & 1u) != 0);
-        lift<REV, WN, false>(win, bw, nh, L.y0 + (uint32_t)(oy - H), L.h == 1, (L.y0 & 1u) != 0);
+        const uint32_t qx = (L.x0 + (uint32_t)(ox - H)) & 1u, qy = (L.y0 + (uint32_t)(oy - H)) & 1u;
+        if (L.w == 1)
+            lift<REV, P, true>(win, nh, nw, L.x0 + (uint32_t)(ox - H), true, (L.x0 & 1u) != 0);
+        else
+            lift_window<REV, true>(win, nh, bw, bh, qx, sink);
+        if (L.h == 1) {
+            lift<REV, P, false>(win, bw, nh, L.y0 + (uint32_t)(oy - H), true, (L.y0 & 1u) != 0);
+        } else {
+            lift_window<REV, false>(win, nh, bw, bh, qy, sink);
+            in_window = false;
+        }
+    }
+    if (in_window && !Sink::kInPlace) {
+        for (int k = tid; k < kBlk * bh; k += kIdwtThreads) {
+            const int y = k / kBlk, x = k % kBlk;
+            if (x < bw) sink(x, y, win[(y + H) * P + x + H]);
+        }
     }
     bw_out = bw;
     bh_out = bh;
@@ -229,9 +448,9 @@ __device__ __forceinline__ uint32_t find_job(const J *jobs, uint32_t njobs, uint
 template <bool REV>
 __device__ __forceinline__ void level_body(const LevelJob *jobs, uint32_t njobs, uint32_t block_offset,
                            uint32_t *coef) {
-    constexpr int H = REV ? 2 : 4;
-    constexpr int WN = kBlk + 2 * H;
-    __shared__ typename Sample<REV>::T win[WN * WN];
+    constexpr int H = Win<REV>::H, N = Win<REV>::N, P = Win<REV>::P;
+    __shared__ typename Sample<REV>::T win[N * P];
+    __shared__ uint32_t rows[N];
     const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x) + block_offset;
     const LevelJob *lj = jobs + find_job(jobs, njobs, blk);
     __shared__ BandLds sb;
@@ -240,19 +459,11 @@ __device__ __forceinline__ void level_body(const LevelJob *jobs, uint32_t njobs,
     const uint32_t nbx = lj->nbx, out = lj->out;
     const int bx = (int)(local % nbx), by = (int)(local / nbx);
     int bw, bh;
-    synth_block<REV>(L, coef, bx, by, win, bw, bh);
-    for (int k = threadIdx.x; k < kBlk * bh; k += kIdwtThreads) {
-        const int y = k / kBlk, x = k % kBlk;
-        if (x >= bw) continue;
-        const auto v = win[(y + H) * WN + x + H];
-        uint32_t bits;
-        if constexpr (REV)
-            bits = (uint32_t)v;
-        else
-            bits = __float_as_uint(v);
-        coef[out + (uint32_t)(by * kBlk + y) * coef_pitch((uint32_t)L.w) +
-             (uint32_t)(bx * kBlk + x)] = bits;
-    }
+    const uint32_t opitch = coef_pitch((uint32_t)L.w);
+    const int bh_blk = min(kBlk, L.h - by * kBlk);
+    const SinkGlobal<REV> sink{coef, out + (uint32_t)(by * kBlk) * opitch + (uint32_t)(bx * kBlk),
+                               opitch, bh_blk};
+    synth_block<REV>(L, coef, bx, by, win, rows, sink, bw, bh);
 }
 
 extern "C" __global__ void __launch_bounds__(kIdwtThreads)
@@ -304,10 +515,10 @@ __device__ __forceinline__ uint32_t out_u16(float v, uint32_t bd) {
 template <bool REV>
 __device__ __forceinline__ void final_body(const FinalJob *jobs, uint32_t njobs, uint32_t block_offset,
                            const LevelJob *lv, const uint32_t *coef) {
-    constexpr int H = REV ? 2 : 4;
-    constexpr int WN = kBlk + 2 * H;
+    constexpr int H = Win<REV>::H, N = Win<REV>::N, P = Win<REV>::P;
     using T = typename Sample<REV>::T;
-    __shared__ T win[WN * WN];
+    __shared__ T win[N * P];
+    __shared__ uint32_t rows[N];
     __shared__ T res[2][kBlk * kBlk];  // all but the last component; the last stays in win
     const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x) + block_offset;
     const FinalJob *fj = jobs + find_job(jobs, njobs, blk);
@@ -332,17 +543,14 @@ __device__ __forceinline__ void final_body(const FinalJob *jobs, uint32_t njobs,
     for (int c = 0; c < F.nc; ++c) {
         // the previous component's last reads of sb ended at a barrier
         const LevelView L = level_view(lv + (c == 0 ? lvl0 : c == 1 ? lvl1 : lvl2), &sb);
-        synth_block<REV>(L, coef, bx, by, win, bw, bh);
-        if (c + 1 < F.nc) {
-            for (int k = threadIdx.x; k < kBlk * bh; k += kIdwtThreads) {
-                const int y = k / kBlk, x = k % kBlk;
-                if (x < bw) res[c][k] = win[(y + H) * WN + x + H];
-            }
-            __syncthreads();
-        }
+        if (c + 1 < F.nc)
+            synth_block<REV>(L, coef, bx, by, win, rows, SinkPlane<REV>{res[c]}, bw, bh);
+        else
+            synth_block<REV>(L, coef, bx, by, win, rows, SinkWin<REV>{win}, bw, bh);
     }
     // Each thread converts and stores four horizontally adjacent pixels.
     const bool words = F.bps == 1 && ((reinterpret_cast<uintptr_t>(F.dst) | F.pitch) & 3) == 0;
+    const bool int8_out = REV && F.bps == 1 && F.same8;
     for (int k = threadIdx.x; k < (kBlk / 4) * bh; k += kIdwtThreads) {
         const int y = k / (kBlk / 4), x4 = (k % (kBlk / 4)) * 4;
         if (x4 >= bw) continue;
@@ -356,7 +564,7 @@ __device__ __forceinline__ void final_body(const FinalJob *jobs, uint32_t njobs,
                 int32_t s[3] = {0, 0, 0};
 #pragma unroll
                 for (int c = 0; c < 3; ++c)
-                    if (c < F.nc) s[c] = c + 1 < F.nc ? res[c < 2 ? c : 1][ki] : win[(y + H) * WN + x + H];
+                    if (c < F.nc) s[c] = c + 1 < F.nc ? res[c < 2 ? c : 1][ki] : win[(y + H) * P + x + H];
                 if (F.mct) {
                     const int32_t i1 = s[0] - ((s[2] + s[1]) >> 2);
                     const int32_t r = s[2] + i1, b = s[1] + i1;
@@ -367,12 +575,20 @@ __device__ __forceinline__ void final_body(const FinalJob *jobs, uint32_t njobs,
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     const uint32_t pc = c == 0 ? F.prec0 : c == 1 ? F.prec1 : F.prec2;
-                    v[c] = (float)(s[c] + (1 << (pc - 1)));
+                    s[c] += 1 << (pc - 1);
+                    v[c] = (float)s[c];
+                }
+                if (int8_out) {
+                    // integers: rounding is the identity, `as u8` a clamp
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) o[q][c] = (uint32_t)min(max(s[c], 0), 255);
+                    o[q][3] = 255;
+                    continue;
                 }
             } else {
 #pragma unroll
                 for (int c = 0; c < 3; ++c)
-                    v[c] = c < F.nc ? (c + 1 < F.nc ? res[c < 2 ? c : 1][ki] : win[(y + H) * WN + x + H]) : 0.0f;
+                    v[c] = c < F.nc ? (c + 1 < F.nc ? res[c < 2 ? c : 1][ki] : win[(y + H) * P + x + H]) : 0.0f;
                 if (F.mct) {
                     const float y0 = v[0], y1 = v[1], y2 = v[2];
                     v[0] = fmaf(y2, 1.402f, y0);

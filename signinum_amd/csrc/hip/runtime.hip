@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <exception>
 #include <memory>
@@ -21,6 +22,7 @@
 #include "ht_decode.hip"
 #include "ht_rows.hip"
 #include "idwt.hip"
+#include "cost_model.hip"
 
 using namespace sgn;
 
@@ -38,7 +40,49 @@ struct sgn_hip_session {
     hipStream_t stream = nullptr;
     uint16_t *d_tables = nullptr;
     uint64_t dispatches = 0;
+    int cus = 0;                    // compute units of the device
+    std::vector<uint32_t> resident; // per lane-kernel variant: resident work-groups (0: unknown)
 };
+
+namespace {
+
+// Lane-kernel variants the autotuner chooses from: work-group shape, quad
+// pairs per segment (and MagSgn ring slots), and one-shot grid versus
+// persistent waves pulling 64-block chunks from a device work counter.
+using RowsLaunch = void (*)(hipStream_t, uint32_t, const CbJob *, uint32_t, const uint8_t *,
+                            int32_t *, const uint16_t *, uint8_t *, uint32_t *);
+
+template <int W, int S, int M, bool PERSIST>
+void rows_launch(hipStream_t st, uint32_t grid, const CbJob *jobs, uint32_t n,
+                 const uint8_t *arena, int32_t *coef, const uint16_t *tables, uint8_t *status,
+                 uint32_t *queue) {
+    hipLaunchKernelGGL((sgn_ht_rows<W, S, M, PERSIST>), dim3(grid), dim3(W * 64), 0, st, jobs, n,
+                       arena, coef, tables, status, queue);
+}
+
+struct RowsVariant {
+    const char *name;
+    int waves;
+    bool persist;
+    RowsLaunch launch;
+    const void *kernel;
+};
+
+const RowsVariant kRowsVariants[] = {
+    {"w4-seg8", 4, false, rows_launch<4, 8, 8, false>,
+     reinterpret_cast<const void *>(sgn_ht_rows<4, 8, 8, false>)},
+    {"w4-seg8-queue", 4, true, rows_launch<4, 8, 8, true>,
+     reinterpret_cast<const void *>(sgn_ht_rows<4, 8, 8, true>)},
+    {"w4-seg4-queue", 4, true, rows_launch<4, 4, 4, true>,
+     reinterpret_cast<const void *>(sgn_ht_rows<4, 4, 4, true>)},
+    {"w2-seg16-queue", 2, true, rows_launch<2, 16, 16, true>,
+     reinterpret_cast<const void *>(sgn_ht_rows<2, 16, 16, true>)},
+    {"w1-seg8-queue", 1, true, rows_launch<1, 8, 8, true>,
+     reinterpret_cast<const void *>(sgn_ht_rows<1, 8, 8, true>)},
+};
+constexpr int kRowsVariantCount = int(sizeof(kRowsVariants) / sizeof(kRowsVariants[0]));
+
+}  // namespace
 
 namespace {
 
@@ -70,6 +114,18 @@ Status dev_upload(T *&dptr, const std::vector<T> &v) {
 
 }  // namespace
 
+struct sgn_cost_model {
+    int device = 0;
+    float *d_params = nullptr;             // kMlpParams
+    float *d_m = nullptr, *d_v = nullptr;  // Adam moments
+    uint32_t steps = 0;
+    ~sgn_cost_model() {
+        void *ptrs[] = {d_params, d_m, d_v};
+        for (void *p : ptrs)
+            if (p) (void)hipFree(p);
+    }
+};
+
 struct sgn_j2k_plan {
     size_t ntiles = 0;
     // host tables
@@ -94,6 +150,8 @@ struct sgn_j2k_plan {
     LevelJob *d_final_lv = nullptr;
     FinalJob *d_fin[2] = {nullptr, nullptr};
     uint8_t *d_status_lane = nullptr, *d_status_wave = nullptr;
+    uint32_t *d_queue = nullptr;  // work counter of the persistent lane kernel
+    int lane_variant = 0;         // index into kRowsVariants
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
     uint32_t launches_batched = 0;
@@ -102,7 +160,8 @@ struct sgn_j2k_plan {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
         void *ptrs[] = {d_arena, d_coef, d_cb_lane, d_cb_lane_plain, d_cb_wave, d_lv[0],
-                        d_lv[1], d_final_lv, d_fin[0], d_fin[1], d_status_lane, d_status_wave};
+                        d_lv[1], d_final_lv, d_fin[0], d_fin[1], d_status_lane, d_status_wave,
+                        d_queue};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
     }
@@ -272,6 +331,7 @@ Status upload_plan(const sgn_tile_req *jobs, sgn_j2k_plan &P) {
     }
     if ((st = dev_upload(P.d_final_lv, P.final_lv))) return st;
     if (!P.cb_lane.empty()) HIP_TRY(hipMalloc(&P.d_status_lane, P.cb_lane.size()));
+    HIP_TRY(hipMalloc(&P.d_queue, 16));
     if (!P.cb_wave.empty()) HIP_TRY(hipMalloc(&P.d_status_wave, P.cb_wave.size()));
     return kOk;
 }
@@ -287,13 +347,47 @@ uint32_t final_blocks(const std::vector<FinalJob> &v, uint32_t a, uint32_t b) {
 }
 
 // Enqueue helpers; each returns the number of kernel launches issued.
+// Work-groups of variant v that the device holds at once.
+uint32_t resident_groups(sgn_hip_session *s, int v) {
+    if (s->resident.empty()) s->resident.assign(kRowsVariantCount, 0);
+    if (!s->resident[v]) {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kRowsVariants[v].kernel,
+                                                         kRowsVariants[v].waves * 64, 0) !=
+                hipSuccess ||
+            per_cu < 1)
+            per_cu = 1;
+        if (!s->cus && (hipDeviceGetAttribute(&s->cus, hipDeviceAttributeMultiprocessorCount,
+                                              s->device) != hipSuccess ||
+                        s->cus < 1))
+            s->cus = 1;
+        s->resident[v] = uint32_t(per_cu) * uint32_t(s->cus);
+    }
+    return s->resident[v];
+}
+
+uint32_t launch_lane_variant(sgn_hip_session *s, sgn_j2k_plan &P, int v, const CbJob *d_jobs,
+                             uint32_t count, uint8_t *status) {
+    if (!count) return 0;
+    const RowsVariant &rv = kRowsVariants[v];
+    const uint32_t per_group = 64u * uint32_t(rv.waves);
+    uint32_t grid = (count + per_group - 1) / per_group;
+    uint32_t k = 1;
+    if (rv.persist) {
+        // the counter is cleared by a kernel, never a memset node (graph capture)
+        hipLaunchKernelGGL(sgn_fill_bytes, dim3(1), dim3(64), 0, s->stream,
+                           reinterpret_cast<uint8_t *>(P.d_queue), size_t(4), size_t(1), 0u);
+        grid = std::min(grid, resident_groups(s, v));
+        ++k;
+    }
+    rv.launch(s->stream, grid, d_jobs, count, P.d_arena, reinterpret_cast<int32_t *>(P.d_coef),
+              s->d_tables, status, P.d_queue);
+    return k;
+}
+
 uint32_t launch_lane(sgn_hip_session *s, sgn_j2k_plan &P, const CbJob *d_jobs, uint32_t count,
                      uint8_t *status) {
-    if (!count) return 0;
-    const uint32_t grid = (count + kRowsWg - 1) / kRowsWg;
-    hipLaunchKernelGGL(sgn_ht_cleanup_rows, dim3(grid), dim3(kRowsWg), 0, s->stream, d_jobs,
-                       count, P.d_arena, reinterpret_cast<int32_t *>(P.d_coef), s->d_tables, status);
-    return 1;
+    return launch_lane_variant(s, P, P.lane_variant, d_jobs, count, status);
 }
 
 uint32_t launch_wave(sgn_hip_session *s, sgn_j2k_plan &P, const CbJob *d_jobs, uint32_t count,
@@ -346,7 +440,8 @@ uint32_t enqueue_plain(sgn_hip_session *s, sgn_j2k_plan &P) {
     uint32_t k = 0;
     for (size_t t = 0; t < P.ntiles; ++t) {
         const uint32_t a0 = P.lane_plain_begin[t], a1 = P.lane_plain_begin[t + 1];
-        k += launch_lane(s, P, P.d_cb_lane_plain + a0, a1 - a0, P.d_status_lane + a0);
+        k += launch_lane_variant(s, P, 0, P.d_cb_lane_plain + a0, a1 - a0,
+                                 P.d_status_lane + a0);
         const uint32_t b0 = P.wave_begin[t], b1 = P.wave_begin[t + 1];
         k += launch_wave(s, P, P.d_cb_wave + b0, b1 - b0, P.d_status_wave + b0);
         for (const LevelRange &rg : P.tile_levels[t])
@@ -404,6 +499,13 @@ Status check_status(sgn_j2k_plan &P, int mode, size_t *first_fail) {
         return kErrBlock;
     }
     return kOk;
+}
+
+void drop_graph(sgn_j2k_plan &P) {
+    if (P.graph_exec) (void)hipGraphExecDestroy(P.graph_exec);
+    if (P.graph) (void)hipGraphDestroy(P.graph);
+    P.graph_exec = nullptr;
+    P.graph = nullptr;
 }
 
 Status make_plan(sgn_hip_session *s, const sgn_tile_req *jobs, size_t n,
@@ -642,12 +744,71 @@ int sgn_j2k_plan_set_output(sgn_j2k_plan *P, size_t i, void *dst, size_t pitch, 
     }
     HIP_TRY(hipMemcpy(P->d_fin[ff.first] + ff.second, v.data() + ff.second,
                       P->tile_final_count[i] * sizeof(FinalJob), hipMemcpyHostToDevice));
-    if (P->graph_exec) {
-        (void)hipGraphExecDestroy(P->graph_exec);
-        (void)hipGraphDestroy(P->graph);
-        P->graph_exec = nullptr;
-        P->graph = nullptr;
+    drop_graph(*P);
+    return SGN_OK;
+    API_GUARD_END
+}
+
+int sgn_ht_lane_variant_count(void) { return kRowsVariantCount; }
+
+const char *sgn_ht_lane_variant_name(int v) {
+    return (v >= 0 && v < kRowsVariantCount) ? kRowsVariants[v].name : nullptr;
+}
+
+int sgn_j2k_plan_set_lane_variant(sgn_j2k_plan *P, int v) {
+    if (!P || v < 0 || v >= kRowsVariantCount) return SGN_E_ARG;
+    if (P->lane_variant != v) drop_graph(*P);
+    P->lane_variant = v;
+    return SGN_OK;
+}
+
+int sgn_j2k_plan_get_lane_variant(const sgn_j2k_plan *P) { return P ? P->lane_variant : -1; }
+
+int sgn_j2k_plan_autotune(sgn_hip_session *s, sgn_j2k_plan *P, int reps, float *ms, size_t cap,
+                          int *chosen) {
+    API_GUARD_BEGIN
+    if (!s || !P || reps < 1) return SGN_E_ARG;
+    HIP_TRY(hipSetDevice(s->device));
+    const uint32_t n = static_cast<uint32_t>(P->cb_lane.size());
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        set_last_error("hipEventCreate failed");
+        return SGN_E_DEVICE;
     }
+    int best = P->lane_variant;
+    float best_ms = 0.0f;
+    Status st = kOk;
+    for (int v = 0; v < kRowsVariantCount && !st; ++v) {
+        float t = 0.0f;
+        if (n) {
+            s->dispatches += launch_lane_variant(s, *P, v, P->d_cb_lane, n, P->d_status_lane);
+            bool ok = hipEventRecord(e0, s->stream) == hipSuccess;
+            for (int r = 0; r < reps && ok; ++r)
+                s->dispatches += launch_lane_variant(s, *P, v, P->d_cb_lane, n, P->d_status_lane);
+            ok = ok && hipEventRecord(e1, s->stream) == hipSuccess &&
+                 hipEventSynchronize(e1) == hipSuccess &&
+                 hipEventElapsedTime(&t, e0, e1) == hipSuccess && hipGetLastError() == hipSuccess;
+            if (!ok) {
+                set_last_error("lane-kernel variant failed to run");
+                st = kErrDevice;
+                break;
+            }
+            t /= float(reps);
+        }
+        if (ms && size_t(v) < cap) ms[v] = t;
+        if (v == 0 || t < best_ms) {
+            best = v;
+            best_ms = t;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st) return st;
+    if (P->lane_variant != best) drop_graph(*P);
+    P->lane_variant = best;
+    if (chosen) *chosen = best;
     return SGN_OK;
     API_GUARD_END
 }
@@ -835,9 +996,9 @@ static int hook_blocks(sgn_hip_session *s, const uint8_t *data, size_t data_len,
         return SGN_E_DEVICE;
     }
     if (!lane.empty())
-        hipLaunchKernelGGL(sgn_ht_cleanup_rows, dim3((lane.size() + kRowsWg - 1) / kRowsWg),
-                           dim3(kRowsWg), 0, s->stream, d_lane, uint32_t(lane.size()), d_data,
-                           reinterpret_cast<int32_t *>(d_out), s->d_tables, d_st);
+        kRowsVariants[0].launch(s->stream, uint32_t((lane.size() + 255) / 256), d_lane,
+                                uint32_t(lane.size()), d_data, reinterpret_cast<int32_t *>(d_out),
+                                s->d_tables, d_st, nullptr);
     if (!wave.empty())
         hipLaunchKernelGGL(sgn_ht_block_wave, dim3(wave.size()), dim3(64), 0, s->stream, d_wave,
                            uint32_t(wave.size()), d_data, reinterpret_cast<int32_t *>(d_out),
@@ -982,3 +1143,269 @@ int sgn_inverse_mct(sgn_hip_session *s, int reversible, float *p0, float *p1, fl
 }
 
 }  // extern "C"
+
+// ---- decode-cost model ---------------------------------------------------
+
+namespace {
+
+// He-uniform initial weights, zero biases, from a splitmix64 stream.
+std::vector<float> mlp_init(uint64_t seed) {
+    std::vector<float> p(kMlpParams, 0.0f);
+    uint64_t x = seed;
+    auto next = [&]() {
+        x += 0x9E3779B97F4A7C15ull;
+        uint64_t z = x;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return double(z >> 11) * (1.0 / 9007199254740992.0);
+    };
+    auto fill = [&](int off, int count, int fan_in) {
+        const double a = std::sqrt(6.0 / fan_in);
+        for (int k = 0; k < count; ++k) p[off + k] = float((2.0 * next() - 1.0) * a);
+    };
+    fill(kW1, kMlpH * kMlpIn, kMlpIn);
+    fill(kW2, kMlpH * kMlpH, kMlpH);
+    fill(kW3, kMlpH, kMlpH);
+    return p;
+}
+
+// Device buffer that frees itself.
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    Status alloc(size_t n) {
+        if (hipMalloc(&p, std::max<size_t>(1, n) * sizeof(T)) != hipSuccess) {
+            p = nullptr;
+            set_last_error("hipMalloc failed");
+            return kErrDevice;
+        }
+        return kOk;
+    }
+    Status upload(const T *h, size_t n) {
+        Status st = alloc(n);
+        if (st) return st;
+        if (n && hipMemcpy(p, h, n * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) {
+            set_last_error("hipMemcpy failed");
+            return kErrDevice;
+        }
+        return kOk;
+    }
+};
+
+// Enqueues the mean-loss gradient of the model on (X, Y) into d_grad.
+Status mlp_gradient(sgn_hip_session *s, const float *d_params, const float *d_x, const float *d_y,
+                    uint32_t n, float *d_partial, float *d_grad) {
+    const uint32_t groups = (n + kGradWg - 1) / kGradWg;
+    hipLaunchKernelGGL(sgn_mlp_grad_partial, dim3(groups), dim3(kGradWg), 0, s->stream, d_params,
+                       d_x, d_y, n, d_partial);
+    hipLaunchKernelGGL(sgn_mlp_reduce, dim3((kMlpParams + 255) / 256), dim3(256), 0, s->stream,
+                       d_partial, groups, 1.0f / float(n), d_grad);
+    s->dispatches += 2;
+    HIP_TRY(hipGetLastError());
+    return kOk;
+}
+
+}  // namespace
+
+int sgn_cost_model_create(sgn_hip_session *s, uint64_t seed, sgn_cost_model **out) {
+    API_GUARD_BEGIN
+    if (!s || !out) return SGN_E_ARG;
+    *out = nullptr;
+    HIP_TRY(hipSetDevice(s->device));
+    auto m = std::make_unique<sgn_cost_model>();
+    m->device = s->device;
+    const std::vector<float> p = mlp_init(seed);
+    const std::vector<float> z(kMlpParams, 0.0f);
+    HIP_TRY(hipMalloc(&m->d_params, kMlpParams * sizeof(float)));
+    HIP_TRY(hipMalloc(&m->d_m, kMlpParams * sizeof(float)));
+    HIP_TRY(hipMalloc(&m->d_v, kMlpParams * sizeof(float)));
+    HIP_TRY(hipMemcpy(m->d_params, p.data(), kMlpParams * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(m->d_m, z.data(), kMlpParams * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(m->d_v, z.data(), kMlpParams * sizeof(float), hipMemcpyHostToDevice));
+    *out = m.release();
+    return SGN_OK;
+    API_GUARD_END
+}
+
+void sgn_cost_model_destroy(sgn_cost_model *m) { delete m; }
+
+int sgn_cost_model_param_count(void) { return kMlpParams; }
+
+int sgn_cost_model_get_params(const sgn_cost_model *m, float *out, size_t n) {
+    if (!m || !out || n < size_t(kMlpParams)) return SGN_E_ARG;
+    HIP_TRY(hipSetDevice(m->device));
+    HIP_TRY(hipMemcpy(out, m->d_params, kMlpParams * sizeof(float), hipMemcpyDeviceToHost));
+    return SGN_OK;
+}
+
+int sgn_cost_model_set_params(sgn_cost_model *m, const float *p, size_t n) {
+    if (!m || !p || n < size_t(kMlpParams)) return SGN_E_ARG;
+    HIP_TRY(hipSetDevice(m->device));
+    HIP_TRY(hipMemcpy(m->d_params, p, kMlpParams * sizeof(float), hipMemcpyHostToDevice));
+    return SGN_OK;
+}
+
+int sgn_cost_model_gradient(sgn_hip_session *s, const sgn_cost_model *m, const float *X,
+                            const float *Y, size_t n, float *grad) {
+    API_GUARD_BEGIN
+    if (!s || !m || !X || !Y || !grad || n == 0 || n > (size_t(1) << 24)) return SGN_E_ARG;
+    HIP_TRY(hipSetDevice(s->device));
+    DevBuf<float> dx, dy, dpart, dgrad;
+    Status st;
+    if ((st = dx.upload(X, n * kMlpIn)) || (st = dy.upload(Y, n)) ||
+        (st = dpart.alloc(((n + kGradWg - 1) / kGradWg) * kMlpParams)) ||
+        (st = dgrad.alloc(kMlpParams)))
+        return st;
+    if ((st = mlp_gradient(s, m->d_params, dx.p, dy.p, uint32_t(n), dpart.p, dgrad.p))) return st;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemcpy(grad, dgrad.p, kMlpParams * sizeof(float), hipMemcpyDeviceToHost));
+    return SGN_OK;
+    API_GUARD_END
+}
+
+int sgn_cost_model_train(sgn_hip_session *s, sgn_cost_model *m, const float *X, const float *Y,
+                         size_t n, uint32_t epochs, float lr, float *loss) {
+    API_GUARD_BEGIN
+    if (!s || !m || !X || !Y || n == 0 || n > (size_t(1) << 24) || !(lr > 0.0f))
+        return SGN_E_ARG;
+    HIP_TRY(hipSetDevice(s->device));
+    DevBuf<float> dx, dy, dpart, dgrad, dpred;
+    Status st;
+    if ((st = dx.upload(X, n * kMlpIn)) || (st = dy.upload(Y, n)) ||
+        (st = dpart.alloc(((n + kGradWg - 1) / kGradWg) * kMlpParams)) ||
+        (st = dgrad.alloc(kMlpParams)) || (st = dpred.alloc(n)))
+        return st;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    for (uint32_t e = 0; e < epochs; ++e) {
+        if ((st = mlp_gradient(s, m->d_params, dx.p, dy.p, uint32_t(n), dpart.p, dgrad.p)))
+            return st;
+        ++m->steps;
+        const float c1 = 1.0f - std::pow(b1, float(m->steps));
+        const float c2 = 1.0f - std::pow(b2, float(m->steps));
+        hipLaunchKernelGGL(sgn_mlp_adam, dim3((kMlpParams + 255) / 256), dim3(256), 0, s->stream,
+                           m->d_params, dgrad.p, m->d_m, m->d_v, lr, b1, b2, eps, c1, c2);
+        s->dispatches++;
+    }
+    HIP_TRY(hipGetLastError());
+    if (loss) {
+        hipLaunchKernelGGL(sgn_mlp_predict, dim3(uint32_t((n + 255) / 256)), dim3(256), 0,
+                           s->stream, m->d_params, dx.p, uint32_t(n), dpred.p);
+        s->dispatches++;
+        std::vector<float> pred(n);
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        HIP_TRY(hipMemcpy(pred.data(), dpred.p, n * sizeof(float), hipMemcpyDeviceToHost));
+        double acc = 0;
+        for (size_t i = 0; i < n; ++i) acc += 0.5 * double(pred[i] - Y[i]) * (pred[i] - Y[i]);
+        *loss = float(acc / double(n));
+    }
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return SGN_OK;
+    API_GUARD_END
+}
+
+int sgn_cost_model_predict(sgn_hip_session *s, const sgn_cost_model *m, const float *X, size_t n,
+                           float *out) {
+    API_GUARD_BEGIN
+    if (!s || !m || (!X && n) || (!out && n) || n > (size_t(1) << 24)) return SGN_E_ARG;
+    if (!n) return SGN_OK;
+    HIP_TRY(hipSetDevice(s->device));
+    DevBuf<float> dx, dpred;
+    Status st;
+    if ((st = dx.upload(X, n * kMlpIn)) || (st = dpred.alloc(n))) return st;
+    hipLaunchKernelGGL(sgn_mlp_predict, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, s->stream,
+                       m->d_params, dx.p, uint32_t(n), dpred.p);
+    s->dispatches++;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemcpy(out, dpred.p, n * sizeof(float), hipMemcpyDeviceToHost));
+    return SGN_OK;
+    API_GUARD_END
+}
+
+int sgn_j2k_plan_time_code_blocks(sgn_hip_session *s, sgn_j2k_plan *P, size_t stride,
+                                  size_t max_samples, float *features, float *cycles,
+                                  size_t *count) {
+    API_GUARD_BEGIN
+    if (!s || !P || !stride || !count) return SGN_E_ARG;
+    HIP_TRY(hipSetDevice(s->device));
+    std::vector<size_t> pick;
+    for (size_t i = 0; i < P->cb_lane.size() && pick.size() < max_samples; i += stride)
+        if (!(P->cb_lane[i].flags & kCbZero)) pick.push_back(i);
+    *count = pick.size();
+    if (pick.empty()) return SGN_OK;
+    // wave k: lane 0 holds sample k, lanes 1..63 an empty block
+    CbJob empty{};
+    empty.flags = kCbZero;
+    std::vector<CbJob> jobs(pick.size() * 64, empty);
+    for (size_t k = 0; k < pick.size(); ++k) jobs[k * 64] = P->cb_lane[pick[k]];
+    DevBuf<CbJob> djobs;
+    DevBuf<uint8_t> dst;
+    DevBuf<uint64_t> dcyc;
+    Status st;
+    if ((st = djobs.upload(jobs.data(), jobs.size())) || (st = dst.alloc(jobs.size())) ||
+        (st = dcyc.alloc(pick.size())))
+        return st;
+    const uint32_t waves = uint32_t(pick.size());
+    hipLaunchKernelGGL(sgn_ht_rows_timed, dim3((waves + 3) / 4), dim3(256), 0, s->stream, djobs.p,
+                       uint32_t(jobs.size()), P->d_arena, reinterpret_cast<int32_t *>(P->d_coef),
+                       s->d_tables, dst.p, dcyc.p);
+    s->dispatches++;
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    std::vector<uint64_t> cyc(pick.size());
+    HIP_TRY(hipMemcpy(cyc.data(), dcyc.p, cyc.size() * 8, hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < pick.size(); ++k) {
+        if (cycles) cycles[k] = float(cyc[k]);
+        if (features) {
+            const CbFeatures f = cb_features(P->cb_lane[pick[k]]);
+            for (int q = 0; q < kMlpIn; ++q) features[k * kMlpIn + q] = f.f[q];
+        }
+    }
+    return SGN_OK;
+    API_GUARD_END
+}
+
+int sgn_j2k_plan_order_code_blocks(sgn_hip_session *s, sgn_j2k_plan *P, int order,
+                                   const sgn_cost_model *m) {
+    API_GUARD_BEGIN
+    if (!s || !P || order < 0 || order > 2 || (order == 2 && !m)) return SGN_E_ARG;
+    HIP_TRY(hipSetDevice(s->device));
+    std::vector<CbJob> v = P->cb_lane_plain;
+    if (order == 0) {
+        std::stable_sort(v.begin(), v.end(),
+                         [](const CbJob &a, const CbJob &b) { return a.cost > b.cost; });
+    } else if (order == 2 && !v.empty()) {
+        DevBuf<float> dpred;
+        Status st;
+        if ((st = dpred.alloc(v.size()))) return st;
+        hipLaunchKernelGGL(sgn_mlp_predict_blocks, dim3(uint32_t((v.size() + 255) / 256)),
+                           dim3(256), 0, s->stream, m->d_params, P->d_cb_lane_plain,
+                           uint32_t(v.size()), dpred.p);
+        s->dispatches++;
+        std::vector<float> key(v.size());
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        HIP_TRY(hipMemcpy(key.data(), dpred.p, key.size() * 4, hipMemcpyDeviceToHost));
+        std::vector<uint32_t> idx(v.size());
+        for (size_t i = 0; i < idx.size(); ++i) idx[i] = uint32_t(i);
+        std::stable_sort(idx.begin(), idx.end(),
+                         [&](uint32_t a, uint32_t b) { return key[a] > key[b]; });
+        std::vector<CbJob> w(v.size());
+        for (size_t i = 0; i < idx.size(); ++i) w[i] = v[idx[i]];
+        v.swap(w);
+    }
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (!v.empty())
+        HIP_TRY(hipMemcpy(P->d_cb_lane, v.data(), v.size() * sizeof(CbJob), hipMemcpyHostToDevice));
+    P->cb_lane.swap(v);
+    drop_graph(*P);
+    return SGN_OK;
+    API_GUARD_END
+}
+
+#include "server.hip"

@@ -149,7 +149,8 @@ Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denomin
                 j.w = static_cast<uint16_t>(cg.rect.w());
                 j.h = static_cast<uint16_t>(cg.rect.h());
                 j.kmax = static_cast<uint8_t>(K);
-                j.flags = (cp.cbstyle & 0x08) ? kCbStripeCausal : 0;
+                j.flags = static_cast<uint8_t>(((cp.cbstyle & 0x08) ? kCbStripeCausal : 0) |
+                                               ((bg.type & 3u) << kCbBandShift));
                 if (j.w == 0 || j.h == 0) continue;
                 const uint32_t actual = K > sg.zblk ? K - sg.zblk : 0;
                 if (K > 31 && sg.passes > 0) {

@@ -19,7 +19,22 @@ struct CbJob {
 };
 static_assert(sizeof(CbJob) == 32, "CbJob layout");
 
-enum : uint8_t { kCbStripeCausal = 1, kCbZero = 2 };
+// flags: bit 0 stripe-causal, bit 1 decodes to zero, bits 2-3 sub-band
+// (0 LL, 1 HL, 2 LH, 3 HH; a feature of the decode-cost model)
+enum : uint8_t { kCbStripeCausal = 1, kCbZero = 2, kCbBandShift = 2 };
+
+// Input features of the code-block decode-cost model (5 values, each
+// roughly in [0, 1]): coded bytes, coding passes, missing MSBs (Zblk),
+// sub-band and sample count.
+struct CbFeatures {
+    float f[5];
+};
+constexpr inline CbFeatures cb_features(const CbJob &j) {
+    return CbFeatures{{float(uint32_t(j.lcup) + j.lref) * (1.0f / 4096.0f),
+                       float(j.passes) * (1.0f / 3.0f), float(j.zblk) * (1.0f / 16.0f),
+                       float((j.flags >> kCbBandShift) & 3u) * (1.0f / 3.0f),
+                       float(uint32_t(j.w) * j.h) * (1.0f / 4096.0f)}};
+}
 
 // Row pitch (elements) of every band and intermediate level in the
 // coefficient arena: a multiple of four, so rows start 16-byte aligned and

@@ -45,6 +45,12 @@ def lib():
         L.oracle_inverse_mct.argtypes = [ctypes.c_int, vp, vp, vp, sz, ctypes.c_float,
                                          ctypes.c_float, ctypes.c_float]
         L.oracle_get_tables.argtypes = [vp, vp, vp, vp]
+        L.oracle_mlp_predict.argtypes = [vp, vp]
+        L.oracle_mlp_predict.restype = ctypes.c_double
+        L.oracle_mlp_loss.argtypes = [vp, vp, vp, sz]
+        L.oracle_mlp_loss.restype = ctypes.c_double
+        L.oracle_mlp_loss_grad.argtypes = [vp, vp, vp, sz, vp]
+        L.oracle_mlp_loss_grad.restype = ctypes.c_double
         _L = L
     return _L
 
@@ -194,3 +200,28 @@ def code_blocks(cs):
     rc = L.oracle_code_blocks(p, n, arr, cnt.value, ctypes.byref(cnt))
     assert rc == 0, rc
     return list(arr[:cnt.value])
+
+
+def mlp_loss_grad(params, X, Y):
+    """(mean loss, gradient) of the decode-cost MLP in double precision."""
+    p = np.ascontiguousarray(params, np.float64)
+    X = np.ascontiguousarray(X, np.float32).reshape(-1, 5)
+    Y = np.ascontiguousarray(Y, np.float32).reshape(-1)
+    g = np.zeros(1281, np.float64)
+    loss = lib().oracle_mlp_loss_grad(p.ctypes.data, X.ctypes.data, Y.ctypes.data, X.shape[0],
+                                      g.ctypes.data)
+    return loss, g
+
+
+def mlp_loss(params, X, Y):
+    p = np.ascontiguousarray(params, np.float64)
+    X = np.ascontiguousarray(X, np.float32).reshape(-1, 5)
+    Y = np.ascontiguousarray(Y, np.float32).reshape(-1)
+    return lib().oracle_mlp_loss(p.ctypes.data, X.ctypes.data, Y.ctypes.data, X.shape[0])
+
+
+def mlp_predict(params, X):
+    p = np.ascontiguousarray(params, np.float64)
+    X = np.ascontiguousarray(X, np.float32).reshape(-1, 5)
+    return np.array([lib().oracle_mlp_predict(p.ctypes.data, X[i].ctypes.data)
+                     for i in range(X.shape[0])])

@@ -149,6 +149,31 @@ def test_plan_replay_graph(session, oracle):
     assert st["tiles"] == 12 and st["output_pixels"] == 12 * 96 * 96
 
 
+def test_every_lane_kernel_variant_and_autotune(session, oracle):
+    # one-shot and persistent (device work counter) lane kernels of every
+    # shape decode the same batch, twice in a row through a captured graph
+    streams, pix, _cls = synth.lossless_batch_with_pixels(40, size=96)
+    plan = S.TilePlan(session, streams, S.PixelFormat.RGB8)
+    names = S.TilePlan.lane_variants()
+    assert len(names) >= 2
+    for v in range(len(names)):
+        plan.lane_variant = v
+        for _ in range(2):
+            for s in plan.surfaces:
+                s.fill(0)
+            plan.decode(S.MODE_GRAPH)
+            session.synchronize()
+            for s, img in zip(plan.surfaces, pix):
+                assert np.array_equal(s.to_numpy(), img), names[v]
+    times = plan.autotune(reps=2)
+    assert set(times) == set(names) and all(t > 0 for t in times.values())
+    assert times[names[plan.lane_variant]] == min(times.values())
+    plan.decode(S.MODE_GRAPH)
+    session.synchronize()
+    assert all(np.array_equal(s.to_numpy(), img) for s, img in zip(plan.surfaces, pix))
+    plan.close()
+
+
 def test_mixed_plan_formats_and_scales(session, oracle):
     # one plan over tiles of different sizes, bit depths, formats and scales
     rng = np.random.default_rng(77)
