@@ -25,6 +25,105 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def _graph_ms(S, sess, plan, reps=10):
+    for _ in range(2):
+        plan.decode(S.MODE_GRAPH)
+    sess.synchronize()
+    a, b = S.HipEvent(), S.HipEvent()
+    a.record(sess)
+    for _ in range(reps):
+        plan.decode(S.MODE_GRAPH)
+    b.record(sess)
+    sess.synchronize()
+    return a.elapsed_ms(b) / reps
+
+
+def extra_workloads(S, synth, sess, args, c2_streams, c2_cls):
+    """The other decode workloads: the mixed-geometry batch, the 9/7 batch and
+    the serving loop under Poisson arrivals (reported beside the headline)."""
+    from signinum_amd.serving import Server, run_poisson
+    res = {}
+    # mixed sizes / bit depths / code-block sizes / levels / scales
+    t0 = time.time()
+    items = synth.mixed_batch(args.mixed_tiles)
+    plan = S.TilePlan(sess, [i[0] for i in items], [i[1] for i in items], [i[2] for i in items])
+    plan.autotune(reps=2)
+    ms = _graph_ms(S, sess, plan)
+    px = plan.stats()["output_pixels"]
+    for k in range(0, len(items), max(1, len(items) // 16)):
+        cs, fmt, scale, src = items[k]
+        if int(scale) == 1:
+            got = plan.surfaces[k].to_numpy()
+            want = src if src.ndim == 3 else src[..., None]
+            if not np.array_equal(got, want):
+                raise SystemExit(f"mixed batch tile {k} differs from its source")
+    res["mixed_batch"] = {
+        "workload": f"{len(items)} tiles of 240/256/512/1024^2, 75% RGB8 / 25% 12-bit gray, "
+                    "32/64 code-blocks, 3-6 levels, scale 1, 1/2, 1/4 (one mixed plan)",
+        "mpix_s": round(px / (ms * 1e-3) / 1e6, 1), "ms_per_step": round(ms, 3),
+        "output_mpix": round(px / 1e6, 2), "coded_mb": round(sum(len(i[0]) for i in items) / 1e6, 2)}
+    plan.close()
+    log(f"mixed batch: {res['mixed_batch']} ({time.time() - t0:.1f}s)")
+    # irreversible 9/7 + ICT
+    t0 = time.time()
+    lossy = synth.lossy_batch(args.lossy_tiles)
+    plan = S.TilePlan(sess, [c for c, _ in lossy], S.PixelFormat.RGB8)
+    plan.autotune(reps=2)
+    ms = _graph_ms(S, sess, plan)
+    px = plan.stats()["output_pixels"]
+    err = []
+    for k in range(0, len(lossy), max(1, len(lossy) // 8)):
+        d = plan.surfaces[k].to_numpy().astype(np.float64) - lossy[k][1]
+        err.append((np.abs(d).max(), np.mean(d * d)))
+    mse = float(np.mean([e[1] for e in err]))
+    res["irreversible_batch"] = {
+        "workload": f"{len(lossy)} x 512x512 RGB8 tissue, 9/7 + ICT, 5 levels, 64x64, "
+                    "base step 1/64, 2 guard bits (f32 lifting)",
+        "mpix_s": round(px / (ms * 1e-3) / 1e6, 1), "ms_per_step": round(ms, 3),
+        "max_abs_vs_source": int(max(e[0] for e in err)),
+        "psnr_vs_source_db": round(10 * np.log10(255.0 ** 2 / mse), 2) if mse > 0 else None}
+    plan.close()
+    log(f"irreversible batch: {res['irreversible_batch']} ({time.time() - t0:.1f}s)")
+    # serving: Poisson arrivals over a pool of the workloads' tiles
+    t0 = time.time()
+    pool = [(c2_streams[i], S.PixelFormat.RGB8, S.Downscale.NONE, 256 * 256)
+            for i in range(0, len(c2_streams), max(1, len(c2_streams) // 64))]
+    pool += [(c, f, sc, 0) for c, f, sc, _ in items[:64]]
+    pool += [(c, S.PixelFormat.RGB8, S.Downscale.NONE, 0) for c, _ in lossy[:16]]
+    srv = Server(window_us=2000, hbm_budget=64 << 30)
+    recs, wall = run_poisson(srv, pool, rate_per_s=args.serve_rate,
+                             duration_s=args.serve_seconds, seed=7, max_tiles=64)
+    srv.close()
+    lat = np.array([r["latency_ms"] for r in recs])
+    served_px = sum(r["pixels"] for r in recs)
+    # the plain per-tile decode of the same pool, one launch sequence per tile
+    plain_px, plain_ms = 0, 0.0
+    for c, f, sc, _ in pool[::4]:
+        p1 = S.TilePlan(sess, [c], f, sc)
+        a, b = S.HipEvent(), S.HipEvent()
+        p1.decode(S.MODE_PLAIN)
+        a.record(sess)
+        p1.decode(S.MODE_PLAIN)
+        b.record(sess)
+        sess.synchronize()
+        plain_ms += a.elapsed_ms(b)
+        plain_px += p1.stats()["output_pixels"]
+        p1.close()
+    res["serving"] = {
+        "workload": f"open-loop Poisson arrivals at {args.serve_rate:g} requests/s for "
+                    f"{args.serve_seconds:g} s, 1-64 tiles per request from a pool of "
+                    f"{len(pool)} tiles (lossless 256^2 batch, mixed geometry, 9/7), 2 ms "
+                    "window, 64 GiB device budget, outputs kept on the device",
+        "requests": len(recs), "failed": int(sum(r["status"] != 0 for r in recs)),
+        "p50_ms": round(float(np.percentile(lat, 50)), 3),
+        "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "max_ms": round(float(lat.max()), 3),
+        "sustained_mpix_s": round(served_px / wall / 1e6, 1),
+        "plain_per_tile_mpix_s": round(plain_px / (plain_ms * 1e-3) / 1e6, 1)}
+    log(f"serving: {res['serving']} ({time.time() - t0:.1f}s)")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -37,6 +136,11 @@ def main():
     ap.add_argument("--cpu-sample-1t", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--mixed-tiles", type=int, default=512)
+    ap.add_argument("--lossy-tiles", type=int, default=256)
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--serve-rate", type=float, default=400.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build decodes on a single MI355X; "
@@ -171,6 +275,10 @@ def main():
         log(f"cpu baseline: {cpu['value']} MPix/s on {workers} threads, "
             f"{cpu['single_thread_mpix_s']} MPix/s on one")
 
+    extras = {}
+    if not args.no_extras:
+        extras = extra_workloads(S, synth, sess, args, streams, cls)
+
     out = {
         "metric": "decoded MPix/s, 4096x256^2 RGB8 HTJ2K batch vs plain GPU; p50/p99 latency",
         "value": round(mpix, 1),
@@ -203,6 +311,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    out.update(extras)
     print(json.dumps(out), flush=True)
     plan.close()
     sess.close()

@@ -24,8 +24,6 @@
 
 namespace sgn {
 
-constexpr int kVlcSlots = 4;     // VLC ring (64 B)
-constexpr int kMelSlots = 2;     // MEL ring (32 B)
 constexpr int kRowsMaxQ = 34;    // quads per row of a 64-wide block, plus 2
 
 // Synchronous 8-byte load for a stream that ran past its staged range.  The
@@ -98,9 +96,10 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
 
 // Decodes the 64 code-blocks jobs[first .. first + 64) (lane l takes
 // first + l) with the calling wave.  WAVES waves share the work-group's
-// tables; SEG quad pairs are decoded between refills of the MagSgn ring of
-// MS 16-byte slots.  Must be entered by all 64 lanes of the wave.
-template <int WAVES, int SEG, int MS>
+// tables; SEG quad pairs are decoded between refills of the rings of MS
+// (MagSgn), VS (VLC) and LS (MEL) 16-byte slots per lane.  Must be entered by
+// all 64 lanes of the wave.
+template <int WAVES, int SEG, int MS, int VS, int LS>
 __device__ __forceinline__ void rows_wave(const CbJob *__restrict__ jobs, uint32_t n,
                                           uint32_t first, const uint8_t *__restrict__ arena,
                                           int32_t *__restrict__ coef,
@@ -108,7 +107,7 @@ __device__ __forceinline__ void rows_wave(const CbJob *__restrict__ jobs, uint32
                                           uint8_t (*s_rho)[WAVES * 64],
                                           uint8_t (*s_e)[WAVES * 64], uint4 *s_ms, uint4 *s_vlc,
                                           uint4 *s_mel) {
-    constexpr int kSegPairs = SEG, kMsSlots = MS;
+    constexpr int kSegPairs = SEG, kMsSlots = MS, kVlcSlots = VS, kMelSlots = LS;
     const uint16_t *vlc0 = s_tab, *vlc1 = s_tab + 1024, *uvlc0 = s_tab + 2048,
                    *uvlc1 = s_tab + 2368;
     const int tid = threadIdx.x, wv = tid >> 6;
@@ -395,7 +394,7 @@ __device__ __forceinline__ void rows_wave(const CbJob *__restrict__ jobs, uint32
 // waves take the next 64 code-blocks (in the host's cost order) from the
 // device work counter *queue until none are left; otherwise one wave per 64
 // code-blocks.  *queue must be zero at launch.
-template <int WAVES, int SEG, int MS, bool PERSIST>
+template <int WAVES, int SEG, int MS, int VS, int LS, bool PERSIST>
 __global__ void __launch_bounds__(WAVES * 64)
 sgn_ht_rows(const CbJob *__restrict__ jobs, uint32_t n, const uint8_t *__restrict__ arena,
             int32_t *__restrict__ coef, const uint16_t *__restrict__ tables,
@@ -404,13 +403,13 @@ sgn_ht_rows(const CbJob *__restrict__ jobs, uint32_t n, const uint8_t *__restric
     __shared__ uint8_t s_rho[kRowsMaxQ][WAVES * 64];  // significance nibble of the quad above
     __shared__ uint8_t s_e[kRowsMaxQ][WAVES * 64];    // floor(log2(v_n)) per column pair
     __shared__ uint4 s_ms[WAVES * 64 * MS];
-    __shared__ uint4 s_vlc[WAVES * 64 * kVlcSlots];
-    __shared__ uint4 s_mel[WAVES * 64 * kMelSlots];
+    __shared__ uint4 s_vlc[WAVES * 64 * VS];
+    __shared__ uint4 s_mel[WAVES * 64 * LS];
     for (int t = threadIdx.x; t < 2624; t += WAVES * 64) s_tab[t] = tables[t];
     __syncthreads();
     if constexpr (!PERSIST) {
         const uint32_t first = (blockIdx.x * WAVES + (threadIdx.x >> 6)) * 64;
-        rows_wave<WAVES, SEG, MS>(jobs, n, first, arena, coef, status, s_tab, s_rho, s_e, s_ms,
+        rows_wave<WAVES, SEG, MS, VS, LS>(jobs, n, first, arena, coef, status, s_tab, s_rho, s_e, s_ms,
                                   s_vlc, s_mel);
     } else {
         for (;;) {
@@ -418,7 +417,7 @@ sgn_ht_rows(const CbJob *__restrict__ jobs, uint32_t n, const uint8_t *__restric
             if ((threadIdx.x & 63) == 0) first = atomicAdd(queue, 64u);
             first = __shfl(first, 0);
             if (first >= n) break;
-            rows_wave<WAVES, SEG, MS>(jobs, n, first, arena, coef, status, s_tab, s_rho, s_e,
+            rows_wave<WAVES, SEG, MS, VS, LS>(jobs, n, first, arena, coef, status, s_tab, s_rho, s_e,
                                       s_ms, s_vlc, s_mel);
         }
     }
@@ -431,18 +430,18 @@ extern "C" __global__ void __launch_bounds__(256)
 sgn_ht_rows_timed(const CbJob *__restrict__ jobs, uint32_t n, const uint8_t *__restrict__ arena,
                   int32_t *__restrict__ coef, const uint16_t *__restrict__ tables,
                   uint8_t *__restrict__ status, uint64_t *__restrict__ cycles) {
-    constexpr int W = 4, S = 8, M = 8;
+    constexpr int W = 4, S = 8, M = 8, VS = 4, LS = 2;
     __shared__ uint16_t s_tab[1024 + 1024 + 320 + 256];
     __shared__ uint8_t s_rho[kRowsMaxQ][W * 64];
     __shared__ uint8_t s_e[kRowsMaxQ][W * 64];
     __shared__ uint4 s_ms[W * 64 * M];
-    __shared__ uint4 s_vlc[W * 64 * kVlcSlots];
-    __shared__ uint4 s_mel[W * 64 * kMelSlots];
+    __shared__ uint4 s_vlc[W * 64 * VS];
+    __shared__ uint4 s_mel[W * 64 * LS];
     for (int t = threadIdx.x; t < 2624; t += W * 64) s_tab[t] = tables[t];
     __syncthreads();
     const uint32_t wave = blockIdx.x * W + (threadIdx.x >> 6);
     const uint64_t t0 = clock64();
-    rows_wave<W, S, M>(jobs, n, wave * 64, arena, coef, status, s_tab, s_rho, s_e, s_ms, s_vlc,
+    rows_wave<W, S, M, VS, LS>(jobs, n, wave * 64, arena, coef, status, s_tab, s_rho, s_e, s_ms, s_vlc,
                        s_mel);
     const uint64_t t1 = clock64();
     if ((threadIdx.x & 63) == 0 && wave * 64 < n) cycles[wave] = t1 - t0;

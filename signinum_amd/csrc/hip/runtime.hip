@@ -52,12 +52,12 @@ namespace {
 using RowsLaunch = void (*)(hipStream_t, uint32_t, const CbJob *, uint32_t, const uint8_t *,
                             int32_t *, const uint16_t *, uint8_t *, uint32_t *);
 
-template <int W, int S, int M, bool PERSIST>
+template <int W, int S, int M, int VS, int LS, bool PERSIST>
 void rows_launch(hipStream_t st, uint32_t grid, const CbJob *jobs, uint32_t n,
                  const uint8_t *arena, int32_t *coef, const uint16_t *tables, uint8_t *status,
                  uint32_t *queue) {
-    hipLaunchKernelGGL((sgn_ht_rows<W, S, M, PERSIST>), dim3(grid), dim3(W * 64), 0, st, jobs, n,
-                       arena, coef, tables, status, queue);
+    hipLaunchKernelGGL((sgn_ht_rows<W, S, M, VS, LS, PERSIST>), dim3(grid), dim3(W * 64), 0, st,
+                       jobs, n, arena, coef, tables, status, queue);
 }
 
 struct RowsVariant {
@@ -68,18 +68,21 @@ struct RowsVariant {
     const void *kernel;
 };
 
+#define SGN_ROWS_VARIANT(name, W, S, M, VS, LS, Q)                                           \
+    {name, W, Q, rows_launch<W, S, M, VS, LS, Q>,                                          \
+     reinterpret_cast<const void *>(sgn_ht_rows<W, S, M, VS, LS, Q>)}
+
+// name: waves per work-group, quad pairs per segment, small rings (3 work-
+// groups per CU instead of 2), persistent queue
 const RowsVariant kRowsVariants[] = {
-    {"w4-seg8", 4, false, rows_launch<4, 8, 8, false>,
-     reinterpret_cast<const void *>(sgn_ht_rows<4, 8, 8, false>)},
-    {"w4-seg8-queue", 4, true, rows_launch<4, 8, 8, true>,
-     reinterpret_cast<const void *>(sgn_ht_rows<4, 8, 8, true>)},
-    {"w4-seg4-queue", 4, true, rows_launch<4, 4, 4, true>,
-     reinterpret_cast<const void *>(sgn_ht_rows<4, 4, 4, true>)},
-    {"w2-seg16-queue", 2, true, rows_launch<2, 16, 16, true>,
-     reinterpret_cast<const void *>(sgn_ht_rows<2, 16, 16, true>)},
-    {"w1-seg8-queue", 1, true, rows_launch<1, 8, 8, true>,
-     reinterpret_cast<const void *>(sgn_ht_rows<1, 8, 8, true>)},
+    SGN_ROWS_VARIANT("w4-seg8", 4, 8, 8, 4, 2, false),
+    SGN_ROWS_VARIANT("w4-seg8-queue", 4, 8, 8, 4, 2, true),
+    SGN_ROWS_VARIANT("w4-seg4-queue", 4, 4, 4, 4, 2, true),
+    SGN_ROWS_VARIANT("w4-seg4-small-queue", 4, 4, 4, 2, 1, true),
+    SGN_ROWS_VARIANT("w2-seg16-queue", 2, 16, 16, 4, 2, true),
+    SGN_ROWS_VARIANT("w1-seg8-queue", 1, 8, 8, 4, 2, true),
 };
+#undef SGN_ROWS_VARIANT
 constexpr int kRowsVariantCount = int(sizeof(kRowsVariants) / sizeof(kRowsVariants[0]));
 
 }  // namespace

@@ -122,3 +122,46 @@ def lossless_batch_with_pixels(n, size=256, seed=SEED_BASE + 2, levels=5, worker
     with ThreadPoolExecutor(max_workers=workers) as ex:
         pairs = list(ex.map(make, range(n)))
     return [p[1] for p in pairs], [p[0] for p in pairs], cls
+
+
+def mixed_batch(n, seed=SEED_BASE + 3, workers=16):
+    """Tiles of square sizes 240/256/512/1024 (equal shares): 75% RGB8 with the
+    lossless-batch content classes, 25% 12-bit Poisson spot images in uint16;
+    code-blocks 32x32 or 64x64, 3-6 levels, decoded at scale 1, 1/2 or 1/4.
+    Returns [(codestream, PixelFormat, Downscale, source pixels)]."""
+    from . import encode_j2k_ht, PixelFormat, Downscale
+    sizes = (240, 256, 512, 1024)
+
+    def make(i):
+        rng = tile_rng(seed, i)
+        size = sizes[i % 4]
+        cb = 5 if rng.random() < 0.5 else 6
+        levels = int(rng.integers(3, 7))
+        scale = (Downscale.NONE, Downscale.HALF, Downscale.QUARTER)[int(rng.integers(0, 3))]
+        if rng.random() < 0.75:
+            img = batch_tile(seed, i, int(rng.integers(0, 3)), size)
+            cs = encode_j2k_ht(img, bit_depth=8, levels=levels, cb_width_log2=cb,
+                               cb_height_log2=cb, reversible=1, use_mct=1)
+            return cs, PixelFormat.RGB8, scale, img
+        img = spots_gray12(rng, size, size)
+        cs = encode_j2k_ht(img, bit_depth=12, levels=levels, cb_width_log2=cb, cb_height_log2=cb,
+                           reversible=1, use_mct=0)
+        return cs, PixelFormat.GRAY16, scale, img
+
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        return list(ex.map(make, range(n)))
+
+
+def lossy_batch(n, size=512, seed=SEED_BASE + 4, workers=16):
+    """Tissue-like RGB8 tiles encoded with 9/7, ICT, 5 levels, 64x64
+    code-blocks, base step 1/64 scaled by the sub-band gains, 2 guard bits.
+    Returns [(codestream, source pixels)]."""
+    from . import encode_j2k_ht
+
+    def make(i):
+        img = tissue_rgb(tile_rng(seed, i), size, size)
+        return encode_j2k_ht(img, bit_depth=8, levels=5, reversible=0, use_mct=1,
+                             base_step=1.0 / 64, guard_bits=2), img
+
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        return list(ex.map(make, range(n)))

@@ -117,6 +117,24 @@ def test_irreversible_within_one_lsb(session, oracle, size):
     assert err <= 2 and _psnr(out, f64_u8) > 45
 
 
+def test_irreversible_batch_error_vs_f64(session, oracle):
+    """The 9/7 batch workload (512x512 tissue, ICT, base step 1/64, 2 guard
+    bits): every tile within 1 LSB of the oracle's f32 decode; max-abs and
+    PSNR against the f64 restatement reported (tolerance: 2 LSB, 45 dB)."""
+    lossy = synth.lossy_batch(6)
+    surfs = S.decode_tiles_to_device(session, [c for c, _ in lossy], S.PixelFormat.RGB8,
+                                     mode=S.MODE_BATCHED)
+    worst, psnr = 0, []
+    for (cs, _src), s in zip(lossy, surfs):
+        out = s.to_numpy().astype(np.int64)
+        assert np.abs(out - O.decode(cs, 1).astype(np.int64)).max() <= 1
+        f64 = np.clip(np.round(O.decode_planes(cs, use_f64=True)), 0, 255).transpose(1, 2, 0)
+        worst = max(worst, int(np.abs(out - f64).max()))
+        psnr.append(_psnr(out, f64))
+    print(f"9/7 batch vs f64 oracle: max-abs {worst}, min PSNR {min(psnr):.2f} dB")
+    assert worst <= 2 and min(psnr) > 45
+
+
 def test_irreversible_fixture_reduced(session, oracle):
     cs = open(f"{GOLDEN}/openhtj2k_ds0_ht_09_b11.j2k", "rb").read()
     for den in (2, 4):

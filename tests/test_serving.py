@@ -89,7 +89,7 @@ def test_corrupt_tile_fails_only_its_request(oracle):
 
 def test_small_budget_splits_batches(oracle):
     pool = _pool()
-    srv = Server(window_us=0, max_inflight=2, hbm_budget=24 << 20)
+    srv = Server(window_us=0, max_inflight=2, hbm_budget=8 << 20)
     ids = [srv.submit([pool[i % len(pool)] for i in range(k, k + 6)]) for k in range(10)]
     recs = srv.wait(ids)
     assert all(r["status"] == 0 for r in recs.values())
