@@ -140,7 +140,7 @@ def main():
     ap.add_argument("--mixed-tiles", type=int, default=512)
     ap.add_argument("--lossy-tiles", type=int, default=256)
     ap.add_argument("--serve-seconds", type=float, default=2.0)
-    ap.add_argument("--serve-rate", type=float, default=400.0)
+    ap.add_argument("--serve-rate", type=float, default=150.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build decodes on a single MI355X; "
@@ -220,14 +220,16 @@ def main():
     # ---- roofline of the dominant kernel ----
     dom = max(stage_ms, key=stage_ms.get)
     samples = pixels * 3
+    es = 2 if st["narrow_samples"] else 4  # bytes per code-block sample in HBM
     alg_bytes = {
-        # compressed bytes read + int32 coefficients written
-        "blocks": st["coded_bytes"] + 4 * samples,
-        # every band sample of the intermediate levels read once (4 B) and the
-        # intermediate low-pass outputs written once
-        "levels": 4 * samples // 4 + 4 * samples // 16,
-        # final level: its four inputs read (4 B) and RGB8 written
-        "final": 4 * samples + samples,
+        # compressed bytes read + code-block samples written
+        "blocks": st["coded_bytes"] + es * samples,
+        # every band sample of the intermediate levels read once and the
+        # intermediate low-pass outputs (int32) written once
+        "levels": es * samples // 4 + 4 * samples // 16,
+        # final level: its low-pass input (int32) and three detail bands read,
+        # RGB8 written
+        "final": 4 * samples // 4 + es * 3 * samples // 4 + samples,
     }
     achieved = alg_bytes[dom] / (stage_ms[dom] * 1e-3) / 1e9
     kernel_name = {"blocks": "sgn_ht_rows", "levels": "sgn_idwt_level", "final": "sgn_final"}[dom]

@@ -205,6 +205,8 @@ typedef struct {
     uint64_t output_pixels;
     uint64_t coefficient_bytes;/* device coefficient arena */
     uint32_t launches_batched; /* kernel launches per batched decode */
+    uint32_t narrow_samples;   /* 1: code-block samples stored as int16 (every band has at
+                                  most 15 magnitude bit-planes); 0: int32 */
 } sgn_j2k_plan_stats;
 int sgn_j2k_plan_get_stats(const sgn_j2k_plan *plan, sgn_j2k_plan_stats *stats);
 /* Output geometry of tile i: width, height, bytes per row (unpadded). */

@@ -73,7 +73,7 @@ class PlanStats(ctypes.Structure):
     _fields_ = [("tiles", ctypes.c_uint64), ("code_blocks", ctypes.c_uint64),
                 ("coded_bytes", ctypes.c_uint64), ("output_bytes", ctypes.c_uint64),
                 ("output_pixels", ctypes.c_uint64), ("coefficient_bytes", ctypes.c_uint64),
-                ("launches_batched", ctypes.c_uint32)]
+                ("launches_batched", ctypes.c_uint32), ("narrow_samples", ctypes.c_uint32)]
 
 
 class HtCodeBlockJob(ctypes.Structure):

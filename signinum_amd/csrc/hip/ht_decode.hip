@@ -366,7 +366,7 @@ __device__ uint32_t wave_magref(const uint8_t *ref, uint32_t lref, const uint16_
 extern "C" __global__ void __launch_bounds__(64)
 sgn_ht_block_wave(const CbJob *__restrict__ jobs, uint32_t n, const uint8_t *__restrict__ arena,
                   int32_t *__restrict__ coef, const uint16_t *__restrict__ tables,
-                  uint8_t *__restrict__ status, uint32_t raw_out) {
+                  uint8_t *__restrict__ status, uint32_t raw_out, uint32_t narrow) {
     __shared__ uint16_t s_tab[2624];
     __shared__ uint32_t s_blk[kWaveMaxSamples];
     __shared__ uint16_t s_sym[kWaveSymWords];
@@ -427,10 +427,15 @@ sgn_ht_block_wave(const CbJob *__restrict__ jobs, uint32_t n, const uint8_t *__r
     __syncthreads();
     const bool bad = s_fail != 0;
     int32_t *out = coef + j.dst;
+    int16_t *out16 = reinterpret_cast<int16_t *>(coef) + j.dst;  // 16-bit coefficient arena
     for (uint32_t k = lane; k < w * h; k += 64) {
         const uint32_t y = k / w, x = k - y * w;
         uint32_t v = (bad || k >= (uint32_t)kWaveMaxSamples) ? 0u : s_blk[k];
-        out[y * j.stride + x] = raw_out ? (int32_t)v : to_i32(v, j.kmax);
+        const int32_t c = raw_out ? (int32_t)v : to_i32(v, j.kmax);
+        if (narrow)
+            out16[y * j.stride + x] = (int16_t)c;
+        else
+            out[y * j.stride + x] = c;
     }
 }
 

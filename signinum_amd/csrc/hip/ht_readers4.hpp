@@ -182,61 +182,52 @@ __device__ __forceinline__ void vlc_fill(VlcBits &b, Rev4 &r, const Src &src) {
 
 // MEL: MSB-first bits, 7 bits after 0xFF, the segment's last byte ORed with
 // 0x0F and 0xFF past the end (ht_block_decode.rs:783-879).
+// MEL bits in a 64-bit buffer, next bit in bit 63.  A byte after 0xFF
+// carries 7 bits (its stuffed MSB is dropped); the segment's last byte is
+// ORed with 0x0F and bytes past it read as 0xFF (ht_block_decode.rs:860-929).
 struct Mel {
-    uint32_t buf, buf_n;  // fetched bytes not yet used, next in bits 0..7
-    int32_t left;         // segment bytes not yet used
-    uint32_t unstuff, cur, bits_left;
-    int32_t k;
-    uint32_t num_runs;
-    uint64_t runs;
+    uint64_t tmp;
+    uint32_t bits;     // valid bits in tmp
+    int32_t left;      // segment bytes not yet appended
+    uint32_t unstuff;  // the last appended byte was 0xFF
+    int32_t k;         // run-length state, 0..12
 };
 
+__device__ __forceinline__ Mel mel_make(uint32_t len) { return Mel{0, 0, (int32_t)len, 0, 0}; }
+
+// Appends the next four segment bytes.
 template <class Src>
-__device__ __forceinline__ uint32_t mel_bit4(Mel &m, Fwd4 &f, const Src &src) {
-    if (m.bits_left == 0) {
-        if (m.buf_n == 0) {
-            m.buf = fwd4_get(f, src);
-            m.buf_n = 4;
-        }
-        uint32_t b = m.buf & 0xFF;
-        m.buf >>= 8;
-        m.buf_n--;
-        if (m.left <= 1) b |= 0x0F;  // last byte, and 0xFF pad beyond it
-        m.left--;
-        m.cur = b;
-        m.bits_left = 8 - m.unstuff;
-        m.unstuff = b == 0xFF;
+__device__ __forceinline__ void mel_fill(Mel &m, Fwd4 &f, const Src &src) {
+    const uint32_t w = fwd4_get(f, src);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        uint32_t c = (w >> (8 * i)) & 0xFF;
+        if (m.left - i <= 1) c |= 0x0F;
+        const uint32_t nb = 8 - m.unstuff;
+        m.tmp |= (uint64_t)(c & ((1u << nb) - 1)) << (64 - m.bits - nb);
+        m.bits += nb;
+        m.unstuff = c == 0xFF;
     }
-    m.bits_left--;
-    return (m.cur >> m.bits_left) & 1u;
+    m.left -= 4;
 }
 
+// Next MEL run: the number of all-insignificant quads before the next
+// quad that needs a symbol, times two, plus one when that quad follows
+// (ht_block_decode.rs:860-929).  A symbol is one bit (1: a full run of
+// 2^E quads, E from the state k) or a zero bit and E bits of run length.
 template <class Src>
 __device__ __forceinline__ int mel_run4(Mel &m, Fwd4 &f, const Src &src) {
-    if (m.num_runs == 0) {
-        while (m.num_runs < 8) {
-            const int eval = (m.k < 3) ? 0 : (m.k < 6) ? 1 : (m.k < 9) ? 2 : (m.k < 11) ? 3
-                                                                    : (m.k == 11) ? 4 : 5;
-            const uint32_t first = mel_bit4(m, f, src);
-            uint64_t run;
-            if (first) {
-                m.k = m.k + 1 > 12 ? 12 : m.k + 1;
-                run = (uint64_t)((1u << eval) - 1) << 1;
-            } else {
-                uint32_t v = 0;
-                for (int i = 0; i < eval; ++i) v = (v << 1) | mel_bit4(m, f, src);
-                m.k = m.k > 0 ? m.k - 1 : 0;
-                run = ((uint64_t)v << 1) | 1;
-            }
-            m.runs |= run << (m.num_runs * 7);
-            m.num_runs++;
-            if (eval == 5 && first == 0 && m.num_runs >= 8) break;
-        }
-    }
-    const int r = (int)(m.runs & 0x7F);
-    m.runs >>= 7;
-    m.num_runs--;
-    return r;
+    // E for k = 0..12, four bits each
+    constexpr uint64_t kExp = 0x5433222111000ull;
+    if (m.bits < 6) mel_fill(m, f, src);
+    const uint32_t e = (uint32_t)(kExp >> (4 * m.k)) & 0xF;
+    const bool first = (m.tmp >> 63) != 0;
+    const uint32_t v = e ? (uint32_t)((m.tmp << 1) >> (64 - e)) : 0u;
+    const uint32_t n = first ? 1u : 1u + e;
+    m.tmp <<= n;
+    m.bits -= n;
+    m.k = first ? (m.k < 12 ? m.k + 1 : 12) : (m.k > 0 ? m.k - 1 : 0);
+    return first ? (int)(((1u << e) - 1) << 1) : (int)((v << 1) | 1);
 }
 
 // One sample of the magnitude/sign phase (ht_block_decode.rs:1010-1032).
@@ -250,8 +241,8 @@ __device__ __forceinline__ uint32_t ms_value(MsBits &ms, uint32_t inf, uint32_t 
     const uint32_t m_n = uq - ((inf >> (12 + bit)) & 1u);
     ms.tmp >>= m_n;
     ms.bits -= m_n;
-    const uint32_t mask = m_n == 0 ? 0u : (m_n >= 32 ? 0xFFFFFFFFu : ((1u << m_n) - 1));
-    uint32_t v = ms_val & mask;
+    // m_n <= 31: callers reject U_q above Zblk + 2 <= 31
+    uint32_t v = ms_val & ((1u << m_n) - 1u);
     v |= ((inf >> (8 + bit)) & 1u) << m_n;
     v |= 1u;
     vn = v;

@@ -99,10 +99,10 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
 // tables; SEG quad pairs are decoded between refills of the rings of MS
 // (MagSgn), VS (VLC) and LS (MEL) 16-byte slots per lane.  Must be entered by
 // all 64 lanes of the wave.
-template <int WAVES, int SEG, int MS, int VS, int LS>
+template <int WAVES, int SEG, int MS, int VS, int LS, typename C>
 __device__ __forceinline__ void rows_wave(const CbJob *__restrict__ jobs, uint32_t n,
                                           uint32_t first, const uint8_t *__restrict__ arena,
-                                          int32_t *__restrict__ coef,
+                                          C *__restrict__ coef,
                                           uint8_t *__restrict__ status, const uint16_t *s_tab,
                                           uint8_t (*s_rho)[WAVES * 64],
                                           uint8_t (*s_e)[WAVES * 64], uint4 *s_ms, uint4 *s_vlc,
@@ -115,7 +115,7 @@ __device__ __forceinline__ void rows_wave(const CbJob *__restrict__ jobs, uint32
     const bool valid = i < n;
     CbJob j{};
     if (valid) j = jobs[i];
-    int32_t *out = coef + j.dst;
+    C *out = coef + j.dst;
     const uint32_t w = j.w, h = j.h, stride = j.stride;
     const bool vec_store = ((j.dst | stride) & 3u) == 0;
     uint8_t st = kCbOk;
@@ -194,7 +194,7 @@ __device__ __forceinline__ void rows_wave(const CbJob *__restrict__ jobs, uint32
     rev4_init(rvlc, vlc_start, vlc_len, RingSrc<kVlcSlots>{ring_vlc, have_vlc, arena});
     MsBits ms{0, 0, 0};
     VlcBits vlc;
-    Mel mel{0, 0, (int32_t)mel_len, 0, 0, 0, 0, 0, 0};
+    Mel mel = mel_make(mel_len);
     {
         const uint32_t b0 = run_block ? (uint32_t)arena[j.src + lcup - 2] : 0;
         vlc.tmp = b0 >> 4;
@@ -242,99 +242,60 @@ __device__ __forceinline__ void rows_wave(const CbJob *__restrict__ jobs, uint32
         const uint32_t pa = seg * kSegPairs, pb = min(pa + kSegPairs, npairs);
         for (uint32_t pr = pa; pr < pb; ++pr) {
             const uint32_t x = 4 * pr, qj = 2 * pr;
-            // ---- cleanup symbols of the quad pair ----
-            const uint32_t pt0 = (uint32_t)s_rho[qj][tid] << 4;
-            const uint32_t pt1 = (uint32_t)s_rho[qj + 1][tid] << 4;
-            const uint32_t pt2 = (uint32_t)s_rho[qj + 2][tid] << 4;
-            uint32_t t0, t1, u0, u1;
+            // ---- cleanup symbols of the quad pair (one quad per iteration) ----
             // one fill per pair: a pair takes at most 7 + 7 + 3 + 10 bits
             vlc_fill(vlc, rvlc, RingSrc<kVlcSlots>{ring_vlc, have_vlc, arena});
-            if (first_row) {
-                t0 = vlc0[c_q + (vlc.tmp & 0x7F)];
-                if (c_q == 0) {
+            const uint16_t *vtab = first_row ? vlc0 : vlc1;
+            uint32_t t0 = 0, t1 = 0;
+#pragma unroll 1
+            for (uint32_t k = 0; k < 2; ++k) {
+                const uint32_t q = qj + k;
+                uint32_t above = 0;
+                if (!first_row) {
+                    above = (uint32_t)s_rho[q][tid] << 4;
+                    c_q |= ((above & 0xA0) << 2) | (((uint32_t)s_rho[q + 1][tid] << 4 & 0x20) << 4);
+                }
+                uint32_t tq = vtab[c_q + (vlc.tmp & 0x7F)];
+                const bool present = x + 2 * k < w;
+                if (c_q == 0 && present) {
                     run -= 2;
-                    t0 = (run == -1) ? t0 : 0;
+                    tq = (run == -1) ? tq : 0;
                     if (run < 0) run = mel_run4(mel, fmel, RingSrc<kMelSlots>{ring_mel, have_mel, arena});
                 }
-                c_q = ((t0 & 0x10) << 3) | ((t0 & 0xE0) << 2);
-                vlc.tmp >>= t0 & 7;
-                vlc.bits -= t0 & 7;
-                t1 = vlc0[c_q + (vlc.tmp & 0x7F)];
-                if (c_q == 0 && x + 2 < w) {
-                    run -= 2;
-                    t1 = (run == -1) ? t1 : 0;
-                    if (run < 0) run = mel_run4(mel, fmel, RingSrc<kMelSlots>{ring_mel, have_mel, arena});
-                }
-                if (x + 2 >= w) t1 = 0;
-                c_q = ((t1 & 0x10) << 3) | ((t1 & 0xE0) << 2);
-                vlc.tmp >>= t1 & 7;
-                vlc.bits -= t1 & 7;
-                uint32_t mode = ((t0 & 8) << 3) | ((t1 & 8) << 4);
-                if (mode == 0xC0) {
-                    run -= 2;
-                    if (run == -1) mode += 0x40;
-                    if (run < 0) run = mel_run4(mel, fmel, RingSrc<kMelSlots>{ring_mel, have_mel, arena});
-                }
-                uint32_t e = uvlc0[mode + (vlc.tmp & 0x3F)];
-                vlc.tmp >>= e & 7;
-                vlc.bits -= e & 7;
-                e >>= 3;
-                uint32_t len = e & 0xF;
-                const uint32_t tmp = (uint32_t)vlc.tmp & ((1u << len) - 1);
-                vlc.tmp >>= len;
-                vlc.bits -= len;
-                e >>= 4;
-                len = e & 7;
-                e >>= 3;
-                u0 = 1 + (e & 7) + (tmp & ~(0xFFu << len));
-                u1 = 1 + (e >> 3) + (tmp >> len);
-            } else {
-                c_q |= (pt0 & 0xA0) << 2;
-                c_q |= (pt1 & 0x20) << 4;
-                t0 = vlc1[c_q + (vlc.tmp & 0x7F)];
-                if (c_q == 0) {
-                    run -= 2;
-                    t0 = (run == -1) ? t0 : 0;
-                    if (run < 0) run = mel_run4(mel, fmel, RingSrc<kMelSlots>{ring_mel, have_mel, arena});
-                }
-                c_q = ((t0 & 0x40) << 2) | ((t0 & 0x80) << 1);
-                c_q |= pt0 & 0x80;
-                c_q |= (pt1 & 0xA0) << 2;
-                c_q |= (pt2 & 0x20) << 4;
-                vlc.tmp >>= t0 & 7;
-                vlc.bits -= t0 & 7;
-                t1 = vlc1[c_q + (vlc.tmp & 0x7F)];
-                if (c_q == 0 && x + 2 < w) {
-                    run -= 2;
-                    t1 = (run == -1) ? t1 : 0;
-                    if (run < 0) run = mel_run4(mel, fmel, RingSrc<kMelSlots>{ring_mel, have_mel, arena});
-                }
-                if (x + 2 >= w) t1 = 0;
-                c_q = ((t1 & 0x40) << 2) | ((t1 & 0x80) << 1);
-                c_q |= pt1 & 0x80;
-                vlc.tmp >>= t1 & 7;
-                vlc.bits -= t1 & 7;
-                const uint32_t mode = ((t0 & 8) << 3) | ((t1 & 8) << 4);
-                uint32_t e = uvlc1[mode + (vlc.tmp & 0x3F)];
-                vlc.tmp >>= e & 7;
-                vlc.bits -= e & 7;
-                e >>= 3;
-                uint32_t len = e & 0xF;
-                const uint32_t tmp = (uint32_t)vlc.tmp & ((1u << len) - 1);
-                vlc.tmp >>= len;
-                vlc.bits -= len;
-                e >>= 4;
-                len = e & 7;
-                e >>= 3;
-                u0 = (e & 7) + (tmp & ~(0xFFu << len));
-                u1 = (e >> 3) + (tmp >> len);
+                if (!present) tq = 0;
+                c_q = first_row ? (((tq & 0x10) << 3) | ((tq & 0xE0) << 2))
+                                : (((tq & 0x40) << 2) | ((tq & 0x80) << 1) | (above & 0x80));
+                vlc.tmp >>= tq & 7;
+                vlc.bits -= tq & 7;
+                if (k == 0) t0 = tq;
+                else t1 = tq;
             }
+            uint32_t mode = ((t0 & 8) << 3) | ((t1 & 8) << 4);
+            if (first_row && mode == 0xC0) {
+                run -= 2;
+                if (run == -1) mode += 0x40;
+                if (run < 0) run = mel_run4(mel, fmel, RingSrc<kMelSlots>{ring_mel, have_mel, arena});
+            }
+            uint32_t e = (first_row ? uvlc0 : uvlc1)[mode + (vlc.tmp & 0x3F)];
+            vlc.tmp >>= e & 7;
+            vlc.bits -= e & 7;
+            e >>= 3;
+            uint32_t len = e & 0xF;
+            const uint32_t tmp = (uint32_t)vlc.tmp & ((1u << len) - 1);
+            vlc.tmp >>= len;
+            vlc.bits -= len;
+            e >>= 4;
+            len = e & 7;
+            e >>= 3;
+            const uint32_t bias = first_row ? 1u : 0u;
+            const uint32_t u0 = bias + (e & 7) + (tmp & ~(0xFFu << len));
+            const uint32_t u1 = bias + (e >> 3) + (tmp >> len);
             s_rho[qj][tid] = (uint8_t)((t0 >> 4) & 0xF);
             s_rho[qj + 1][tid] = (uint8_t)((t1 >> 4) & 0xF);
 
-            // ---- magnitude / sign of both quads ----
-            uint32_t r0[4] = {0, 0, 0, 0}, r1[4] = {0, 0, 0, 0};  // rows y, y+1 at x..x+3
-#pragma unroll
+            // ---- magnitude / sign, one quad per iteration; each quad's two
+            // columns go out as one 8-byte store per row ----
+#pragma unroll 1
             for (uint32_t k = 0; k < 2; ++k) {
                 const uint32_t xq = x + 2 * k;
                 if (xq >= w) break;
@@ -352,35 +313,39 @@ __device__ __forceinline__ void rows_wave(const CbJob *__restrict__ jobs, uint32
                     st = kCbBad;
                     break;
                 }
-                uint32_t v0, v1, v2, v3;
+                uint32_t v0, v1, v2 = 0, v3 = 0;
                 ms_fill(ms, fms, RingSrc<kMsSlots>{ring_ms, have_ms, arena});
                 const uint32_t a0 = ms_value(ms, inf, 0, uq, p, v0);
                 ms_fill(ms, fms, RingSrc<kMsSlots>{ring_ms, have_ms, arena});
                 const uint32_t a1 = ms_value(ms, inf, 1, uq, p, v1);
-                r0[2 * k] = (uint32_t)to_i32(a0, kmax);
-                r1[2 * k] = (uint32_t)to_i32(a1, kmax);
                 s_e[q][tid] = (uint8_t)max(prev_e3, flog2r(v1));
-                prev_e3 = 0;
-                if (xq + 1 >= w) break;
-                ms_fill(ms, fms, RingSrc<kMsSlots>{ring_ms, have_ms, arena});
-                const uint32_t a2 = ms_value(ms, inf, 2, uq, p, v2);
-                ms_fill(ms, fms, RingSrc<kMsSlots>{ring_ms, have_ms, arena});
-                const uint32_t a3 = ms_value(ms, inf, 3, uq, p, v3);
-                r0[2 * k + 1] = (uint32_t)to_i32(a2, kmax);
-                r1[2 * k + 1] = (uint32_t)to_i32(a3, kmax);
-                prev_e3 = flog2r(v3);
-            }
-            int32_t *o = out + y * stride + x;
-            if (vec_store && x + 4 <= w) {
-                *reinterpret_cast<uint4 *>(o) = make_uint4(r0[0], r0[1], r0[2], r0[3]);
-                if (second)
-                    *reinterpret_cast<uint4 *>(o + stride) = make_uint4(r1[0], r1[1], r1[2], r1[3]);
-            } else {
-#pragma unroll
-                for (uint32_t c = 0; c < 4; ++c) {
-                    if (x + c >= w) break;
-                    o[c] = (int32_t)r0[c];
-                    if (second) o[stride + c] = (int32_t)r1[c];
+                const bool two = xq + 1 < w;
+                uint32_t a2 = 0, a3 = 0;
+                if (two) {
+                    ms_fill(ms, fms, RingSrc<kMsSlots>{ring_ms, have_ms, arena});
+                    a2 = ms_value(ms, inf, 2, uq, p, v2);
+                    ms_fill(ms, fms, RingSrc<kMsSlots>{ring_ms, have_ms, arena});
+                    a3 = ms_value(ms, inf, 3, uq, p, v3);
+                }
+                prev_e3 = two ? flog2r(v3) : 0;
+                C *o = out + y * stride + xq;
+                const C c0 = (C)to_i32(a0, kmax), c1 = (C)to_i32(a1, kmax);
+                const C c2 = (C)to_i32(a2, kmax), c3 = (C)to_i32(a3, kmax);
+                if (vec_store && two) {
+                    if constexpr (sizeof(C) == 4) {
+                        *reinterpret_cast<int2 *>(o) = make_int2(c0, c2);
+                        if (second) *reinterpret_cast<int2 *>(o + stride) = make_int2(c1, c3);
+                    } else {
+                        *reinterpret_cast<short2 *>(o) = make_short2(c0, c2);
+                        if (second) *reinterpret_cast<short2 *>(o + stride) = make_short2(c1, c3);
+                    }
+                } else {
+                    o[0] = c0;
+                    if (second) o[stride] = c1;
+                    if (two) {
+                        o[1] = c2;
+                        if (second) o[stride + 1] = c3;
+                    }
                 }
             }
             if (st != kCbOk) break;
@@ -394,10 +359,10 @@ __device__ __forceinline__ void rows_wave(const CbJob *__restrict__ jobs, uint32
 // waves take the next 64 code-blocks (in the host's cost order) from the
 // device work counter *queue until none are left; otherwise one wave per 64
 // code-blocks.  *queue must be zero at launch.
-template <int WAVES, int SEG, int MS, int VS, int LS, bool PERSIST>
+template <int WAVES, int SEG, int MS, int VS, int LS, bool PERSIST, typename C>
 __global__ void __launch_bounds__(WAVES * 64)
 sgn_ht_rows(const CbJob *__restrict__ jobs, uint32_t n, const uint8_t *__restrict__ arena,
-            int32_t *__restrict__ coef, const uint16_t *__restrict__ tables,
+            C *__restrict__ coef, const uint16_t *__restrict__ tables,
             uint8_t *__restrict__ status, uint32_t *__restrict__ queue) {
     __shared__ uint16_t s_tab[1024 + 1024 + 320 + 256];
     __shared__ uint8_t s_rho[kRowsMaxQ][WAVES * 64];  // significance nibble of the quad above
@@ -409,7 +374,7 @@ sgn_ht_rows(const CbJob *__restrict__ jobs, uint32_t n, const uint8_t *__restric
     __syncthreads();
     if constexpr (!PERSIST) {
         const uint32_t first = (blockIdx.x * WAVES + (threadIdx.x >> 6)) * 64;
-        rows_wave<WAVES, SEG, MS, VS, LS>(jobs, n, first, arena, coef, status, s_tab, s_rho, s_e, s_ms,
+        rows_wave<WAVES, SEG, MS, VS, LS, C>(jobs, n, first, arena, coef, status, s_tab, s_rho, s_e, s_ms,
                                   s_vlc, s_mel);
     } else {
         for (;;) {
@@ -417,7 +382,7 @@ sgn_ht_rows(const CbJob *__restrict__ jobs, uint32_t n, const uint8_t *__restric
             if ((threadIdx.x & 63) == 0) first = atomicAdd(queue, 64u);
             first = __shfl(first, 0);
             if (first >= n) break;
-            rows_wave<WAVES, SEG, MS, VS, LS>(jobs, n, first, arena, coef, status, s_tab, s_rho, s_e,
+            rows_wave<WAVES, SEG, MS, VS, LS, C>(jobs, n, first, arena, coef, status, s_tab, s_rho, s_e,
                                       s_ms, s_vlc, s_mel);
         }
     }
@@ -426,9 +391,10 @@ sgn_ht_rows(const CbJob *__restrict__ jobs, uint32_t n, const uint8_t *__restric
 // Timing run for the decode-cost model: of each wave's 64 jobs only lane
 // 0's is a code-block (the others are empty), and the wave records the
 // shader-clock cycles it spent decoding it.
-extern "C" __global__ void __launch_bounds__(256)
+template <typename C>
+__global__ void __launch_bounds__(256)
 sgn_ht_rows_timed(const CbJob *__restrict__ jobs, uint32_t n, const uint8_t *__restrict__ arena,
-                  int32_t *__restrict__ coef, const uint16_t *__restrict__ tables,
+                  C *__restrict__ coef, const uint16_t *__restrict__ tables,
                   uint8_t *__restrict__ status, uint64_t *__restrict__ cycles) {
     constexpr int W = 4, S = 8, M = 8, VS = 4, LS = 2;
     __shared__ uint16_t s_tab[1024 + 1024 + 320 + 256];
@@ -441,7 +407,7 @@ sgn_ht_rows_timed(const CbJob *__restrict__ jobs, uint32_t n, const uint8_t *__r
     __syncthreads();
     const uint32_t wave = blockIdx.x * W + (threadIdx.x >> 6);
     const uint64_t t0 = clock64();
-    rows_wave<W, S, M, VS, LS>(jobs, n, wave * 64, arena, coef, status, s_tab, s_rho, s_e, s_ms, s_vlc,
+    rows_wave<W, S, M, VS, LS, C>(jobs, n, wave * 64, arena, coef, status, s_tab, s_rho, s_e, s_ms, s_vlc,
                        s_mel);
     const uint64_t t1 = clock64();
     if ((threadIdx.x & 63) == 0 && wave * 64 < n) cycles[wave] = t1 - t0;

@@ -64,7 +64,7 @@ struct BandLds {
 
 struct LevelView {
     const BandLds *bands;
-    uint32_t x0, y0, fin;
+    uint32_t x0, y0, fin, narrow;
     int w, h, levels;
 };
 
@@ -81,38 +81,12 @@ __device__ __forceinline__ LevelView level_view(const LevelJob *__restrict__ lj,
     v.x0 = lj->x0;
     v.y0 = lj->y0;
     v.fin = lj->float_in;
+    v.narrow = lj->narrow;
     v.w = (int)(lj->x1 - lj->x0);
     v.h = (int)(lj->y1 - lj->y0);
     v.levels = lj->levels;
     __syncthreads();
     return v;
-}
-
-// Value of level-local position (lx, ly) after extension.
-template <bool REV>
-__device__ __forceinline__ typename Sample<REV>::T load_level(const LevelView &L,
-                                                              const uint32_t *__restrict__ coef,
-                                                              int lx, int ly) {
-    const uint32_t mx = (uint32_t)reflect(lx, L.w), my = (uint32_t)reflect(ly, L.h);
-    uint32_t b, col, row;
-    if (L.levels == 0) {
-        // no synthesis: the low-pass band is the level itself
-        b = 0;
-        col = mx;
-        row = my;
-    } else {
-        const uint32_t ax = L.x0 + mx, ay = L.y0 + my;
-        b = (ax & 1u) | ((ay & 1u) << 1);
-        col = (ax & 1u) ? (ax >> 1) - (L.x0 >> 1) : (ax >> 1) - ((L.x0 + 1) >> 1);
-        row = (ay & 1u) ? (ay >> 1) - (L.y0 >> 1) : (ay >> 1) - ((L.y0 + 1) >> 1);
-    }
-    const uint32_t v = coef[L.bands->base[b] + row * L.bands->pitch[b] + col];
-    const bool isf = (L.fin >> b) & 1u;
-    if constexpr (REV) {
-        return isf ? (int32_t)__uint_as_float(v) : (int32_t)v;
-    } else {
-        return isf ? __uint_as_float(v) : (float)(int32_t)v * L.bands->delta[b];
-    }
 }
 
 // Lifting over the window.  HORIZ: lines are window rows (all nh of them),
@@ -315,22 +289,14 @@ __device__ __forceinline__ void lift_window(typename Sample<REV>::T *win, int nh
     if (kInPlace) __syncthreads();
 }
 
-// Synthesises block (bx, by) of one level into the window.  Result element
-// (x, y) of the block is at win[(y + H) * P + x + H].  rows: LDS scratch of
-// N words.
-template <bool REV, class Sink>
-__device__ __forceinline__ void synth_block(const LevelView &L, const uint32_t *__restrict__ coef,
-                                            int bx, int by, typename Sample<REV>::T *win,
-                                            uint32_t *rows, const Sink &sink, int &bw_out,
-                                            int &bh_out) {
-    using T = typename Sample<REV>::T;
-    constexpr int H = Win<REV>::H, N = Win<REV>::N, P = Win<REV>::P;
-    const int ox = bx * kBlk, oy = by * kBlk;
-    const int bw = min(kBlk, L.w - ox), bh = min(kBlk, L.h - oy);
-    const int nw = bw + 2 * H, nh = bh + 2 * H;
-    const int tid = threadIdx.x;
-    // window row -> (band row, row parity), after extension
-    for (int yy = tid; yy < nh; yy += kIdwtThreads) {
+// Window-row table of block row `by`: band row and row parity (bit 31) of
+// every window row after extension.  No barrier.
+template <bool REV>
+__device__ __forceinline__ void window_rows(const LevelView &L, int by, uint32_t *rows) {
+    constexpr int H = Win<REV>::H;
+    const int oy = by * kBlk;
+    const int nh = min(kBlk, L.h - oy) + 2 * H;
+    for (int yy = threadIdx.x; yy < nh; yy += kIdwtThreads) {
         const int my = reflect(oy - H + yy, L.h);
         uint32_t row = (uint32_t)my, par = 0;
         if (L.levels) {
@@ -340,62 +306,173 @@ __device__ __forceinline__ void synth_block(const LevelView &L, const uint32_t *
         }
         rows[yy] = row | (par << 31);
     }
-    __syncthreads();
-    // Each thread owns one window column and walks down it: the column's
-    // band, band column and the two band tables it can hit are set up once,
-    // and all its loads are issued before the first one is used.  Pass 0:
-    // columns 0..63, a quarter of the rows per wave; pass 1: the 2H halo
-    // columns, 256 / 2H threads per column.
-    constexpr int kRowsPer = (N + 3) / 4;  // rows per thread in pass 0
+}
+
+// The window's raw coefficient words held in registers between the loads
+// and their conversion into LDS, so that the loads of one component can be
+// in flight while the previous component is lifted.  Each thread owns one
+// window column and walks down it: pass 0 covers columns 0..63, a quarter
+// of the rows per wave; pass 1 the 2H halo columns, 256 / 2H threads each.
+template <bool REV>
+struct WinFetch {
+    static constexpr int kHalo = 2 * Win<REV>::H;
+    static constexpr int kRows0 = (Win<REV>::N + 3) / 4;
+    static constexpr int kStep1 = kIdwtThreads / kHalo;
+    static constexpr int kRows1 = (Win<REV>::N + kStep1 - 1) / kStep1;
+    uint32_t raw0[kRows0];
+    uint32_t raw1[kRows1];
+};
+
+// Column `xx` of the window: the band column and the two band slots (row
+// parity 0 and 1) it reads.
+struct WinColumn {
+    uint32_t base0, pitch0, base1, pitch1;
+    bool n0, n1;  // slot holds int16 samples
+};
+
+__device__ __forceinline__ WinColumn window_column(const LevelView &L, int ox, int xx, int H) {
+    const int mx = reflect(ox - H + xx, L.w);
+    uint32_t col = (uint32_t)mx, xpar = 0;
+    if (L.levels) {
+        const uint32_t ax = L.x0 + (uint32_t)mx;
+        xpar = ax & 1u;
+        col = xpar ? (ax >> 1) - (L.x0 >> 1) : (ax >> 1) - ((L.x0 + 1) >> 1);
+    }
+    WinColumn c;
+    c.base0 = L.bands->base[xpar] + col;
+    c.pitch0 = L.bands->pitch[xpar];
+    c.base1 = L.bands->base[xpar | 2] + col;
+    c.pitch1 = L.bands->pitch[xpar | 2];
+    c.n0 = (L.narrow >> xpar) & 1u;
+    c.n1 = (L.narrow >> (xpar | 2)) & 1u;
+    return c;
+}
+
+// Coefficient word at index `idx` of a slot: int16 samples (narrow) are
+// sign-extended; int32 and float words are returned as stored.
+__device__ __forceinline__ uint32_t coef_word(const uint32_t *__restrict__ coef, uint32_t idx,
+                                              bool narrow) {
+    return narrow ? (uint32_t)(int32_t)reinterpret_cast<const int16_t *>(coef)[idx] : coef[idx];
+}
+
+// Issues every load of the window of block (bx, by); `rows` holds the
+// block row's window-row table.
+template <bool REV>
+__device__ __forceinline__ void fetch_window(const LevelView &L, const uint32_t *__restrict__ coef,
+                                             int bx, int by, const uint32_t *rows,
+                                             WinFetch<REV> &f) {
+    constexpr int H = Win<REV>::H;
+    using F = WinFetch<REV>;
+    const int ox = bx * kBlk, oy = by * kBlk;
+    const int nw = min(kBlk, L.w - ox) + 2 * H, nh = min(kBlk, L.h - oy) + 2 * H;
+    const int tid = threadIdx.x;
+    {
+        const int xx = tid & 63, y0 = tid >> 6;
+        const WinColumn c = window_column(L, ox, xx < nw ? xx : 0, H);
 #pragma unroll
-    for (int pass = 0; pass < 2; ++pass) {
-        constexpr int kHalo = 2 * H;
-        const int xx = pass == 0 ? (tid & 63) : 64 + (tid % kHalo);
-        const int y0 = pass == 0 ? (tid >> 6) : tid / kHalo;
-        const int ystep = pass == 0 ? 4 : kIdwtThreads / kHalo;
-        if (xx >= nw || y0 >= nh) continue;
-        const int mx = reflect(ox - H + xx, L.w);
-        uint32_t col = (uint32_t)mx, xpar = 0;
-        if (L.levels) {
-            const uint32_t ax = L.x0 + (uint32_t)mx;
-            xpar = ax & 1u;
-            col = xpar ? (ax >> 1) - (L.x0 >> 1) : (ax >> 1) - ((L.x0 + 1) >> 1);
+        for (int k = 0; k < F::kRows0; ++k) {
+            const int yy = y0 + 4 * k;
+            f.raw0[k] = 0;
+            if (xx < nw && yy < nh) {
+                const uint32_t r = rows[yy];
+                const bool y1 = (r >> 31) != 0;
+                const uint32_t row = r & 0x7FFFFFFFu;
+                f.raw0[k] = coef_word(coef, (y1 ? c.base1 : c.base0) + row * (y1 ? c.pitch1 : c.pitch0),
+                                      y1 ? c.n1 : c.n0);
+            }
         }
-        const uint32_t base0 = L.bands->base[xpar] + col, pitch0 = L.bands->pitch[xpar];
-        const uint32_t base1 = L.bands->base[xpar | 2] + col, pitch1 = L.bands->pitch[xpar | 2];
-        const bool f0 = (L.fin >> xpar) & 1u, f1 = (L.fin >> (xpar | 2)) & 1u;
-        float d0 = 1.0f, d1 = 1.0f;
+    }
+    {
+        const int xx = 64 + tid % F::kHalo, y0 = tid / F::kHalo;
+        const WinColumn c = window_column(L, ox, xx < nw ? xx : 0, H);
+#pragma unroll
+        for (int k = 0; k < F::kRows1; ++k) {
+            const int yy = y0 + F::kStep1 * k;
+            f.raw1[k] = 0;
+            if (xx < nw && yy < nh) {
+                const uint32_t r = rows[yy];
+                const bool y1 = (r >> 31) != 0;
+                const uint32_t row = r & 0x7FFFFFFFu;
+                f.raw1[k] = coef_word(coef, (y1 ? c.base1 : c.base0) + row * (y1 ? c.pitch1 : c.pitch0),
+                                      y1 ? c.n1 : c.n0);
+            }
+        }
+    }
+}
+
+// Converts the fetched words (dequantising with the band step for 9/7) and
+// writes them into the window.  No barrier.
+template <bool REV>
+__device__ __forceinline__ void commit_window(const LevelView &L, int bx, int by,
+                                              const uint32_t *rows, const WinFetch<REV> &f,
+                                              typename Sample<REV>::T *win) {
+    using T = typename Sample<REV>::T;
+    using F = WinFetch<REV>;
+    constexpr int H = Win<REV>::H, P = Win<REV>::P;
+    const int ox = bx * kBlk, oy = by * kBlk;
+    const int nw = min(kBlk, L.w - ox) + 2 * H, nh = min(kBlk, L.h - oy) + 2 * H;
+    const int tid = threadIdx.x;
+    // per column: band-slot flags and steps for row parity 0 and 1
+    auto column = [&](int xx, bool &f0, bool &f1, float &d0, float &d1) {
+        uint32_t xpar = 0;
+        if (L.levels) xpar = (L.x0 + (uint32_t)reflect(ox - H + xx, L.w)) & 1u;
+        f0 = (L.fin >> xpar) & 1u;
+        f1 = (L.fin >> (xpar | 2)) & 1u;
+        d0 = d1 = 1.0f;
         if constexpr (!REV) {
             d0 = L.bands->delta[xpar];
             d1 = L.bands->delta[xpar | 2];
         }
-        uint32_t raw[kRowsPer];
+    };
+    auto put = [&](int xx, int yy, uint32_t v, bool f0, bool f1, float d0, float d1) {
+        const bool y1 = (rows[yy] >> 31) != 0;
+        const bool isf = y1 ? f1 : f0;
+        T val;
+        if constexpr (REV)
+            val = isf ? (int32_t)__uint_as_float(v) : (int32_t)v;
+        else
+            val = isf ? __uint_as_float(v) : (float)(int32_t)v * (y1 ? d1 : d0);
+        win[yy * P + xx] = val;
+    };
+    {
+        const int xx = tid & 63, y0 = tid >> 6;
+        if (xx < nw) {
+            bool f0, f1;
+            float d0, d1;
+            column(xx, f0, f1, d0, d1);
 #pragma unroll
-        for (int k = 0; k < kRowsPer; ++k) {
-            const int yy = y0 + k * ystep;
-            raw[k] = 0;
-            if (yy < nh) {
-                const uint32_t r = rows[yy];
-                const bool y1 = (r >> 31) != 0;
-                const uint32_t row = r & 0x7FFFFFFFu;
-                raw[k] = coef[(y1 ? base1 : base0) + row * (y1 ? pitch1 : pitch0)];
+            for (int k = 0; k < F::kRows0; ++k) {
+                const int yy = y0 + 4 * k;
+                if (yy < nh) put(xx, yy, f.raw0[k], f0, f1, d0, d1);
             }
         }
+    }
+    {
+        const int xx = 64 + tid % F::kHalo, y0 = tid / F::kHalo;
+        if (xx < nw) {
+            bool f0, f1;
+            float d0, d1;
+            column(xx, f0, f1, d0, d1);
 #pragma unroll
-        for (int k = 0; k < kRowsPer; ++k) {
-            const int yy = y0 + k * ystep;
-            if (yy >= nh) continue;
-            const bool y1 = (rows[yy] >> 31) != 0;
-            const bool isf = y1 ? f1 : f0;
-            const uint32_t v = raw[k];
-            T val;
-            if constexpr (REV)
-                val = isf ? (int32_t)__uint_as_float(v) : (int32_t)v;
-            else
-                val = isf ? __uint_as_float(v) : (float)(int32_t)v * (y1 ? d1 : d0);
-            win[yy * P + xx] = val;
+            for (int k = 0; k < F::kRows1; ++k) {
+                const int yy = y0 + F::kStep1 * k;
+                if (yy < nh) put(xx, yy, f.raw1[k], f0, f1, d0, d1);
+            }
         }
     }
+}
+
+// Lifts the committed window of block (bx, by) and hands the result to
+// `sink`.  Starts with a barrier (the window writes of every thread must be
+// visible); the sink's writes are complete for the calling thread only.
+template <bool REV, class Sink>
+__device__ __forceinline__ void lift_block(const LevelView &L, int bx, int by,
+                                           typename Sample<REV>::T *win, const Sink &sink,
+                                           int &bw_out, int &bh_out) {
+    constexpr int H = Win<REV>::H, P = Win<REV>::P;
+    const int ox = bx * kBlk, oy = by * kBlk;
+    const int bw = min(kBlk, L.w - ox), bh = min(kBlk, L.h - oy);
+    const int nw = bw + 2 * H, nh = bh + 2 * H;
     __syncthreads();
     bool in_window = true;  // result still in the window, not yet in the sink
     if (L.levels) {
@@ -412,13 +489,29 @@ __device__ __forceinline__ void synth_block(const LevelView &L, const uint32_t *
         }
     }
     if (in_window && !Sink::kInPlace) {
-        for (int k = tid; k < kBlk * bh; k += kIdwtThreads) {
+        for (int k = threadIdx.x; k < kBlk * bh; k += kIdwtThreads) {
             const int y = k / kBlk, x = k % kBlk;
             if (x < bw) sink(x, y, win[(y + H) * P + x + H]);
         }
     }
     bw_out = bw;
     bh_out = bh;
+}
+
+// Synthesises block (bx, by) of one level: window-row table, loads, window,
+// lifting.  Result element (x, y) of an in-place sink's block is at
+// win[(y + H) * P + x + H].  rows: LDS scratch of N words.
+template <bool REV, class Sink>
+__device__ __forceinline__ void synth_block(const LevelView &L, const uint32_t *__restrict__ coef,
+                                            int bx, int by, typename Sample<REV>::T *win,
+                                            uint32_t *rows, const Sink &sink, int &bw_out,
+                                            int &bh_out) {
+    window_rows<REV>(L, by, rows);
+    __syncthreads();
+    WinFetch<REV> f;
+    fetch_window<REV>(L, coef, bx, by, rows, f);
+    commit_window<REV>(L, bx, by, rows, f, win);
+    lift_block<REV>(L, bx, by, win, sink, bw_out, bh_out);
 }
 
 // Work-groups of a launch go to the 8 XCDs round-robin (work-group w runs on
@@ -518,7 +611,7 @@ __device__ __forceinline__ void final_body(const FinalJob *jobs, uint32_t njobs,
     constexpr int H = Win<REV>::H, N = Win<REV>::N, P = Win<REV>::P;
     using T = typename Sample<REV>::T;
     __shared__ T win[N * P];
-    __shared__ uint32_t rows[N];
+    __shared__ uint32_t rows[2][N];    // window-row tables, alternating by component
     __shared__ T res[2][kBlk * kBlk];  // all but the last component; the last stays in win
     const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x) + block_offset;
     const FinalJob *fj = jobs + find_job(jobs, njobs, blk);
@@ -539,14 +632,35 @@ __device__ __forceinline__ void final_body(const FinalJob *jobs, uint32_t njobs,
     F.same8 = fj->same8;
     const uint32_t lvl0 = fj->level[0], lvl1 = fj->level[1], lvl2 = fj->level[2];
     int bw = 0, bh = 0;
-    __shared__ BandLds sb;
+    __shared__ BandLds sb[2];
+    // Components are pipelined: the loads of component c + 1 are issued
+    // before component c is lifted and land in registers meanwhile.
+    auto view = [&](int c) {
+        return level_view(lv + (c == 0 ? lvl0 : c == 1 ? lvl1 : lvl2), &sb[c & 1]);
+    };
+    LevelView L = view(0);
+    window_rows<REV>(L, by, rows[0]);
+    __syncthreads();
+    WinFetch<REV> f;
+    fetch_window<REV>(L, coef, bx, by, rows[0], f);
     for (int c = 0; c < F.nc; ++c) {
-        // the previous component's last reads of sb ended at a barrier
-        const LevelView L = level_view(lv + (c == 0 ? lvl0 : c == 1 ? lvl1 : lvl2), &sb);
+        // the previous component's vertical pass may still read the window
+        if (c) __syncthreads();
+        commit_window<REV>(L, bx, by, rows[c & 1], f, win);
+        LevelView next = L;
+        if (c + 1 < F.nc) {
+            // sb and rows of parity (c + 1) were last read before the
+            // previous component's lifting began
+            next = view(c + 1);
+            window_rows<REV>(next, by, rows[(c + 1) & 1]);
+            __syncthreads();
+            fetch_window<REV>(next, coef, bx, by, rows[(c + 1) & 1], f);
+        }
         if (c + 1 < F.nc)
-            synth_block<REV>(L, coef, bx, by, win, rows, SinkPlane<REV>{res[c]}, bw, bh);
+            lift_block<REV>(L, bx, by, win, SinkPlane<REV>{res[c]}, bw, bh);
         else
-            synth_block<REV>(L, coef, bx, by, win, rows, SinkWin<REV>{win}, bw, bh);
+            lift_block<REV>(L, bx, by, win, SinkWin<REV>{win}, bw, bh);
+        L = next;
     }
     // Each thread converts and stores four horizontally adjacent pixels.
     const bool words = F.bps == 1 && ((reinterpret_cast<uintptr_t>(F.dst) | F.pitch) & 3) == 0;

@@ -9,7 +9,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
+#include <atomic>
+#include <condition_variable>
+#include <functional>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <exception>
 #include <memory>
@@ -41,7 +46,9 @@ struct sgn_hip_session {
     uint16_t *d_tables = nullptr;
     uint64_t dispatches = 0;
     int cus = 0;                    // compute units of the device
-    std::vector<uint32_t> resident; // per lane-kernel variant: resident work-groups (0: unknown)
+    // per lane-kernel variant and sample type (2 v + narrow): resident
+    // work-groups (0: unknown)
+    std::vector<uint32_t> resident;
 };
 
 namespace {
@@ -50,27 +57,32 @@ namespace {
 // pairs per segment (and MagSgn ring slots), and one-shot grid versus
 // persistent waves pulling 64-block chunks from a device work counter.
 using RowsLaunch = void (*)(hipStream_t, uint32_t, const CbJob *, uint32_t, const uint8_t *,
-                            int32_t *, const uint16_t *, uint8_t *, uint32_t *);
+                            void *, const uint16_t *, uint8_t *, uint32_t *);
 
-template <int W, int S, int M, int VS, int LS, bool PERSIST>
+template <int W, int S, int M, int VS, int LS, bool PERSIST, typename C>
 void rows_launch(hipStream_t st, uint32_t grid, const CbJob *jobs, uint32_t n,
-                 const uint8_t *arena, int32_t *coef, const uint16_t *tables, uint8_t *status,
+                 const uint8_t *arena, void *coef, const uint16_t *tables, uint8_t *status,
                  uint32_t *queue) {
-    hipLaunchKernelGGL((sgn_ht_rows<W, S, M, VS, LS, PERSIST>), dim3(grid), dim3(W * 64), 0, st,
-                       jobs, n, arena, coef, tables, status, queue);
+    hipLaunchKernelGGL((sgn_ht_rows<W, S, M, VS, LS, PERSIST, C>), dim3(grid), dim3(W * 64), 0,
+                       st, jobs, n, arena, static_cast<C *>(coef), tables, status, queue);
 }
 
+// launch[0] / kernel[0]: int32 samples; [1]: int16 samples (narrow plans)
 struct RowsVariant {
     const char *name;
     int waves;
     bool persist;
-    RowsLaunch launch;
-    const void *kernel;
+    RowsLaunch launch[2];
+    const void *kernel[2];
 };
 
 #define SGN_ROWS_VARIANT(name, W, S, M, VS, LS, Q)                                           \
-    {name, W, Q, rows_launch<W, S, M, VS, LS, Q>,                                          \
-     reinterpret_cast<const void *>(sgn_ht_rows<W, S, M, VS, LS, Q>)}
+    {name,                                                                                 \
+     W,                                                                                    \
+     Q,                                                                                    \
+     {rows_launch<W, S, M, VS, LS, Q, int32_t>, rows_launch<W, S, M, VS, LS, Q, int16_t>}, \
+     {reinterpret_cast<const void *>(sgn_ht_rows<W, S, M, VS, LS, Q, int32_t>),            \
+      reinterpret_cast<const void *>(sgn_ht_rows<W, S, M, VS, LS, Q, int16_t>)}}
 
 // name: waves per work-group, quad pairs per segment, small rings (3 work-
 // groups per CU instead of 2), persistent queue
@@ -145,6 +157,10 @@ struct sgn_j2k_plan {
     std::vector<size_t> tile_row_bytes;
     uint32_t depth_max = 0;
     uint64_t coef_elems = 0, arena_len = 0, out_bytes = 0, out_pixels = 0;
+    // Code-block samples stored as int16 (every band has at most 15
+    // magnitude bit-planes, so every sample fits); intermediate level
+    // outputs stay int32.  Each band keeps its int32-sized slot.
+    bool narrow = false;
     // device
     uint8_t *d_arena = nullptr;
     uint32_t *d_coef = nullptr;
@@ -159,42 +175,100 @@ struct sgn_j2k_plan {
     hipGraph_t graph = nullptr;
     uint32_t launches_batched = 0;
 
+    void *d_block = nullptr;  // the one allocation every device pointer above lives in
+
     ~sgn_j2k_plan() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
-        void *ptrs[] = {d_arena, d_coef, d_cb_lane, d_cb_lane_plain, d_cb_wave, d_lv[0],
-                        d_lv[1], d_final_lv, d_fin[0], d_fin[1], d_status_lane, d_status_wave,
-                        d_queue};
-        for (void *p : ptrs)
-            if (p) (void)hipFree(p);
+        if (d_block) (void)hipFree(d_block);
     }
 };
 
 namespace {
+
+// Host worker threads shared by every plan build (created once, on first
+// use): parse work of many small plans does not pay a thread start each.
+class WorkerPool {
+  public:
+    static WorkerPool &get() {
+        static WorkerPool pool;
+        return pool;
+    }
+    // Runs fn(i) for i in [0, n) on the pool and the calling thread.
+    void run(size_t n, const std::function<void(size_t)> &fn) {
+        if (n == 0) return;
+        if (n == 1 || threads_.empty()) {
+            for (size_t i = 0; i < n; ++i) fn(i);
+            return;
+        }
+        std::lock_guard<std::mutex> one(run_mu_);  // one batch of work at a time
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            fn_ = &fn;
+            n_ = n;
+            next_.store(0);
+            busy_ = threads_.size();
+            ++gen_;
+        }
+        cv_.notify_all();
+        drain();
+        std::unique_lock<std::mutex> g(mu_);
+        done_cv_.wait(g, [&] { return busy_ == 0; });
+        fn_ = nullptr;
+    }
+
+  private:
+    WorkerPool() {
+        const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+        const unsigned k = std::min(16u, hw) - 1;
+        for (unsigned t = 0; t < k; ++t) threads_.emplace_back([this] { loop(); });
+    }
+    ~WorkerPool() {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto &t : threads_) t.join();
+    }
+    void drain() {
+        for (size_t i = next_.fetch_add(1); i < n_; i = next_.fetch_add(1)) (*fn_)(i);
+    }
+    void loop() {
+        uint64_t seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> g(mu_);
+                cv_.wait(g, [&] { return stop_ || gen_ != seen; });
+                if (stop_) return;
+                seen = gen_;
+            }
+            drain();
+            std::lock_guard<std::mutex> g(mu_);
+            if (--busy_ == 0) done_cv_.notify_all();
+        }
+    }
+    std::vector<std::thread> threads_;
+    std::mutex mu_, run_mu_;
+    std::condition_variable cv_, done_cv_;
+    const std::function<void(size_t)> *fn_ = nullptr;
+    size_t n_ = 0, busy_ = 0;
+    std::atomic<size_t> next_{0};
+    uint64_t gen_ = 0;
+    bool stop_ = false;
+};
 
 Status build_plan(const sgn_tile_req *jobs, size_t n, sgn_j2k_plan &P, int32_t *tile_status) {
     P.ntiles = n;
     std::vector<LocalPlan> lps(n);
     std::vector<int32_t> st(n, 0);
     std::vector<std::string> errs(n);
-    const size_t nthreads =
-        std::max<size_t>(1, std::min<size_t>(n, std::min(16u, std::max(1u, std::thread::hardware_concurrency()))));
-    auto work = [&](size_t t0) {
-        for (size_t i = t0; i < n; i += nthreads) {
-            st[i] = build_local_plan(jobs[i].bytes, jobs[i].len, jobs[i].fmt,
-                                     jobs[i].denominator,
-                                     reinterpret_cast<uint64_t>(jobs[i].dst), jobs[i].pitch,
-                                     jobs[i].capacity, lps[i]);
-            if (st[i]) errs[i] = last_error();
-        }
-    };
-    if (nthreads == 1) {
-        work(0);
-    } else {
-        std::vector<std::thread> th;
-        for (size_t t = 0; t < nthreads; ++t) th.emplace_back(work, t);
-        for (auto &t : th) t.join();
-    }
+    WorkerPool::get().run(n, [&](size_t i) {
+        st[i] = build_local_plan(jobs[i].bytes, jobs[i].len, jobs[i].fmt, jobs[i].denominator,
+                                 reinterpret_cast<uint64_t>(jobs[i].dst), jobs[i].pitch,
+                                 jobs[i].capacity, lps[i]);
+        if (st[i]) errs[i] = last_error();
+    });
     int32_t first_bad = 0;
     size_t bad_index = 0;
     for (size_t i = 0; i < n; ++i) {
@@ -223,6 +297,13 @@ Status build_plan(const sgn_tile_req *jobs, size_t n, sgn_j2k_plan &P, int32_t *
     }
     P.arena_len = arena;
     P.coef_elems = coef;
+    {
+        uint32_t kmax = 0;
+        for (const LocalPlan &lp : lps) kmax = std::max(kmax, lp.max_k);
+        const char *env = std::getenv("SGN_COEF_NARROW");
+        const bool allow = !(env && env[0] == '0');
+        P.narrow = allow && kmax <= 15 && 2 * coef < (1ull << 32);
+    }
     P.lane_plain_begin.assign(1, 0);
     P.wave_begin.assign(1, 0);
     P.tile_levels.assign(n, {});
@@ -238,9 +319,13 @@ Status build_plan(const sgn_tile_req *jobs, size_t n, sgn_j2k_plan &P, int32_t *
         P.tile_row_bytes[i] = lp.row_bytes;
         P.out_bytes += lp.out_bytes;
         P.out_pixels += uint64_t(lp.width) * lp.height;
-        for (CbJob j : lp.cbs) {
+        const uint32_t cbase = static_cast<uint32_t>(coef_base[i]);
+        for (size_t k = 0; k < lp.cbs.size(); ++k) {
+            CbJob j = lp.cbs[k];
             j.src += static_cast<uint32_t>(arena_base[i]);
-            j.dst += static_cast<uint32_t>(coef_base[i]);
+            // int16 index of a narrow sample: twice its band's int32 offset
+            // plus the offset inside the band
+            j.dst = P.narrow ? j.dst + lp.cb_band[k] + 2 * cbase : j.dst + cbase;
             j.tile = static_cast<uint32_t>(i);
             if (lane_eligible(j))
                 P.cb_lane_plain.push_back(j);
@@ -249,12 +334,16 @@ Status build_plan(const sgn_tile_req *jobs, size_t n, sgn_j2k_plan &P, int32_t *
         }
         P.lane_plain_begin.push_back(static_cast<uint32_t>(P.cb_lane_plain.size()));
         P.wave_begin.push_back(static_cast<uint32_t>(P.cb_wave.size()));
+        auto rebase = [&](LevelJob &l) {
+            if (!P.narrow) l.narrow = 0;
+            for (int b = 0; b < 4; ++b)
+                l.band[b] = ((l.narrow >> b) & 1) ? 2 * (l.band[b] + cbase) : l.band[b] + cbase;
+        };
         for (LevelJob &l : lp.levels) {
-            for (int b = 0; b < 4; ++b) l.band[b] += static_cast<uint32_t>(coef_base[i]);
-            l.out += static_cast<uint32_t>(coef_base[i]);
+            rebase(l);
+            l.out += cbase;
         }
-        for (LevelJob &l : lp.finals_lv)
-            for (int b = 0; b < 4; ++b) l.band[b] += static_cast<uint32_t>(coef_base[i]);
+        for (LevelJob &l : lp.finals_lv) rebase(l);
         for (size_t d = 0; d < lp.level_depth.size(); ++d)
             P.depth_max = std::max<uint32_t>(P.depth_max, lp.level_depth[d]);
     }
@@ -312,30 +401,69 @@ Status build_plan(const sgn_tile_req *jobs, size_t n, sgn_j2k_plan &P, int32_t *
     return kOk;
 }
 
+// One device allocation per plan: codestream arena, job tables, status
+// words, work counter and coefficient arena, carved at 256-byte alignment;
+// everything but the last three goes up in one copy.
 Status upload_plan(const sgn_tile_req *jobs, sgn_j2k_plan &P) {
-    if (P.arena_len) {
-        HIP_TRY(hipMalloc(&P.d_arena, P.arena_len));
-        std::vector<uint8_t> staging(P.arena_len, 0);
-        uint64_t off = 0;
-        for (size_t i = 0; i < P.ntiles; ++i) {
-            std::memcpy(staging.data() + off, jobs[i].bytes, jobs[i].len);
-            off += (jobs[i].len + 15) & ~size_t(15);
-        }
-        HIP_TRY(hipMemcpy(P.d_arena, staging.data(), P.arena_len, hipMemcpyHostToDevice));
+    uint64_t off = 0;
+    auto take = [&](uint64_t bytes) {
+        const uint64_t o = off;
+        off = (off + bytes + 255) & ~uint64_t(255);
+        return o;
+    };
+    auto bytes_of = [](const auto &v) { return uint64_t(v.size()) * sizeof(v[0]); };
+    const uint64_t o_arena = take(P.arena_len);
+    const uint64_t o_lane = take(bytes_of(P.cb_lane)), o_plain = take(bytes_of(P.cb_lane_plain));
+    const uint64_t o_wave = take(bytes_of(P.cb_wave));
+    const uint64_t o_lv0 = take(bytes_of(P.lv[0])), o_lv1 = take(bytes_of(P.lv[1]));
+    const uint64_t o_fin0 = take(bytes_of(P.fin[0])), o_fin1 = take(bytes_of(P.fin[1]));
+    const uint64_t o_flv = take(bytes_of(P.final_lv));
+    const uint64_t host_bytes = off;
+    const uint64_t o_stl = take(P.cb_lane.size()), o_stw = take(P.cb_wave.size());
+    const uint64_t o_queue = take(16), o_coef = take(P.coef_elems * sizeof(uint32_t));
+    if (hipMalloc(&P.d_block, off) != hipSuccess) {
+        P.d_block = nullptr;
+        set_last_error("hipMalloc failed for the plan (" + std::to_string(off) + " bytes)");
+        return kErrDevice;
     }
-    if (P.coef_elems) HIP_TRY(hipMalloc(&P.d_coef, P.coef_elems * sizeof(uint32_t)));
-    Status st;
-    if ((st = dev_upload(P.d_cb_lane, P.cb_lane))) return st;
-    if ((st = dev_upload(P.d_cb_lane_plain, P.cb_lane_plain))) return st;
-    if ((st = dev_upload(P.d_cb_wave, P.cb_wave))) return st;
-    for (int r = 0; r < 2; ++r) {
-        if ((st = dev_upload(P.d_lv[r], P.lv[r]))) return st;
-        if ((st = dev_upload(P.d_fin[r], P.fin[r]))) return st;
+    std::unique_ptr<uint8_t[]> staging(new uint8_t[std::max<uint64_t>(1, host_bytes)]);
+    uint8_t *h = staging.get();
+    uint64_t a = o_arena;
+    for (size_t i = 0; i < P.ntiles; ++i) {
+        std::memcpy(h + a, jobs[i].bytes, jobs[i].len);
+        const uint64_t padded = (jobs[i].len + 15) & ~uint64_t(15);
+        std::memset(h + a + jobs[i].len, 0, padded - jobs[i].len);
+        a += padded;
     }
-    if ((st = dev_upload(P.d_final_lv, P.final_lv))) return st;
-    if (!P.cb_lane.empty()) HIP_TRY(hipMalloc(&P.d_status_lane, P.cb_lane.size()));
-    HIP_TRY(hipMalloc(&P.d_queue, 16));
-    if (!P.cb_wave.empty()) HIP_TRY(hipMalloc(&P.d_status_wave, P.cb_wave.size()));
+    auto put = [&](uint64_t o, const auto &v) {
+        if (!v.empty()) std::memcpy(h + o, v.data(), bytes_of(v));
+    };
+    put(o_lane, P.cb_lane);
+    put(o_plain, P.cb_lane_plain);
+    put(o_wave, P.cb_wave);
+    put(o_lv0, P.lv[0]);
+    put(o_lv1, P.lv[1]);
+    put(o_fin0, P.fin[0]);
+    put(o_fin1, P.fin[1]);
+    put(o_flv, P.final_lv);
+    HIP_TRY(hipMemcpy(P.d_block, h, host_bytes, hipMemcpyHostToDevice));
+    uint8_t *d = static_cast<uint8_t *>(P.d_block);
+    auto at = [&](uint64_t o, auto *&ptr, bool present) {
+        ptr = present ? reinterpret_cast<std::remove_reference_t<decltype(ptr)>>(d + o) : nullptr;
+    };
+    at(o_arena, P.d_arena, P.arena_len != 0);
+    at(o_lane, P.d_cb_lane, !P.cb_lane.empty());
+    at(o_plain, P.d_cb_lane_plain, !P.cb_lane_plain.empty());
+    at(o_wave, P.d_cb_wave, !P.cb_wave.empty());
+    at(o_lv0, P.d_lv[0], !P.lv[0].empty());
+    at(o_lv1, P.d_lv[1], !P.lv[1].empty());
+    at(o_fin0, P.d_fin[0], !P.fin[0].empty());
+    at(o_fin1, P.d_fin[1], !P.fin[1].empty());
+    at(o_flv, P.d_final_lv, !P.final_lv.empty());
+    at(o_stl, P.d_status_lane, !P.cb_lane.empty());
+    at(o_stw, P.d_status_wave, !P.cb_wave.empty());
+    at(o_queue, P.d_queue, true);
+    at(o_coef, P.d_coef, P.coef_elems != 0);
     return kOk;
 }
 
@@ -351,22 +479,23 @@ uint32_t final_blocks(const std::vector<FinalJob> &v, uint32_t a, uint32_t b) {
 
 // Enqueue helpers; each returns the number of kernel launches issued.
 // Work-groups of variant v that the device holds at once.
-uint32_t resident_groups(sgn_hip_session *s, int v) {
-    if (s->resident.empty()) s->resident.assign(kRowsVariantCount, 0);
-    if (!s->resident[v]) {
+uint32_t resident_groups(sgn_hip_session *s, int v, bool narrow) {
+    if (s->resident.empty()) s->resident.assign(2 * kRowsVariantCount, 0);
+    const int slot = 2 * v + (narrow ? 1 : 0);
+    if (!s->resident[slot]) {
         int per_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kRowsVariants[v].kernel,
-                                                         kRowsVariants[v].waves * 64, 0) !=
-                hipSuccess ||
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, kRowsVariants[v].kernel[narrow ? 1 : 0], kRowsVariants[v].waves * 64,
+                0) != hipSuccess ||
             per_cu < 1)
             per_cu = 1;
         if (!s->cus && (hipDeviceGetAttribute(&s->cus, hipDeviceAttributeMultiprocessorCount,
                                               s->device) != hipSuccess ||
                         s->cus < 1))
             s->cus = 1;
-        s->resident[v] = uint32_t(per_cu) * uint32_t(s->cus);
+        s->resident[slot] = uint32_t(per_cu) * uint32_t(s->cus);
     }
-    return s->resident[v];
+    return s->resident[slot];
 }
 
 uint32_t launch_lane_variant(sgn_hip_session *s, sgn_j2k_plan &P, int v, const CbJob *d_jobs,
@@ -380,11 +509,11 @@ uint32_t launch_lane_variant(sgn_hip_session *s, sgn_j2k_plan &P, int v, const C
         // the counter is cleared by a kernel, never a memset node (graph capture)
         hipLaunchKernelGGL(sgn_fill_bytes, dim3(1), dim3(64), 0, s->stream,
                            reinterpret_cast<uint8_t *>(P.d_queue), size_t(4), size_t(1), 0u);
-        grid = std::min(grid, resident_groups(s, v));
+        grid = std::min(grid, resident_groups(s, v, P.narrow));
         ++k;
     }
-    rv.launch(s->stream, grid, d_jobs, count, P.d_arena, reinterpret_cast<int32_t *>(P.d_coef),
-              s->d_tables, status, P.d_queue);
+    rv.launch[P.narrow ? 1 : 0](s->stream, grid, d_jobs, count, P.d_arena, P.d_coef, s->d_tables,
+                                status, P.d_queue);
     return k;
 }
 
@@ -397,7 +526,8 @@ uint32_t launch_wave(sgn_hip_session *s, sgn_j2k_plan &P, const CbJob *d_jobs, u
                      uint8_t *status) {
     if (!count) return 0;
     hipLaunchKernelGGL(sgn_ht_block_wave, dim3(count), dim3(64), 0, s->stream, d_jobs, count,
-                       P.d_arena, reinterpret_cast<int32_t *>(P.d_coef), s->d_tables, status, 0u);
+                       P.d_arena, reinterpret_cast<int32_t *>(P.d_coef), s->d_tables, status, 0u,
+                       P.narrow ? 1u : 0u);
     return 1;
 }
 
@@ -860,6 +990,7 @@ int sgn_j2k_plan_get_stats(const sgn_j2k_plan *P, sgn_j2k_plan_stats *st) {
     st->output_pixels = P->out_pixels;
     st->coefficient_bytes = P->coef_elems * 4;
     st->launches_batched = P->launches_batched;
+    st->narrow_samples = P->narrow ? 1u : 0u;
     return SGN_OK;
 }
 
@@ -999,13 +1130,13 @@ static int hook_blocks(sgn_hip_session *s, const uint8_t *data, size_t data_len,
         return SGN_E_DEVICE;
     }
     if (!lane.empty())
-        kRowsVariants[0].launch(s->stream, uint32_t((lane.size() + 255) / 256), d_lane,
-                                uint32_t(lane.size()), d_data, reinterpret_cast<int32_t *>(d_out),
-                                s->d_tables, d_st, nullptr);
+        kRowsVariants[0].launch[0](s->stream, uint32_t((lane.size() + 255) / 256), d_lane,
+                                   uint32_t(lane.size()), d_data, d_out, s->d_tables, d_st,
+                                   nullptr);
     if (!wave.empty())
         hipLaunchKernelGGL(sgn_ht_block_wave, dim3(wave.size()), dim3(64), 0, s->stream, d_wave,
                            uint32_t(wave.size()), d_data, reinterpret_cast<int32_t *>(d_out),
-                           s->d_tables, d_st + lane.size(), raw ? 1u : 0u);
+                           s->d_tables, d_st + lane.size(), raw ? 1u : 0u, 0u);
     s->dispatches += (lane.empty() ? 0 : 1) + (wave.empty() ? 0 : 1);
     if (hipStreamSynchronize(s->stream) != hipSuccess ||
         hipMemcpy(out, d_out, out_len * 4, hipMemcpyDeviceToHost) != hipSuccess) {
@@ -1355,9 +1486,14 @@ int sgn_j2k_plan_time_code_blocks(sgn_hip_session *s, sgn_j2k_plan *P, size_t st
         (st = dcyc.alloc(pick.size())))
         return st;
     const uint32_t waves = uint32_t(pick.size());
-    hipLaunchKernelGGL(sgn_ht_rows_timed, dim3((waves + 3) / 4), dim3(256), 0, s->stream, djobs.p,
-                       uint32_t(jobs.size()), P->d_arena, reinterpret_cast<int32_t *>(P->d_coef),
-                       s->d_tables, dst.p, dcyc.p);
+    if (P->narrow)
+        hipLaunchKernelGGL(sgn_ht_rows_timed<int16_t>, dim3((waves + 3) / 4), dim3(256), 0,
+                           s->stream, djobs.p, uint32_t(jobs.size()), P->d_arena,
+                           reinterpret_cast<int16_t *>(P->d_coef), s->d_tables, dst.p, dcyc.p);
+    else
+        hipLaunchKernelGGL(sgn_ht_rows_timed<int32_t>, dim3((waves + 3) / 4), dim3(256), 0,
+                           s->stream, djobs.p, uint32_t(jobs.size()), P->d_arena,
+                           reinterpret_cast<int32_t *>(P->d_coef), s->d_tables, dst.p, dcyc.p);
     s->dispatches++;
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s->stream));

@@ -26,7 +26,8 @@ struct ServeRequest {
     std::vector<uint32_t> den;
     std::vector<uint32_t> width, height;
     std::vector<size_t> row_bytes;
-    std::vector<void *> out;  // device surfaces (one per tile)
+    std::vector<void *> out;  // device surfaces (one per tile), carved from out_block
+    void *out_block = nullptr;
     uint64_t footprint = 0;   // admission estimate, bytes
     uint64_t out_bytes = 0;
     bool out_counted = false; // out_bytes included in the server's device usage
@@ -77,8 +78,7 @@ struct sgn_server {
         }
         running.clear();
         for (auto &kv : reqs)
-            for (void *p : kv.second->out)
-                if (p) (void)hipFree(p);
+            if (kv.second->out_block) (void)hipFree(kv.second->out_block);
         for (sgn_hip_session *s : slots) sgn_hip_session_destroy(s);
     }
 };
@@ -200,11 +200,19 @@ Status launch_batch(sgn_server &S, bool force) {
     size_t ti = 0;
     for (uint64_t id : B->ids) {
         ServeRequest &r = *S.reqs[id];
-        for (size_t t = 0; t < r.bytes.size(); ++t, ++ti) {
-            const size_t bytes = std::max<size_t>(16, r.row_bytes[t] * r.height[t]);
-            if (!r.out[t]) HIP_TRY(hipMalloc(&r.out[t], bytes));
-            tiles[ti].dst = r.out[t];
+        if (!r.out_block) {
+            // one allocation per request, surfaces at 256-byte alignment
+            size_t total = 0;
+            for (size_t t = 0; t < r.bytes.size(); ++t)
+                total += (std::max<size_t>(16, r.row_bytes[t] * r.height[t]) + 255) & ~size_t(255);
+            HIP_TRY(hipMalloc(&r.out_block, total));
+            size_t o = 0;
+            for (size_t t = 0; t < r.bytes.size(); ++t) {
+                r.out[t] = static_cast<uint8_t *>(r.out_block) + o;
+                o += (std::max<size_t>(16, r.row_bytes[t] * r.height[t]) + 255) & ~size_t(255);
+            }
         }
+        for (size_t t = 0; t < r.bytes.size(); ++t, ++ti) tiles[ti].dst = r.out[t];
         if (!r.out_counted) S.hbm_in_use += r.out_bytes;
         r.out_counted = true;
     }
@@ -376,8 +384,7 @@ int sgn_server_release(sgn_server *S, uint64_t id) {
     if (it == S->reqs.end() || it->second->state != ServeRequest::kDone) return SGN_E_ARG;
     ServeRequest &r = *it->second;
     (void)hipSetDevice(S->device);
-    for (void *p : r.out)
-        if (p) (void)hipFree(p);
+    if (r.out_block) (void)hipFree(r.out_block);
     if (r.out_counted) S->hbm_in_use -= r.out_bytes;
     S->reqs.erase(it);
     return SGN_OK;

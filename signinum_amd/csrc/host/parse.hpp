@@ -44,7 +44,9 @@ struct LocalPlan {
     std::vector<uint8_t> level_depth;// r of each intermediate level job (1-based)
     std::vector<LevelJob> finals_lv; // last level per tile and component
     std::vector<FinalJob> finals;    // one per codestream tile
-    uint64_t coef_elems = 0;
+    std::vector<uint32_t> cb_band;   // element offset of each code-block's band
+    uint64_t coef_elems = 0;         // int32 elements of the coefficient arena
+    uint32_t max_k = 0;              // largest magnitude bit-plane count of any band
     uint32_t width = 0, height = 0;
     size_t row_bytes = 0;
     uint64_t out_bytes = 0;

@@ -134,6 +134,7 @@ Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denomin
                 }
                 band_delta[b] = d;
                 band_k[b] = K;
+                lp.max_k = std::max(lp.max_k, K);
             }
             // code-blocks (decode.rs:1117-1187,1554-1718)
             for (size_t k = 0; k < g.cbs.size(); ++k) {
@@ -168,6 +169,7 @@ Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denomin
                 }
                 j.cost = (j.flags & kCbZero) ? 0 : sg.lcup + sg.lref + (uint32_t(j.w) * j.h) / 4;
                 lp.cbs.push_back(j);
+                lp.cb_band.push_back(band_off[cg.band]);
             }
             // synthesis levels (idwt.rs:54-193)
             uint32_t prev_out = band_off[g.res[0].band[0]];
@@ -186,6 +188,7 @@ Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denomin
                 lj.y1 = rr.y1;
                 lj.reversible = rev;
                 lj.levels = 0;
+                lj.narrow = 1;  // slot 0 is the code-block band itself
                 fj.level[c] = static_cast<uint32_t>(lp.finals_lv.size());
                 lp.finals_lv.push_back(lj);
             }
@@ -203,6 +206,9 @@ Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denomin
                 lj.bw[0] = prev_w;
                 lj.delta[0] = prev_delta;
                 lj.float_in = prev_float ? 1 : 0;
+                // slots filled by code-blocks: the detail bands, and the
+                // low-pass band at the first level
+                lj.narrow = static_cast<uint8_t>(r == 1 ? 0xF : 0xE);
                 for (int b = 0; b < 3; ++b) {
                     const uint32_t bi = rs.band[b];
                     lj.band[b + 1] = band_off[bi];

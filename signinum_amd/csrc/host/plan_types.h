@@ -9,7 +9,9 @@ namespace sgn {
 // crates/signinum-j2k-native/src/lib.rs:149-174).
 struct CbJob {
     uint32_t src;     // byte offset of cleanup||refinement data in the arena
-    uint32_t dst;     // element offset of the block's first sample (coefficient arena)
+    uint32_t dst;     // index of the block's first sample in the coefficient arena, in
+                      // elements of the plan's code-block sample type (int32, or int16
+                      // in a narrow plan)
     uint32_t stride;  // band row width in elements
     uint16_t lcup, lref;
     uint16_t w, h;
@@ -44,7 +46,7 @@ constexpr inline uint32_t coef_pitch(uint32_t w) { return (w + 3u) & ~3u; }
 // One 2D synthesis level of one tile-component (IDWT step,
 // J2kDirectIdwtStep in crates/signinum-j2k-native/src/direct_plan.rs).
 struct LevelJob {
-    uint32_t band[4];   // element offsets: LL, HL, LH, HH
+    uint32_t band[4];   // element offsets: LL, HL, LH, HH (int16 units for `narrow` slots)
     uint32_t bw[4];     // band row pitches (coef_pitch of the band width)
     float delta[4];     // dequantisation steps (irreversible), 1 for float inputs
     uint32_t out;       // element offset of the output (intermediate levels)
@@ -54,7 +56,7 @@ struct LevelJob {
     uint8_t reversible;
     uint8_t float_in;         // bit b set: band b holds float samples, not integers
     uint8_t levels;           // 1: one synthesis level, 0: copy LL only
-    uint8_t pad;
+    uint8_t narrow;           // bit b set: band b holds int16 code-block samples
 };
 
 // Final stage of one tile: last synthesis level of every component fused

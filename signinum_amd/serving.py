@@ -26,9 +26,9 @@ class Server:
 
     def submit(self, tiles):
         """tiles: iterable of (codestream bytes, PixelFormat, Downscale).
-        Returns the request id."""
+        Returns the request id.  The server copies the codestreams."""
         tiles = list(tiles)
-        keep = [np.frombuffer(bytes(t[0]), np.uint8) for t in tiles]
+        keep = [np.frombuffer(t[0], np.uint8) for t in tiles]
         arr = (_lib.ServeTile * len(tiles))()
         for i, ((_b, fmt, den), k) in enumerate(zip(tiles, keep)):
             arr[i] = _lib.ServeTile(k.ctypes.data, k.size, int(PixelFormat(fmt)),
@@ -110,26 +110,35 @@ class Server:
 def run_poisson(server, pool, rate_per_s, duration_s, seed=0, max_tiles=64, release=True):
     """Open-loop load: requests arrive as a Poisson process at `rate_per_s`;
     each takes 1..max_tiles tiles drawn from `pool` (list of (bytes, fmt,
-    scale, pixels)).  Returns per-request records and the wall time from the
-    first arrival to the last completion."""
+    scale, ...)).  A request's latency runs from its scheduled arrival (not
+    from the moment the loop got round to submitting it) to the pump that
+    observed its completion.  Returns (records with "latency_ms", wall time
+    from the first arrival to the last completion)."""
     rng = np.random.default_rng(seed)
     t0 = time.perf_counter()
     next_at = t0 + rng.exponential(1.0 / rate_per_s)
     t_stop = t0 + duration_s
-    records, ids = [], []
+    records, arrival = [], {}
+    submitted = 0
     while True:
         now = time.perf_counter()
-        if now >= next_at and now < t_stop:
+        if now >= next_at and next_at < t_stop:
             k = int(rng.integers(1, max_tiles + 1))
             pick = rng.integers(0, len(pool), k)
-            ids.append(server.submit([pool[i][:3] for i in pick]))
+            rid = server.submit([pool[i][:3] for i in pick])
+            arrival[rid] = next_at
+            submitted += 1
             next_at += rng.exponential(1.0 / rate_per_s)
             continue
         server.pump(flush=now >= t_stop)
-        for r in server.poll():
+        done = server.poll()
+        t_done = time.perf_counter()
+        for r in done:
+            r["server_latency_ms"] = r["latency_ms"]
+            r["latency_ms"] = (t_done - arrival[r["id"]]) * 1e3
             records.append(r)
             if release:
                 server.release(r["id"])
-        if now >= t_stop and len(records) == len(ids):
+        if next_at >= t_stop and len(records) == submitted:
             break
     return records, time.perf_counter() - t0

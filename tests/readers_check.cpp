@@ -99,7 +99,7 @@ int main() {
             ref.remaining = mlen;
             sgn::Fwd4 f;
             sgn::fwd4_init(f, off, mlen, 0xFF, src);
-            sgn::Mel m{0, 0, (int32_t)mlen, 0, 0, 0, 0, 0, 0};
+            sgn::Mel m = sgn::mel_make((uint32_t)mlen);
             for (int step = 0; step < 150; ++step) {
                 const int a = mel_get_run(&ref);
                 const int c = sgn::mel_run4(m, f, src);

@@ -216,6 +216,52 @@ def test_mixed_plan_formats_and_scales(session, oracle):
     plan.close()
 
 
+def _decode_plan(session, streams, fmt, narrow):
+    old = os.environ.get("SGN_COEF_NARROW")
+    try:
+        if narrow:
+            os.environ.pop("SGN_COEF_NARROW", None)
+        else:
+            os.environ["SGN_COEF_NARROW"] = "0"
+        plan = S.TilePlan(session, streams, fmt)
+    finally:
+        if old is None:
+            os.environ.pop("SGN_COEF_NARROW", None)
+        else:
+            os.environ["SGN_COEF_NARROW"] = old
+    plan.decode(S.MODE_GRAPH)
+    session.synchronize()
+    outs = [s.to_numpy() for s in plan.surfaces]
+    flag = plan.stats()["narrow_samples"]
+    plan.close()
+    return outs, flag
+
+
+def test_int16_and_int32_sample_arenas_agree(session, oracle):
+    """Plans whose bands all have at most 15 magnitude bit-planes keep
+    code-block samples as int16; the same plan forced to int32 decodes to the
+    same output, and a plan with a deeper band stays int32."""
+    streams, pix, _cls = synth.lossless_batch_with_pixels(16, size=128)
+    narrow, f1 = _decode_plan(session, streams, S.PixelFormat.RGB8, True)
+    wide, f0 = _decode_plan(session, streams, S.PixelFormat.RGB8, False)
+    assert (f1, f0) == (1, 0)
+    for a, b, img in zip(narrow, wide, pix):
+        assert np.array_equal(a, img) and np.array_equal(b, img)
+    lossy = [c for c, _ in synth.lossy_batch(3, 256)]
+    narrow, f1 = _decode_plan(session, lossy, S.PixelFormat.RGB8, True)
+    wide, _ = _decode_plan(session, lossy, S.PixelFormat.RGB8, False)
+    for cs, a, b in zip(lossy, narrow, wide):
+        assert np.array_equal(a, b)
+        assert np.abs(a.astype(np.int64) - O.decode(cs, 1).astype(np.int64)).max() <= 1
+    # 6 guard bits on 12-bit samples: up to 19 bit-planes, int32 samples
+    rng = np.random.default_rng(12)
+    deep = [S.encode_j2k_ht(synth.spots_gray12(rng, 96, 80), bit_depth=12, use_mct=0,
+                            guard_bits=6, levels=4)]
+    out, flag = _decode_plan(session, deep, S.PixelFormat.GRAY16, True)
+    assert flag == 0
+    assert np.array_equal(out[0], O.decode(deep[0], int(S.PixelFormat.GRAY16), 1))
+
+
 @pytest.mark.parametrize("width", [96, 95])
 def test_surface_copy(session, width):
     # odd widths exercise the unaligned byte tail of the copy kernel
