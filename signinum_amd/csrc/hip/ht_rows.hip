@@ -25,6 +25,10 @@
 namespace sgn {
 
 constexpr int kRowsMaxQ = 34;    // quads per row of a 64-wide block, plus 2
+// Bytes of a block row that a lane gathers in registers and stores at once
+// (a multiple of the 32-byte HBM write granule).  The coefficient pitch is
+// a multiple of this many int16 samples.
+constexpr uint32_t kRowChunkBytes = 32;
 
 // Synchronous 8-byte load for a stream that ran past its staged range.  The
 // wait sits inside the rarely taken branch, so the common path never waits
@@ -118,6 +122,17 @@ __device__ __forceinline__ void rows_wave(const CbJob *__restrict__ jobs, uint32
     C *out = coef + j.dst;
     const uint32_t w = j.w, h = j.h, stride = j.stride;
     const bool vec_store = ((j.dst | stride) & 3u) == 0;
+    // Rows go out in aligned 32-byte chunks (the HBM write granule): a
+    // lane keeps the chunk it is filling in registers, so that no partly
+    // written line of its block rows is ever evicted from L2 and written
+    // back more than once.
+    constexpr uint32_t kCW = kRowChunkBytes / 4;              // words per row chunk
+    constexpr uint32_t kChunkCols = kRowChunkBytes / sizeof(C);  // samples per row chunk
+    constexpr uint32_t kChunkPairs = kChunkCols / 4;            // quad pairs per row chunk
+    const bool chunk_store = ((j.dst | stride) & (kChunkCols - 1)) == 0;
+    uint32_t ch0[kCW], ch1[kCW];  // rows y, y+1 of the chunk being filled, packed samples
+#pragma unroll
+    for (uint32_t t = 0; t < kCW; ++t) ch0[t] = ch1[t] = 0;
     uint8_t st = kCbOk;
     bool run_block = valid;
     uint32_t scup = 0;
@@ -242,60 +257,99 @@ __device__ __forceinline__ void rows_wave(const CbJob *__restrict__ jobs, uint32
         const uint32_t pa = seg * kSegPairs, pb = min(pa + kSegPairs, npairs);
         for (uint32_t pr = pa; pr < pb; ++pr) {
             const uint32_t x = 4 * pr, qj = 2 * pr;
-            // ---- cleanup symbols of the quad pair (one quad per iteration) ----
+            // ---- cleanup symbols of the quad pair ----
+            const uint32_t pt0 = (uint32_t)s_rho[qj][tid] << 4;
+            const uint32_t pt1 = (uint32_t)s_rho[qj + 1][tid] << 4;
+            const uint32_t pt2 = (uint32_t)s_rho[qj + 2][tid] << 4;
+            uint32_t t0, t1, u0, u1;
             // one fill per pair: a pair takes at most 7 + 7 + 3 + 10 bits
             vlc_fill(vlc, rvlc, RingSrc<kVlcSlots>{ring_vlc, have_vlc, arena});
-            const uint16_t *vtab = first_row ? vlc0 : vlc1;
-            uint32_t t0 = 0, t1 = 0;
-#pragma unroll 1
-            for (uint32_t k = 0; k < 2; ++k) {
-                const uint32_t q = qj + k;
-                uint32_t above = 0;
-                if (!first_row) {
-                    above = (uint32_t)s_rho[q][tid] << 4;
-                    c_q |= ((above & 0xA0) << 2) | (((uint32_t)s_rho[q + 1][tid] << 4 & 0x20) << 4);
-                }
-                uint32_t tq = vtab[c_q + (vlc.tmp & 0x7F)];
-                const bool present = x + 2 * k < w;
-                if (c_q == 0 && present) {
+            if (first_row) {
+                t0 = vlc0[c_q + (vlc.tmp & 0x7F)];
+                if (c_q == 0) {
                     run -= 2;
-                    tq = (run == -1) ? tq : 0;
+                    t0 = (run == -1) ? t0 : 0;
                     if (run < 0) run = mel_run4(mel, fmel, RingSrc<kMelSlots>{ring_mel, have_mel, arena});
                 }
-                if (!present) tq = 0;
-                c_q = first_row ? (((tq & 0x10) << 3) | ((tq & 0xE0) << 2))
-                                : (((tq & 0x40) << 2) | ((tq & 0x80) << 1) | (above & 0x80));
-                vlc.tmp >>= tq & 7;
-                vlc.bits -= tq & 7;
-                if (k == 0) t0 = tq;
-                else t1 = tq;
+                c_q = ((t0 & 0x10) << 3) | ((t0 & 0xE0) << 2);
+                vlc.tmp >>= t0 & 7;
+                vlc.bits -= t0 & 7;
+                t1 = vlc0[c_q + (vlc.tmp & 0x7F)];
+                if (c_q == 0 && x + 2 < w) {
+                    run -= 2;
+                    t1 = (run == -1) ? t1 : 0;
+                    if (run < 0) run = mel_run4(mel, fmel, RingSrc<kMelSlots>{ring_mel, have_mel, arena});
+                }
+                if (x + 2 >= w) t1 = 0;
+                c_q = ((t1 & 0x10) << 3) | ((t1 & 0xE0) << 2);
+                vlc.tmp >>= t1 & 7;
+                vlc.bits -= t1 & 7;
+                uint32_t mode = ((t0 & 8) << 3) | ((t1 & 8) << 4);
+                if (mode == 0xC0) {
+                    run -= 2;
+                    if (run == -1) mode += 0x40;
+                    if (run < 0) run = mel_run4(mel, fmel, RingSrc<kMelSlots>{ring_mel, have_mel, arena});
+                }
+                uint32_t e = uvlc0[mode + (vlc.tmp & 0x3F)];
+                vlc.tmp >>= e & 7;
+                vlc.bits -= e & 7;
+                e >>= 3;
+                uint32_t len = e & 0xF;
+                const uint32_t tmp = (uint32_t)vlc.tmp & ((1u << len) - 1);
+                vlc.tmp >>= len;
+                vlc.bits -= len;
+                e >>= 4;
+                len = e & 7;
+                e >>= 3;
+                u0 = 1 + (e & 7) + (tmp & ~(0xFFu << len));
+                u1 = 1 + (e >> 3) + (tmp >> len);
+            } else {
+                c_q |= (pt0 & 0xA0) << 2;
+                c_q |= (pt1 & 0x20) << 4;
+                t0 = vlc1[c_q + (vlc.tmp & 0x7F)];
+                if (c_q == 0) {
+                    run -= 2;
+                    t0 = (run == -1) ? t0 : 0;
+                    if (run < 0) run = mel_run4(mel, fmel, RingSrc<kMelSlots>{ring_mel, have_mel, arena});
+                }
+                c_q = ((t0 & 0x40) << 2) | ((t0 & 0x80) << 1);
+                c_q |= pt0 & 0x80;
+                c_q |= (pt1 & 0xA0) << 2;
+                c_q |= (pt2 & 0x20) << 4;
+                vlc.tmp >>= t0 & 7;
+                vlc.bits -= t0 & 7;
+                t1 = vlc1[c_q + (vlc.tmp & 0x7F)];
+                if (c_q == 0 && x + 2 < w) {
+                    run -= 2;
+                    t1 = (run == -1) ? t1 : 0;
+                    if (run < 0) run = mel_run4(mel, fmel, RingSrc<kMelSlots>{ring_mel, have_mel, arena});
+                }
+                if (x + 2 >= w) t1 = 0;
+                c_q = ((t1 & 0x40) << 2) | ((t1 & 0x80) << 1);
+                c_q |= pt1 & 0x80;
+                vlc.tmp >>= t1 & 7;
+                vlc.bits -= t1 & 7;
+                const uint32_t mode = ((t0 & 8) << 3) | ((t1 & 8) << 4);
+                uint32_t e = uvlc1[mode + (vlc.tmp & 0x3F)];
+                vlc.tmp >>= e & 7;
+                vlc.bits -= e & 7;
+                e >>= 3;
+                uint32_t len = e & 0xF;
+                const uint32_t tmp = (uint32_t)vlc.tmp & ((1u << len) - 1);
+                vlc.tmp >>= len;
+                vlc.bits -= len;
+                e >>= 4;
+                len = e & 7;
+                e >>= 3;
+                u0 = (e & 7) + (tmp & ~(0xFFu << len));
+                u1 = (e >> 3) + (tmp >> len);
             }
-            uint32_t mode = ((t0 & 8) << 3) | ((t1 & 8) << 4);
-            if (first_row && mode == 0xC0) {
-                run -= 2;
-                if (run == -1) mode += 0x40;
-                if (run < 0) run = mel_run4(mel, fmel, RingSrc<kMelSlots>{ring_mel, have_mel, arena});
-            }
-            uint32_t e = (first_row ? uvlc0 : uvlc1)[mode + (vlc.tmp & 0x3F)];
-            vlc.tmp >>= e & 7;
-            vlc.bits -= e & 7;
-            e >>= 3;
-            uint32_t len = e & 0xF;
-            const uint32_t tmp = (uint32_t)vlc.tmp & ((1u << len) - 1);
-            vlc.tmp >>= len;
-            vlc.bits -= len;
-            e >>= 4;
-            len = e & 7;
-            e >>= 3;
-            const uint32_t bias = first_row ? 1u : 0u;
-            const uint32_t u0 = bias + (e & 7) + (tmp & ~(0xFFu << len));
-            const uint32_t u1 = bias + (e >> 3) + (tmp >> len);
             s_rho[qj][tid] = (uint8_t)((t0 >> 4) & 0xF);
             s_rho[qj + 1][tid] = (uint8_t)((t1 >> 4) & 0xF);
 
-            // ---- magnitude / sign, one quad per iteration; each quad's two
-            // columns go out as one 8-byte store per row ----
-#pragma unroll 1
+            // ---- magnitude / sign of both quads ----
+            uint32_t r0[4] = {0, 0, 0, 0}, r1[4] = {0, 0, 0, 0};  // rows y, y+1 at x..x+3
+#pragma unroll
             for (uint32_t k = 0; k < 2; ++k) {
                 const uint32_t xq = x + 2 * k;
                 if (xq >= w) break;
@@ -313,39 +367,76 @@ __device__ __forceinline__ void rows_wave(const CbJob *__restrict__ jobs, uint32
                     st = kCbBad;
                     break;
                 }
-                uint32_t v0, v1, v2 = 0, v3 = 0;
+                uint32_t v0, v1, v2, v3;
                 ms_fill(ms, fms, RingSrc<kMsSlots>{ring_ms, have_ms, arena});
                 const uint32_t a0 = ms_value(ms, inf, 0, uq, p, v0);
                 ms_fill(ms, fms, RingSrc<kMsSlots>{ring_ms, have_ms, arena});
                 const uint32_t a1 = ms_value(ms, inf, 1, uq, p, v1);
+                r0[2 * k] = (uint32_t)to_i32(a0, kmax);
+                r1[2 * k] = (uint32_t)to_i32(a1, kmax);
                 s_e[q][tid] = (uint8_t)max(prev_e3, flog2r(v1));
-                const bool two = xq + 1 < w;
-                uint32_t a2 = 0, a3 = 0;
-                if (two) {
-                    ms_fill(ms, fms, RingSrc<kMsSlots>{ring_ms, have_ms, arena});
-                    a2 = ms_value(ms, inf, 2, uq, p, v2);
-                    ms_fill(ms, fms, RingSrc<kMsSlots>{ring_ms, have_ms, arena});
-                    a3 = ms_value(ms, inf, 3, uq, p, v3);
+                prev_e3 = 0;
+                if (xq + 1 >= w) break;
+                ms_fill(ms, fms, RingSrc<kMsSlots>{ring_ms, have_ms, arena});
+                const uint32_t a2 = ms_value(ms, inf, 2, uq, p, v2);
+                ms_fill(ms, fms, RingSrc<kMsSlots>{ring_ms, have_ms, arena});
+                const uint32_t a3 = ms_value(ms, inf, 3, uq, p, v3);
+                r0[2 * k + 1] = (uint32_t)to_i32(a2, kmax);
+                r1[2 * k + 1] = (uint32_t)to_i32(a3, kmax);
+                prev_e3 = flog2r(v3);
+            }
+            C *o = out + y * stride + x;
+            if (chunk_store && (x | (kChunkCols - 1)) < w) {
+                // the pair's samples join the row chunks; a full chunk goes
+                // out as two 16-byte stores per row
+                constexpr int kW = kCW / kChunkPairs;  // words per row per pair
+#pragma unroll
+                for (int t = 0; t < (int)kCW - kW; ++t) {
+                    ch0[t] = ch0[t + kW];
+                    ch1[t] = ch1[t + kW];
                 }
-                prev_e3 = two ? flog2r(v3) : 0;
-                C *o = out + y * stride + xq;
-                const C c0 = (C)to_i32(a0, kmax), c1 = (C)to_i32(a1, kmax);
-                const C c2 = (C)to_i32(a2, kmax), c3 = (C)to_i32(a3, kmax);
-                if (vec_store && two) {
-                    if constexpr (sizeof(C) == 4) {
-                        *reinterpret_cast<int2 *>(o) = make_int2(c0, c2);
-                        if (second) *reinterpret_cast<int2 *>(o + stride) = make_int2(c1, c3);
-                    } else {
-                        *reinterpret_cast<short2 *>(o) = make_short2(c0, c2);
-                        if (second) *reinterpret_cast<short2 *>(o + stride) = make_short2(c1, c3);
+                if constexpr (sizeof(C) == 4) {
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        ch0[kCW - 4 + t] = r0[t];
+                        ch1[kCW - 4 + t] = r1[t];
                     }
                 } else {
-                    o[0] = c0;
-                    if (second) o[stride] = c1;
-                    if (two) {
-                        o[1] = c2;
-                        if (second) o[stride + 1] = c3;
+                    ch0[kCW - 2] = (r0[0] & 0xFFFFu) | (r0[1] << 16);
+                    ch0[kCW - 1] = (r0[2] & 0xFFFFu) | (r0[3] << 16);
+                    ch1[kCW - 2] = (r1[0] & 0xFFFFu) | (r1[1] << 16);
+                    ch1[kCW - 1] = (r1[2] & 0xFFFFu) | (r1[3] << 16);
+                }
+                if ((pr & (kChunkPairs - 1)) == kChunkPairs - 1) {
+                    uint4 *c0 = reinterpret_cast<uint4 *>(out + y * stride + (x & ~(kChunkCols - 1)));
+#pragma unroll
+                    for (uint32_t t = 0; t < kCW / 4; ++t)
+                        c0[t] = make_uint4(ch0[4 * t], ch0[4 * t + 1], ch0[4 * t + 2], ch0[4 * t + 3]);
+                    if (second) {
+                        uint4 *c1 = reinterpret_cast<uint4 *>(reinterpret_cast<C *>(c0) + stride);
+#pragma unroll
+                        for (uint32_t t = 0; t < kCW / 4; ++t)
+                            c1[t] = make_uint4(ch1[4 * t], ch1[4 * t + 1], ch1[4 * t + 2], ch1[4 * t + 3]);
                     }
+                }
+            } else if (vec_store && x + 4 <= w) {
+                if constexpr (sizeof(C) == 4) {
+                    *reinterpret_cast<uint4 *>(o) = make_uint4(r0[0], r0[1], r0[2], r0[3]);
+                    if (second)
+                        *reinterpret_cast<uint4 *>(o + stride) = make_uint4(r1[0], r1[1], r1[2], r1[3]);
+                } else {
+                    *reinterpret_cast<uint2 *>(o) = make_uint2((r0[0] & 0xFFFFu) | (r0[1] << 16),
+                                                               (r0[2] & 0xFFFFu) | (r0[3] << 16));
+                    if (second)
+                        *reinterpret_cast<uint2 *>(o + stride) =
+                            make_uint2((r1[0] & 0xFFFFu) | (r1[1] << 16), (r1[2] & 0xFFFFu) | (r1[3] << 16));
+                }
+            } else {
+#pragma unroll
+                for (uint32_t c = 0; c < 4; ++c) {
+                    if (x + c >= w) break;
+                    o[c] = (C)(int32_t)r0[c];
+                    if (second) o[stride + c] = (C)(int32_t)r1[c];
                 }
             }
             if (st != kCbOk) break;

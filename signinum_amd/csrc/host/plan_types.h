@@ -39,9 +39,10 @@ constexpr inline CbFeatures cb_features(const CbJob &j) {
 }
 
 // Row pitch (elements) of every band and intermediate level in the
-// coefficient arena: a multiple of four, so rows start 16-byte aligned and
-// four samples of a row move as one 16-byte access.
-constexpr inline uint32_t coef_pitch(uint32_t w) { return (w + 3u) & ~3u; }
+// coefficient arena: a multiple of 16, so that rows of int16 samples start
+// 32-byte aligned (the HT kernels store aligned 32-byte row chunks) and rows
+// of int32 samples 64-byte aligned.
+constexpr inline uint32_t coef_pitch(uint32_t w) { return (w + 15u) & ~15u; }
 
 // One 2D synthesis level of one tile-component (IDWT step,
 // J2kDirectIdwtStep in crates/signinum-j2k-native/src/direct_plan.rs).
