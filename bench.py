@@ -47,7 +47,7 @@ def extra_workloads(S, synth, sess, args, c2_streams, c2_cls):
     t0 = time.time()
     items = synth.mixed_batch(args.mixed_tiles)
     plan = S.TilePlan(sess, [i[0] for i in items], [i[1] for i in items], [i[2] for i in items])
-    plan.autotune(reps=2)
+    plan.autotune_order(reps=2)
     ms = _graph_ms(S, sess, plan)
     px = plan.stats()["output_pixels"]
     for k in range(0, len(items), max(1, len(items) // 16)):
@@ -68,7 +68,7 @@ def extra_workloads(S, synth, sess, args, c2_streams, c2_cls):
     t0 = time.time()
     lossy = synth.lossy_batch(args.lossy_tiles)
     plan = S.TilePlan(sess, [c for c, _ in lossy], S.PixelFormat.RGB8)
-    plan.autotune(reps=2)
+    plan.autotune_order(reps=2)
     ms = _graph_ms(S, sess, plan)
     px = plan.stats()["output_pixels"]
     err = []
@@ -165,9 +165,14 @@ def main():
     pixels = st["output_pixels"]
     log(f"plan: {plan_ms:.1f} ms host parse + upload; {st['code_blocks']} code-blocks")
 
-    tune = None
+    tune = order_ms = None
+    order_name = "length"
     if not args.no_autotune:
+        order_ms, chosen = plan.autotune_order(reps=3)
+        order_name = {0: "length", 1: "input", 3: "geometry", 4: "geometry-cost"}[chosen]
         tune = plan.autotune(reps=3)
+        log("autotune (code-block order, best variant ms): " +
+            ", ".join(f"{k} {v:.3f}" for k, v in order_ms.items()) + f" -> {order_name}")
         log("autotune (HT lane kernel, ms): " +
             ", ".join(f"{k} {v:.3f}" for k, v in tune.items()) +
             f" -> {S.TilePlan.lane_variants()[plan.lane_variant]}")
@@ -308,6 +313,8 @@ def main():
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
         "lane_kernel": S.TilePlan.lane_variants()[plan.lane_variant],
         "autotune_ms": None if tune is None else {k: round(v, 4) for k, v in tune.items()},
+        "code_block_order": order_name,
+        "order_ms": None if order_ms is None else {k: round(v, 4) for k, v in order_ms.items()},
         "plan_ms": round(plan_ms, 1),
         "end_to_end_mpix_s": round(pixels / ((plan_ms + ms_per_step) * 1e-3) / 1e6, 1),
         "roofline": roofline,

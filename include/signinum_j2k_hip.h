@@ -295,9 +295,26 @@ int sgn_j2k_plan_time_code_blocks(sgn_hip_session *session, sgn_j2k_plan *plan, 
                                   size_t *count);
 /* Order of the plan's lane code-blocks: 0 by the compressed-length
  * heuristic (coded bytes + samples / 4, the default), 1 input order, 2 by the
- * model's predicted cost (largest first). */
+ * model's predicted cost (largest first), 3 by block shape (largest first,
+ * input order within a shape), 4 by block shape and then by the tile's summed
+ * heuristic cost (a tile's blocks stay together), 5 as 4 with the model's
+ * predicted costs.  2 and 5 need `model`. */
+enum {
+    SGN_ORDER_LENGTH = 0,
+    SGN_ORDER_INPUT = 1,
+    SGN_ORDER_MODEL = 2,
+    SGN_ORDER_GEOMETRY = 3,
+    SGN_ORDER_GEOMETRY_COST = 4,
+    SGN_ORDER_GEOMETRY_MODEL = 5
+};
 int sgn_j2k_plan_order_code_blocks(sgn_hip_session *session, sgn_j2k_plan *plan, int order,
                                    const sgn_cost_model *model);
+/* Autotunes order and lane-kernel variant together: for each of the length,
+ * input, geometry and geometry-cost orders runs sgn_j2k_plan_autotune, keeps
+ * the fastest (order, variant) pair on the plan.  ms (optional, 4 entries, in
+ * that order) receives each order's best time; *chosen the chosen order. */
+int sgn_j2k_plan_autotune_order(sgn_hip_session *session, sgn_j2k_plan *plan, int reps,
+                                float *ms, int *chosen);
 
 /* ---- serving ---------------------------------------------------------------
  * No counterpart in the reference (its entry points decode per call): a

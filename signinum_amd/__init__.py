@@ -439,12 +439,26 @@ class TilePlan:
         return {names[i]: float(ms[i]) for i in range(n)}
 
     ORDER_LENGTH, ORDER_INPUT, ORDER_MODEL = 0, 1, 2
+    ORDER_GEOMETRY, ORDER_GEOMETRY_COST, ORDER_GEOMETRY_MODEL = 3, 4, 5
 
     def order_code_blocks(self, order, model=None):
         """Orders the lane code-blocks: ORDER_LENGTH (compressed-length
-        heuristic), ORDER_INPUT, or ORDER_MODEL (predicted decode cost)."""
+        heuristic), ORDER_INPUT, ORDER_MODEL (predicted decode cost),
+        ORDER_GEOMETRY (block shape, input order within a shape),
+        ORDER_GEOMETRY_COST / ORDER_GEOMETRY_MODEL (block shape, then tiles by
+        summed heuristic / predicted block cost)."""
         _check(_lib.load().sgn_j2k_plan_order_code_blocks(
             self.session.handle, self.handle, int(order), model.handle if model else None))
+
+    def autotune_order(self, reps=3):
+        """Autotunes code-block order and lane-kernel variant together; keeps
+        the fastest pair and returns ({order name: best ms}, chosen order)."""
+        ms = (ctypes.c_float * 4)()
+        chosen = ctypes.c_int()
+        _check(_lib.load().sgn_j2k_plan_autotune_order(self.session.handle, self.handle,
+                                                       int(reps), ms, ctypes.byref(chosen)))
+        names = ("length", "input", "geometry", "geometry-cost")
+        return {names[i]: float(ms[i]) for i in range(4)}, chosen.value
 
     def time_code_blocks(self, stride=1, max_samples=1 << 20):
         """(features [n, 5] float32, cycles [n] float32) of code-blocks decoded

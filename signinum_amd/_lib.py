@@ -156,6 +156,8 @@ SIGNATURES = {
                                                      ctypes.POINTER(c_size_t)]),
     "sgn_j2k_plan_order_code_blocks": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
                                                       ctypes.c_int, ctypes.c_void_p]),
+    "sgn_j2k_plan_autotune_order": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                                    ctypes.c_void_p, ctypes.c_void_p]),
     "sgn_server_default_config": (None, [ctypes.POINTER(ServerConfig)]),
     "sgn_server_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ServerConfig),
                                          ctypes.POINTER(ctypes.c_void_p)]),

@@ -132,8 +132,8 @@ __device__ __forceinline__ uint32_t has_ff(uint32_t x) {  // nonzero iff a byte 
 // 7-bit byte (ORed into the next position, as byte-wise appending does),
 // stays inside the 64-bit accumulator.
 template <class Src>
-__device__ __forceinline__ void ms_fill(MsBits &b, Fwd4 &f, const Src &src) {
-    while (b.bits < 32) {
+__device__ __forceinline__ void ms_fill(MsBits &b, Fwd4 &f, const Src &src, uint32_t need = 32) {
+    while (b.bits < need) {
         const uint32_t w = fwd4_get(f, src);
         if (!b.unstuff && !has_ff(w & 0x00FFFFFFu)) {
             b.tmp |= (uint64_t)w << b.bits;

@@ -367,19 +367,29 @@ __device__ __forceinline__ void rows_wave(const CbJob *__restrict__ jobs, uint32
                     st = kCbBad;
                     break;
                 }
+                // Bits the quad's significant samples take (m_n = U_q or
+                // U_q - 1 each).  Up to 32 of them come in one top-up, and
+                // the per-sample top-ups below stay idle; larger quads top
+                // up before every sample.
+                const uint32_t sig = (inf >> 4) & (xq + 1 < w ? 0xFu : 0x3u);
+                const uint32_t need =
+                    (uint32_t)__builtin_popcount(sig) * uq -
+                    (uint32_t)__builtin_popcount(sig & (inf >> 12));
+                const bool one_fill = need <= 32;
+                if (one_fill) ms_fill(ms, fms, RingSrc<kMsSlots>{ring_ms, have_ms, arena}, need);
                 uint32_t v0, v1, v2, v3;
-                ms_fill(ms, fms, RingSrc<kMsSlots>{ring_ms, have_ms, arena});
+                if (!one_fill) ms_fill(ms, fms, RingSrc<kMsSlots>{ring_ms, have_ms, arena});
                 const uint32_t a0 = ms_value(ms, inf, 0, uq, p, v0);
-                ms_fill(ms, fms, RingSrc<kMsSlots>{ring_ms, have_ms, arena});
+                if (!one_fill) ms_fill(ms, fms, RingSrc<kMsSlots>{ring_ms, have_ms, arena});
                 const uint32_t a1 = ms_value(ms, inf, 1, uq, p, v1);
                 r0[2 * k] = (uint32_t)to_i32(a0, kmax);
                 r1[2 * k] = (uint32_t)to_i32(a1, kmax);
                 s_e[q][tid] = (uint8_t)max(prev_e3, flog2r(v1));
                 prev_e3 = 0;
                 if (xq + 1 >= w) break;
-                ms_fill(ms, fms, RingSrc<kMsSlots>{ring_ms, have_ms, arena});
+                if (!one_fill) ms_fill(ms, fms, RingSrc<kMsSlots>{ring_ms, have_ms, arena});
                 const uint32_t a2 = ms_value(ms, inf, 2, uq, p, v2);
-                ms_fill(ms, fms, RingSrc<kMsSlots>{ring_ms, have_ms, arena});
+                if (!one_fill) ms_fill(ms, fms, RingSrc<kMsSlots>{ring_ms, have_ms, arena});
                 const uint32_t a3 = ms_value(ms, inf, 3, uq, p, v3);
                 r0[2 * k + 1] = (uint32_t)to_i32(a2, kmax);
                 r1[2 * k + 1] = (uint32_t)to_i32(a3, kmax);

@@ -245,9 +245,9 @@ struct SinkGlobal {  // the level's output in the coefficient arena
 // back into the window; else every block column in [0, bw), output rows
 // [H, H + bh), handed to `sink`.  q: parity of window position 0 along the
 // direction.  All segments are read before any in-place write.
-template <bool REV, bool HORIZ, class Sink>
-__device__ __forceinline__ void lift_window(typename Sample<REV>::T *win, int nh, int bw, int bh,
-                                            uint32_t q, const Sink &sink) {
+template <bool REV, bool HORIZ, int Q, class Sink>
+__device__ __forceinline__ void lift_window_q(typename Sample<REV>::T *win, int nh, int bw, int bh,
+                                              const Sink &sink) {
     using T = typename Sample<REV>::T;
     constexpr int H = Win<REV>::H, N = Win<REV>::N, P = Win<REV>::P, R = Win<REV>::R;
     // items: HORIZ (row, segment) with the row fastest; else (column, segment)
@@ -267,8 +267,7 @@ __device__ __forceinline__ void lift_window(typename Sample<REV>::T *win, int nh
 #pragma unroll
         for (int j = 0; j < R; ++j)
             v[j] = HORIZ ? win[line * P + 8 * seg + j] : win[(8 * seg + j) * P + H + line];
-        if (q) lift_segment<REV, 1>(v);
-        else lift_segment<REV, 0>(v);
+        lift_segment<REV, Q>(v);
 #pragma unroll
         for (int j = 0; j < 8; ++j) out[it][j] = v[H + j];
     }
@@ -287,6 +286,18 @@ __device__ __forceinline__ void lift_window(typename Sample<REV>::T *win, int nh
         }
     }
     if (kInPlace) __syncthreads();
+}
+
+// The window parity q is uniform; dispatching on it outside the segment loop
+// keeps each instance's lifting in fixed registers (a merged, select-based
+// form would index the segment registers at run time).
+template <bool REV, bool HORIZ, class Sink>
+__device__ __forceinline__ void lift_window(typename Sample<REV>::T *win, int nh, int bw, int bh,
+                                            uint32_t q, const Sink &sink) {
+    if (__builtin_amdgcn_readfirstlane(q))
+        lift_window_q<REV, HORIZ, 1>(win, nh, bw, bh, sink);
+    else
+        lift_window_q<REV, HORIZ, 0>(win, nh, bw, bh, sink);
 }
 
 // Window-row table of block row `by`: band row and row parity (bit 31) of
@@ -319,8 +330,10 @@ struct WinFetch {
     static constexpr int kRows0 = (Win<REV>::N + 3) / 4;
     static constexpr int kStep1 = kIdwtThreads / kHalo;
     static constexpr int kRows1 = (Win<REV>::N + kStep1 - 1) / kStep1;
+    static_assert(3 + 4 * (kRows0 - 1) < Win<REV>::N, "pass-0 rows stay inside the window");
     uint32_t raw0[kRows0];
     uint32_t raw1[kRows1];
+    uint32_t y1m0, y1m1;  // bit k: row k of the pass read the row-parity-1 slots
 };
 
 // Column `xx` of the window: the band column and the two band slots (row
@@ -349,115 +362,123 @@ __device__ __forceinline__ WinColumn window_column(const LevelView &L, int ox, i
 }
 
 // Coefficient word at index `idx` of a slot: int16 samples (narrow) are
-// sign-extended; int32 and float words are returned as stored.
+// sign-extended; int32 and float words are returned as stored.  Both kinds
+// come from one aligned 32-bit load (an int16 sample's dword lies inside its
+// band's slot), so lanes of mixed slots take one load path.
 __device__ __forceinline__ uint32_t coef_word(const uint32_t *__restrict__ coef, uint32_t idx,
                                               bool narrow) {
-    return narrow ? (uint32_t)(int32_t)reinterpret_cast<const int16_t *>(coef)[idx] : coef[idx];
+    const uint64_t byte = (uint64_t)idx << (narrow ? 1 : 2);
+    const uint32_t w = *reinterpret_cast<const uint32_t *>(
+        reinterpret_cast<const uint8_t *>(coef) + (byte & ~uint64_t(3)));
+    const uint32_t up = 16u - 8u * (uint32_t)(byte & 2);  // 16: low half, 0: high half
+    return narrow ? (uint32_t)((int32_t)(w << up) >> 16) : w;
 }
 
 // Issues every load of the window of block (bx, by); `rows` holds the
-// block row's window-row table.
+// block row's window-row table.  Positions outside the window (xx >= nw or
+// yy >= nh) load arena word 0 instead: every lane takes the same path, and
+// those values only reach outputs outside the block.
 template <bool REV>
 __device__ __forceinline__ void fetch_window(const LevelView &L, const uint32_t *__restrict__ coef,
                                              int bx, int by, const uint32_t *rows,
                                              WinFetch<REV> &f) {
-    constexpr int H = Win<REV>::H;
+    constexpr int H = Win<REV>::H, N = Win<REV>::N;
     using F = WinFetch<REV>;
     const int ox = bx * kBlk, oy = by * kBlk;
     const int nw = min(kBlk, L.w - ox) + 2 * H, nh = min(kBlk, L.h - oy) + 2 * H;
     const int tid = threadIdx.x;
     {
         const int xx = tid & 63, y0 = tid >> 6;
-        const WinColumn c = window_column(L, ox, xx < nw ? xx : 0, H);
+        const bool colok = xx < nw;
+        const WinColumn c = window_column(L, ox, colok ? xx : 0, H);
+        uint32_t rr[F::kRows0];
+#pragma unroll
+        for (int k = 0; k < F::kRows0; ++k) rr[k] = rows[y0 + 4 * k];
+        uint32_t m = 0;
 #pragma unroll
         for (int k = 0; k < F::kRows0; ++k) {
-            const int yy = y0 + 4 * k;
-            f.raw0[k] = 0;
-            if (xx < nw && yy < nh) {
-                const uint32_t r = rows[yy];
-                const bool y1 = (r >> 31) != 0;
-                const uint32_t row = r & 0x7FFFFFFFu;
-                f.raw0[k] = coef_word(coef, (y1 ? c.base1 : c.base0) + row * (y1 ? c.pitch1 : c.pitch0),
-                                      y1 ? c.n1 : c.n0);
-            }
+            const bool ok = colok && y0 + 4 * k < nh;
+            const bool y1 = (rr[k] >> 31) != 0;
+            const uint32_t row = rr[k] & 0x7FFFFFFFu;
+            const uint32_t idx = ok ? (y1 ? c.base1 : c.base0) + row * (y1 ? c.pitch1 : c.pitch0) : 0u;
+            f.raw0[k] = coef_word(coef, idx, ok && (y1 ? c.n1 : c.n0));
+            m |= (y1 ? 1u : 0u) << k;
         }
+        f.y1m0 = m;
     }
     {
         const int xx = 64 + tid % F::kHalo, y0 = tid / F::kHalo;
-        const WinColumn c = window_column(L, ox, xx < nw ? xx : 0, H);
+        const bool colok = xx < nw;
+        const WinColumn c = window_column(L, ox, colok ? xx : 0, H);
+        uint32_t m = 0;
 #pragma unroll
         for (int k = 0; k < F::kRows1; ++k) {
             const int yy = y0 + F::kStep1 * k;
-            f.raw1[k] = 0;
-            if (xx < nw && yy < nh) {
-                const uint32_t r = rows[yy];
-                const bool y1 = (r >> 31) != 0;
-                const uint32_t row = r & 0x7FFFFFFFu;
-                f.raw1[k] = coef_word(coef, (y1 ? c.base1 : c.base0) + row * (y1 ? c.pitch1 : c.pitch0),
-                                      y1 ? c.n1 : c.n0);
-            }
+            const bool ok = colok && yy < nh;
+            const uint32_t r = rows[min(yy, N - 1)];
+            const bool y1 = (r >> 31) != 0;
+            const uint32_t row = r & 0x7FFFFFFFu;
+            const uint32_t idx = ok ? (y1 ? c.base1 : c.base0) + row * (y1 ? c.pitch1 : c.pitch0) : 0u;
+            f.raw1[k] = coef_word(coef, idx, ok && (y1 ? c.n1 : c.n0));
+            m |= (y1 ? 1u : 0u) << k;
         }
+        f.y1m1 = m;
     }
 }
 
 // Converts the fetched words (dequantising with the band step for 9/7) and
-// writes them into the window.  No barrier.
+// writes them into the window.  5/3 levels hold integers only; 9/7 slots may
+// hold f32 samples (float_in).  No barrier.
 template <bool REV>
 __device__ __forceinline__ void commit_window(const LevelView &L, int bx, int by,
                                               const uint32_t *rows, const WinFetch<REV> &f,
                                               typename Sample<REV>::T *win) {
     using T = typename Sample<REV>::T;
     using F = WinFetch<REV>;
-    constexpr int H = Win<REV>::H, P = Win<REV>::P;
+    constexpr int H = Win<REV>::H, N = Win<REV>::N, P = Win<REV>::P;
     const int ox = bx * kBlk, oy = by * kBlk;
-    const int nw = min(kBlk, L.w - ox) + 2 * H, nh = min(kBlk, L.h - oy) + 2 * H;
+    const int nh = min(kBlk, L.h - oy) + 2 * H;
     const int tid = threadIdx.x;
-    // per column: band-slot flags and steps for row parity 0 and 1
-    auto column = [&](int xx, bool &f0, bool &f1, float &d0, float &d1) {
-        uint32_t xpar = 0;
-        if (L.levels) xpar = (L.x0 + (uint32_t)reflect(ox - H + xx, L.w)) & 1u;
-        f0 = (L.fin >> xpar) & 1u;
-        f1 = (L.fin >> (xpar | 2)) & 1u;
-        d0 = d1 = 1.0f;
-        if constexpr (!REV) {
-            d0 = L.bands->delta[xpar];
-            d1 = L.bands->delta[xpar | 2];
-        }
+    (void)rows;
+    // per column: band-slot flags and steps for row parity 0 and 1 (9/7)
+    struct Col {
+        bool f0, f1;
+        float d0, d1;
     };
-    auto put = [&](int xx, int yy, uint32_t v, bool f0, bool f1, float d0, float d1) {
-        const bool y1 = (rows[yy] >> 31) != 0;
-        const bool isf = y1 ? f1 : f0;
-        T val;
-        if constexpr (REV)
-            val = isf ? (int32_t)__uint_as_float(v) : (int32_t)v;
-        else
-            val = isf ? __uint_as_float(v) : (float)(int32_t)v * (y1 ? d1 : d0);
-        win[yy * P + xx] = val;
+    auto column = [&](int xx) {
+        Col c{false, false, 1.0f, 1.0f};
+        if constexpr (!REV) {
+            uint32_t xpar = 0;
+            if (L.levels) xpar = (L.x0 + (uint32_t)reflect(ox - H + xx, L.w)) & 1u;
+            c.f0 = (L.fin >> xpar) & 1u;
+            c.f1 = (L.fin >> (xpar | 2)) & 1u;
+            c.d0 = L.bands->delta[xpar];
+            c.d1 = L.bands->delta[xpar | 2];
+        }
+        return c;
+    };
+    auto convert = [&](const Col &c, uint32_t v, bool y1) -> T {
+        if constexpr (REV) {
+            return (int32_t)v;
+        } else {
+            const bool isf = y1 ? c.f1 : c.f0;
+            return isf ? __uint_as_float(v) : (float)(int32_t)v * (y1 ? c.d1 : c.d0);
+        }
     };
     {
         const int xx = tid & 63, y0 = tid >> 6;
-        if (xx < nw) {
-            bool f0, f1;
-            float d0, d1;
-            column(xx, f0, f1, d0, d1);
+        const Col c = column(xx);
 #pragma unroll
-            for (int k = 0; k < F::kRows0; ++k) {
-                const int yy = y0 + 4 * k;
-                if (yy < nh) put(xx, yy, f.raw0[k], f0, f1, d0, d1);
-            }
-        }
+        for (int k = 0; k < F::kRows0; ++k)
+            win[(y0 + 4 * k) * P + xx] = convert(c, f.raw0[k], (f.y1m0 >> k) & 1u);
     }
     {
         const int xx = 64 + tid % F::kHalo, y0 = tid / F::kHalo;
-        if (xx < nw) {
-            bool f0, f1;
-            float d0, d1;
-            column(xx, f0, f1, d0, d1);
+        const Col c = column(xx);
 #pragma unroll
-            for (int k = 0; k < F::kRows1; ++k) {
-                const int yy = y0 + F::kStep1 * k;
-                if (yy < nh) put(xx, yy, f.raw1[k], f0, f1, d0, d1);
-            }
+        for (int k = 0; k < F::kRows1; ++k) {
+            const int yy = y0 + F::kStep1 * k;
+            if (yy < nh && yy < N) win[yy * P + xx] = convert(c, f.raw1[k], (f.y1m1 >> k) & 1u);
         }
     }
 }

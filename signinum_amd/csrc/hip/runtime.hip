@@ -1201,17 +1201,28 @@ int sgn_idwt_level(sgn_hip_session *s, int reversible, uint32_t x0, uint32_t y0,
     const float *src[4] = {ll, hl, lh, hh};
     std::vector<uint32_t> host;
     LevelJob lj{};
+    // 5/3 levels take integers: each f32 input is truncated toward zero,
+    // NaN to 0, saturating at the int32 range; 9/7 levels take the f32 bits
+    auto to_int = [](float f) -> int32_t {
+        if (!(f == f)) return 0;
+        if (f >= 2147483647.0f) return 2147483647;
+        if (f <= -2147483648.0f) return -2147483647 - 1;
+        return static_cast<int32_t>(f);
+    };
     for (int b = 0; b < 4; ++b) {
         lj.band[b] = static_cast<uint32_t>(host.size());
         lj.bw[b] = (b & 1) ? wh : wl;
         lj.delta[b] = 1.0f;
         for (size_t i = 0; i < n[b]; ++i) {
             uint32_t bits;
-            std::memcpy(&bits, &src[b][i], 4);
+            if (reversible)
+                bits = static_cast<uint32_t>(to_int(src[b][i]));
+            else
+                std::memcpy(&bits, &src[b][i], 4);
             host.push_back(bits);
         }
     }
-    lj.float_in = 0xF;
+    lj.float_in = reversible ? 0 : 0xF;
     lj.out = static_cast<uint32_t>(host.size());
     const uint32_t opitch = coef_pitch(w);  // the kernel writes rows at this pitch
     host.resize(host.size() + size_t(opitch) * h, 0);
@@ -1513,13 +1524,14 @@ int sgn_j2k_plan_time_code_blocks(sgn_hip_session *s, sgn_j2k_plan *P, size_t st
 int sgn_j2k_plan_order_code_blocks(sgn_hip_session *s, sgn_j2k_plan *P, int order,
                                    const sgn_cost_model *m) {
     API_GUARD_BEGIN
-    if (!s || !P || order < 0 || order > 2 || (order == 2 && !m)) return SGN_E_ARG;
+    const bool model = order == SGN_ORDER_MODEL || order == SGN_ORDER_GEOMETRY_MODEL;
+    if (!s || !P || order < 0 || order > SGN_ORDER_GEOMETRY_MODEL || (model && !m))
+        return SGN_E_ARG;
     HIP_TRY(hipSetDevice(s->device));
     std::vector<CbJob> v = P->cb_lane_plain;
-    if (order == 0) {
-        std::stable_sort(v.begin(), v.end(),
-                         [](const CbJob &a, const CbJob &b) { return a.cost > b.cost; });
-    } else if (order == 2 && !v.empty()) {
+    // per-block cost key: the model's prediction or the length heuristic
+    std::vector<float> key(v.size());
+    if (model && !v.empty()) {
         DevBuf<float> dpred;
         Status st;
         if ((st = dpred.alloc(v.size()))) return st;
@@ -1527,22 +1539,74 @@ int sgn_j2k_plan_order_code_blocks(sgn_hip_session *s, sgn_j2k_plan *P, int orde
                            dim3(256), 0, s->stream, m->d_params, P->d_cb_lane_plain,
                            uint32_t(v.size()), dpred.p);
         s->dispatches++;
-        std::vector<float> key(v.size());
         HIP_TRY(hipStreamSynchronize(s->stream));
         HIP_TRY(hipMemcpy(key.data(), dpred.p, key.size() * 4, hipMemcpyDeviceToHost));
-        std::vector<uint32_t> idx(v.size());
-        for (size_t i = 0; i < idx.size(); ++i) idx[i] = uint32_t(i);
+    } else {
+        for (size_t i = 0; i < v.size(); ++i) key[i] = float(v[i].cost);
+    }
+    std::vector<uint32_t> idx(v.size());
+    for (size_t i = 0; i < idx.size(); ++i) idx[i] = uint32_t(i);
+    auto area = [&](uint32_t i) { return uint32_t(v[i].w) * v[i].h; };
+    if (order == SGN_ORDER_LENGTH || order == SGN_ORDER_MODEL) {
         std::stable_sort(idx.begin(), idx.end(),
                          [&](uint32_t a, uint32_t b) { return key[a] > key[b]; });
-        std::vector<CbJob> w(v.size());
-        for (size_t i = 0; i < idx.size(); ++i) w[i] = v[idx[i]];
-        v.swap(w);
+    } else if (order == SGN_ORDER_GEOMETRY) {
+        // equal block shapes together, tile order kept inside each shape:
+        // a wave's lanes get equal-sized blocks of neighbouring tiles
+        std::stable_sort(idx.begin(), idx.end(),
+                         [&](uint32_t a, uint32_t b) { return area(a) > area(b); });
+    } else if (order != SGN_ORDER_INPUT) {
+        // shapes as above; inside a shape, tiles by their summed block cost
+        // (a tile's blocks stay together, so neighbouring lanes still read
+        // and write neighbouring memory)
+        std::vector<double> tile_cost(P->ntiles, 0.0);
+        for (size_t i = 0; i < v.size(); ++i)
+            if (v[i].tile < tile_cost.size()) tile_cost[v[i].tile] += key[i];
+        auto tc = [&](uint32_t i) { return v[i].tile < tile_cost.size() ? tile_cost[v[i].tile] : 0.0; };
+        std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) {
+            if (area(a) != area(b)) return area(a) > area(b);
+            return tc(a) > tc(b);
+        });
     }
+    std::vector<CbJob> w(v.size());
+    for (size_t i = 0; i < idx.size(); ++i) w[i] = v[idx[i]];
+    v.swap(w);
     HIP_TRY(hipStreamSynchronize(s->stream));
     if (!v.empty())
         HIP_TRY(hipMemcpy(P->d_cb_lane, v.data(), v.size() * sizeof(CbJob), hipMemcpyHostToDevice));
     P->cb_lane.swap(v);
     drop_graph(*P);
+    return SGN_OK;
+    API_GUARD_END
+}
+
+int sgn_j2k_plan_autotune_order(sgn_hip_session *s, sgn_j2k_plan *P, int reps, float *ms,
+                                int *chosen) {
+    API_GUARD_BEGIN
+    if (!s || !P || reps < 1) return SGN_E_ARG;
+    static const int kOrders[4] = {SGN_ORDER_LENGTH, SGN_ORDER_INPUT, SGN_ORDER_GEOMETRY,
+                                   SGN_ORDER_GEOMETRY_COST};
+    int best_order = -1, best_variant = 0;
+    float best_ms = 0.0f;
+    for (int k = 0; k < 4; ++k) {
+        int rc = sgn_j2k_plan_order_code_blocks(s, P, kOrders[k], nullptr);
+        if (rc) return rc;
+        std::vector<float> t(kRowsVariantCount, 0.0f);
+        int v = 0;
+        rc = sgn_j2k_plan_autotune(s, P, reps, t.data(), t.size(), &v);
+        if (rc) return rc;
+        if (ms) ms[k] = t[v];
+        if (best_order < 0 || t[v] < best_ms) {
+            best_order = kOrders[k];
+            best_variant = v;
+            best_ms = t[v];
+        }
+    }
+    int rc = sgn_j2k_plan_order_code_blocks(s, P, best_order, nullptr);
+    if (rc) return rc;
+    if (P->lane_variant != best_variant) drop_graph(*P);
+    P->lane_variant = best_variant;
+    if (chosen) *chosen = best_order;
     return SGN_OK;
     API_GUARD_END
 }

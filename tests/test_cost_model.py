@@ -87,7 +87,7 @@ def test_device_training_reduces_loss_and_predicts_like_oracle(session, oracle):
 def test_model_gain_over_length_heuristic(session, oracle):
     """Trained on lane-kernel timings, the model predicts held-out decode
     times better than the compressed-length heuristic fitted to the same
-    data; the kernel time of both orders is reported."""
+    data; the kernel time of the orders built on each is reported."""
     streams, pix, _cls = synth.lossless_batch_with_pixels(768, size=256)
     plan = S.TilePlan(session, streams, S.PixelFormat.RGB8)
     feats, cyc = plan.time_code_blocks(stride=3, max_samples=20000)
@@ -119,10 +119,14 @@ def test_model_gain_over_length_heuristic(session, oracle):
         session.synchronize()
         return ev[0].elapsed_ms(ev[1]) / 5
 
+    plan.order_code_blocks(plan.ORDER_GEOMETRY_COST)
+    plan.autotune(reps=2)
     t = {}
     for name, o, m in (("input order", plan.ORDER_INPUT, None),
                        ("length heuristic", plan.ORDER_LENGTH, None),
-                       ("model", plan.ORDER_MODEL, model)):
+                       ("model", plan.ORDER_MODEL, model),
+                       ("shape + tile length heuristic", plan.ORDER_GEOMETRY_COST, None),
+                       ("shape + tile model", plan.ORDER_GEOMETRY_MODEL, model)):
         plan.order_code_blocks(o, m)
         t[name] = lane_ms()
         plan.decode(S.MODE_GRAPH)

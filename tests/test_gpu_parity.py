@@ -189,6 +189,21 @@ def test_every_lane_kernel_variant_and_autotune(session, oracle):
     plan.decode(S.MODE_GRAPH)
     session.synchronize()
     assert all(np.array_equal(s.to_numpy(), img) for s, img in zip(plan.surfaces, pix))
+    # every code-block order decodes the same; the order autotuner keeps one
+    for order in (S.TilePlan.ORDER_INPUT, S.TilePlan.ORDER_GEOMETRY,
+                  S.TilePlan.ORDER_GEOMETRY_COST, S.TilePlan.ORDER_LENGTH):
+        plan.order_code_blocks(order)
+        for s in plan.surfaces:
+            s.fill(0)
+        plan.decode(S.MODE_GRAPH)
+        session.synchronize()
+        assert all(np.array_equal(s.to_numpy(), img) for s, img in zip(plan.surfaces, pix)), order
+    order_ms, chosen = plan.autotune_order(reps=2)
+    assert set(order_ms) == {"length", "input", "geometry", "geometry-cost"}
+    assert chosen in (0, 1, 3, 4)
+    plan.decode(S.MODE_GRAPH)
+    session.synchronize()
+    assert all(np.array_equal(s.to_numpy(), img) for s, img in zip(plan.surfaces, pix))
     plan.close()
 
 
