@@ -559,14 +559,22 @@ __device__ __forceinline__ uint32_t find_job(const J *jobs, uint32_t njobs, uint
     return lo;
 }
 
+// Job of work-group `blk`: from the block table when there is one, else by
+// binary search over the jobs' block bases.
+template <typename J>
+__device__ __forceinline__ const J *job_of(const J *jobs, uint32_t njobs,
+                                           const uint32_t *__restrict__ block_job, uint32_t blk) {
+    return jobs + (block_job ? block_job[blk] : find_job(jobs, njobs, blk));
+}
+
 template <bool REV>
 __device__ __forceinline__ void level_body(const LevelJob *jobs, uint32_t njobs, uint32_t block_offset,
-                           uint32_t *coef) {
+                                           const uint32_t *block_job, uint32_t *coef) {
     constexpr int H = Win<REV>::H, N = Win<REV>::N, P = Win<REV>::P;
     __shared__ typename Sample<REV>::T win[N * P];
     __shared__ uint32_t rows[N];
     const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x) + block_offset;
-    const LevelJob *lj = jobs + find_job(jobs, njobs, blk);
+    const LevelJob *lj = job_of(jobs, njobs, block_job, blk);
     __shared__ BandLds sb;
     const LevelView L = level_view(lj, &sb);
     const uint32_t local = blk - lj->block_base;
@@ -581,13 +589,15 @@ __device__ __forceinline__ void level_body(const LevelJob *jobs, uint32_t njobs,
 }
 
 extern "C" __global__ void __launch_bounds__(kIdwtThreads)
-sgn_idwt_level_rev(const LevelJob *jobs, uint32_t njobs, uint32_t block_offset, uint32_t *coef) {
-    level_body<true>(jobs, njobs, block_offset, coef);
+sgn_idwt_level_rev(const LevelJob *jobs, uint32_t njobs, uint32_t block_offset,
+                   const uint32_t *block_job, uint32_t *coef) {
+    level_body<true>(jobs, njobs, block_offset, block_job, coef);
 }
 
 extern "C" __global__ void __launch_bounds__(kIdwtThreads)
-sgn_idwt_level_irr(const LevelJob *jobs, uint32_t njobs, uint32_t block_offset, uint32_t *coef) {
-    level_body<false>(jobs, njobs, block_offset, coef);
+sgn_idwt_level_irr(const LevelJob *jobs, uint32_t njobs, uint32_t block_offset,
+                   const uint32_t *block_job, uint32_t *coef) {
+    level_body<false>(jobs, njobs, block_offset, block_job, coef);
 }
 
 // ---- final stage -------------------------------------------------------
@@ -628,14 +638,16 @@ __device__ __forceinline__ uint32_t out_u16(float v, uint32_t bd) {
 
 template <bool REV>
 __device__ __forceinline__ void final_body(const FinalJob *jobs, uint32_t njobs, uint32_t block_offset,
-                           const LevelJob *lv, const uint32_t *coef) {
+                                           const uint32_t *block_job, const LevelJob *lv,
+                                           const uint32_t *coef) {
     constexpr int H = Win<REV>::H, N = Win<REV>::N, P = Win<REV>::P;
     using T = typename Sample<REV>::T;
     __shared__ T win[N * P];
     __shared__ uint32_t rows[2][N];    // window-row tables, alternating by component
-    __shared__ T res[2][kBlk * kBlk];  // all but the last component; the last stays in win
+    __shared__ __attribute__((aligned(16))) T res[2][kBlk * kBlk];  // all but the last component;
+                                                                    // the last stays in win
     const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x) + block_offset;
-    const FinalJob *fj = jobs + find_job(jobs, njobs, blk);
+    const FinalJob *fj = job_of(jobs, njobs, block_job, blk);
     const uint32_t local = blk - fj->block_base;
     const uint32_t nbx = fj->nbx;
     const int bx = (int)(local % nbx), by = (int)(local / nbx);
@@ -691,15 +703,29 @@ __device__ __forceinline__ void final_body(const FinalJob *jobs, uint32_t njobs,
         if (x4 >= bw) continue;
         const int np = min(4, bw - x4);
         uint32_t o[4][4];
+        // the four pixels of each LDS plane in one 16-byte read
+        T pl[2][4];
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            if (c + 1 < F.nc) {
+                const int4 q4 = *reinterpret_cast<const int4 *>(&res[c][y * kBlk + x4]);
+                pl[c][0] = __builtin_bit_cast(T, q4.x);
+                pl[c][1] = __builtin_bit_cast(T, q4.y);
+                pl[c][2] = __builtin_bit_cast(T, q4.z);
+                pl[c][3] = __builtin_bit_cast(T, q4.w);
+            } else {
+                pl[c][0] = pl[c][1] = pl[c][2] = pl[c][3] = T(0);
+            }
+        }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int x = x4 + q, ki = y * kBlk + x;
+            const int x = x4 + q;
             float v[3];
             if constexpr (REV) {
                 int32_t s[3] = {0, 0, 0};
 #pragma unroll
                 for (int c = 0; c < 3; ++c)
-                    if (c < F.nc) s[c] = c + 1 < F.nc ? res[c < 2 ? c : 1][ki] : win[(y + H) * P + x + H];
+                    if (c < F.nc) s[c] = c + 1 < F.nc ? pl[c < 2 ? c : 1][q] : win[(y + H) * P + x + H];
                 if (F.mct) {
                     const int32_t i1 = s[0] - ((s[2] + s[1]) >> 2);
                     const int32_t r = s[2] + i1, b = s[1] + i1;
@@ -723,7 +749,7 @@ __device__ __forceinline__ void final_body(const FinalJob *jobs, uint32_t njobs,
             } else {
 #pragma unroll
                 for (int c = 0; c < 3; ++c)
-                    v[c] = c < F.nc ? (c + 1 < F.nc ? res[c < 2 ? c : 1][ki] : win[(y + H) * P + x + H]) : 0.0f;
+                    v[c] = c < F.nc ? (c + 1 < F.nc ? pl[c < 2 ? c : 1][q] : win[(y + H) * P + x + H]) : 0.0f;
                 if (F.mct) {
                     const float y0 = v[0], y1 = v[1], y2 = v[2];
                     v[0] = fmaf(y2, 1.402f, y0);
@@ -776,15 +802,15 @@ __device__ __forceinline__ void final_body(const FinalJob *jobs, uint32_t njobs,
 }
 
 extern "C" __global__ void __launch_bounds__(kIdwtThreads)
-sgn_final_rev(const FinalJob *jobs, uint32_t njobs, uint32_t block_offset, const LevelJob *lv,
-              const uint32_t *coef) {
-    final_body<true>(jobs, njobs, block_offset, lv, coef);
+sgn_final_rev(const FinalJob *jobs, uint32_t njobs, uint32_t block_offset,
+              const uint32_t *block_job, const LevelJob *lv, const uint32_t *coef) {
+    final_body<true>(jobs, njobs, block_offset, block_job, lv, coef);
 }
 
 extern "C" __global__ void __launch_bounds__(kIdwtThreads)
-sgn_final_irr(const FinalJob *jobs, uint32_t njobs, uint32_t block_offset, const LevelJob *lv,
-              const uint32_t *coef) {
-    final_body<false>(jobs, njobs, block_offset, lv, coef);
+sgn_final_irr(const FinalJob *jobs, uint32_t njobs, uint32_t block_offset,
+              const uint32_t *block_job, const LevelJob *lv, const uint32_t *coef) {
+    final_body<false>(jobs, njobs, block_offset, block_job, lv, coef);
 }
 
 // Byte fill used instead of memset nodes (safe under graph capture).

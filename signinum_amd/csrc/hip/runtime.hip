@@ -151,6 +151,8 @@ struct sgn_j2k_plan {
     std::vector<std::vector<LevelRange>> tile_levels;     // per tile
     std::vector<LevelJob> final_lv;
     std::vector<FinalJob> fin[2];
+    // work-group (64x64 output block) -> index of its job in lv[rev] / fin[rev]
+    std::vector<uint32_t> lv_block_job[2], fin_block_job[2];
     std::vector<std::pair<uint8_t, uint32_t>> tile_final_first;  // (rev, first index)
     std::vector<uint32_t> tile_final_count;
     std::vector<uint32_t> tile_width, tile_height;
@@ -168,6 +170,7 @@ struct sgn_j2k_plan {
     LevelJob *d_lv[2] = {nullptr, nullptr};
     LevelJob *d_final_lv = nullptr;
     FinalJob *d_fin[2] = {nullptr, nullptr};
+    uint32_t *d_lv_block_job[2] = {nullptr, nullptr}, *d_fin_block_job[2] = {nullptr, nullptr};
     uint8_t *d_status_lane = nullptr, *d_status_wave = nullptr;
     uint32_t *d_queue = nullptr;  // work counter of the persistent lane kernel
     int lane_variant = 0;         // index into kRowsVariants
@@ -398,6 +401,17 @@ Status build_plan(const sgn_tile_req *jobs, size_t n, sgn_j2k_plan &P, int32_t *
             P.tile_final_count[i]++;
         }
     }
+    // block -> job tables: one load per work-group instead of a search
+    for (int rev = 0; rev < 2; ++rev) {
+        P.lv_block_job[rev].clear();
+        for (size_t j = 0; j < P.lv[rev].size(); ++j)
+            P.lv_block_job[rev].insert(P.lv_block_job[rev].end(),
+                                       size_t(P.lv[rev][j].nbx) * P.lv[rev][j].nby, uint32_t(j));
+        P.fin_block_job[rev].clear();
+        for (size_t j = 0; j < P.fin[rev].size(); ++j)
+            P.fin_block_job[rev].insert(P.fin_block_job[rev].end(),
+                                        size_t(P.fin[rev][j].nbx) * P.fin[rev][j].nby, uint32_t(j));
+    }
     return kOk;
 }
 
@@ -418,6 +432,8 @@ Status upload_plan(const sgn_tile_req *jobs, sgn_j2k_plan &P) {
     const uint64_t o_lv0 = take(bytes_of(P.lv[0])), o_lv1 = take(bytes_of(P.lv[1]));
     const uint64_t o_fin0 = take(bytes_of(P.fin[0])), o_fin1 = take(bytes_of(P.fin[1]));
     const uint64_t o_flv = take(bytes_of(P.final_lv));
+    const uint64_t o_lbj0 = take(bytes_of(P.lv_block_job[0])), o_lbj1 = take(bytes_of(P.lv_block_job[1]));
+    const uint64_t o_fbj0 = take(bytes_of(P.fin_block_job[0])), o_fbj1 = take(bytes_of(P.fin_block_job[1]));
     const uint64_t host_bytes = off;
     const uint64_t o_stl = take(P.cb_lane.size()), o_stw = take(P.cb_wave.size());
     const uint64_t o_queue = take(16), o_coef = take(P.coef_elems * sizeof(uint32_t));
@@ -446,6 +462,10 @@ Status upload_plan(const sgn_tile_req *jobs, sgn_j2k_plan &P) {
     put(o_fin0, P.fin[0]);
     put(o_fin1, P.fin[1]);
     put(o_flv, P.final_lv);
+    put(o_lbj0, P.lv_block_job[0]);
+    put(o_lbj1, P.lv_block_job[1]);
+    put(o_fbj0, P.fin_block_job[0]);
+    put(o_fbj1, P.fin_block_job[1]);
     HIP_TRY(hipMemcpy(P.d_block, h, host_bytes, hipMemcpyHostToDevice));
     uint8_t *d = static_cast<uint8_t *>(P.d_block);
     auto at = [&](uint64_t o, auto *&ptr, bool present) {
@@ -460,6 +480,10 @@ Status upload_plan(const sgn_tile_req *jobs, sgn_j2k_plan &P) {
     at(o_fin0, P.d_fin[0], !P.fin[0].empty());
     at(o_fin1, P.d_fin[1], !P.fin[1].empty());
     at(o_flv, P.d_final_lv, !P.final_lv.empty());
+    at(o_lbj0, P.d_lv_block_job[0], !P.lv_block_job[0].empty());
+    at(o_lbj1, P.d_lv_block_job[1], !P.lv_block_job[1].empty());
+    at(o_fbj0, P.d_fin_block_job[0], !P.fin_block_job[0].empty());
+    at(o_fbj1, P.d_fin_block_job[1], !P.fin_block_job[1].empty());
     at(o_stl, P.d_status_lane, !P.cb_lane.empty());
     at(o_stw, P.d_status_wave, !P.cb_wave.empty());
     at(o_queue, P.d_queue, true);
@@ -537,10 +561,10 @@ uint32_t launch_levels(sgn_hip_session *s, sgn_j2k_plan &P, int rev, uint32_t a,
     const uint32_t off = P.lv[rev][a].block_base;
     if (rev)
         hipLaunchKernelGGL(sgn_idwt_level_rev, dim3(blocks), dim3(kIdwtThreads), 0, s->stream,
-                           P.d_lv[1] + a, b - a, off, P.d_coef);
+                           P.d_lv[1], uint32_t(P.lv[1].size()), off, P.d_lv_block_job[1], P.d_coef);
     else
         hipLaunchKernelGGL(sgn_idwt_level_irr, dim3(blocks), dim3(kIdwtThreads), 0, s->stream,
-                           P.d_lv[0] + a, b - a, off, P.d_coef);
+                           P.d_lv[0], uint32_t(P.lv[0].size()), off, P.d_lv_block_job[0], P.d_coef);
     return 1;
 }
 
@@ -550,10 +574,12 @@ uint32_t launch_finals(sgn_hip_session *s, sgn_j2k_plan &P, int rev, uint32_t a,
     const uint32_t off = P.fin[rev][a].block_base;
     if (rev)
         hipLaunchKernelGGL(sgn_final_rev, dim3(blocks), dim3(kIdwtThreads), 0, s->stream,
-                           P.d_fin[1] + a, b - a, off, P.d_final_lv, P.d_coef);
+                           P.d_fin[1], uint32_t(P.fin[1].size()), off, P.d_fin_block_job[1],
+                           P.d_final_lv, P.d_coef);
     else
         hipLaunchKernelGGL(sgn_final_irr, dim3(blocks), dim3(kIdwtThreads), 0, s->stream,
-                           P.d_fin[0] + a, b - a, off, P.d_final_lv, P.d_coef);
+                           P.d_fin[0], uint32_t(P.fin[0].size()), off, P.d_fin_block_job[0],
+                           P.d_final_lv, P.d_coef);
     return 1;
 }
 
@@ -1243,10 +1269,10 @@ int sgn_idwt_level(sgn_hip_session *s, int reversible, uint32_t x0, uint32_t y0,
     const uint32_t blocks = uint32_t(lj.nbx) * lj.nby;
     if (reversible)
         hipLaunchKernelGGL(sgn_idwt_level_rev, dim3(blocks), dim3(kIdwtThreads), 0, s->stream,
-                           d_job, 1u, 0u, d_coef);
+                           d_job, 1u, 0u, static_cast<const uint32_t *>(nullptr), d_coef);
     else
         hipLaunchKernelGGL(sgn_idwt_level_irr, dim3(blocks), dim3(kIdwtThreads), 0, s->stream,
-                           d_job, 1u, 0u, d_coef);
+                           d_job, 1u, 0u, static_cast<const uint32_t *>(nullptr), d_coef);
     s->dispatches++;
     HIP_TRY(hipStreamSynchronize(s->stream));
     std::vector<uint32_t> res(size_t(w) * h);
