@@ -56,32 +56,47 @@ struct Sample<false> {
 // Per-level constants.  The per-band tables live in LDS (indexed by the
 // band of each sample); the rest are wave-uniform scalars.  Nothing is
 // copied into a by-value struct that per-lane indexing would spill.
+// Band slots are addressed in bytes from `LevelView::origin`, the lowest
+// slot of the level (a tile-component's slots span less than 2 GiB, which
+// the planner checks): relb = slot start, pitchb = row pitch, both in bytes.
 struct BandLds {
-    uint32_t base[4];
-    uint32_t pitch[4];
+    uint32_t relb[4];
+    uint32_t pitchb[4];
     float delta[4];
 };
 
 struct LevelView {
     const BandLds *bands;
+    const uint8_t *origin;
     uint32_t x0, y0, fin, narrow;
     int w, h, levels;
 };
 
 // Stages the band table of `lj` into `sb`; ends with a barrier.
-__device__ __forceinline__ LevelView level_view(const LevelJob *__restrict__ lj, BandLds *sb) {
+__device__ __forceinline__ LevelView level_view(const LevelJob *__restrict__ lj, const uint32_t *coef,
+                                               BandLds *sb) {
     const int t = threadIdx.x;
+    const uint32_t narrow = lj->narrow;
+    const uint32_t used = lj->levels ? 0xFu : 0x1u;
+    auto slot_byte = [&](int b) {
+        return (uint64_t)lj->band[b] << (((narrow >> b) & 1u) ? 1 : 2);
+    };
+    uint64_t lo = slot_byte(0);
+#pragma unroll
+    for (int b = 1; b < 4; ++b)
+        if ((used >> b) & 1u) lo = min(lo, slot_byte(b));
     if (t < 4) {
-        sb->base[t] = lj->band[t];
-        sb->pitch[t] = lj->bw[t];
+        sb->relb[t] = ((used >> t) & 1u) ? (uint32_t)(slot_byte(t) - lo) : 0u;
+        sb->pitchb[t] = lj->bw[t] << (((narrow >> t) & 1u) ? 1 : 2);
         sb->delta[t] = lj->delta[t];
     }
     LevelView v;
     v.bands = sb;
+    v.origin = reinterpret_cast<const uint8_t *>(coef) + lo;
     v.x0 = lj->x0;
     v.y0 = lj->y0;
     v.fin = lj->float_in;
-    v.narrow = lj->narrow;
+    v.narrow = narrow;
     v.w = (int)(lj->x1 - lj->x0);
     v.h = (int)(lj->y1 - lj->y0);
     v.levels = lj->levels;
@@ -300,14 +315,17 @@ __device__ __forceinline__ void lift_window(typename Sample<REV>::T *win, int nh
         lift_window_q<REV, HORIZ, 0>(win, nh, bw, bh, sink);
 }
 
-// Window-row table of block row `by`: band row and row parity (bit 31) of
-// every window row after extension.  No barrier.
+// Window-row table of block row `by`: for every window row after extension
+// and both column parities xp, the byte offset of the row inside band slot
+// xp | 2 * row parity, with the row parity in bit 0 (offsets are even).
+// rows holds 2 * N words.  No barrier.
 template <bool REV>
 __device__ __forceinline__ void window_rows(const LevelView &L, int by, uint32_t *rows) {
     constexpr int H = Win<REV>::H;
     const int oy = by * kBlk;
     const int nh = min(kBlk, L.h - oy) + 2 * H;
-    for (int yy = threadIdx.x; yy < nh; yy += kIdwtThreads) {
+    for (int e = threadIdx.x; e < 2 * nh; e += kIdwtThreads) {
+        const int yy = e >> 1, xp = e & 1;
         const int my = reflect(oy - H + yy, L.h);
         uint32_t row = (uint32_t)my, par = 0;
         if (L.levels) {
@@ -315,7 +333,7 @@ __device__ __forceinline__ void window_rows(const LevelView &L, int by, uint32_t
             par = ay & 1u;
             row = par ? (ay >> 1) - (L.y0 >> 1) : (ay >> 1) - ((L.y0 + 1) >> 1);
         }
-        rows[yy] = row | (par << 31);
+        rows[e] = row * L.bands->pitchb[xp | (par << 1)] + par;
     }
 }
 
@@ -336,11 +354,12 @@ struct WinFetch {
     uint32_t y1m0, y1m1;  // bit k: row k of the pass read the row-parity-1 slots
 };
 
-// Column `xx` of the window: the band column and the two band slots (row
-// parity 0 and 1) it reads.
+// Column `xx` of the window: its parity and, for row parity 0 and 1, the
+// byte offset of the column in the slot (cb1 less one, which cancels the
+// parity bit of the row entry) and the right shift that sign-extends an
+// int16 sample (16) or keeps a 32-bit word (0).
 struct WinColumn {
-    uint32_t base0, pitch0, base1, pitch1;
-    bool n0, n1;  // slot holds int16 samples
+    uint32_t xp, cb0, cb1, d0, d1;
 };
 
 __device__ __forceinline__ WinColumn window_column(const LevelView &L, int ox, int xx, int H) {
@@ -351,37 +370,37 @@ __device__ __forceinline__ WinColumn window_column(const LevelView &L, int ox, i
         xpar = ax & 1u;
         col = xpar ? (ax >> 1) - (L.x0 >> 1) : (ax >> 1) - ((L.x0 + 1) >> 1);
     }
+    const uint32_t n0 = (L.narrow >> xpar) & 1u, n1 = (L.narrow >> (xpar | 2)) & 1u;
     WinColumn c;
-    c.base0 = L.bands->base[xpar] + col;
-    c.pitch0 = L.bands->pitch[xpar];
-    c.base1 = L.bands->base[xpar | 2] + col;
-    c.pitch1 = L.bands->pitch[xpar | 2];
-    c.n0 = (L.narrow >> xpar) & 1u;
-    c.n1 = (L.narrow >> (xpar | 2)) & 1u;
+    c.xp = xpar;
+    c.cb0 = L.bands->relb[xpar] + (col << (n0 ? 1 : 2));
+    c.cb1 = L.bands->relb[xpar | 2] + (col << (n1 ? 1 : 2)) - 1u;
+    c.d0 = n0 ? 16u : 0u;
+    c.d1 = n1 ? 16u : 0u;
     return c;
 }
 
-// Coefficient word at index `idx` of a slot: int16 samples (narrow) are
-// sign-extended; int32 and float words are returned as stored.  Both kinds
-// come from one aligned 32-bit load (an int16 sample's dword lies inside its
-// band's slot), so lanes of mixed slots take one load path.
-__device__ __forceinline__ uint32_t coef_word(const uint32_t *__restrict__ coef, uint32_t idx,
-                                              bool narrow) {
-    const uint64_t byte = (uint64_t)idx << (narrow ? 1 : 2);
-    const uint32_t w = *reinterpret_cast<const uint32_t *>(
-        reinterpret_cast<const uint8_t *>(coef) + (byte & ~uint64_t(3)));
-    const uint32_t up = 16u - 8u * (uint32_t)(byte & 2);  // 16: low half, 0: high half
-    return narrow ? (uint32_t)((int32_t)(w << up) >> 16) : w;
+// Sample word of row-table entry e in column c: an aligned 32-bit load,
+// shifted so that an int16 sample comes out sign-extended and a 32-bit word
+// unchanged.  ok == false loads the level's first word (a value that only
+// reaches outputs outside the block).
+__device__ __forceinline__ uint32_t window_word(const LevelView &L, const WinColumn &c, uint32_t e,
+                                                bool ok) {
+    const bool y1 = (e & 1u) != 0;
+    const uint32_t off = ok ? (y1 ? c.cb1 : c.cb0) + e : 0u;
+    const uint32_t w = *reinterpret_cast<const uint32_t *>(L.origin + (off & ~3u));
+    const uint32_t down = y1 ? c.d1 : c.d0;
+    const uint32_t up = down - ((off & 2u) << 3);
+    return (uint32_t)((int32_t)(w << up) >> down);
 }
 
 // Issues every load of the window of block (bx, by); `rows` holds the
 // block row's window-row table.  Positions outside the window (xx >= nw or
-// yy >= nh) load arena word 0 instead: every lane takes the same path, and
-// those values only reach outputs outside the block.
+// yy >= nh) load the level's first word instead: every lane takes the same
+// path, and those values only reach outputs outside the block.
 template <bool REV>
-__device__ __forceinline__ void fetch_window(const LevelView &L, const uint32_t *__restrict__ coef,
-                                             int bx, int by, const uint32_t *rows,
-                                             WinFetch<REV> &f) {
+__device__ __forceinline__ void fetch_window(const LevelView &L, int bx, int by,
+                                             const uint32_t *rows, WinFetch<REV> &f) {
     constexpr int H = Win<REV>::H, N = Win<REV>::N;
     using F = WinFetch<REV>;
     const int ox = bx * kBlk, oy = by * kBlk;
@@ -393,16 +412,12 @@ __device__ __forceinline__ void fetch_window(const LevelView &L, const uint32_t 
         const WinColumn c = window_column(L, ox, colok ? xx : 0, H);
         uint32_t rr[F::kRows0];
 #pragma unroll
-        for (int k = 0; k < F::kRows0; ++k) rr[k] = rows[y0 + 4 * k];
+        for (int k = 0; k < F::kRows0; ++k) rr[k] = rows[2 * (y0 + 4 * k) + c.xp];
         uint32_t m = 0;
 #pragma unroll
         for (int k = 0; k < F::kRows0; ++k) {
-            const bool ok = colok && y0 + 4 * k < nh;
-            const bool y1 = (rr[k] >> 31) != 0;
-            const uint32_t row = rr[k] & 0x7FFFFFFFu;
-            const uint32_t idx = ok ? (y1 ? c.base1 : c.base0) + row * (y1 ? c.pitch1 : c.pitch0) : 0u;
-            f.raw0[k] = coef_word(coef, idx, ok && (y1 ? c.n1 : c.n0));
-            m |= (y1 ? 1u : 0u) << k;
+            f.raw0[k] = window_word(L, c, rr[k], colok && y0 + 4 * k < nh);
+            m |= (rr[k] & 1u) << k;
         }
         f.y1m0 = m;
     }
@@ -414,13 +429,9 @@ __device__ __forceinline__ void fetch_window(const LevelView &L, const uint32_t 
 #pragma unroll
         for (int k = 0; k < F::kRows1; ++k) {
             const int yy = y0 + F::kStep1 * k;
-            const bool ok = colok && yy < nh;
-            const uint32_t r = rows[min(yy, N - 1)];
-            const bool y1 = (r >> 31) != 0;
-            const uint32_t row = r & 0x7FFFFFFFu;
-            const uint32_t idx = ok ? (y1 ? c.base1 : c.base0) + row * (y1 ? c.pitch1 : c.pitch0) : 0u;
-            f.raw1[k] = coef_word(coef, idx, ok && (y1 ? c.n1 : c.n0));
-            m |= (y1 ? 1u : 0u) << k;
+            const uint32_t e = rows[2 * min(yy, N - 1) + c.xp];
+            f.raw1[k] = window_word(L, c, e, colok && yy < nh);
+            m |= (e & 1u) << k;
         }
         f.y1m1 = m;
     }
@@ -521,7 +532,7 @@ __device__ __forceinline__ void lift_block(const LevelView &L, int bx, int by,
 
 // Synthesises block (bx, by) of one level: window-row table, loads, window,
 // lifting.  Result element (x, y) of an in-place sink's block is at
-// win[(y + H) * P + x + H].  rows: LDS scratch of N words.
+// win[(y + H) * P + x + H].  rows: LDS scratch of 2 N words.
 template <bool REV, class Sink>
 __device__ __forceinline__ void synth_block(const LevelView &L, const uint32_t *__restrict__ coef,
                                             int bx, int by, typename Sample<REV>::T *win,
@@ -530,7 +541,7 @@ __device__ __forceinline__ void synth_block(const LevelView &L, const uint32_t *
     window_rows<REV>(L, by, rows);
     __syncthreads();
     WinFetch<REV> f;
-    fetch_window<REV>(L, coef, bx, by, rows, f);
+    fetch_window<REV>(L, bx, by, rows, f);
     commit_window<REV>(L, bx, by, rows, f, win);
     lift_block<REV>(L, bx, by, win, sink, bw_out, bh_out);
 }
@@ -572,11 +583,11 @@ __device__ __forceinline__ void level_body(const LevelJob *jobs, uint32_t njobs,
                                            const uint32_t *block_job, uint32_t *coef) {
     constexpr int H = Win<REV>::H, N = Win<REV>::N, P = Win<REV>::P;
     __shared__ typename Sample<REV>::T win[N * P];
-    __shared__ uint32_t rows[N];
+    __shared__ uint32_t rows[2 * N];
     const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x) + block_offset;
     const LevelJob *lj = job_of(jobs, njobs, block_job, blk);
     __shared__ BandLds sb;
-    const LevelView L = level_view(lj, &sb);
+    const LevelView L = level_view(lj, coef, &sb);
     const uint32_t local = blk - lj->block_base;
     const uint32_t nbx = lj->nbx, out = lj->out;
     const int bx = (int)(local % nbx), by = (int)(local / nbx);
@@ -643,7 +654,7 @@ __device__ __forceinline__ void final_body(const FinalJob *jobs, uint32_t njobs,
     constexpr int H = Win<REV>::H, N = Win<REV>::N, P = Win<REV>::P;
     using T = typename Sample<REV>::T;
     __shared__ T win[N * P];
-    __shared__ uint32_t rows[2][N];    // window-row tables, alternating by component
+    __shared__ uint32_t rows[2][2 * N];  // window-row tables, alternating by component
     __shared__ __attribute__((aligned(16))) T res[2][kBlk * kBlk];  // all but the last component;
                                                                     // the last stays in win
     const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x) + block_offset;
@@ -669,13 +680,13 @@ __device__ __forceinline__ void final_body(const FinalJob *jobs, uint32_t njobs,
     // Components are pipelined: the loads of component c + 1 are issued
     // before component c is lifted and land in registers meanwhile.
     auto view = [&](int c) {
-        return level_view(lv + (c == 0 ? lvl0 : c == 1 ? lvl1 : lvl2), &sb[c & 1]);
+        return level_view(lv + (c == 0 ? lvl0 : c == 1 ? lvl1 : lvl2), coef, &sb[c & 1]);
     };
     LevelView L = view(0);
     window_rows<REV>(L, by, rows[0]);
     __syncthreads();
     WinFetch<REV> f;
-    fetch_window<REV>(L, coef, bx, by, rows[0], f);
+    fetch_window<REV>(L, bx, by, rows[0], f);
     for (int c = 0; c < F.nc; ++c) {
         // the previous component's vertical pass may still read the window
         if (c) __syncthreads();
@@ -687,7 +698,7 @@ __device__ __forceinline__ void final_body(const FinalJob *jobs, uint32_t njobs,
             next = view(c + 1);
             window_rows<REV>(next, by, rows[(c + 1) & 1]);
             __syncthreads();
-            fetch_window<REV>(next, coef, bx, by, rows[(c + 1) & 1], f);
+            fetch_window<REV>(next, bx, by, rows[(c + 1) & 1], f);
         }
         if (c + 1 < F.nc)
             lift_block<REV>(L, bx, by, win, SinkPlane<REV>{res[c]}, bw, bh);

@@ -112,6 +112,7 @@ Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denomin
             const CompParams &cp = t.comp[c];
             const uint32_t R = g.nres - sk;
             const bool rev = cp.transform == 1;
+            const uint64_t comp_start = lp.coef_elems;
             // coefficient layout: bands of the retained resolutions
             std::vector<uint32_t> band_off(g.bands.size(), 0);
             std::vector<float> band_delta(g.bands.size(), 1.0f);
@@ -236,6 +237,12 @@ Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denomin
                     fj.level[c] = static_cast<uint32_t>(lp.finals_lv.size());
                     lp.finals_lv.push_back(lj);
                 }
+            }
+            // the IDWT kernels address a component's slots with 32-bit byte
+            // offsets from its lowest slot
+            if ((lp.coef_elems - comp_start) * 4 > (uint64_t(1) << 31)) {
+                set_last_error("tile-component too large (coefficients over 2 GiB)");
+                return kErrUnsupported;
             }
             const Rect fr = g.res[R - 1].rect;
             if (c == 0)
