@@ -253,7 +253,9 @@ int sgn_ht_decode_code_blocks_raw(sgn_hip_session *session, const uint8_t *data,
 
 /* decode_single_decomposition_idwt (lib.rs:877; J2kSingleDecompositionIdwtJob
  * lib.rs:767-780): one 2D synthesis level over rect [x0,x1) x [y0,y1);
- * band arrays are row-major with the Annex B band sizes. */
+ * band arrays are row-major with the Annex B band sizes.  For 5/3 the f32
+ * inputs are integers; each is truncated toward zero (NaN to 0, saturating
+ * at the int32 range) before integer lifting. */
 int sgn_idwt_level(sgn_hip_session *session, int reversible, uint32_t x0, uint32_t y0,
                    uint32_t x1, uint32_t y1, const float *ll, const float *hl, const float *lh,
                    const float *hh, float *out);
