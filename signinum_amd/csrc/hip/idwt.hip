@@ -230,11 +230,23 @@ struct SinkWin {
 };
 
 template <bool REV>
-struct SinkPlane {  // a 64x64 LDS plane
+struct SinkPlane {  // a 64x64 LDS plane of samples
     typename Sample<REV>::T *plane;
     static constexpr bool kInPlace = false;
     __device__ __forceinline__ void operator()(int x, int y, typename Sample<REV>::T v) const {
         plane[y * kBlk + x] = v;
+    }
+};
+
+// A 64x64 LDS plane of int16 samples (5/3 only); `ovf` records a sample
+// outside the int16 range (the plane then holds a wrong value).
+struct SinkPlane16 {
+    int16_t *plane;
+    mutable bool ovf;
+    static constexpr bool kInPlace = false;
+    __device__ __forceinline__ void operator()(int x, int y, int32_t v) const {
+        plane[y * kBlk + x] = (int16_t)v;
+        ovf |= v != (int32_t)(int16_t)v;
     }
 };
 
@@ -647,17 +659,22 @@ __device__ __forceinline__ uint32_t out_u16(float v, uint32_t bd) {
     return (uint32_t)(cl > 65535.0f ? 65535.0f : cl);
 }
 
-template <bool REV>
-__device__ __forceinline__ void final_body(const FinalJob *jobs, uint32_t njobs, uint32_t block_offset,
+// Final stage of block `blk`.  PT: sample type of the two LDS planes that
+// hold all but the last component (the last stays in the window).  With
+// int16 planes (5/3) the work-group's LDS drops from 52 to 36 KB (four
+// work-groups per CU instead of three); a block with a sample outside the
+// int16 range writes no output and appends itself to `redo` (count in
+// redo_n), which the int32-plane pass then decodes.
+template <bool REV, typename PT>
+__device__ __forceinline__ void final_body(const FinalJob *jobs, uint32_t njobs, uint32_t blk,
                                            const uint32_t *block_job, const LevelJob *lv,
-                                           const uint32_t *coef) {
+                                           const uint32_t *coef, uint32_t *redo, uint32_t *redo_n) {
     constexpr int H = Win<REV>::H, N = Win<REV>::N, P = Win<REV>::P;
+    constexpr bool kNarrowPlanes = sizeof(PT) == 2;
     using T = typename Sample<REV>::T;
     __shared__ T win[N * P];
-    __shared__ uint32_t rows[2][2 * N];  // window-row tables, alternating by component
-    __shared__ __attribute__((aligned(16))) T res[2][kBlk * kBlk];  // all but the last component;
-                                                                    // the last stays in win
-    const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x) + block_offset;
+    __shared__ uint32_t rows[2 * N];  // window-row table
+    __shared__ __attribute__((aligned(16))) PT res[2][kBlk * kBlk];
     const FinalJob *fj = job_of(jobs, njobs, block_job, blk);
     const uint32_t local = blk - fj->block_base;
     const uint32_t nbx = fj->nbx;
@@ -676,35 +693,33 @@ __device__ __forceinline__ void final_body(const FinalJob *jobs, uint32_t njobs,
     F.same8 = fj->same8;
     const uint32_t lvl0 = fj->level[0], lvl1 = fj->level[1], lvl2 = fj->level[2];
     int bw = 0, bh = 0;
-    __shared__ BandLds sb[2];
-    // Components are pipelined: the loads of component c + 1 are issued
-    // before component c is lifted and land in registers meanwhile.
-    auto view = [&](int c) {
-        return level_view(lv + (c == 0 ? lvl0 : c == 1 ? lvl1 : lvl2), coef, &sb[c & 1]);
-    };
-    LevelView L = view(0);
-    window_rows<REV>(L, by, rows[0]);
-    __syncthreads();
-    WinFetch<REV> f;
-    fetch_window<REV>(L, bx, by, rows[0], f);
+    __shared__ BandLds sb;
+    bool ovf = false;
     for (int c = 0; c < F.nc; ++c) {
-        // the previous component's vertical pass may still read the window
-        if (c) __syncthreads();
-        commit_window<REV>(L, bx, by, rows[c & 1], f, win);
-        LevelView next = L;
+        // level_view ends with a barrier: the previous component's vertical
+        // pass, the last reader of the window, is over before it is rewritten
+        const LevelView L = level_view(lv + (c == 0 ? lvl0 : c == 1 ? lvl1 : lvl2), coef, &sb);
         if (c + 1 < F.nc) {
-            // sb and rows of parity (c + 1) were last read before the
-            // previous component's lifting began
-            next = view(c + 1);
-            window_rows<REV>(next, by, rows[(c + 1) & 1]);
-            __syncthreads();
-            fetch_window<REV>(next, bx, by, rows[(c + 1) & 1], f);
+            if constexpr (kNarrowPlanes) {
+                const SinkPlane16 sink{reinterpret_cast<int16_t *>(res[c]), false};
+                synth_block<REV>(L, coef, bx, by, win, rows, sink, bw, bh);
+                ovf |= sink.ovf;
+            } else {
+                synth_block<REV>(L, coef, bx, by, win, rows,
+                                 SinkPlane<REV>{reinterpret_cast<T *>(res[c])}, bw, bh);
+            }
+        } else {
+            synth_block<REV>(L, coef, bx, by, win, rows, SinkWin<REV>{win}, bw, bh);
         }
-        if (c + 1 < F.nc)
-            lift_block<REV>(L, bx, by, win, SinkPlane<REV>{res[c]}, bw, bh);
-        else
-            lift_block<REV>(L, bx, by, win, SinkWin<REV>{win}, bw, bh);
-        L = next;
+    }
+    if constexpr (kNarrowPlanes) {
+        const bool any = __syncthreads_or(ovf) != 0;
+        if (any) {  // the int32 pass decodes this block
+            if (threadIdx.x == 0) redo[atomicAdd(redo_n, 1u)] = blk;
+            return;
+        }
+    } else {
+        __syncthreads();
     }
     // Each thread converts and stores four horizontally adjacent pixels.
     const bool words = F.bps == 1 && ((reinterpret_cast<uintptr_t>(F.dst) | F.pitch) & 3) == 0;
@@ -714,16 +729,24 @@ __device__ __forceinline__ void final_body(const FinalJob *jobs, uint32_t njobs,
         if (x4 >= bw) continue;
         const int np = min(4, bw - x4);
         uint32_t o[4][4];
-        // the four pixels of each LDS plane in one 16-byte read
+        // the four pixels of each LDS plane in one 16-byte (8-byte) read
         T pl[2][4];
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
             if (c + 1 < F.nc) {
-                const int4 q4 = *reinterpret_cast<const int4 *>(&res[c][y * kBlk + x4]);
-                pl[c][0] = __builtin_bit_cast(T, q4.x);
-                pl[c][1] = __builtin_bit_cast(T, q4.y);
-                pl[c][2] = __builtin_bit_cast(T, q4.z);
-                pl[c][3] = __builtin_bit_cast(T, q4.w);
+                if constexpr (kNarrowPlanes) {
+                    const uint2 q2 = *reinterpret_cast<const uint2 *>(&res[c][y * kBlk + x4]);
+                    pl[c][0] = (int32_t)(int16_t)(q2.x & 0xFFFFu);
+                    pl[c][1] = (int32_t)q2.x >> 16;
+                    pl[c][2] = (int32_t)(int16_t)(q2.y & 0xFFFFu);
+                    pl[c][3] = (int32_t)q2.y >> 16;
+                } else {
+                    const int4 q4 = *reinterpret_cast<const int4 *>(&res[c][y * kBlk + x4]);
+                    pl[c][0] = __builtin_bit_cast(T, q4.x);
+                    pl[c][1] = __builtin_bit_cast(T, q4.y);
+                    pl[c][2] = __builtin_bit_cast(T, q4.z);
+                    pl[c][3] = __builtin_bit_cast(T, q4.w);
+                }
             } else {
                 pl[c][0] = pl[c][1] = pl[c][2] = pl[c][3] = T(0);
             }
@@ -815,13 +838,37 @@ __device__ __forceinline__ void final_body(const FinalJob *jobs, uint32_t njobs,
 extern "C" __global__ void __launch_bounds__(kIdwtThreads)
 sgn_final_rev(const FinalJob *jobs, uint32_t njobs, uint32_t block_offset,
               const uint32_t *block_job, const LevelJob *lv, const uint32_t *coef) {
-    final_body<true>(jobs, njobs, block_offset, block_job, lv, coef);
+    final_body<true, int32_t>(jobs, njobs, xcd_remap(blockIdx.x, gridDim.x) + block_offset,
+                              block_job, lv, coef, nullptr, nullptr);
+}
+
+// 5/3 final stage with int16 component planes.  *redo_n must be zero at
+// launch; blocks with a sample outside the int16 range are listed in `redo`
+// (capacity: the launch's block count) for sgn_final_rev_redo.
+extern "C" __global__ void __launch_bounds__(kIdwtThreads)
+sgn_final_rev16(const FinalJob *jobs, uint32_t njobs, uint32_t block_offset,
+                const uint32_t *block_job, const LevelJob *lv, const uint32_t *coef,
+                uint32_t *redo, uint32_t *redo_n) {
+    final_body<true, int16_t>(jobs, njobs, xcd_remap(blockIdx.x, gridDim.x) + block_offset,
+                              block_job, lv, coef, redo, redo_n);
+}
+
+// The blocks sgn_final_rev16 listed, with int32 planes; a fixed grid walks
+// the list (usually empty) and every work-group exits when it ends.
+extern "C" __global__ void __launch_bounds__(kIdwtThreads)
+sgn_final_rev_redo(const FinalJob *jobs, uint32_t njobs, const uint32_t *block_job,
+                   const LevelJob *lv, const uint32_t *coef, const uint32_t *redo,
+                   const uint32_t *redo_n) {
+    const uint32_t n = *redo_n;
+    for (uint32_t k = blockIdx.x; k < n; k += gridDim.x)
+        final_body<true, int32_t>(jobs, njobs, redo[k], block_job, lv, coef, nullptr, nullptr);
 }
 
 extern "C" __global__ void __launch_bounds__(kIdwtThreads)
 sgn_final_irr(const FinalJob *jobs, uint32_t njobs, uint32_t block_offset,
               const uint32_t *block_job, const LevelJob *lv, const uint32_t *coef) {
-    final_body<false>(jobs, njobs, block_offset, block_job, lv, coef);
+    final_body<false, float>(jobs, njobs, xcd_remap(blockIdx.x, gridDim.x) + block_offset,
+                             block_job, lv, coef, nullptr, nullptr);
 }
 
 // Byte fill used instead of memset nodes (safe under graph capture).

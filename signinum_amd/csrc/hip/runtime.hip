@@ -171,6 +171,7 @@ struct sgn_j2k_plan {
     LevelJob *d_final_lv = nullptr;
     FinalJob *d_fin[2] = {nullptr, nullptr};
     uint32_t *d_lv_block_job[2] = {nullptr, nullptr}, *d_fin_block_job[2] = {nullptr, nullptr};
+    uint32_t *d_fin_redo = nullptr;  // 5/3 final blocks left to the int32-plane pass
     uint8_t *d_status_lane = nullptr, *d_status_wave = nullptr;
     uint32_t *d_queue = nullptr;  // work counter of the persistent lane kernel
     int lane_variant = 0;         // index into kRowsVariants
@@ -436,6 +437,7 @@ Status upload_plan(const sgn_tile_req *jobs, sgn_j2k_plan &P) {
     const uint64_t o_fbj0 = take(bytes_of(P.fin_block_job[0])), o_fbj1 = take(bytes_of(P.fin_block_job[1]));
     const uint64_t host_bytes = off;
     const uint64_t o_stl = take(P.cb_lane.size()), o_stw = take(P.cb_wave.size());
+    const uint64_t o_ff = take(P.fin_block_job[1].size() * sizeof(uint32_t));
     const uint64_t o_queue = take(16), o_coef = take(P.coef_elems * sizeof(uint32_t));
     if (hipMalloc(&P.d_block, off) != hipSuccess) {
         P.d_block = nullptr;
@@ -486,6 +488,7 @@ Status upload_plan(const sgn_tile_req *jobs, sgn_j2k_plan &P) {
     at(o_fbj1, P.d_fin_block_job[1], !P.fin_block_job[1].empty());
     at(o_stl, P.d_status_lane, !P.cb_lane.empty());
     at(o_stw, P.d_status_wave, !P.cb_wave.empty());
+    at(o_ff, P.d_fin_redo, !P.fin_block_job[1].empty());
     at(o_queue, P.d_queue, true);
     at(o_coef, P.d_coef, P.coef_elems != 0);
     return kOk;
@@ -572,14 +575,27 @@ uint32_t launch_finals(sgn_hip_session *s, sgn_j2k_plan &P, int rev, uint32_t a,
     const uint32_t blocks = final_blocks(P.fin[rev], a, b);
     if (!blocks) return 0;
     const uint32_t off = P.fin[rev][a].block_base;
-    if (rev)
-        hipLaunchKernelGGL(sgn_final_rev, dim3(blocks), dim3(kIdwtThreads), 0, s->stream,
+    if (rev) {
+        // int16-plane pass over every block, then the int32-plane pass over
+        // the blocks it listed (word 1 of the work-counter area counts them;
+        // cleared by a kernel, graph-safe)
+        uint32_t *redo_n = P.d_queue + 1;
+        hipLaunchKernelGGL(sgn_fill_bytes, dim3(1), dim3(64), 0, s->stream,
+                           reinterpret_cast<uint8_t *>(redo_n), size_t(4), size_t(1), 0u);
+        hipLaunchKernelGGL(sgn_final_rev16, dim3(blocks), dim3(kIdwtThreads), 0, s->stream,
                            P.d_fin[1], uint32_t(P.fin[1].size()), off, P.d_fin_block_job[1],
-                           P.d_final_lv, P.d_coef);
-    else
-        hipLaunchKernelGGL(sgn_final_irr, dim3(blocks), dim3(kIdwtThreads), 0, s->stream,
-                           P.d_fin[0], uint32_t(P.fin[0].size()), off, P.d_fin_block_job[0],
-                           P.d_final_lv, P.d_coef);
+                           P.d_final_lv, P.d_coef, P.d_fin_redo, redo_n);
+        const uint32_t cus = s->cus ? uint32_t(s->cus) : 256u;
+        hipLaunchKernelGGL(sgn_final_rev_redo, dim3(std::min(blocks, 2 * cus)), dim3(kIdwtThreads),
+                           0, s->stream, P.d_fin[1], uint32_t(P.fin[1].size()),
+                           P.d_fin_block_job[1], P.d_final_lv, P.d_coef,
+                           static_cast<const uint32_t *>(P.d_fin_redo),
+                           static_cast<const uint32_t *>(redo_n));
+        return 3;
+    }
+    hipLaunchKernelGGL(sgn_final_irr, dim3(blocks), dim3(kIdwtThreads), 0, s->stream,
+                       P.d_fin[0], uint32_t(P.fin[0].size()), off, P.d_fin_block_job[0],
+                       P.d_final_lv, P.d_coef);
     return 1;
 }
 

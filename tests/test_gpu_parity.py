@@ -81,6 +81,26 @@ def test_other_formats(session, oracle, fmt):
     assert np.array_equal(_gpu(session, cs, fmt), O.decode(cs, int(fmt)))
 
 
+def test_sixteen_bit_rct_uses_the_int32_plane_pass(session, oracle):
+    """16-bit RGB with RCT: colour-difference components reach +-65535, past
+    the int16 planes of the fast 5/3 final pass, so their blocks are redone
+    by the int32 pass; the left half is dark (fits int16), the right bright."""
+    rng = np.random.default_rng(161)
+    h, w = 150, 200
+    img = np.zeros((h, w, 3), np.uint16)
+    img[:, : w // 2] = rng.integers(0, 600, (h, w // 2, 3))
+    img[:, w // 2:, 0] = rng.integers(60000, 65536, (h, w - w // 2))
+    img[:, w // 2:, 1] = rng.integers(0, 2000, (h, w - w // 2))
+    img[:, w // 2:, 2] = rng.integers(30000, 65536, (h, w - w // 2))
+    cs = S.encode_j2k_lossless(img, bit_depth=16, levels=4)
+    out = _gpu(session, cs, S.PixelFormat.RGB16)
+    assert np.array_equal(out, O.decode(cs, int(S.PixelFormat.RGB16)))
+    assert np.array_equal(out, img)
+    # the same tiles inside a batch plan
+    surfs = S.decode_tiles_to_device(session, [cs, cs], S.PixelFormat.RGB16, mode=S.MODE_GRAPH)
+    assert all(np.array_equal(s.to_numpy(), img) for s in surfs)
+
+
 def test_multi_tile_codestream(session, oracle):
     rng = np.random.default_rng(21)
     img = synth.gradient_noise_rgb(rng, 300, 200)
