@@ -180,6 +180,7 @@ struct sgn_j2k_plan {
     uint32_t launches_batched = 0;
 
     void *d_block = nullptr;  // the one allocation every device pointer above lives in
+    uint64_t device_bytes = 0;  // its size
 
     ~sgn_j2k_plan() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
@@ -439,6 +440,7 @@ Status upload_plan(const sgn_tile_req *jobs, sgn_j2k_plan &P) {
     const uint64_t o_stl = take(P.cb_lane.size()), o_stw = take(P.cb_wave.size());
     const uint64_t o_ff = take(P.fin_block_job[1].size() * sizeof(uint32_t));
     const uint64_t o_queue = take(16), o_coef = take(P.coef_elems * sizeof(uint32_t));
+    P.device_bytes = off;
     if (hipMalloc(&P.d_block, off) != hipSuccess) {
         P.d_block = nullptr;
         set_last_error("hipMalloc failed for the plan (" + std::to_string(off) + " bytes)");

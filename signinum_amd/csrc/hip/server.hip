@@ -46,13 +46,7 @@ struct ServeBatch {
     uint64_t plan_bytes = 0;
 };
 
-uint64_t plan_device_bytes(const sgn_j2k_plan &P) {
-    return P.arena_len + P.coef_elems * 4 +
-           (P.cb_lane.size() * 2 + P.cb_wave.size()) * sizeof(CbJob) +
-           (P.lv[0].size() + P.lv[1].size() + P.final_lv.size()) * sizeof(LevelJob) +
-           (P.fin[0].size() + P.fin[1].size()) * sizeof(FinalJob) + P.cb_lane.size() +
-           P.cb_wave.size() + 16;
-}
+uint64_t plan_device_bytes(const sgn_j2k_plan &P) { return P.device_bytes; }
 
 }  // namespace
 
