@@ -527,19 +527,25 @@ uint32_t resident_groups(sgn_hip_session *s, int v, bool narrow) {
     return s->resident[slot];
 }
 
+// Clears the plan's device counters (word 0: lane work counter, word 1:
+// final redo count) with a kernel, never a memset node (graph capture).
+uint32_t clear_counters(sgn_hip_session *s, sgn_j2k_plan &P, uint32_t words) {
+    hipLaunchKernelGGL(sgn_fill_bytes, dim3(1), dim3(64), 0, s->stream,
+                       reinterpret_cast<uint8_t *>(P.d_queue), size_t(4) * words, size_t(words), 0u);
+    return 1;
+}
+
+// clear: reset the work counter first (false: the caller already did)
 uint32_t launch_lane_variant(sgn_hip_session *s, sgn_j2k_plan &P, int v, const CbJob *d_jobs,
-                             uint32_t count, uint8_t *status) {
+                             uint32_t count, uint8_t *status, bool clear = true) {
     if (!count) return 0;
     const RowsVariant &rv = kRowsVariants[v];
     const uint32_t per_group = 64u * uint32_t(rv.waves);
     uint32_t grid = (count + per_group - 1) / per_group;
     uint32_t k = 1;
     if (rv.persist) {
-        // the counter is cleared by a kernel, never a memset node (graph capture)
-        hipLaunchKernelGGL(sgn_fill_bytes, dim3(1), dim3(64), 0, s->stream,
-                           reinterpret_cast<uint8_t *>(P.d_queue), size_t(4), size_t(1), 0u);
+        if (clear) k += clear_counters(s, P, 1);
         grid = std::min(grid, resident_groups(s, v, P.narrow));
-        ++k;
     }
     rv.launch[P.narrow ? 1 : 0](s->stream, grid, d_jobs, count, P.d_arena, P.d_coef, s->d_tables,
                                 status, P.d_queue);
@@ -547,8 +553,8 @@ uint32_t launch_lane_variant(sgn_hip_session *s, sgn_j2k_plan &P, int v, const C
 }
 
 uint32_t launch_lane(sgn_hip_session *s, sgn_j2k_plan &P, const CbJob *d_jobs, uint32_t count,
-                     uint8_t *status) {
-    return launch_lane_variant(s, P, P.lane_variant, d_jobs, count, status);
+                     uint8_t *status, bool clear = true) {
+    return launch_lane_variant(s, P, P.lane_variant, d_jobs, count, status, clear);
 }
 
 uint32_t launch_wave(sgn_hip_session *s, sgn_j2k_plan &P, const CbJob *d_jobs, uint32_t count,
@@ -573,7 +579,8 @@ uint32_t launch_levels(sgn_hip_session *s, sgn_j2k_plan &P, int rev, uint32_t a,
     return 1;
 }
 
-uint32_t launch_finals(sgn_hip_session *s, sgn_j2k_plan &P, int rev, uint32_t a, uint32_t b) {
+uint32_t launch_finals(sgn_hip_session *s, sgn_j2k_plan &P, int rev, uint32_t a, uint32_t b,
+                       bool clear = true) {
     const uint32_t blocks = final_blocks(P.fin[rev], a, b);
     if (!blocks) return 0;
     const uint32_t off = P.fin[rev][a].block_base;
@@ -582,8 +589,12 @@ uint32_t launch_finals(sgn_hip_session *s, sgn_j2k_plan &P, int rev, uint32_t a,
         // the blocks it listed (word 1 of the work-counter area counts them;
         // cleared by a kernel, graph-safe)
         uint32_t *redo_n = P.d_queue + 1;
-        hipLaunchKernelGGL(sgn_fill_bytes, dim3(1), dim3(64), 0, s->stream,
-                           reinterpret_cast<uint8_t *>(redo_n), size_t(4), size_t(1), 0u);
+        uint32_t k = 2;
+        if (clear) {
+            hipLaunchKernelGGL(sgn_fill_bytes, dim3(1), dim3(64), 0, s->stream,
+                               reinterpret_cast<uint8_t *>(redo_n), size_t(4), size_t(1), 0u);
+            ++k;
+        }
         hipLaunchKernelGGL(sgn_final_rev16, dim3(blocks), dim3(kIdwtThreads), 0, s->stream,
                            P.d_fin[1], uint32_t(P.fin[1].size()), off, P.d_fin_block_job[1],
                            P.d_final_lv, P.d_coef, P.d_fin_redo, redo_n);
@@ -593,7 +604,7 @@ uint32_t launch_finals(sgn_hip_session *s, sgn_j2k_plan &P, int rev, uint32_t a,
                            P.d_fin_block_job[1], P.d_final_lv, P.d_coef,
                            static_cast<const uint32_t *>(P.d_fin_redo),
                            static_cast<const uint32_t *>(redo_n));
-        return 3;
+        return k;
     }
     hipLaunchKernelGGL(sgn_final_irr, dim3(blocks), dim3(kIdwtThreads), 0, s->stream,
                        P.d_fin[0], uint32_t(P.fin[0].size()), off, P.d_fin_block_job[0],
@@ -602,14 +613,15 @@ uint32_t launch_finals(sgn_hip_session *s, sgn_j2k_plan &P, int rev, uint32_t a,
 }
 
 uint32_t enqueue_batched(sgn_hip_session *s, sgn_j2k_plan &P) {
-    uint32_t k = 0;
-    k += launch_lane(s, P, P.d_cb_lane, static_cast<uint32_t>(P.cb_lane.size()), P.d_status_lane);
+    uint32_t k = clear_counters(s, P, 2);  // one node for both counters
+    k += launch_lane(s, P, P.d_cb_lane, static_cast<uint32_t>(P.cb_lane.size()), P.d_status_lane,
+                     false);
     k += launch_wave(s, P, P.d_cb_wave, static_cast<uint32_t>(P.cb_wave.size()), P.d_status_wave);
     for (uint32_t d = 1; d <= P.depth_max; ++d)
         for (int rev = 0; rev < 2; ++rev)
             k += launch_levels(s, P, rev, P.lv_group[rev][d - 1], P.lv_group[rev][d]);
     for (int rev = 0; rev < 2; ++rev)
-        k += launch_finals(s, P, rev, 0, static_cast<uint32_t>(P.fin[rev].size()));
+        k += launch_finals(s, P, rev, 0, static_cast<uint32_t>(P.fin[rev].size()), false);
     return k;
 }
 
