@@ -108,8 +108,7 @@ __device__ __forceinline__ void rows_wave(const CbJob *__restrict__ jobs, uint32
                                           uint32_t first, const uint8_t *__restrict__ arena,
                                           C *__restrict__ coef,
                                           uint8_t *__restrict__ status, const uint16_t *s_tab,
-                                          uint8_t (*s_rho)[WAVES * 64],
-                                          uint8_t (*s_e)[WAVES * 64], uint4 *s_ms, uint4 *s_vlc,
+                                          uint8_t (*s_q)[WAVES * 64], uint4 *s_ms, uint4 *s_vlc,
                                           uint4 *s_mel) {
     constexpr int kSegPairs = SEG, kMsSlots = MS, kVlcSlots = VS, kMelSlots = LS;
     const uint16_t *vlc0 = s_tab, *vlc1 = s_tab + 1024, *uvlc0 = s_tab + 2048,
@@ -223,8 +222,7 @@ __device__ __forceinline__ void rows_wave(const CbJob *__restrict__ jobs, uint32
     const uint32_t total = run_block ? nrows * nseg : 0;
     if (run_block)
         for (uint32_t q = 0; q < nq + 2; ++q) {
-            s_rho[q][tid] = 0;
-            s_e[q][tid] = 0;
+            s_q[q][tid] = 0;
         }
     int run = 0;
     if (run_block) run = mel_run4(mel, fmel, RingSrc<kMelSlots>{ring_mel, have_mel, arena});
@@ -258,9 +256,10 @@ __device__ __forceinline__ void rows_wave(const CbJob *__restrict__ jobs, uint32
         for (uint32_t pr = pa; pr < pb; ++pr) {
             const uint32_t x = 4 * pr, qj = 2 * pr;
             // ---- cleanup symbols of the quad pair ----
-            const uint32_t pt0 = (uint32_t)s_rho[qj][tid] << 4;
-            const uint32_t pt1 = (uint32_t)s_rho[qj + 1][tid] << 4;
-            const uint32_t pt2 = (uint32_t)s_rho[qj + 2][tid] << 4;
+            // quad state of the row above: significance bits 1 and 3 of
+            // the quad at 0x20 / 0x80 (the only ones the contexts use) and
+            // the exponent bound of its column pair in bits 0..4
+            const uint32_t pt0 = s_q[qj][tid], pt1 = s_q[qj + 1][tid], pt2 = s_q[qj + 2][tid];
             uint32_t t0, t1, u0, u1;
             // one fill per pair: a pair takes at most 7 + 7 + 3 + 10 bits
             vlc_fill(vlc, rvlc, RingSrc<kVlcSlots>{ring_vlc, have_vlc, arena});
@@ -344,8 +343,6 @@ __device__ __forceinline__ void rows_wave(const CbJob *__restrict__ jobs, uint32
                 u0 = (e & 7) + (tmp & ~(0xFFu << len));
                 u1 = (e >> 3) + (tmp >> len);
             }
-            s_rho[qj][tid] = (uint8_t)((t0 >> 4) & 0xF);
-            s_rho[qj + 1][tid] = (uint8_t)((t1 >> 4) & 0xF);
 
             // ---- magnitude / sign of both quads ----
             uint32_t r0[4] = {0, 0, 0, 0}, r1[4] = {0, 0, 0, 0};  // rows y, y+1 at x..x+3
@@ -359,7 +356,7 @@ __device__ __forceinline__ void rows_wave(const CbJob *__restrict__ jobs, uint32
                 if (!first_row) {
                     uint32_t gamma = inf & 0xF0;
                     gamma &= gamma - 0x10;
-                    uint32_t emax = max((uint32_t)s_e[q][tid], (uint32_t)s_e[q + 1][tid]);
+                    uint32_t emax = max((uint32_t)s_q[q][tid] & 0x1Fu, (uint32_t)s_q[q + 1][tid] & 0x1Fu);
                     emax = max(emax, 1u);
                     uq += gamma ? emax : 1u;
                 }
@@ -384,7 +381,7 @@ __device__ __forceinline__ void rows_wave(const CbJob *__restrict__ jobs, uint32
                 const uint32_t a1 = ms_value(ms, inf, 1, uq, p, v1);
                 r0[2 * k] = (uint32_t)to_i32(a0, kmax);
                 r1[2 * k] = (uint32_t)to_i32(a1, kmax);
-                s_e[q][tid] = (uint8_t)max(prev_e3, flog2r(v1));
+                s_q[q][tid] = (uint8_t)((inf & 0xA0u) | max(prev_e3, flog2r(v1)));
                 prev_e3 = 0;
                 if (xq + 1 >= w) break;
                 if (!one_fill) ms_fill(ms, fms, RingSrc<kMsSlots>{ring_ms, have_ms, arena});
@@ -451,7 +448,7 @@ __device__ __forceinline__ void rows_wave(const CbJob *__restrict__ jobs, uint32
             }
             if (st != kCbOk) break;
         }
-        if (pb == npairs) s_e[nq][tid] = (uint8_t)prev_e3;
+        if (pb == npairs) s_q[nq][tid] = (uint8_t)prev_e3;
     }
     if (valid) status[i] = st;
 }
@@ -460,14 +457,13 @@ __device__ __forceinline__ void rows_wave(const CbJob *__restrict__ jobs, uint32
 // waves take the next 64 code-blocks (in the host's cost order) from the
 // device work counter *queue until none are left; otherwise one wave per 64
 // code-blocks.  *queue must be zero at launch.
-template <int WAVES, int SEG, int MS, int VS, int LS, bool PERSIST, typename C>
-__global__ void __launch_bounds__(WAVES * 64)
+template <int WAVES, int SEG, int MS, int VS, int LS, bool PERSIST, typename C, int MINW = 1>
+__global__ void __launch_bounds__(WAVES * 64, MINW)
 sgn_ht_rows(const CbJob *__restrict__ jobs, uint32_t n, const uint8_t *__restrict__ arena,
             C *__restrict__ coef, const uint16_t *__restrict__ tables,
             uint8_t *__restrict__ status, uint32_t *__restrict__ queue) {
     __shared__ uint16_t s_tab[1024 + 1024 + 320 + 256];
-    __shared__ uint8_t s_rho[kRowsMaxQ][WAVES * 64];  // significance nibble of the quad above
-    __shared__ uint8_t s_e[kRowsMaxQ][WAVES * 64];    // floor(log2(v_n)) per column pair
+    __shared__ uint8_t s_q[kRowsMaxQ][WAVES * 64];  // quad state of the row above
     __shared__ uint4 s_ms[WAVES * 64 * MS];
     __shared__ uint4 s_vlc[WAVES * 64 * VS];
     __shared__ uint4 s_mel[WAVES * 64 * LS];
@@ -475,7 +471,7 @@ sgn_ht_rows(const CbJob *__restrict__ jobs, uint32_t n, const uint8_t *__restric
     __syncthreads();
     if constexpr (!PERSIST) {
         const uint32_t first = (blockIdx.x * WAVES + (threadIdx.x >> 6)) * 64;
-        rows_wave<WAVES, SEG, MS, VS, LS, C>(jobs, n, first, arena, coef, status, s_tab, s_rho, s_e, s_ms,
+        rows_wave<WAVES, SEG, MS, VS, LS, C>(jobs, n, first, arena, coef, status, s_tab, s_q, s_ms,
                                   s_vlc, s_mel);
     } else {
         for (;;) {
@@ -483,7 +479,7 @@ sgn_ht_rows(const CbJob *__restrict__ jobs, uint32_t n, const uint8_t *__restric
             if ((threadIdx.x & 63) == 0) first = atomicAdd(queue, 64u);
             first = __shfl(first, 0);
             if (first >= n) break;
-            rows_wave<WAVES, SEG, MS, VS, LS, C>(jobs, n, first, arena, coef, status, s_tab, s_rho, s_e,
+            rows_wave<WAVES, SEG, MS, VS, LS, C>(jobs, n, first, arena, coef, status, s_tab, s_q,
                                       s_ms, s_vlc, s_mel);
         }
     }
@@ -499,8 +495,7 @@ sgn_ht_rows_timed(const CbJob *__restrict__ jobs, uint32_t n, const uint8_t *__r
                   uint8_t *__restrict__ status, uint64_t *__restrict__ cycles) {
     constexpr int W = 4, S = 8, M = 8, VS = 4, LS = 2;
     __shared__ uint16_t s_tab[1024 + 1024 + 320 + 256];
-    __shared__ uint8_t s_rho[kRowsMaxQ][W * 64];
-    __shared__ uint8_t s_e[kRowsMaxQ][W * 64];
+    __shared__ uint8_t s_q[kRowsMaxQ][W * 64];
     __shared__ uint4 s_ms[W * 64 * M];
     __shared__ uint4 s_vlc[W * 64 * VS];
     __shared__ uint4 s_mel[W * 64 * LS];
@@ -508,7 +503,7 @@ sgn_ht_rows_timed(const CbJob *__restrict__ jobs, uint32_t n, const uint8_t *__r
     __syncthreads();
     const uint32_t wave = blockIdx.x * W + (threadIdx.x >> 6);
     const uint64_t t0 = clock64();
-    rows_wave<W, S, M, VS, LS, C>(jobs, n, wave * 64, arena, coef, status, s_tab, s_rho, s_e, s_ms, s_vlc,
+    rows_wave<W, S, M, VS, LS, C>(jobs, n, wave * 64, arena, coef, status, s_tab, s_q, s_ms, s_vlc,
                        s_mel);
     const uint64_t t1 = clock64();
     if ((threadIdx.x & 63) == 0 && wave * 64 < n) cycles[wave] = t1 - t0;

@@ -59,12 +59,12 @@ namespace {
 using RowsLaunch = void (*)(hipStream_t, uint32_t, const CbJob *, uint32_t, const uint8_t *,
                             void *, const uint16_t *, uint8_t *, uint32_t *);
 
-template <int W, int S, int M, int VS, int LS, bool PERSIST, typename C>
+template <int W, int S, int M, int VS, int LS, bool PERSIST, typename C, int MINW>
 void rows_launch(hipStream_t st, uint32_t grid, const CbJob *jobs, uint32_t n,
                  const uint8_t *arena, void *coef, const uint16_t *tables, uint8_t *status,
                  uint32_t *queue) {
-    hipLaunchKernelGGL((sgn_ht_rows<W, S, M, VS, LS, PERSIST, C>), dim3(grid), dim3(W * 64), 0,
-                       st, jobs, n, arena, static_cast<C *>(coef), tables, status, queue);
+    hipLaunchKernelGGL((sgn_ht_rows<W, S, M, VS, LS, PERSIST, C, MINW>), dim3(grid), dim3(W * 64),
+                       0, st, jobs, n, arena, static_cast<C *>(coef), tables, status, queue);
 }
 
 // launch[0] / kernel[0]: int32 samples; [1]: int16 samples (narrow plans)
@@ -76,23 +76,28 @@ struct RowsVariant {
     const void *kernel[2];
 };
 
-#define SGN_ROWS_VARIANT(name, W, S, M, VS, LS, Q)                                           \
+// MINW: waves per SIMD the register allocation must allow (1: no bound)
+#define SGN_ROWS_VARIANT(name, W, S, M, VS, LS, Q, MINW)                                     \
     {name,                                                                                 \
      W,                                                                                    \
      Q,                                                                                    \
-     {rows_launch<W, S, M, VS, LS, Q, int32_t>, rows_launch<W, S, M, VS, LS, Q, int16_t>}, \
-     {reinterpret_cast<const void *>(sgn_ht_rows<W, S, M, VS, LS, Q, int32_t>),            \
-      reinterpret_cast<const void *>(sgn_ht_rows<W, S, M, VS, LS, Q, int16_t>)}}
+     {rows_launch<W, S, M, VS, LS, Q, int32_t, MINW>,                                      \
+      rows_launch<W, S, M, VS, LS, Q, int16_t, MINW>},                                     \
+     {reinterpret_cast<const void *>(sgn_ht_rows<W, S, M, VS, LS, Q, int32_t, MINW>),      \
+      reinterpret_cast<const void *>(sgn_ht_rows<W, S, M, VS, LS, Q, int16_t, MINW>)}}
 
 // name: waves per work-group, quad pairs per segment, small rings (3 work-
 // groups per CU instead of 2), persistent queue
 const RowsVariant kRowsVariants[] = {
-    SGN_ROWS_VARIANT("w4-seg8", 4, 8, 8, 4, 2, false),
-    SGN_ROWS_VARIANT("w4-seg8-queue", 4, 8, 8, 4, 2, true),
-    SGN_ROWS_VARIANT("w4-seg4-queue", 4, 4, 4, 4, 2, true),
-    SGN_ROWS_VARIANT("w4-seg4-small-queue", 4, 4, 4, 2, 1, true),
-    SGN_ROWS_VARIANT("w2-seg16-queue", 2, 16, 16, 4, 2, true),
-    SGN_ROWS_VARIANT("w1-seg8-queue", 1, 8, 8, 4, 2, true),
+    SGN_ROWS_VARIANT("w4-seg8", 4, 8, 8, 4, 2, false, 1),
+    SGN_ROWS_VARIANT("w4-seg8-queue", 4, 8, 8, 4, 2, true, 1),
+    SGN_ROWS_VARIANT("w4-seg4-queue", 4, 4, 4, 4, 2, true, 1),
+    SGN_ROWS_VARIANT("w4-seg4-small-queue", 4, 4, 4, 2, 1, true, 1),
+    SGN_ROWS_VARIANT("w2-seg16-queue", 2, 16, 16, 4, 2, true, 1),
+    SGN_ROWS_VARIANT("w1-seg8-queue", 1, 8, 8, 4, 2, true, 1),
+    // eight waves sharing the tables, registers bounded for four waves per
+    // SIMD: two work-groups (16 waves) per CU
+    SGN_ROWS_VARIANT("w8-seg4-small-queue", 8, 4, 4, 2, 1, true, 4),
 };
 #undef SGN_ROWS_VARIANT
 constexpr int kRowsVariantCount = int(sizeof(kRowsVariants) / sizeof(kRowsVariants[0]));
