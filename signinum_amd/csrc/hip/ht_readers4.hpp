@@ -230,9 +230,12 @@ __device__ __forceinline__ int mel_run4(Mel &m, Fwd4 &f, const Src &src) {
     return first ? (int)(((1u << e) - 1) << 1) : (int)((v << 1) | 1);
 }
 
-// One sample of the magnitude/sign phase (ht_block_decode.rs:1010-1032).
-__device__ __forceinline__ uint32_t ms_value(MsBits &ms, uint32_t inf, uint32_t bit, uint32_t uq,
-                                             uint32_t p, uint32_t &vn) {
+// One sample of the magnitude/sign phase (ht_block_decode.rs:1010-1032),
+// its sign-magnitude word converted to the integer sample in the same step
+// (coefficient_to_i32, ht_block_decode.rs:225-234): shift = 31 - K (0 for
+// K >= 31).  Insignificant samples give 0.
+__device__ __forceinline__ int32_t ms_sample(MsBits &ms, uint32_t inf, uint32_t bit, uint32_t uq,
+                                             uint32_t p, uint32_t shift, uint32_t &vn) {
     if ((inf & (1u << (4 + bit))) == 0) {
         vn = 0;
         return 0;
@@ -246,7 +249,8 @@ __device__ __forceinline__ uint32_t ms_value(MsBits &ms, uint32_t inf, uint32_t 
     v |= ((inf >> (8 + bit)) & 1u) << m_n;
     v |= 1u;
     vn = v;
-    return (ms_val << 31) | ((v + 2) << (p - 1));
+    const int32_t mag = (int32_t)((((v + 2) << (p - 1)) & 0x7FFFFFFFu) >> shift);
+    return (ms_val & 1u) ? -mag : mag;
 }
 
 __device__ __forceinline__ uint32_t flog2r(uint32_t v) { return v ? 31u - __builtin_clz(v) : 0u; }

@@ -216,6 +216,7 @@ __device__ __forceinline__ void rows_wave(const CbJob *__restrict__ jobs, uint32
         vlc.unstuff = ((b0 | 0x0F) > 0x8F) ? 1u : 0u;
     }
     const uint32_t p = 30 - zblk, mmsbp2 = zblk + 2;
+    const uint32_t kshift = kmax >= 31 ? 0u : 31u - kmax;  // sign-magnitude word -> sample
     const uint32_t nq = (w + 1) >> 1;
     const uint32_t npairs = (w + 3) >> 2, nrows = (h + 1) >> 1;
     const uint32_t nseg = (npairs + kSegPairs - 1) / kSegPairs;
@@ -376,20 +377,20 @@ __device__ __forceinline__ void rows_wave(const CbJob *__restrict__ jobs, uint32
                 if (one_fill) ms_fill(ms, fms, RingSrc<kMsSlots>{ring_ms, have_ms, arena}, need);
                 uint32_t v0, v1, v2, v3;
                 if (!one_fill) ms_fill(ms, fms, RingSrc<kMsSlots>{ring_ms, have_ms, arena});
-                const uint32_t a0 = ms_value(ms, inf, 0, uq, p, v0);
+                const int32_t a0 = ms_sample(ms, inf, 0, uq, p, kshift, v0);
                 if (!one_fill) ms_fill(ms, fms, RingSrc<kMsSlots>{ring_ms, have_ms, arena});
-                const uint32_t a1 = ms_value(ms, inf, 1, uq, p, v1);
-                r0[2 * k] = (uint32_t)to_i32(a0, kmax);
-                r1[2 * k] = (uint32_t)to_i32(a1, kmax);
+                const int32_t a1 = ms_sample(ms, inf, 1, uq, p, kshift, v1);
+                r0[2 * k] = (uint32_t)a0;
+                r1[2 * k] = (uint32_t)a1;
                 s_q[q][tid] = (uint8_t)((inf & 0xA0u) | max(prev_e3, flog2r(v1)));
                 prev_e3 = 0;
                 if (xq + 1 >= w) break;
                 if (!one_fill) ms_fill(ms, fms, RingSrc<kMsSlots>{ring_ms, have_ms, arena});
-                const uint32_t a2 = ms_value(ms, inf, 2, uq, p, v2);
+                const int32_t a2 = ms_sample(ms, inf, 2, uq, p, kshift, v2);
                 if (!one_fill) ms_fill(ms, fms, RingSrc<kMsSlots>{ring_ms, have_ms, arena});
-                const uint32_t a3 = ms_value(ms, inf, 3, uq, p, v3);
-                r0[2 * k + 1] = (uint32_t)to_i32(a2, kmax);
-                r1[2 * k + 1] = (uint32_t)to_i32(a3, kmax);
+                const int32_t a3 = ms_sample(ms, inf, 3, uq, p, kshift, v3);
+                r0[2 * k + 1] = (uint32_t)a2;
+                r1[2 * k + 1] = (uint32_t)a3;
                 prev_e3 = flog2r(v3);
             }
             C *o = out + y * stride + x;
