@@ -131,24 +131,28 @@ __device__ __forceinline__ uint32_t has_ff(uint32_t x) {  // nonzero iff a byte 
 // starts below bit 32, so every appended bit, including the top bit of a
 // 7-bit byte (ORed into the next position, as byte-wise appending does),
 // stays inside the 64-bit accumulator.
+// Appends the next four-byte group (needs b.bits <= 32).
+template <class Src>
+__device__ __forceinline__ void ms_append(MsBits &b, Fwd4 &f, const Src &src) {
+    const uint32_t w = fwd4_get(f, src);
+    if (!b.unstuff && !has_ff(w & 0x00FFFFFFu)) {
+        b.tmp |= (uint64_t)w << b.bits;
+        b.bits += 32;
+        b.unstuff = (w >> 24) == 0xFF;
+        return;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const uint32_t c = (w >> (8 * i)) & 0xFF;
+        b.tmp |= (uint64_t)c << b.bits;
+        b.bits += 8 - b.unstuff;
+        b.unstuff = c == 0xFF;
+    }
+}
+
 template <class Src>
 __device__ __forceinline__ void ms_fill(MsBits &b, Fwd4 &f, const Src &src, uint32_t need = 32) {
-    while (b.bits < need) {
-        const uint32_t w = fwd4_get(f, src);
-        if (!b.unstuff && !has_ff(w & 0x00FFFFFFu)) {
-            b.tmp |= (uint64_t)w << b.bits;
-            b.bits += 32;
-            b.unstuff = (w >> 24) == 0xFF;
-            continue;
-        }
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const uint32_t c = (w >> (8 * i)) & 0xFF;
-            b.tmp |= (uint64_t)c << b.bits;
-            b.bits += 8 - b.unstuff;
-            b.unstuff = c == 0xFF;
-        }
-    }
+    while (b.bits < need) ms_append(b, f, src);
 }
 
 // VLC: backward; a byte whose low 7 bits are ones after a byte > 0x8F

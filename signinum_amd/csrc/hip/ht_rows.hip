@@ -92,6 +92,17 @@ __device__ __forceinline__ void stage_ring(uint4 *wave_ring, const uint8_t *aren
     }
 }
 
+// MagSgn top-up for the whole wave: while any active lane holds fewer than
+// `need` bits, every active lane with room appends a group.  Lanes refill
+// together instead of one at a time at different quads, so the refill runs
+// less often and with most lanes active; the bits each lane reads are the
+// same (only when its groups are appended changes).
+template <class Src>
+__device__ __forceinline__ void ms_topup(MsBits &b, Fwd4 &f, const Src &src, uint32_t need) {
+    while (__any(b.bits < need))
+        if (b.bits <= 32) ms_append(b, f, src);
+}
+
 __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor(v, o));
@@ -374,7 +385,7 @@ __device__ __forceinline__ void rows_wave(const CbJob *__restrict__ jobs, uint32
                     (uint32_t)__builtin_popcount(sig) * uq -
                     (uint32_t)__builtin_popcount(sig & (inf >> 12));
                 const bool one_fill = need <= 32;
-                if (one_fill) ms_fill(ms, fms, RingSrc<kMsSlots>{ring_ms, have_ms, arena}, need);
+                ms_topup(ms, fms, RingSrc<kMsSlots>{ring_ms, have_ms, arena}, one_fill ? need : 0u);
                 uint32_t v0, v1, v2, v3;
                 if (!one_fill) ms_fill(ms, fms, RingSrc<kMsSlots>{ring_ms, have_ms, arena});
                 const int32_t a0 = ms_sample(ms, inf, 0, uq, p, kshift, v0);

@@ -95,9 +95,6 @@ const RowsVariant kRowsVariants[] = {
     SGN_ROWS_VARIANT("w4-seg4-small-queue", 4, 4, 4, 2, 1, true, 1),
     SGN_ROWS_VARIANT("w2-seg16-queue", 2, 16, 16, 4, 2, true, 1),
     SGN_ROWS_VARIANT("w1-seg8-queue", 1, 8, 8, 4, 2, true, 1),
-    // eight waves sharing the tables, registers bounded for four waves per
-    // SIMD: two work-groups (16 waves) per CU
-    SGN_ROWS_VARIANT("w8-seg4-small-queue", 8, 4, 4, 2, 1, true, 4),
 };
 #undef SGN_ROWS_VARIANT
 constexpr int kRowsVariantCount = int(sizeof(kRowsVariants) / sizeof(kRowsVariants[0]));
