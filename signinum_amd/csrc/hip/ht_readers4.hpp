@@ -219,11 +219,10 @@ __device__ __forceinline__ void mel_fill(Mel &m, Fwd4 &f, const Src &src) {
 // quad that needs a symbol, times two, plus one when that quad follows
 // (ht_block_decode.rs:860-929).  A symbol is one bit (1: a full run of
 // 2^E quads, E from the state k) or a zero bit and E bits of run length.
-template <class Src>
-__device__ __forceinline__ int mel_run4(Mel &m, Fwd4 &f, const Src &src) {
+// Decodes one symbol from a buffer holding at least 6 bits.
+__device__ __forceinline__ int mel_symbol(Mel &m) {
     // E for k = 0..12, four bits each
     constexpr uint64_t kExp = 0x5433222111000ull;
-    if (m.bits < 6) mel_fill(m, f, src);
     const uint32_t e = (uint32_t)(kExp >> (4 * m.k)) & 0xF;
     const bool first = (m.tmp >> 63) != 0;
     const uint32_t v = e ? (uint32_t)((m.tmp << 1) >> (64 - e)) : 0u;
@@ -232,6 +231,12 @@ __device__ __forceinline__ int mel_run4(Mel &m, Fwd4 &f, const Src &src) {
     m.bits -= n;
     m.k = first ? (m.k < 12 ? m.k + 1 : 12) : (m.k > 0 ? m.k - 1 : 0);
     return first ? (int)(((1u << e) - 1) << 1) : (int)((v << 1) | 1);
+}
+
+template <class Src>
+__device__ __forceinline__ int mel_run4(Mel &m, Fwd4 &f, const Src &src) {
+    if (m.bits < 6) mel_fill(m, f, src);
+    return mel_symbol(m);
 }
 
 // One sample of the magnitude/sign phase (ht_block_decode.rs:1010-1032),

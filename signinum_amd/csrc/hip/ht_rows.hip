@@ -103,6 +103,14 @@ __device__ __forceinline__ void ms_topup(MsBits &b, Fwd4 &f, const Src &src, uin
         if (b.bits <= 32) ms_append(b, f, src);
 }
 
+// Next MEL run with the refill shared by the active lanes, as ms_topup.
+template <class Src>
+__device__ __forceinline__ int mel_run_wave(Mel &m, Fwd4 &f, const Src &src) {
+    while (__any(m.bits < 6))
+        if (m.bits <= 32) mel_fill(m, f, src);
+    return mel_symbol(m);
+}
+
 __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor(v, o));
@@ -237,7 +245,7 @@ __device__ __forceinline__ void rows_wave(const CbJob *__restrict__ jobs, uint32
             s_q[q][tid] = 0;
         }
     int run = 0;
-    if (run_block) run = mel_run4(mel, fmel, RingSrc<kMelSlots>{ring_mel, have_mel, arena});
+    if (run_block) run = mel_run_wave(mel, fmel, RingSrc<kMelSlots>{ring_mel, have_mel, arena});
     uint32_t c_q = 0, prev_e3 = 0;
     const uint32_t segs = wave_max(total);
 
@@ -280,7 +288,7 @@ __device__ __forceinline__ void rows_wave(const CbJob *__restrict__ jobs, uint32
                 if (c_q == 0) {
                     run -= 2;
                     t0 = (run == -1) ? t0 : 0;
-                    if (run < 0) run = mel_run4(mel, fmel, RingSrc<kMelSlots>{ring_mel, have_mel, arena});
+                    if (run < 0) run = mel_run_wave(mel, fmel, RingSrc<kMelSlots>{ring_mel, have_mel, arena});
                 }
                 c_q = ((t0 & 0x10) << 3) | ((t0 & 0xE0) << 2);
                 vlc.tmp >>= t0 & 7;
@@ -289,7 +297,7 @@ __device__ __forceinline__ void rows_wave(const CbJob *__restrict__ jobs, uint32
                 if (c_q == 0 && x + 2 < w) {
                     run -= 2;
                     t1 = (run == -1) ? t1 : 0;
-                    if (run < 0) run = mel_run4(mel, fmel, RingSrc<kMelSlots>{ring_mel, have_mel, arena});
+                    if (run < 0) run = mel_run_wave(mel, fmel, RingSrc<kMelSlots>{ring_mel, have_mel, arena});
                 }
                 if (x + 2 >= w) t1 = 0;
                 c_q = ((t1 & 0x10) << 3) | ((t1 & 0xE0) << 2);
@@ -299,7 +307,7 @@ __device__ __forceinline__ void rows_wave(const CbJob *__restrict__ jobs, uint32
                 if (mode == 0xC0) {
                     run -= 2;
                     if (run == -1) mode += 0x40;
-                    if (run < 0) run = mel_run4(mel, fmel, RingSrc<kMelSlots>{ring_mel, have_mel, arena});
+                    if (run < 0) run = mel_run_wave(mel, fmel, RingSrc<kMelSlots>{ring_mel, have_mel, arena});
                 }
                 uint32_t e = uvlc0[mode + (vlc.tmp & 0x3F)];
                 vlc.tmp >>= e & 7;
@@ -321,7 +329,7 @@ __device__ __forceinline__ void rows_wave(const CbJob *__restrict__ jobs, uint32
                 if (c_q == 0) {
                     run -= 2;
                     t0 = (run == -1) ? t0 : 0;
-                    if (run < 0) run = mel_run4(mel, fmel, RingSrc<kMelSlots>{ring_mel, have_mel, arena});
+                    if (run < 0) run = mel_run_wave(mel, fmel, RingSrc<kMelSlots>{ring_mel, have_mel, arena});
                 }
                 c_q = ((t0 & 0x40) << 2) | ((t0 & 0x80) << 1);
                 c_q |= pt0 & 0x80;
@@ -333,7 +341,7 @@ __device__ __forceinline__ void rows_wave(const CbJob *__restrict__ jobs, uint32
                 if (c_q == 0 && x + 2 < w) {
                     run -= 2;
                     t1 = (run == -1) ? t1 : 0;
-                    if (run < 0) run = mel_run4(mel, fmel, RingSrc<kMelSlots>{ring_mel, have_mel, arena});
+                    if (run < 0) run = mel_run_wave(mel, fmel, RingSrc<kMelSlots>{ring_mel, have_mel, arena});
                 }
                 if (x + 2 >= w) t1 = 0;
                 c_q = ((t1 & 0x40) << 2) | ((t1 & 0x80) << 1);
