@@ -93,7 +93,7 @@ __device__ __forceinline__ void stage_ring(uint4 *wave_ring, const uint8_t *aren
 }
 
 // MagSgn top-up for the whole wave: while any active lane holds fewer than
-// `need` bits, every active lane with room appends a group.  Lanes refill
+// `need` bits (need <= 33), every active lane with room appends a group.  Lanes refill
 // together instead of one at a time at different quads, so the refill runs
 // less often and with most lanes active; the bits each lane reads are the
 // same (only when its groups are appended changes).
@@ -385,14 +385,15 @@ __device__ __forceinline__ void rows_wave(const CbJob *__restrict__ jobs, uint32
                     break;
                 }
                 // Bits the quad's significant samples take (m_n = U_q or
-                // U_q - 1 each).  Up to 32 of them come in one top-up, and
-                // the per-sample top-ups below stay idle; larger quads top
-                // up before every sample.
+                // U_q - 1 each).  Up to 32 of them come in one top-up (to one
+                // bit more: a sample reads its sign from the buffer's next bit
+                // even when m_n is 0), and the per-sample top-ups below stay
+                // idle; larger quads top up to 32 bits before every sample.
                 const uint32_t sig = (inf >> 4) & (xq + 1 < w ? 0xFu : 0x3u);
-                const uint32_t need =
-                    (uint32_t)__builtin_popcount(sig) * uq -
-                    (uint32_t)__builtin_popcount(sig & (inf >> 12));
-                const bool one_fill = need <= 32;
+                const uint32_t qbits = (uint32_t)__builtin_popcount(sig) * uq -
+                                       (uint32_t)__builtin_popcount(sig & (inf >> 12));
+                const bool one_fill = qbits <= 32;
+                const uint32_t need = sig ? qbits + 1u : 0u;
                 ms_topup(ms, fms, RingSrc<kMsSlots>{ring_ms, have_ms, arena}, one_fill ? need : 0u);
                 uint32_t v0, v1, v2, v3;
                 if (!one_fill) ms_fill(ms, fms, RingSrc<kMsSlots>{ring_ms, have_ms, arena});
