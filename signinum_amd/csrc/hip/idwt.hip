@@ -70,6 +70,7 @@ struct LevelView {
     const uint8_t *origin;
     uint32_t x0, y0, fin, narrow;
     int w, h, levels;
+    bool aligned;  // every used slot and row pitch 16-byte aligned
 };
 
 // Stages the band table of `lj` into `sb`; ends with a barrier.
@@ -90,7 +91,14 @@ __device__ __forceinline__ LevelView level_view(const LevelJob *__restrict__ lj,
         sb->pitchb[t] = lj->bw[t] << (((narrow >> t) & 1u) ? 1 : 2);
         sb->delta[t] = lj->delta[t];
     }
+    uint64_t mis = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+        if ((used >> b) & 1u)
+            mis |= (reinterpret_cast<uintptr_t>(coef) + slot_byte(b)) |
+                   ((uint64_t)lj->bw[b] << (((narrow >> b) & 1u) ? 1 : 2));
     LevelView v;
+    v.aligned = (mis & 15u) == 0;
     v.bands = sb;
     v.origin = reinterpret_cast<const uint8_t *>(coef) + lo;
     v.x0 = lj->x0;
@@ -169,14 +177,26 @@ __device__ __forceinline__ void lift(T *win, int nlines, int n, uint32_t origin,
 }
 
 // Window geometry: halo H (2 for 5/3, 4 for 9/7), N = 64 + 2H samples per
-// side, row pitch P = N + 1 (odd, so a wave walking down 64 rows of one
-// column, or along 64 columns of one row, touches 64 distinct LDS banks).
+// side.  A window row holds the 80 positions ox - 8 .. ox + 72 of its block
+// (ox: the block's first column), so that each sub-band contributes 40
+// samples that start on a 4-sample boundary (fetch_chunks); lifting position
+// 0 (ox - H) is row position X0 = 8 - H, and the lifting functions take the
+// window from that column on (`wb` = window + X0).  Only positions
+// [X0, X0 + N) are read; the row pitch P is the smallest odd number above
+// X0 + N (a wave walking down 64 rows of one column, or along 64 columns of
+// one row, touches 64 distinct LDS banks), and the positions a row's chunks
+// write past P land on the next row's unread positions below X0.  kWords:
+// the array, whose last row takes all 80 positions.
 template <bool REV>
 struct Win {
     static constexpr int H = REV ? 2 : 4;
     static constexpr int N = kBlk + 2 * H;
-    static constexpr int P = N + 1;
+    static constexpr int X0 = 8 - H;
+    static constexpr int P = (X0 + N) | 1;
     static constexpr int R = 8 + 2 * H;  // inputs of one 8-sample lifting segment
+    static constexpr int kRow = 80;      // row positions the chunk fetch writes
+    static constexpr int kWords = (N - 1) * P + kRow;
+    static_assert(P >= X0 + N && kRow - P <= X0, "row spill stays in unread positions");
 };
 
 // Both lifting passes of an 8-sample segment in registers: v[0..R) holds the
@@ -449,6 +469,121 @@ __device__ __forceinline__ void fetch_window(const LevelView &L, int bx, int by,
     }
 }
 
+// Loads the window of block (bx, by) of a level with an even origin x0,
+// width above H and 16-byte aligned slots, four samples of a sub-band row at
+// a time (one 8-byte int16 or 16-byte int32 load), and writes it into `win`
+// (the window from column X0 on).  With x0 even, row position x of the
+// window is absolute column x0 + ox - 8 + x: even positions hold low-pass
+// column ox/2 - 4 + x/2, odd ones high-pass column ox/2 - 4 + (x - 1)/2, so
+// chunk k of either band (columns ox/2 - 4 + 4k .. + 3) fills positions
+// 8k + 2j + parity, j < 4.  An item is one window row and one k, both
+// bands.  Lanes 2m and 2m + 1 take chunks 2q and 2q + 1 of window row r0 + m:
+// a half-wave covers 16 consecutive rows, reading 16 bytes (int16) or 32
+// bytes (int32) of each per band, and its LDS writes, 8 positions apart
+// along a row and P (odd) apart across rows, fall on distinct banks.  (Lanes
+// walking the chunks of one row write 8 words apart, three to a bank; lanes
+// walking the rows of one chunk read a separate 8 bytes per row.)  Window
+// rows past the block's last (yy >= nh) repeat the last row of their parity.
+// Positions left of the level (block column 0) and right of it (last block
+// column) are then mirrored from inside (whole-sample symmetric extension,
+// F.3.7); chunks outside the band rows load the level's first word instead.
+// No barrier at the end.
+template <bool REV>
+__device__ __forceinline__ void fetch_chunks(const LevelView &L, int bx, int by,
+                                             const uint32_t *rows,
+                                             typename Sample<REV>::T *win) {
+    using T = typename Sample<REV>::T;
+    constexpr int H = Win<REV>::H, N = Win<REV>::N, P = Win<REV>::P, X0 = Win<REV>::X0;
+    constexpr int kChunks = 10;
+    constexpr int kItems = N * kChunks;
+    constexpr int kPer = (kItems + kIdwtThreads - 1) / kIdwtThreads;
+    const int ox = bx * kBlk, oy = by * kBlk;
+    const int bw = min(kBlk, L.w - ox), nh = min(kBlk, L.h - oy) + 2 * H;
+    const int c0 = (ox >> 1) - 4;
+    const int wl = (L.w + 1) >> 1, wh = L.w >> 1;  // low- and high-pass columns
+    const int tid = threadIdx.x;
+    uint32_t raw[kPer][2][4];
+    int dst[kPer];
+    uint32_t slot[kPer];  // bit b: band slot of the item's bands, for the conversion
+#pragma unroll
+    for (int it = 0; it < kPer; ++it) {
+        const int i = tid + it * kIdwtThreads;
+        dst[it] = -1;
+        if (i >= kItems) continue;
+        const int t = i >> 1;
+        const int yy = t % N, k = 2 * (t / N) + (i & 1);
+        // the last window row < nh of yy's parity
+        const int yl = yy < nh ? yy : nh - 1 - ((nh - 1 - yy) & 1);
+        const uint32_t e0 = rows[2 * yl], e1 = rows[2 * yl + 1];
+        const uint32_t par = e0 & 1u;
+        const int c = c0 + 4 * k;
+        dst[it] = yy * P + 8 * k - X0;
+        slot[it] = par << 1;
+#pragma unroll
+        for (int xp = 0; xp < 2; ++xp) {
+            const uint32_t b = (uint32_t)xp | (par << 1);
+            const bool nar = ((L.narrow >> b) & 1u) != 0;
+            const bool ok = (unsigned)c < (unsigned)(xp ? wh : wl);
+            const uint32_t off =
+                ok ? L.bands->relb[b] + ((xp ? e1 : e0) - par) + ((uint32_t)c << (nar ? 1 : 2)) : 0u;
+            const uint8_t *src = L.origin + off;
+            if (nar) {
+                const uint2 v = *reinterpret_cast<const uint2 *>(src);
+                raw[it][xp][0] = v.x;
+                raw[it][xp][1] = v.y;
+            } else {
+                const uint4 v = *reinterpret_cast<const uint4 *>(src);
+                raw[it][xp][0] = v.x;
+                raw[it][xp][1] = v.y;
+                raw[it][xp][2] = v.z;
+                raw[it][xp][3] = v.w;
+            }
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < kPer; ++it) {
+        if (dst[it] < 0) continue;
+        T *d = win + dst[it];
+#pragma unroll
+        for (int xp = 0; xp < 2; ++xp) {
+            const uint32_t b = (uint32_t)xp | slot[it];
+            const bool nar = ((L.narrow >> b) & 1u) != 0;
+            int32_t s[4];
+            if (nar) {
+                s[0] = (int32_t)(raw[it][xp][0] << 16) >> 16;
+                s[1] = (int32_t)raw[it][xp][0] >> 16;
+                s[2] = (int32_t)(raw[it][xp][1] << 16) >> 16;
+                s[3] = (int32_t)raw[it][xp][1] >> 16;
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) s[j] = (int32_t)raw[it][xp][j];
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if constexpr (REV) {
+                    d[2 * j + xp] = s[j];
+                } else {
+                    const bool isf = !nar && ((L.fin >> b) & 1u);
+                    d[2 * j + xp] = isf ? __int_as_float(s[j]) : (float)s[j] * L.bands->delta[b];
+                }
+            }
+        }
+    }
+    const bool left = ox == 0, right = ox + bw == L.w;
+    if (left || right) {
+        __syncthreads();
+        const int xl = 8 + (L.w - 1 - ox);  // row position of the level's last column
+        for (int i = tid; i < N * 2 * H; i += kIdwtThreads) {
+            const int yy = i % N, s = i / N;
+            const bool lft = s < H;
+            const int m = lft ? s + 1 : s - H + 1;
+            if (lft ? !left : !right) continue;
+            const int to = lft ? 8 - m : xl + m, from = lft ? 8 + m : xl - m;
+            win[yy * P + to - X0] = win[yy * P + from - X0];
+        }
+    }
+}
+
 // Converts the fetched words (dequantising with the band step for 9/7) and
 // writes them into the window.  5/3 levels hold integers only; 9/7 slots may
 // hold f32 samples (float_in).  No barrier.
@@ -543,8 +678,9 @@ __device__ __forceinline__ void lift_block(const LevelView &L, int bx, int by,
 }
 
 // Synthesises block (bx, by) of one level: window-row table, loads, window,
-// lifting.  Result element (x, y) of an in-place sink's block is at
-// win[(y + H) * P + x + H].  rows: LDS scratch of 2 N words.
+// lifting.  win: the window from column X0 on.  Result element (x, y) of an
+// in-place sink's block is at win[(y + H) * P + x + H].  rows: LDS scratch
+// of 2 N words.
 template <bool REV, class Sink>
 __device__ __forceinline__ void synth_block(const LevelView &L, const uint32_t *__restrict__ coef,
                                             int bx, int by, typename Sample<REV>::T *win,
@@ -552,9 +688,13 @@ __device__ __forceinline__ void synth_block(const LevelView &L, const uint32_t *
                                             int &bh_out) {
     window_rows<REV>(L, by, rows);
     __syncthreads();
-    WinFetch<REV> f;
-    fetch_window<REV>(L, bx, by, rows, f);
-    commit_window<REV>(L, bx, by, rows, f, win);
+    if (L.levels && L.aligned && (L.x0 & 1u) == 0 && L.w > Win<REV>::H) {
+        fetch_chunks<REV>(L, bx, by, rows, win);
+    } else {
+        WinFetch<REV> f;
+        fetch_window<REV>(L, bx, by, rows, f);
+        commit_window<REV>(L, bx, by, rows, f, win);
+    }
     lift_block<REV>(L, bx, by, win, sink, bw_out, bh_out);
 }
 
@@ -593,8 +733,9 @@ __device__ __forceinline__ const J *job_of(const J *jobs, uint32_t njobs,
 template <bool REV>
 __device__ __forceinline__ void level_body(const LevelJob *jobs, uint32_t njobs, uint32_t block_offset,
                                            const uint32_t *block_job, uint32_t *coef) {
-    constexpr int H = Win<REV>::H, N = Win<REV>::N, P = Win<REV>::P;
-    __shared__ typename Sample<REV>::T win[N * P];
+    constexpr int N = Win<REV>::N;
+    __shared__ typename Sample<REV>::T win_lds[Win<REV>::kWords];
+    typename Sample<REV>::T *const win = win_lds + Win<REV>::X0;
     __shared__ uint32_t rows[2 * N];
     const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x) + block_offset;
     const LevelJob *lj = job_of(jobs, njobs, block_job, blk);
@@ -672,7 +813,8 @@ __device__ __forceinline__ void final_body(const FinalJob *jobs, uint32_t njobs,
     constexpr int H = Win<REV>::H, N = Win<REV>::N, P = Win<REV>::P;
     constexpr bool kNarrowPlanes = sizeof(PT) == 2;
     using T = typename Sample<REV>::T;
-    __shared__ T win[N * P];
+    __shared__ T win_lds[Win<REV>::kWords];
+    T *const win = win_lds + Win<REV>::X0;  // lifting position 0
     __shared__ uint32_t rows[2 * N];  // window-row table
     __shared__ __attribute__((aligned(16))) PT res[2][kBlk * kBlk];
     const FinalJob *fj = job_of(jobs, njobs, block_job, blk);
