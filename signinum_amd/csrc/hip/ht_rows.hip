@@ -99,15 +99,20 @@ __device__ __forceinline__ void stage_ring(uint4 *wave_ring, const uint8_t *aren
 // same (only when its groups are appended changes).
 template <class Src>
 __device__ __forceinline__ void ms_topup(MsBits &b, Fwd4 &f, const Src &src, uint32_t need) {
-    while (__any(b.bits < need))
+    if (!__any(b.bits < need)) return;
+    do {
         if (b.bits <= 32) ms_append(b, f, src);
+    } while (__any(b.bits < need));
 }
 
 // Next MEL run with the refill shared by the active lanes, as ms_topup.
 template <class Src>
 __device__ __forceinline__ int mel_run_wave(Mel &m, Fwd4 &f, const Src &src) {
-    while (__any(m.bits < 6))
-        if (m.bits <= 32) mel_fill(m, f, src);
+    if (__any(m.bits < 6)) {
+        do {
+            if (m.bits <= 32) mel_fill(m, f, src);
+        } while (__any(m.bits < 6));
+    }
     return mel_symbol(m);
 }
 
