@@ -23,6 +23,18 @@ namespace sgn {
 
 constexpr int kBlk = 64;
 constexpr int kIdwtThreads = 256;
+
+// Output block and work-group shape of a synthesis kernel: Big for every
+// level and the final stage, Small (32x32 blocks, one wave) for level
+// launches whose outputs are all at most 32x32, whose work-groups are
+// latency-bound: more of them fit a CU.
+template <int BLK, int THREADS>
+struct Geo {
+    static constexpr int kBlk = BLK, kThreads = THREADS;
+    static_assert(BLK % 8 == 0 && THREADS % BLK == 0 && THREADS % 64 == 0, "block shape");
+};
+using Big = Geo<kBlk, kIdwtThreads>;
+using Small = Geo<32, 64>;
 constexpr float kKappa = 1.2301741f;
 constexpr float kNegAlpha = 1.5861343f;
 constexpr float kNegBeta = 0.052980117f;
@@ -116,11 +128,12 @@ __device__ __forceinline__ LevelView level_view(const LevelJob *__restrict__ lj,
 // elements run along x; otherwise lines are the 64 block columns and
 // elements run along y.  Window pitch WN and the pair count are
 // compile-time constants, so index arithmetic needs no runtime division.
-template <bool REV, int WN, bool HORIZ, typename T>
+template <bool REV, int WN, bool HORIZ, typename T, class G = Big>
 __device__ __forceinline__ void lift(T *win, int nlines, int n, uint32_t origin, bool single,
                                      bool odd_single) {
     constexpr int H = REV ? 2 : 4;
     constexpr int PAIRS = WN / 2;
+    constexpr int kBlk = G::kBlk, kIdwtThreads = G::kThreads;
     const int tid = threadIdx.x;
     auto at = [&](int line, int e) -> T & {
         return HORIZ ? win[line * WN + e] : win[e * WN + H + line];
@@ -187,14 +200,14 @@ __device__ __forceinline__ void lift(T *win, int nlines, int n, uint32_t origin,
 // one row, touches 64 distinct LDS banks), and the positions a row's chunks
 // write past P land on the next row's unread positions below X0.  kWords:
 // the array, whose last row takes all 80 positions.
-template <bool REV>
+template <bool REV, class G = Big>
 struct Win {
     static constexpr int H = REV ? 2 : 4;
-    static constexpr int N = kBlk + 2 * H;
+    static constexpr int N = G::kBlk + 2 * H;
     static constexpr int X0 = 8 - H;
     static constexpr int P = (X0 + N) | 1;
-    static constexpr int R = 8 + 2 * H;  // inputs of one 8-sample lifting segment
-    static constexpr int kRow = 80;      // row positions the chunk fetch writes
+    static constexpr int R = 8 + 2 * H;    // inputs of one 8-sample lifting segment
+    static constexpr int kRow = G::kBlk + 16;  // row positions the chunk fetch writes
     static constexpr int kWords = (N - 1) * P + kRow;
     static_assert(P >= X0 + N && kRow - P <= X0, "row spill stays in unread positions");
 };
@@ -240,12 +253,12 @@ __device__ __forceinline__ void lift_segment(T (&v)[Win<REV>::R]) {
 // Destinations of the synthesised block (block coordinates x, y < 64).
 // In-place sinks write the window itself; the others let the vertical pass
 // write its results straight to their consumer.
-template <bool REV>
+template <bool REV, class G = Big>
 struct SinkWin {
     typename Sample<REV>::T *win;
     static constexpr bool kInPlace = true;
     __device__ __forceinline__ void operator()(int x, int y, typename Sample<REV>::T v) const {
-        win[(y + Win<REV>::H) * Win<REV>::P + x + Win<REV>::H] = v;
+        win[(y + Win<REV, G>::H) * Win<REV, G>::P + x + Win<REV, G>::H] = v;
     }
 };
 
@@ -292,11 +305,12 @@ struct SinkGlobal {  // the level's output in the coefficient arena
 // back into the window; else every block column in [0, bw), output rows
 // [H, H + bh), handed to `sink`.  q: parity of window position 0 along the
 // direction.  All segments are read before any in-place write.
-template <bool REV, bool HORIZ, int Q, class Sink>
+template <bool REV, bool HORIZ, int Q, class Sink, class G = Big>
 __device__ __forceinline__ void lift_window_q(typename Sample<REV>::T *win, int nh, int bw, int bh,
                                               const Sink &sink) {
     using T = typename Sample<REV>::T;
-    constexpr int H = Win<REV>::H, N = Win<REV>::N, P = Win<REV>::P, R = Win<REV>::R;
+    constexpr int kBlk = G::kBlk, kIdwtThreads = G::kThreads;
+    constexpr int H = Win<REV, G>::H, N = Win<REV, G>::N, P = Win<REV, G>::P, R = Win<REV, G>::R;
     // items: HORIZ (row, segment) with the row fastest; else (column, segment)
     constexpr int kLines = HORIZ ? N : kBlk;
     constexpr int kItems = kLines * (kBlk / 8);
@@ -338,22 +352,23 @@ __device__ __forceinline__ void lift_window_q(typename Sample<REV>::T *win, int 
 // The window parity q is uniform; dispatching on it outside the segment loop
 // keeps each instance's lifting in fixed registers (a merged, select-based
 // form would index the segment registers at run time).
-template <bool REV, bool HORIZ, class Sink>
+template <bool REV, bool HORIZ, class Sink, class G = Big>
 __device__ __forceinline__ void lift_window(typename Sample<REV>::T *win, int nh, int bw, int bh,
                                             uint32_t q, const Sink &sink) {
     if (__builtin_amdgcn_readfirstlane(q))
-        lift_window_q<REV, HORIZ, 1>(win, nh, bw, bh, sink);
+        lift_window_q<REV, HORIZ, 1, Sink, G>(win, nh, bw, bh, sink);
     else
-        lift_window_q<REV, HORIZ, 0>(win, nh, bw, bh, sink);
+        lift_window_q<REV, HORIZ, 0, Sink, G>(win, nh, bw, bh, sink);
 }
 
 // Window-row table of block row `by`: for every window row after extension
 // and both column parities xp, the byte offset of the row inside band slot
 // xp | 2 * row parity, with the row parity in bit 0 (offsets are even).
 // rows holds 2 * N words.  No barrier.
-template <bool REV>
+template <bool REV, class G = Big>
 __device__ __forceinline__ void window_rows(const LevelView &L, int by, uint32_t *rows) {
-    constexpr int H = Win<REV>::H;
+    constexpr int kBlk = G::kBlk, kIdwtThreads = G::kThreads;
+    constexpr int H = Win<REV, G>::H;
     const int oy = by * kBlk;
     const int nh = min(kBlk, L.h - oy) + 2 * H;
     for (int e = threadIdx.x; e < 2 * nh; e += kIdwtThreads) {
@@ -372,15 +387,18 @@ __device__ __forceinline__ void window_rows(const LevelView &L, int by, uint32_t
 // The window's raw coefficient words held in registers between the loads
 // and their conversion into LDS, so that the loads of one component can be
 // in flight while the previous component is lifted.  Each thread owns one
-// window column and walks down it: pass 0 covers columns 0..63, a quarter
-// of the rows per wave; pass 1 the 2H halo columns, 256 / 2H threads each.
-template <bool REV>
+// window column and walks down it: pass 0 covers the block's columns with
+// kGroups threads per column (each every kGroups-th row); pass 1 the 2H halo
+// columns, threads / 2H per column.
+template <bool REV, class G = Big>
 struct WinFetch {
-    static constexpr int kHalo = 2 * Win<REV>::H;
-    static constexpr int kRows0 = (Win<REV>::N + 3) / 4;
-    static constexpr int kStep1 = kIdwtThreads / kHalo;
-    static constexpr int kRows1 = (Win<REV>::N + kStep1 - 1) / kStep1;
-    static_assert(3 + 4 * (kRows0 - 1) < Win<REV>::N, "pass-0 rows stay inside the window");
+    static constexpr int kGroups = G::kThreads / G::kBlk;  // threads per window column
+    static constexpr int kHalo = 2 * Win<REV, G>::H;
+    static constexpr int kRows0 = (Win<REV, G>::N + kGroups - 1) / kGroups;
+    static constexpr int kStep1 = G::kThreads / kHalo;
+    static constexpr int kRows1 = (Win<REV, G>::N + kStep1 - 1) / kStep1;
+    static_assert(kGroups - 1 + kGroups * (kRows0 - 1) < Win<REV, G>::N,
+                  "pass-0 rows stay inside the window");
     uint32_t raw0[kRows0];
     uint32_t raw1[kRows1];
     uint32_t y1m0, y1m1;  // bit k: row k of the pass read the row-parity-1 slots
@@ -430,31 +448,32 @@ __device__ __forceinline__ uint32_t window_word(const LevelView &L, const WinCol
 // block row's window-row table.  Positions outside the window (xx >= nw or
 // yy >= nh) load the level's first word instead: every lane takes the same
 // path, and those values only reach outputs outside the block.
-template <bool REV>
+template <bool REV, class G = Big>
 __device__ __forceinline__ void fetch_window(const LevelView &L, int bx, int by,
-                                             const uint32_t *rows, WinFetch<REV> &f) {
-    constexpr int H = Win<REV>::H, N = Win<REV>::N;
-    using F = WinFetch<REV>;
+                                             const uint32_t *rows, WinFetch<REV, G> &f) {
+    constexpr int kBlk = G::kBlk;
+    constexpr int H = Win<REV, G>::H, N = Win<REV, G>::N;
+    using F = WinFetch<REV, G>;
     const int ox = bx * kBlk, oy = by * kBlk;
     const int nw = min(kBlk, L.w - ox) + 2 * H, nh = min(kBlk, L.h - oy) + 2 * H;
     const int tid = threadIdx.x;
     {
-        const int xx = tid & 63, y0 = tid >> 6;
+        const int xx = tid % kBlk, y0 = tid / kBlk;
         const bool colok = xx < nw;
         const WinColumn c = window_column(L, ox, colok ? xx : 0, H);
         uint32_t rr[F::kRows0];
 #pragma unroll
-        for (int k = 0; k < F::kRows0; ++k) rr[k] = rows[2 * (y0 + 4 * k) + c.xp];
+        for (int k = 0; k < F::kRows0; ++k) rr[k] = rows[2 * (y0 + F::kGroups * k) + c.xp];
         uint32_t m = 0;
 #pragma unroll
         for (int k = 0; k < F::kRows0; ++k) {
-            f.raw0[k] = window_word(L, c, rr[k], colok && y0 + 4 * k < nh);
+            f.raw0[k] = window_word(L, c, rr[k], colok && y0 + F::kGroups * k < nh);
             m |= (rr[k] & 1u) << k;
         }
         f.y1m0 = m;
     }
     {
-        const int xx = 64 + tid % F::kHalo, y0 = tid / F::kHalo;
+        const int xx = kBlk + tid % F::kHalo, y0 = tid / F::kHalo;
         const bool colok = xx < nw;
         const WinColumn c = window_column(L, ox, colok ? xx : 0, H);
         uint32_t m = 0;
@@ -488,13 +507,14 @@ __device__ __forceinline__ void fetch_window(const LevelView &L, int bx, int by,
 // column) are then mirrored from inside (whole-sample symmetric extension,
 // F.3.7); chunks outside the band rows load the level's first word instead.
 // No barrier at the end.
-template <bool REV>
+template <bool REV, class G = Big>
 __device__ __forceinline__ void fetch_chunks(const LevelView &L, int bx, int by,
                                              const uint32_t *rows,
                                              typename Sample<REV>::T *win) {
     using T = typename Sample<REV>::T;
-    constexpr int H = Win<REV>::H, N = Win<REV>::N, P = Win<REV>::P, X0 = Win<REV>::X0;
-    constexpr int kChunks = 10;
+    constexpr int kBlk = G::kBlk, kIdwtThreads = G::kThreads;
+    constexpr int H = Win<REV, G>::H, N = Win<REV, G>::N, P = Win<REV, G>::P, X0 = Win<REV, G>::X0;
+    constexpr int kChunks = Win<REV, G>::kRow / 8;
     constexpr int kItems = N * kChunks;
     constexpr int kPer = (kItems + kIdwtThreads - 1) / kIdwtThreads;
     const int ox = bx * kBlk, oy = by * kBlk;
@@ -587,13 +607,15 @@ __device__ __forceinline__ void fetch_chunks(const LevelView &L, int bx, int by,
 // Converts the fetched words (dequantising with the band step for 9/7) and
 // writes them into the window.  5/3 levels hold integers only; 9/7 slots may
 // hold f32 samples (float_in).  No barrier.
-template <bool REV>
+template <bool REV, class G = Big>
 __device__ __forceinline__ void commit_window(const LevelView &L, int bx, int by,
-                                              const uint32_t *rows, const WinFetch<REV> &f,
+                                              const uint32_t *rows, const WinFetch<REV, G> &f,
                                               typename Sample<REV>::T *win) {
     using T = typename Sample<REV>::T;
-    using F = WinFetch<REV>;
-    constexpr int H = Win<REV>::H, N = Win<REV>::N, P = Win<REV>::P;
+    using F = WinFetch<REV, G>;
+    constexpr int kBlk = G::kBlk, kIdwtThreads = G::kThreads;
+    (void)kIdwtThreads;
+    constexpr int H = Win<REV, G>::H, N = Win<REV, G>::N, P = Win<REV, G>::P;
     const int ox = bx * kBlk, oy = by * kBlk;
     const int nh = min(kBlk, L.h - oy) + 2 * H;
     const int tid = threadIdx.x;
@@ -624,14 +646,14 @@ __device__ __forceinline__ void commit_window(const LevelView &L, int bx, int by
         }
     };
     {
-        const int xx = tid & 63, y0 = tid >> 6;
+        const int xx = tid % kBlk, y0 = tid / kBlk;
         const Col c = column(xx);
 #pragma unroll
         for (int k = 0; k < F::kRows0; ++k)
-            win[(y0 + 4 * k) * P + xx] = convert(c, f.raw0[k], (f.y1m0 >> k) & 1u);
+            win[(y0 + F::kGroups * k) * P + xx] = convert(c, f.raw0[k], (f.y1m0 >> k) & 1u);
     }
     {
-        const int xx = 64 + tid % F::kHalo, y0 = tid / F::kHalo;
+        const int xx = kBlk + tid % F::kHalo, y0 = tid / F::kHalo;
         const Col c = column(xx);
 #pragma unroll
         for (int k = 0; k < F::kRows1; ++k) {
@@ -644,11 +666,13 @@ __device__ __forceinline__ void commit_window(const LevelView &L, int bx, int by
 // Lifts the committed window of block (bx, by) and hands the result to
 // `sink`.  Starts with a barrier (the window writes of every thread must be
 // visible); the sink's writes are complete for the calling thread only.
-template <bool REV, class Sink>
+template <bool REV, class Sink, class G = Big>
 __device__ __forceinline__ void lift_block(const LevelView &L, int bx, int by,
                                            typename Sample<REV>::T *win, const Sink &sink,
                                            int &bw_out, int &bh_out) {
-    constexpr int H = Win<REV>::H, P = Win<REV>::P;
+    constexpr int kBlk = G::kBlk, kIdwtThreads = G::kThreads;
+    constexpr int H = Win<REV, G>::H, P = Win<REV, G>::P;
+    using T = typename Sample<REV>::T;
     const int ox = bx * kBlk, oy = by * kBlk;
     const int bw = min(kBlk, L.w - ox), bh = min(kBlk, L.h - oy);
     const int nw = bw + 2 * H, nh = bh + 2 * H;
@@ -657,13 +681,13 @@ __device__ __forceinline__ void lift_block(const LevelView &L, int bx, int by,
     if (L.levels) {
         const uint32_t qx = (L.x0 + (uint32_t)(ox - H)) & 1u, qy = (L.y0 + (uint32_t)(oy - H)) & 1u;
         if (L.w == 1)
-            lift<REV, P, true>(win, nh, nw, L.x0 + (uint32_t)(ox - H), true, (L.x0 & 1u) != 0);
+            lift<REV, P, true, T, G>(win, nh, nw, L.x0 + (uint32_t)(ox - H), true, (L.x0 & 1u) != 0);
         else
-            lift_window<REV, true>(win, nh, bw, bh, qx, sink);
+            lift_window<REV, true, Sink, G>(win, nh, bw, bh, qx, sink);
         if (L.h == 1) {
-            lift<REV, P, false>(win, bw, nh, L.y0 + (uint32_t)(oy - H), true, (L.y0 & 1u) != 0);
+            lift<REV, P, false, T, G>(win, bw, nh, L.y0 + (uint32_t)(oy - H), true, (L.y0 & 1u) != 0);
         } else {
-            lift_window<REV, false>(win, nh, bw, bh, qy, sink);
+            lift_window<REV, false, Sink, G>(win, nh, bw, bh, qy, sink);
             in_window = false;
         }
     }
@@ -681,21 +705,21 @@ __device__ __forceinline__ void lift_block(const LevelView &L, int bx, int by,
 // lifting.  win: the window from column X0 on.  Result element (x, y) of an
 // in-place sink's block is at win[(y + H) * P + x + H].  rows: LDS scratch
 // of 2 N words.
-template <bool REV, class Sink>
+template <bool REV, class Sink, class G = Big>
 __device__ __forceinline__ void synth_block(const LevelView &L, const uint32_t *__restrict__ coef,
                                             int bx, int by, typename Sample<REV>::T *win,
                                             uint32_t *rows, const Sink &sink, int &bw_out,
                                             int &bh_out) {
-    window_rows<REV>(L, by, rows);
+    window_rows<REV, G>(L, by, rows);
     __syncthreads();
-    if (L.levels && L.aligned && (L.x0 & 1u) == 0 && L.w > Win<REV>::H) {
-        fetch_chunks<REV>(L, bx, by, rows, win);
+    if (L.levels && L.aligned && (L.x0 & 1u) == 0 && L.w > Win<REV, G>::H) {
+        fetch_chunks<REV, G>(L, bx, by, rows, win);
     } else {
-        WinFetch<REV> f;
-        fetch_window<REV>(L, bx, by, rows, f);
-        commit_window<REV>(L, bx, by, rows, f, win);
+        WinFetch<REV, G> f;
+        fetch_window<REV, G>(L, bx, by, rows, f);
+        commit_window<REV, G>(L, bx, by, rows, f, win);
     }
-    lift_block<REV>(L, bx, by, win, sink, bw_out, bh_out);
+    lift_block<REV, Sink, G>(L, bx, by, win, sink, bw_out, bh_out);
 }
 
 // Work-groups of a launch go to the 8 XCDs round-robin (work-group w runs on
@@ -730,12 +754,16 @@ __device__ __forceinline__ const J *job_of(const J *jobs, uint32_t njobs,
     return jobs + (block_job ? block_job[blk] : find_job(jobs, njobs, blk));
 }
 
-template <bool REV>
+// One output block of one level.  G = Small serves launches whose levels
+// are all at most 32x32 (one block each; the plan's block tables count
+// 64x64 blocks, which are then the same single block).
+template <bool REV, class G = Big>
 __device__ __forceinline__ void level_body(const LevelJob *jobs, uint32_t njobs, uint32_t block_offset,
                                            const uint32_t *block_job, uint32_t *coef) {
-    constexpr int N = Win<REV>::N;
-    __shared__ typename Sample<REV>::T win_lds[Win<REV>::kWords];
-    typename Sample<REV>::T *const win = win_lds + Win<REV>::X0;
+    constexpr int kBlk = G::kBlk;
+    constexpr int N = Win<REV, G>::N;
+    __shared__ typename Sample<REV>::T win_lds[Win<REV, G>::kWords];
+    typename Sample<REV>::T *const win = win_lds + Win<REV, G>::X0;
     __shared__ uint32_t rows[2 * N];
     const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x) + block_offset;
     const LevelJob *lj = job_of(jobs, njobs, block_job, blk);
@@ -749,7 +777,7 @@ __device__ __forceinline__ void level_body(const LevelJob *jobs, uint32_t njobs,
     const int bh_blk = min(kBlk, L.h - by * kBlk);
     const SinkGlobal<REV> sink{coef, out + (uint32_t)(by * kBlk) * opitch + (uint32_t)(bx * kBlk),
                                opitch, bh_blk};
-    synth_block<REV>(L, coef, bx, by, win, rows, sink, bw, bh);
+    synth_block<REV, SinkGlobal<REV>, G>(L, coef, bx, by, win, rows, sink, bw, bh);
 }
 
 extern "C" __global__ void __launch_bounds__(kIdwtThreads)
@@ -762,6 +790,22 @@ extern "C" __global__ void __launch_bounds__(kIdwtThreads)
 sgn_idwt_level_irr(const LevelJob *jobs, uint32_t njobs, uint32_t block_offset,
                    const uint32_t *block_job, uint32_t *coef) {
     level_body<false>(jobs, njobs, block_offset, block_job, coef);
+}
+
+// The same for launches whose levels are all at most 32x32: one wave per
+// level, a window of a fifth of the size, so several times more
+// work-groups fit a CU (these launches are bound by each work-group's
+// chain of memory round trips, not by bandwidth).
+extern "C" __global__ void __launch_bounds__(Small::kThreads)
+sgn_idwt_level_rev_small(const LevelJob *jobs, uint32_t njobs, uint32_t block_offset,
+                         const uint32_t *block_job, uint32_t *coef) {
+    level_body<true, Small>(jobs, njobs, block_offset, block_job, coef);
+}
+
+extern "C" __global__ void __launch_bounds__(Small::kThreads)
+sgn_idwt_level_irr_small(const LevelJob *jobs, uint32_t njobs, uint32_t block_offset,
+                         const uint32_t *block_job, uint32_t *coef) {
+    level_body<false, Small>(jobs, njobs, block_offset, block_job, coef);
 }
 
 // ---- final stage -------------------------------------------------------

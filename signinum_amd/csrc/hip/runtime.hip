@@ -568,16 +568,26 @@ uint32_t launch_wave(sgn_hip_session *s, sgn_j2k_plan &P, const CbJob *d_jobs, u
     return 1;
 }
 
+// True when every level of [a, b) is at most 32x32 (one Small block).
+bool small_levels(const std::vector<LevelJob> &v, uint32_t a, uint32_t b) {
+    for (uint32_t j = a; j < b; ++j)
+        if (v[j].x1 - v[j].x0 > uint32_t(Small::kBlk) || v[j].y1 - v[j].y0 > uint32_t(Small::kBlk))
+            return false;
+    return true;
+}
+
 uint32_t launch_levels(sgn_hip_session *s, sgn_j2k_plan &P, int rev, uint32_t a, uint32_t b) {
     const uint32_t blocks = group_blocks(P.lv[rev], a, b);
     if (!blocks) return 0;
     const uint32_t off = P.lv[rev][a].block_base;
-    if (rev)
-        hipLaunchKernelGGL(sgn_idwt_level_rev, dim3(blocks), dim3(kIdwtThreads), 0, s->stream,
-                           P.d_lv[1], uint32_t(P.lv[1].size()), off, P.d_lv_block_job[1], P.d_coef);
-    else
-        hipLaunchKernelGGL(sgn_idwt_level_irr, dim3(blocks), dim3(kIdwtThreads), 0, s->stream,
-                           P.d_lv[0], uint32_t(P.lv[0].size()), off, P.d_lv_block_job[0], P.d_coef);
+    // levels of at most 32x32 take the one-wave kernel (same block count:
+    // each is one block for either kernel)
+    const bool small = small_levels(P.lv[rev], a, b);
+    const auto kernel = rev ? (small ? sgn_idwt_level_rev_small : sgn_idwt_level_rev)
+                            : (small ? sgn_idwt_level_irr_small : sgn_idwt_level_irr);
+    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(small ? Small::kThreads : kIdwtThreads), 0,
+                       s->stream, P.d_lv[rev], uint32_t(P.lv[rev].size()), off,
+                       P.d_lv_block_job[rev], P.d_coef);
     return 1;
 }
 
