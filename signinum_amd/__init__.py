@@ -190,12 +190,18 @@ class DeviceSurface:
     surface.rs:11-127): a pitched device image in memory owned by the
     library (sgn_device_alloc), released with the surface."""
 
-    def __init__(self, session, width, height, pixel_format, pitch_bytes=None):
+    def __init__(self, session, width, height, pixel_format, pitch_bytes=None, _block=None,
+                 _offset=0):
         self.session = session
         self.width = int(width)
         self.height = int(height)
         self.pixel_format = PixelFormat(pixel_format)
         self.pitch_bytes = int(pitch_bytes or self.width * self.pixel_format.bytes_per_pixel)
+        # a surface of a batch lives in the batch's shared allocation
+        self._block = _block
+        if _block is not None:
+            self._ptr = _block.ptr + int(_offset)
+            return
         p = ctypes.c_void_p()
         _check(_lib.load().sgn_device_alloc(session.handle, max(1, self.byte_len), ctypes.byref(p)))
         self._ptr = p.value
@@ -243,7 +249,10 @@ class DeviceSurface:
         return out.reshape(self.height, self.width, self.pixel_format.channels)
 
     def close(self):
-        if getattr(self, "_ptr", None):
+        if getattr(self, "_block", None) is not None:
+            self._block = None  # the shared allocation goes with its last surface
+            self._ptr = None
+        elif getattr(self, "_ptr", None):
             _lib.load().sgn_device_free(self.session.handle, self._ptr)
             self._ptr = None
 
@@ -256,6 +265,37 @@ class DeviceSurface:
 
 def _alloc_surface(session, width, height, fmt):
     return DeviceSurface(session, width, height, fmt)
+
+
+class _DeviceBlock:
+    """One device allocation shared by the surfaces of a batch; freed when
+    the last of them is released."""
+
+    def __init__(self, session, nbytes):
+        self.session = session
+        p = ctypes.c_void_p()
+        _check(_lib.load().sgn_device_alloc(session.handle, max(1, int(nbytes)), ctypes.byref(p)))
+        self.ptr = p.value
+
+    def __del__(self):
+        try:
+            _lib.load().sgn_device_free(self.session.handle, self.ptr)
+        except Exception:
+            pass
+
+
+def _alloc_surfaces(session, dims, fmts):
+    """Surfaces of (width, height) and format each, carved from one device
+    allocation at 256-byte aligned offsets (one allocation instead of one
+    per tile)."""
+    sizes = [max(1, w * PixelFormat(f).bytes_per_pixel * h) for (w, h), f in zip(dims, fmts)]
+    offs, total = [], 0
+    for n in sizes:
+        offs.append(total)
+        total += (n + 255) & ~255
+    block = _DeviceBlock(session, total)
+    return [DeviceSurface(session, w, h, f, _block=block, _offset=o)
+            for (w, h), f, o in zip(dims, fmts, offs)]
 
 
 class HipEvent:
@@ -385,13 +425,13 @@ class TilePlan:
         self.scale = self.scales[0] if n else Downscale.NONE
         self._keep = [np.frombuffer(b, np.uint8) for b in self._inputs]
         if surfaces is None:
-            surfaces = []
+            dims = []
             for i, b in enumerate(self._inputs):
                 try:
-                    w, h = scaled_dims(b, self.scales[i])
+                    dims.append(scaled_dims(b, self.scales[i]))
                 except J2kError as e:
                     raise TileBatchError(i, e)
-                surfaces.append(_alloc_surface(session, w, h, self.formats[i]))
+            surfaces = _alloc_surfaces(session, dims, self.formats)
         self.surfaces = surfaces
         reqs = (_lib.TileReq * max(1, n))()
         for i, (k, s) in enumerate(zip(self._keep, surfaces)):
@@ -571,13 +611,13 @@ def decode_tiles_to_device(session, inputs, fmt, scale=Downscale.NONE, mode=MODE
     scale = Downscale(scale)
     inputs = [bytes(b) for b in inputs]
     keep = [np.frombuffer(b, np.uint8) for b in inputs]
-    surfaces = []
+    dims = []
     for b in inputs:
         try:
-            w, h = scaled_dims(b, scale)
+            dims.append(scaled_dims(b, scale))
         except J2kError as e:
-            raise TileBatchError(len(surfaces), e)
-        surfaces.append(_alloc_surface(session, w, h, fmt))
+            raise TileBatchError(len(dims), e)
+    surfaces = _alloc_surfaces(session, dims, [fmt] * len(inputs))
     n = len(inputs)
     jobs = (TileJob * max(1, n))()
     for i, (k, s) in enumerate(zip(keep, surfaces)):

@@ -450,13 +450,19 @@ Status upload_plan(const sgn_tile_req *jobs, sgn_j2k_plan &P) {
     }
     std::unique_ptr<uint8_t[]> staging(new uint8_t[std::max<uint64_t>(1, host_bytes)]);
     uint8_t *h = staging.get();
-    uint64_t a = o_arena;
-    for (size_t i = 0; i < P.ntiles; ++i) {
-        std::memcpy(h + a, jobs[i].bytes, jobs[i].len);
-        const uint64_t padded = (jobs[i].len + 15) & ~uint64_t(15);
-        std::memset(h + a + jobs[i].len, 0, padded - jobs[i].len);
-        a += padded;
+    // codestreams at their 16-byte aligned arena offsets, copied by the
+    // worker pool (a fresh staging buffer also takes its first-touch page
+    // faults there, in parallel)
+    std::vector<uint64_t> at_tile(P.ntiles);
+    for (uint64_t i = 0, a = o_arena; i < P.ntiles; ++i) {
+        at_tile[i] = a;
+        a += (jobs[i].len + 15) & ~uint64_t(15);
     }
+    WorkerPool::get().run(P.ntiles, [&](size_t i) {
+        std::memcpy(h + at_tile[i], jobs[i].bytes, jobs[i].len);
+        const uint64_t padded = (jobs[i].len + 15) & ~uint64_t(15);
+        std::memset(h + at_tile[i] + jobs[i].len, 0, padded - jobs[i].len);
+    });
     auto put = [&](uint64_t o, const auto &v) {
         if (!v.empty()) std::memcpy(h + o, v.data(), bytes_of(v));
     };
