@@ -18,6 +18,8 @@
 #include <cstring>
 #include <exception>
 #include <memory>
+#include <new>
+#include <sys/mman.h>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -265,6 +267,50 @@ class WorkerPool {
     bool stop_ = false;
 };
 
+// out = in ordered by cost, largest first, ties in input order (the order
+// of a stable sort): a counting sort over the bounded cost range (a
+// code-block's cost is its coded bytes plus a quarter of its samples).
+void sort_by_cost(const std::vector<CbJob> &in, std::vector<CbJob> &out) {
+    uint32_t top = 0;
+    for (const CbJob &j : in) top = std::max(top, j.cost);
+    if (top > (1u << 20)) {
+        out = in;
+        std::stable_sort(out.begin(), out.end(),
+                         [](const CbJob &a, const CbJob &b) { return a.cost > b.cost; });
+        return;
+    }
+    std::vector<uint32_t> at(size_t(top) + 2, 0);
+    for (const CbJob &j : in) at[top - j.cost + 1]++;
+    for (size_t k = 1; k < at.size(); ++k) at[k] += at[k - 1];
+    out.resize(in.size());
+    for (const CbJob &j : in) out[at[top - j.cost]++] = j;
+}
+
+// Host staging memory for a plan upload: an anonymous mapping advised to
+// use transparent huge pages, so that filling a few hundred MB takes a few
+// hundred page faults rather than one per 4 KB.
+class HostStaging {
+  public:
+    explicit HostStaging(size_t bytes) {
+        constexpr size_t kHuge = size_t(2) << 20;
+        len_ = std::max<size_t>(1, bytes) + kHuge;
+        void *m = mmap(nullptr, len_, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS, -1, 0);
+        if (m == MAP_FAILED) throw std::bad_alloc();
+        base_ = static_cast<uint8_t *>(m);
+        p_ = reinterpret_cast<uint8_t *>((reinterpret_cast<uintptr_t>(base_) + kHuge - 1) &
+                                         ~uintptr_t(kHuge - 1));
+        (void)madvise(p_, std::max<size_t>(1, bytes), MADV_HUGEPAGE);
+    }
+    ~HostStaging() { munmap(base_, len_); }
+    HostStaging(const HostStaging &) = delete;
+    HostStaging &operator=(const HostStaging &) = delete;
+    uint8_t *get() const { return p_; }
+
+  private:
+    uint8_t *base_ = nullptr, *p_ = nullptr;
+    size_t len_ = 0;
+};
+
 Status build_plan(const sgn_tile_req *jobs, size_t n, sgn_j2k_plan &P, int32_t *tile_status) {
     P.ntiles = n;
     std::vector<LocalPlan> lps(n);
@@ -356,9 +402,7 @@ Status build_plan(const sgn_tile_req *jobs, size_t n, sgn_j2k_plan &P, int32_t *
     }
     // code-blocks for the batched lane kernel, ordered by predicted cost so
     // that a wave's lanes finish together
-    P.cb_lane = P.cb_lane_plain;
-    std::stable_sort(P.cb_lane.begin(), P.cb_lane.end(),
-                     [](const CbJob &a, const CbJob &b) { return a.cost > b.cost; });
+    sort_by_cost(P.cb_lane_plain, P.cb_lane);
     // synthesis levels grouped by (transform, depth), tiles in order
     for (int rev = 0; rev < 2; ++rev) {
         P.lv_group[rev].assign(1, 0);
@@ -448,7 +492,7 @@ Status upload_plan(const sgn_tile_req *jobs, sgn_j2k_plan &P) {
         set_last_error("hipMalloc failed for the plan (" + std::to_string(off) + " bytes)");
         return kErrDevice;
     }
-    std::unique_ptr<uint8_t[]> staging(new uint8_t[std::max<uint64_t>(1, host_bytes)]);
+    HostStaging staging(host_bytes);
     uint8_t *h = staging.get();
     // codestreams at their 16-byte aligned arena offsets, copied by the
     // worker pool (a fresh staging buffer also takes its first-touch page
