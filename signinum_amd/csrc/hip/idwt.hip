@@ -910,9 +910,54 @@ __device__ __forceinline__ void final_body(const FinalJob *jobs, uint32_t njobs,
     // Each thread converts and stores four horizontally adjacent pixels.
     const bool words = F.bps == 1 && ((reinterpret_cast<uintptr_t>(F.dst) | F.pitch) & 3) == 0;
     const bool int8_out = REV && F.bps == 1 && F.same8;
+    // 5/3 with RCT into 8-bit RGB (the common case), integers throughout:
+    // R = Cr + I, G = I, B = Cb + I with I = Y - ((Cb + Cr) >> 2), each
+    // shifted by 2^(P-1) and clamped (mct.rs:71-156, decode.rs:1720-1752,
+    // lib.rs:2202-2295), three words per four pixels.
+    const bool rct_rgb8 = REV && int8_out && words && F.nc == 3 && F.mct && F.ch == 3;
+    if (rct_rgb8) {
+        const int32_t a0 = 1 << (F.prec0 - 1), a1 = 1 << (F.prec1 - 1), a2 = 1 << (F.prec2 - 1);
+        for (int k = threadIdx.x; k < (kBlk / 4) * bh; k += kIdwtThreads) {
+            const int y = k / (kBlk / 4), x4 = (k % (kBlk / 4)) * 4;
+            if (x4 + 4 > bw) continue;  // partial groups: the general loop below
+            int32_t c0[4], c1[4], c2[4];
+            if constexpr (kNarrowPlanes) {
+                const uint2 q0 = *reinterpret_cast<const uint2 *>(&res[0][y * kBlk + x4]);
+                const uint2 q1 = *reinterpret_cast<const uint2 *>(&res[1][y * kBlk + x4]);
+                c0[0] = (int32_t)(q0.x << 16) >> 16;
+                c0[1] = (int32_t)q0.x >> 16;
+                c0[2] = (int32_t)(q0.y << 16) >> 16;
+                c0[3] = (int32_t)q0.y >> 16;
+                c1[0] = (int32_t)(q1.x << 16) >> 16;
+                c1[1] = (int32_t)q1.x >> 16;
+                c1[2] = (int32_t)(q1.y << 16) >> 16;
+                c1[3] = (int32_t)q1.y >> 16;
+            } else {
+                const int4 q0 = *reinterpret_cast<const int4 *>(&res[0][y * kBlk + x4]);
+                const int4 q1 = *reinterpret_cast<const int4 *>(&res[1][y * kBlk + x4]);
+                c0[0] = q0.x, c0[1] = q0.y, c0[2] = q0.z, c0[3] = q0.w;
+                c1[0] = q1.x, c1[1] = q1.y, c1[2] = q1.z, c1[3] = q1.w;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) c2[q] = (int32_t)win[(y + H) * P + x4 + q + H];
+            uint32_t r[4], g[4], b[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int32_t i1 = c0[q] - ((c2[q] + c1[q]) >> 2);
+                r[q] = (uint32_t)min(max(c2[q] + i1 + a0, 0), 255);
+                g[q] = (uint32_t)min(max(i1 + a1, 0), 255);
+                b[q] = (uint32_t)min(max(c1[q] + i1 + a2, 0), 255);
+            }
+            uint32_t *pw = reinterpret_cast<uint32_t *>(F.dst + (size_t)(by * kBlk + y) * F.pitch +
+                                                        (size_t)(bx * kBlk + x4) * 3);
+            pw[0] = r[0] | (g[0] << 8) | (b[0] << 16) | (r[1] << 24);
+            pw[1] = g[1] | (b[1] << 8) | (r[2] << 16) | (g[2] << 24);
+            pw[2] = b[2] | (r[3] << 8) | (g[3] << 16) | (b[3] << 24);
+        }
+    }
     for (int k = threadIdx.x; k < (kBlk / 4) * bh; k += kIdwtThreads) {
         const int y = k / (kBlk / 4), x4 = (k % (kBlk / 4)) * 4;
-        if (x4 >= bw) continue;
+        if (x4 >= bw || (rct_rgb8 && x4 + 4 <= bw)) continue;
         const int np = min(4, bw - x4);
         uint32_t o[4][4];
         // the four pixels of each LDS plane in one 16-byte (8-byte) read
