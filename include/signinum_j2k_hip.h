@@ -327,7 +327,10 @@ int sgn_j2k_plan_autotune_order(sgn_hip_session *session, sgn_j2k_plan *plan, in
  * hbm_budget.  Up to max_inflight batches run at once, each on its own
  * stream, so new batches overlap running ones.  Outputs stay in device
  * memory until the request is released.  The caller drives the server with
- * sgn_server_pump(); one server is used from one thread at a time. */
+ * sgn_server_pump(); one server is used from one thread at a time.  Device
+ * blocks of finished batches and released requests are kept for reuse (up
+ * to hbm_budget / 4 bytes of them; freed when the server is destroyed),
+ * because freeing device memory waits for every batch in flight. */
 typedef struct sgn_server sgn_server;
 typedef struct {
     uint32_t window_us;    /* batching window (default 2000) */

@@ -17,7 +17,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <exception>
+#include <map>
 #include <memory>
+#include <unordered_map>
 #include <new>
 #include <sys/mman.h>
 #include <mutex>
@@ -145,6 +147,57 @@ struct sgn_cost_model {
     }
 };
 
+// Device blocks kept for reuse instead of freed: a serving loop allocates
+// and releases plan and output blocks continuously, and hipFree waits for
+// the whole device, so a freed block would stall every batch in flight.
+// get() hands out a cached block of at least `bytes` (at most twice that,
+// plus 1 MiB) or allocates one; put() caches it, freeing the largest cached
+// blocks while more than `cap` bytes are cached.  Not thread-safe: one
+// owner (the server) drives it.
+class DeviceBlocks {
+  public:
+    explicit DeviceBlocks(uint64_t cap) : cap_(cap) {}
+    ~DeviceBlocks() { trim(0); }
+    DeviceBlocks(const DeviceBlocks &) = delete;
+    DeviceBlocks &operator=(const DeviceBlocks &) = delete;
+    hipError_t get(uint64_t bytes, void **out) {
+        bytes = std::max<uint64_t>(bytes, 256);
+        auto it = free_.lower_bound(bytes);
+        if (it != free_.end() && it->first <= 2 * bytes + (uint64_t(1) << 20)) {
+            *out = it->second;
+            size_[*out] = it->first;
+            cached_ -= it->first;
+            free_.erase(it);
+            return hipSuccess;
+        }
+        const hipError_t e = hipMalloc(out, bytes);
+        if (e == hipSuccess) size_[*out] = bytes;
+        return e;
+    }
+    void put(void *p) {
+        if (!p) return;
+        const auto it = size_.find(p);
+        if (it == size_.end()) return;
+        free_.emplace(it->second, p);
+        cached_ += it->second;
+        size_.erase(it);
+        trim(cap_);
+    }
+
+  private:
+    void trim(uint64_t keep) {
+        while (cached_ > keep && !free_.empty()) {
+            auto last = std::prev(free_.end());
+            (void)hipFree(last->second);
+            cached_ -= last->first;
+            free_.erase(last);
+        }
+    }
+    uint64_t cap_, cached_ = 0;
+    std::multimap<uint64_t, void *> free_;         // cached: size -> block
+    std::unordered_map<void *, uint64_t> size_;   // handed out: block -> size
+};
+
 struct sgn_j2k_plan {
     size_t ntiles = 0;
     // host tables
@@ -185,11 +238,17 @@ struct sgn_j2k_plan {
 
     void *d_block = nullptr;  // the one allocation every device pointer above lives in
     uint64_t device_bytes = 0;  // its size
+    DeviceBlocks *blocks = nullptr;  // where d_block came from (else hipMalloc)
 
     ~sgn_j2k_plan() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
-        if (d_block) (void)hipFree(d_block);
+        if (d_block) {
+            if (blocks)
+                blocks->put(d_block);
+            else
+                (void)hipFree(d_block);
+        }
     }
 };
 
@@ -487,7 +546,7 @@ Status upload_plan(const sgn_tile_req *jobs, sgn_j2k_plan &P) {
     const uint64_t o_ff = take(P.fin_block_job[1].size() * sizeof(uint32_t));
     const uint64_t o_queue = take(16), o_coef = take(P.coef_elems * sizeof(uint32_t));
     P.device_bytes = off;
-    if (hipMalloc(&P.d_block, off) != hipSuccess) {
+    if ((P.blocks ? P.blocks->get(off, &P.d_block) : hipMalloc(&P.d_block, off)) != hipSuccess) {
         P.d_block = nullptr;
         set_last_error("hipMalloc failed for the plan (" + std::to_string(off) + " bytes)");
         return kErrDevice;
@@ -760,13 +819,15 @@ void drop_graph(sgn_j2k_plan &P) {
 }
 
 Status make_plan(sgn_hip_session *s, const sgn_tile_req *jobs, size_t n,
-                 std::unique_ptr<sgn_j2k_plan> &out, int32_t *tile_status) {
+                 std::unique_ptr<sgn_j2k_plan> &out, int32_t *tile_status,
+                 DeviceBlocks *blocks = nullptr) {
     if (!s || (!jobs && n)) {
         set_last_error("null session or jobs");
         return kErrArg;
     }
     HIP_TRY(hipSetDevice(s->device));
     auto P = std::make_unique<sgn_j2k_plan>();
+    P->blocks = blocks;
     Status st = build_plan(jobs, n, *P, tile_status);
     if (st) return st;
     st = upload_plan(jobs, *P);

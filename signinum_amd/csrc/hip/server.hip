@@ -53,6 +53,9 @@ uint64_t plan_device_bytes(const sgn_j2k_plan &P) { return P.device_bytes; }
 struct sgn_server {
     sgn_server_config cfg{};
     int device = 0;
+    // plan and output blocks, reused across batches (declared before the
+    // members whose blocks it takes back)
+    std::unique_ptr<DeviceBlocks> blocks;
     std::vector<sgn_hip_session *> slots;  // one stream per in-flight batch
     std::vector<bool> slot_busy;
     std::unordered_map<uint64_t, std::unique_ptr<ServeRequest>> reqs;
@@ -71,8 +74,11 @@ struct sgn_server {
             }
         }
         running.clear();
-        for (auto &kv : reqs)
-            if (kv.second->out_block) (void)hipFree(kv.second->out_block);
+        if (blocks)
+            for (auto &kv : reqs)
+                if (kv.second->out_block) blocks->put(kv.second->out_block);
+        reqs.clear();
+        blocks.reset();
         for (sgn_hip_session *s : slots) sgn_hip_session_destroy(s);
     }
 };
@@ -199,7 +205,7 @@ Status launch_batch(sgn_server &S, bool force) {
             size_t total = 0;
             for (size_t t = 0; t < r.bytes.size(); ++t)
                 total += (std::max<size_t>(16, r.row_bytes[t] * r.height[t]) + 255) & ~size_t(255);
-            HIP_TRY(hipMalloc(&r.out_block, total));
+            HIP_TRY(S.blocks->get(total, &r.out_block));
             size_t o = 0;
             for (size_t t = 0; t < r.bytes.size(); ++t) {
                 r.out[t] = static_cast<uint8_t *>(r.out_block) + o;
@@ -212,7 +218,7 @@ Status launch_batch(sgn_server &S, bool force) {
     }
     std::vector<int32_t> ts(tiles.size(), 0);
     sgn_hip_session *sess = S.slots[size_t(slot)];
-    Status st = make_plan(sess, tiles.data(), tiles.size(), B->plan, ts.data());
+    Status st = make_plan(sess, tiles.data(), tiles.size(), B->plan, ts.data(), S.blocks.get());
     if (st) {
         // parse failures fail their own requests; the rest retry next pump
         size_t r = 0;
@@ -275,6 +281,8 @@ int sgn_server_create(int device, const sgn_server_config *cfg, sgn_server **out
         return SGN_E_ARG;
     }
     S->device = device;
+    // cached free blocks stay within a quarter of the budget
+    S->blocks = std::make_unique<DeviceBlocks>(S->cfg.hbm_budget / 4);
     for (uint32_t i = 0; i < S->cfg.max_inflight; ++i) {
         sgn_hip_session *s = nullptr;
         const int rc = sgn_hip_session_create(device, &s);
@@ -378,7 +386,7 @@ int sgn_server_release(sgn_server *S, uint64_t id) {
     if (it == S->reqs.end() || it->second->state != ServeRequest::kDone) return SGN_E_ARG;
     ServeRequest &r = *it->second;
     (void)hipSetDevice(S->device);
-    if (r.out_block) (void)hipFree(r.out_block);
+    if (r.out_block) S->blocks->put(r.out_block);
     if (r.out_counted) S->hbm_in_use -= r.out_bytes;
     S->reqs.erase(it);
     return SGN_OK;
