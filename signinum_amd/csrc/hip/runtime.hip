@@ -102,6 +102,9 @@ const RowsVariant kRowsVariants[] = {
 };
 #undef SGN_ROWS_VARIANT
 constexpr int kRowsVariantCount = int(sizeof(kRowsVariants) / sizeof(kRowsVariants[0]));
+// variant of a plan nobody autotuned: w4-seg4-small-queue, the autotuner's
+// choice on every batch measured (small rings, three work-groups per CU)
+constexpr int kDefaultLaneVariant = 3;
 
 }  // namespace
 
@@ -231,7 +234,7 @@ struct sgn_j2k_plan {
     uint32_t *d_fin_redo = nullptr;  // 5/3 final blocks left to the int32-plane pass
     uint8_t *d_status_lane = nullptr, *d_status_wave = nullptr;
     uint32_t *d_queue = nullptr;  // work counter of the persistent lane kernel
-    int lane_variant = 0;         // index into kRowsVariants
+    int lane_variant = kDefaultLaneVariant;  // index into kRowsVariants
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
     uint32_t launches_batched = 0;
