@@ -95,6 +95,10 @@ int sgn_device_upload(sgn_hip_session *session, void *dev, const void *host, siz
 /* Pitched download of `rows` rows of `row_bytes` each. */
 int sgn_device_download_2d(sgn_hip_session *session, void *host, size_t host_stride,
                            const void *dev, size_t dev_pitch, size_t row_bytes, size_t rows);
+/* Page-locked host memory (hipHostMalloc): codestreams placed in it go to
+ * the device by DMA without a staging copy.  Free with sgn_host_free. */
+int sgn_host_alloc(size_t bytes, void **out);
+void sgn_host_free(void *ptr);
 int sgn_hip_event_create(void **event);
 void sgn_hip_event_destroy(void *event);
 /* Records the event on the session stream. */
@@ -134,7 +138,13 @@ typedef struct {
 /* decode_tiles_into / TileBatchDecodeManyDevice::decode_tiles_to_device
  * (crates/signinum-j2k/src/batch.rs:110-147; crates/signinum-j2k-cuda/src/
  * codec.rs:190-210).  On failure *first_fail holds the first failing tile
- * index (TileBatchError, batch.rs:47-67).  Synchronous. */
+ * index (TileBatchError, batch.rs:47-67).  Synchronous.
+ * SGN_MODE_BATCHED pipelines batches of more than 1024 tiles in chunks
+ * (SGN_PIPELINE_TILES in the environment sets the chunk): the host parses
+ * chunk k+1 while chunk k's bytes travel and chunk k-1 decodes.  Codestreams
+ * that lie back to back (gaps under 4 KB) in page-locked host memory (see
+ * sgn_host_alloc) are copied to the device straight from it; others are
+ * staged through the session's pinned buffers. */
 int sgn_j2k_decode_tiles_device(sgn_hip_session *session, const sgn_tile_job *jobs, size_t n,
                                 int fmt, uint32_t denominator, int mode, size_t *first_fail);
 
@@ -207,6 +217,10 @@ typedef struct {
     uint32_t launches_batched; /* kernel launches per batched decode */
     uint32_t narrow_samples;   /* 1: code-block samples stored as int16 (every band has at
                                   most 15 magnitude bit-planes); 0: int32 */
+    /* host time of the plan build (ms): per-tile parse on the worker pool,
+     * merge into the batch tables, staging fill, copy submission (until the
+     * copies completed, for a synchronous build) */
+    float host_parse_ms, host_merge_ms, host_stage_ms, host_copy_ms;
 } sgn_j2k_plan_stats;
 int sgn_j2k_plan_get_stats(const sgn_j2k_plan *plan, sgn_j2k_plan_stats *stats);
 /* Output geometry of tile i: width, height, bytes per row (unpadded). */
@@ -251,6 +265,25 @@ int sgn_ht_decode_code_blocks_raw(sgn_hip_session *session, const uint8_t *data,
                                   size_t data_len, const sgn_ht_code_block_job *jobs, size_t n,
                                   uint32_t *out, size_t out_len);
 
+/* HtCodeBlockBatchJob (lib.rs:188-197): a code-block at (output_x,
+ * output_y) of its sub-band; code_block.data_offset is relative to `data`,
+ * code_block.output_offset is ignored (the position is the block's place
+ * in the band) and code_block.output_stride is the band width. */
+typedef struct {
+    uint32_t output_x, output_y;
+    sgn_ht_code_block_job code_block;
+} sgn_ht_code_block_batch_job;
+
+/* decode_sub_band (lib.rs:864; HtSubBandDecodeJob lib.rs:200-209, called at
+ * decode.rs:1641-1650): decodes every block of one width x height sub-band
+ * on the device into `out` (row-major, out_len >= width * height) as
+ * f32 = coefficient * step.  Samples outside the listed blocks are left as
+ * the caller passed them. */
+int sgn_ht_decode_sub_band(sgn_hip_session *session, const uint8_t *data, size_t data_len,
+                           uint32_t width, uint32_t height,
+                           const sgn_ht_code_block_batch_job *jobs, size_t n, float *out,
+                           size_t out_len);
+
 /* decode_single_decomposition_idwt (lib.rs:877; J2kSingleDecompositionIdwtJob
  * lib.rs:767-780): one 2D synthesis level over rect [x0,x1) x [y0,y1);
  * band arrays are row-major with the Annex B band sizes.  For 5/3 the f32
@@ -264,6 +297,17 @@ int sgn_idwt_level(sgn_hip_session *session, int reversible, uint32_t x0, uint32
  * planes updated in place, addends applied after the transform. */
 int sgn_inverse_mct(sgn_hip_session *session, int reversible, float *plane0, float *plane1,
                     float *plane2, size_t n, float addend0, float addend1, float addend2);
+
+/* decode_store_component (lib.rs:899; J2kStoreComponentJob lib.rs:805-828,
+ * called at decode.rs:1792-1810): on the device,
+ * output[(output_y + y) * output_width + output_x + x] =
+ *     input[(source_y + y) * input_width + source_x + x] + addend
+ * for x < copy_width, y < copy_height; the rest of `output` is unchanged. */
+int sgn_store_component(sgn_hip_session *session, const float *input, size_t input_len,
+                        uint32_t input_width, uint32_t source_x, uint32_t source_y,
+                        uint32_t copy_width, uint32_t copy_height, float *output,
+                        size_t output_len, uint32_t output_width, uint32_t output_x,
+                        uint32_t output_y, float addend);
 
 /* ---- decode-cost model ---------------------------------------------------
  * No counterpart in the reference: a 5-32-32-1 ReLU MLP over code-block

@@ -724,6 +724,35 @@ class HtCodeBlockDecoder:
                                                  out.ctypes.data, out_len))
         return out
 
+    def decode_sub_band(self, data, width, height, jobs, out=None):
+        """decode_sub_band (lib.rs:864): jobs are (output_x, output_y,
+        HtCodeBlockJob) of one width x height sub-band; returns the band as
+        float32 (height, width), samples outside the blocks taken from `out`
+        (zeros by default)."""
+        arr = (_lib.HtCodeBlockBatchJob * max(1, len(jobs)))()
+        for i, (x, y, j) in enumerate(jobs):
+            arr[i] = _lib.HtCodeBlockBatchJob(int(x), int(y), j)
+        band = np.zeros(width * height, np.float32) if out is None else \
+            np.ascontiguousarray(out, np.float32).reshape(-1).copy()
+        ptr, n, _keep = _buf(data)
+        _check(_lib.load().sgn_ht_decode_sub_band(self.session.handle, ptr, n, int(width),
+                                                  int(height), arr, len(jobs),
+                                                  band.ctypes.data, band.size))
+        return band.reshape(height, width)
+
+    def decode_store_component(self, input, input_width, source_x, source_y, copy_width,
+                               copy_height, output, output_width, output_x, output_y, addend):
+        """decode_store_component (lib.rs:899, J2kStoreComponentJob
+        lib.rs:805-828): returns a copy of `output` with the rectangle
+        stored."""
+        src = np.ascontiguousarray(input, np.float32).reshape(-1)
+        dst = np.ascontiguousarray(output, np.float32).reshape(-1).copy()
+        _check(_lib.load().sgn_store_component(
+            self.session.handle, src.ctypes.data, src.size, int(input_width), int(source_x),
+            int(source_y), int(copy_width), int(copy_height), dst.ctypes.data, dst.size,
+            int(output_width), int(output_x), int(output_y), float(addend)))
+        return dst
+
     def decode_single_decomposition_idwt(self, reversible, rect, ll, hl, lh, hh):
         x0, y0, x1, y1 = rect
         out = np.zeros((y1 - y0) * (x1 - x0), np.float32)

@@ -73,7 +73,9 @@ class PlanStats(ctypes.Structure):
     _fields_ = [("tiles", ctypes.c_uint64), ("code_blocks", ctypes.c_uint64),
                 ("coded_bytes", ctypes.c_uint64), ("output_bytes", ctypes.c_uint64),
                 ("output_pixels", ctypes.c_uint64), ("coefficient_bytes", ctypes.c_uint64),
-                ("launches_batched", ctypes.c_uint32), ("narrow_samples", ctypes.c_uint32)]
+                ("launches_batched", ctypes.c_uint32), ("narrow_samples", ctypes.c_uint32),
+                ("host_parse_ms", ctypes.c_float), ("host_merge_ms", ctypes.c_float),
+                ("host_stage_ms", ctypes.c_float), ("host_copy_ms", ctypes.c_float)]
 
 
 class HtCodeBlockJob(ctypes.Structure):
@@ -91,6 +93,11 @@ class EncodeOptions(ctypes.Structure):
         "guard_bits")] + [("base_step", ctypes.c_float)] +
         [(n, ctypes.c_uint32) for n in ("precinct_log2", "sop", "eph", "tile_width",
                                          "tile_height")])
+
+
+class HtCodeBlockBatchJob(ctypes.Structure):
+    _fields_ = [("output_x", ctypes.c_uint32), ("output_y", ctypes.c_uint32),
+                ("code_block", HtCodeBlockJob)]
 
 
 # name -> (restype, argtypes)
@@ -115,6 +122,8 @@ SIGNATURES = {
                                          c_size_t]),
     "sgn_device_download_2d": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, c_size_t,
                                               ctypes.c_void_p, c_size_t, c_size_t, c_size_t]),
+    "sgn_host_alloc": (ctypes.c_int, [c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
+    "sgn_host_free": (None, [ctypes.c_void_p]),
     "sgn_hip_event_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p)]),
     "sgn_hip_event_destroy": (None, [ctypes.c_void_p]),
     "sgn_hip_event_record": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
@@ -199,6 +208,13 @@ SIGNATURES = {
                                                      ctypes.c_void_p, c_size_t]),
     "sgn_idwt_level": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_uint32] * 4 +
                        [ctypes.c_void_p] * 5),
+    "sgn_ht_decode_sub_band": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, c_size_t,
+                                              ctypes.c_uint32, ctypes.c_uint32,
+                                              ctypes.POINTER(HtCodeBlockBatchJob), c_size_t,
+                                              ctypes.c_void_p, c_size_t]),
+    "sgn_store_component": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, c_size_t] +
+                            [ctypes.c_uint32] * 5 + [ctypes.c_void_p, c_size_t] +
+                            [ctypes.c_uint32] * 3 + [ctypes.c_float]),
     "sgn_inverse_mct": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                        ctypes.c_void_p, ctypes.c_void_p, c_size_t,
                                        ctypes.c_float, ctypes.c_float, ctypes.c_float]),

@@ -1157,4 +1157,17 @@ extern "C" __global__ void sgn_inverse_mct_planes(float *p0, float *p1, float *p
     p2[i] = b + a2;
 }
 
+// Component store (J2kStoreComponentJob hook, lib.rs:805-828; the copy of
+// decode.rs:1792-1810): out[(oy + y) * ow + ox + x] =
+// in[(sy + y) * iw + sx + x] + addend over a copy_w x copy_h rectangle.
+// One thread per sample, rows along x so loads and stores coalesce.
+extern "C" __global__ void sgn_store_plane(const float *in, uint32_t iw, uint32_t sx, uint32_t sy,
+                                           uint32_t copy_w, uint32_t copy_h, float *out,
+                                           uint32_t ow, uint32_t ox, uint32_t oy, float addend) {
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= copy_w) return;
+    for (uint32_t y = blockIdx.y; y < copy_h; y += gridDim.y)
+        out[size_t(oy + y) * ow + ox + x] = in[size_t(sy + y) * iw + sx + x] + addend;
+}
+
 }  // namespace sgn

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <type_traits>
 #include <atomic>
+#include <chrono>
 #include <condition_variable>
 #include <functional>
 #include <cmath>
@@ -21,7 +22,6 @@
 #include <memory>
 #include <unordered_map>
 #include <new>
-#include <sys/mman.h>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -44,6 +44,19 @@ using namespace sgn;
         }                                                                          \
     } while (0)
 
+class DeviceBlocks;
+
+// Pinned host memory (hipHostMalloc) that plan uploads stage their tables
+// and non-pinned codestream bytes in, kept by the session and grown on
+// demand.  `done` is recorded after the last copy out of it was enqueued;
+// a new fill waits for it.
+struct PinnedSlot {
+    uint8_t *p = nullptr;
+    size_t cap = 0;
+    hipEvent_t done = nullptr;
+    bool pending = false;
+};
+
 struct sgn_hip_session {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -53,6 +66,12 @@ struct sgn_hip_session {
     // per lane-kernel variant and sample type (2 v + narrow): resident
     // work-groups (0: unknown)
     std::vector<uint32_t> resident;
+    // pipelined batch decode: host-to-device copies run on copy_stream
+    // while the previous chunk decodes on `stream`; two staging slots
+    hipStream_t copy_stream = nullptr;
+    hipStream_t aux_stream = nullptr;  // second decode stream: chunks alternate
+    PinnedSlot slot[2];
+    DeviceBlocks *blocks = nullptr;  // device blocks of pipelined chunk plans, reused
 };
 
 namespace {
@@ -219,6 +238,19 @@ struct sgn_j2k_plan {
     std::vector<size_t> tile_row_bytes;
     uint32_t depth_max = 0;
     uint64_t coef_elems = 0, arena_len = 0, out_bytes = 0, out_pixels = 0;
+    uint64_t coded_bytes = 0;  // sum of the codestream lengths
+    // Codestream bytes reach the device arena as runs: tiles whose bytes
+    // follow each other in host memory (gaps under 4 KB) form one run and
+    // keep their relative offsets, so a batch laid out in one pinned buffer
+    // goes up in one DMA straight from the caller's memory.
+    struct CopyRun {
+        const uint8_t *src;
+        uint64_t arena_off, len;
+    };
+    std::vector<CopyRun> runs;
+    // host time of the last build (ms): per-tile parse, merge of the tile
+    // plans into the batch tables, staging fill and copy submission
+    double t_parse = 0, t_merge = 0, t_stage = 0, t_copy = 0;
     // Code-block samples stored as int16 (every band has at most 15
     // magnitude bit-planes, so every sample fits); intermediate level
     // outputs stay int32.  Each band keeps its int32-sized slot.
@@ -348,32 +380,8 @@ void sort_by_cost(const std::vector<CbJob> &in, std::vector<CbJob> &out) {
     for (const CbJob &j : in) out[at[top - j.cost]++] = j;
 }
 
-// Host staging memory for a plan upload: an anonymous mapping advised to
-// use transparent huge pages, so that filling a few hundred MB takes a few
-// hundred page faults rather than one per 4 KB.
-class HostStaging {
-  public:
-    explicit HostStaging(size_t bytes) {
-        constexpr size_t kHuge = size_t(2) << 20;
-        len_ = std::max<size_t>(1, bytes) + kHuge;
-        void *m = mmap(nullptr, len_, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS, -1, 0);
-        if (m == MAP_FAILED) throw std::bad_alloc();
-        base_ = static_cast<uint8_t *>(m);
-        p_ = reinterpret_cast<uint8_t *>((reinterpret_cast<uintptr_t>(base_) + kHuge - 1) &
-                                         ~uintptr_t(kHuge - 1));
-        (void)madvise(p_, std::max<size_t>(1, bytes), MADV_HUGEPAGE);
-    }
-    ~HostStaging() { munmap(base_, len_); }
-    HostStaging(const HostStaging &) = delete;
-    HostStaging &operator=(const HostStaging &) = delete;
-    uint8_t *get() const { return p_; }
-
-  private:
-    uint8_t *base_ = nullptr, *p_ = nullptr;
-    size_t len_ = 0;
-};
-
 Status build_plan(const sgn_tile_req *jobs, size_t n, sgn_j2k_plan &P, int32_t *tile_status) {
+    const auto t0 = std::chrono::steady_clock::now();
     P.ntiles = n;
     std::vector<LocalPlan> lps(n);
     std::vector<int32_t> st(n, 0);
@@ -397,15 +405,40 @@ Status build_plan(const sgn_tile_req *jobs, size_t n, sgn_j2k_plan &P, int32_t *
         set_last_error("tile " + std::to_string(bad_index) + ": " + errs[bad_index]);
         return static_cast<Status>(first_bad);
     }
+    const auto t1 = std::chrono::steady_clock::now();
+    P.t_parse = std::chrono::duration<double, std::milli>(t1 - t0).count();
     // ---- merge with rebasing ----
     uint64_t arena = 0, coef = 0;
     std::vector<uint64_t> arena_base(n), coef_base(n);
+    P.runs.clear();
+    P.coded_bytes = 0;
     for (size_t i = 0; i < n; ++i) {
-        arena_base[i] = arena;
+        const uint8_t *b = jobs[i].bytes;
+        const uint64_t len = jobs[i].len;
+        P.coded_bytes += len;
+        bool joined = false;
+        if (!P.runs.empty()) {
+            sgn_j2k_plan::CopyRun &r = P.runs.back();
+            const uint8_t *end = r.src + r.len;
+            if (b >= end && uint64_t(b - end) < 4096) {
+                arena_base[i] = r.arena_off + uint64_t(b - r.src);
+                r.len = uint64_t(b - r.src) + len;
+                arena = r.arena_off + r.len;
+                joined = true;
+            }
+        }
+        if (!joined) {
+            arena = (arena + 15) & ~uint64_t(15);
+            arena_base[i] = arena;
+            P.runs.push_back({b, arena, len});
+            arena += len;
+        }
         coef_base[i] = coef;
-        arena += (jobs[i].len + 15) & ~size_t(15);
         coef += lps[i].coef_elems;
     }
+    // the HT kernels read whole aligned 16-byte units: pad the arena so the
+    // unit holding its last byte lies inside the allocation
+    arena = (arena + 31) & ~uint64_t(15);
     if (arena >= (1ull << 32) || coef >= (1ull << 32)) {
         set_last_error("batch too large for 32-bit plan offsets; split it");
         return kErrArg;
@@ -522,13 +555,55 @@ Status build_plan(const sgn_tile_req *jobs, size_t n, sgn_j2k_plan &P, int32_t *
             P.fin_block_job[rev].insert(P.fin_block_job[rev].end(),
                                         size_t(P.fin[rev][j].nbx) * P.fin[rev][j].nby, uint32_t(j));
     }
+    P.t_merge = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1)
+                    .count();
+    return kOk;
+}
+
+// True when p lies in page-locked host memory (hipHostMalloc or
+// registered), which the DMA engines read directly.  A query on pageable
+// memory fails; its error is cleared so later launch checks do not see it.
+bool host_pinned(const void *p) {
+    hipPointerAttribute_t a{};
+    const hipError_t e = hipPointerGetAttributes(&a, p);
+    (void)hipGetLastError();
+    return e == hipSuccess && a.type == hipMemoryTypeHost;
+}
+
+// Reserves `bytes` of the session's pinned staging slot k, after the copies
+// out of its previous contents have finished.
+Status slot_reserve(sgn_hip_session *s, int k, size_t bytes, uint8_t **out) {
+    PinnedSlot &sl = s->slot[k];
+    if (sl.pending) {
+        HIP_TRY(hipEventSynchronize(sl.done));
+        sl.pending = false;
+    }
+    if (!sl.done) HIP_TRY(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+    if (sl.cap < bytes) {
+        if (sl.p) HIP_TRY(hipHostFree(sl.p));
+        sl.p = nullptr;
+        sl.cap = 0;
+        const size_t cap = (std::max<size_t>(bytes, size_t(16) << 20) + (size_t(2) << 20) - 1) &
+                           ~((size_t(2) << 20) - 1);
+        HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&sl.p), cap, hipHostMallocDefault));
+        sl.cap = cap;
+    }
+    *out = sl.p;
     return kOk;
 }
 
 // One device allocation per plan: codestream arena, job tables, status
-// words, work counter and coefficient arena, carved at 256-byte alignment;
-// everything but the last three goes up in one copy.
-Status upload_plan(const sgn_tile_req *jobs, sgn_j2k_plan &P) {
+// words, work counter and coefficient arena, carved at 256-byte alignment.
+// Codestream runs of at least 64 KB in pinned host memory are copied
+// straight from the caller's memory; the other runs and the tables are
+// staged in the session's pinned slot `slot` and go up in as few copies as
+// the run layout allows.  Copies are enqueued on `copy`; the caller orders
+// the decode after them (same stream or an event).  With sync the call
+// returns after the copies completed.
+Status upload_plan(sgn_hip_session *s, const sgn_tile_req *jobs, sgn_j2k_plan &P,
+                   hipStream_t copy, int slot, bool sync) {
+    (void)jobs;
+    const auto t0 = std::chrono::steady_clock::now();
     uint64_t off = 0;
     auto take = [&](uint64_t bytes) {
         const uint64_t o = off;
@@ -537,6 +612,7 @@ Status upload_plan(const sgn_tile_req *jobs, sgn_j2k_plan &P) {
     };
     auto bytes_of = [](const auto &v) { return uint64_t(v.size()) * sizeof(v[0]); };
     const uint64_t o_arena = take(P.arena_len);
+    const uint64_t o_tables = off;
     const uint64_t o_lane = take(bytes_of(P.cb_lane)), o_plain = take(bytes_of(P.cb_lane_plain));
     const uint64_t o_wave = take(bytes_of(P.cb_wave));
     const uint64_t o_lv0 = take(bytes_of(P.lv[0])), o_lv1 = take(bytes_of(P.lv[1]));
@@ -554,20 +630,33 @@ Status upload_plan(const sgn_tile_req *jobs, sgn_j2k_plan &P) {
         set_last_error("hipMalloc failed for the plan (" + std::to_string(off) + " bytes)");
         return kErrDevice;
     }
-    HostStaging staging(host_bytes);
-    uint8_t *h = staging.get();
-    // codestreams at their 16-byte aligned arena offsets, copied by the
-    // worker pool (a fresh staging buffer also takes its first-touch page
-    // faults there, in parallel)
-    std::vector<uint64_t> at_tile(P.ntiles);
-    for (uint64_t i = 0, a = o_arena; i < P.ntiles; ++i) {
-        at_tile[i] = a;
-        a += (jobs[i].len + 15) & ~uint64_t(15);
+    // which runs go up straight from the caller's memory
+    std::vector<uint8_t> direct(P.runs.size(), 0);
+    for (size_t r = 0; r < P.runs.size(); ++r)
+        direct[r] = P.runs[r].len >= (uint64_t(64) << 10) && host_pinned(P.runs[r].src);
+    // the staging slot mirrors [0, host_bytes) of the device block; arena
+    // bytes of direct runs are left unfilled there and never copied
+    uint8_t *h = nullptr;
+    {
+        const Status st = slot_reserve(s, slot, host_bytes, &h);
+        if (st) return st;
     }
-    WorkerPool::get().run(P.ntiles, [&](size_t i) {
-        std::memcpy(h + at_tile[i], jobs[i].bytes, jobs[i].len);
-        const uint64_t padded = (jobs[i].len + 15) & ~uint64_t(15);
-        std::memset(h + at_tile[i] + jobs[i].len, 0, padded - jobs[i].len);
+    // staged codestream bytes, copied by the worker pool in 1 MB pieces
+    struct Piece {
+        const uint8_t *src;
+        uint64_t off, len;
+    };
+    std::vector<Piece> pieces;
+    constexpr uint64_t kPiece = uint64_t(1) << 20;
+    for (size_t r = 0; r < P.runs.size(); ++r) {
+        if (direct[r]) continue;
+        const sgn_j2k_plan::CopyRun &run = P.runs[r];
+        for (uint64_t q = 0; q < run.len; q += kPiece)
+            pieces.push_back({run.src + q, o_arena + run.arena_off + q,
+                              std::min(kPiece, run.len - q)});
+    }
+    WorkerPool::get().run(pieces.size(), [&](size_t i) {
+        std::memcpy(h + pieces[i].off, pieces[i].src, pieces[i].len);
     });
     auto put = [&](uint64_t o, const auto &v) {
         if (!v.empty()) std::memcpy(h + o, v.data(), bytes_of(v));
@@ -584,8 +673,45 @@ Status upload_plan(const sgn_tile_req *jobs, sgn_j2k_plan &P) {
     put(o_lbj1, P.lv_block_job[1]);
     put(o_fbj0, P.fin_block_job[0]);
     put(o_fbj1, P.fin_block_job[1]);
-    HIP_TRY(hipMemcpy(P.d_block, h, host_bytes, hipMemcpyHostToDevice));
+    const auto t1 = std::chrono::steady_clock::now();
     uint8_t *d = static_cast<uint8_t *>(P.d_block);
+    // direct runs first, then the staged range: consecutive staged runs and
+    // the tables in one copy each
+    for (size_t r = 0; r < P.runs.size(); ++r)
+        if (direct[r])
+            HIP_TRY(hipMemcpyAsync(d + o_arena + P.runs[r].arena_off, P.runs[r].src,
+                                   P.runs[r].len, hipMemcpyHostToDevice, copy));
+    {
+        uint64_t a = UINT64_MAX, e = 0;
+        auto flush = [&]() -> hipError_t {
+            if (a == UINT64_MAX) return hipSuccess;
+            const hipError_t err =
+                hipMemcpyAsync(d + a, h + a, e - a, hipMemcpyHostToDevice, copy);
+            a = UINT64_MAX;
+            return err;
+        };
+        for (size_t r = 0; r < P.runs.size(); ++r) {
+            if (direct[r]) {
+                HIP_TRY(flush());
+                continue;
+            }
+            if (a == UINT64_MAX) a = o_arena + P.runs[r].arena_off;
+            e = o_arena + P.runs[r].arena_off + P.runs[r].len;
+        }
+        HIP_TRY(flush());
+        if (host_bytes > o_tables)
+            HIP_TRY(hipMemcpyAsync(d + o_tables, h + o_tables, host_bytes - o_tables,
+                                   hipMemcpyHostToDevice, copy));
+    }
+    HIP_TRY(hipEventRecord(s->slot[slot].done, copy));
+    s->slot[slot].pending = true;
+    if (sync) {
+        HIP_TRY(hipEventSynchronize(s->slot[slot].done));
+        s->slot[slot].pending = false;
+    }
+    const auto t2 = std::chrono::steady_clock::now();
+    P.t_stage = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    P.t_copy = std::chrono::duration<double, std::milli>(t2 - t1).count();
     auto at = [&](uint64_t o, auto *&ptr, bool present) {
         ptr = present ? reinterpret_cast<std::remove_reference_t<decltype(ptr)>>(d + o) : nullptr;
     };
@@ -833,9 +959,102 @@ Status make_plan(sgn_hip_session *s, const sgn_tile_req *jobs, size_t n,
     P->blocks = blocks;
     Status st = build_plan(jobs, n, *P, tile_status);
     if (st) return st;
-    st = upload_plan(jobs, *P);
+    st = upload_plan(s, jobs, *P, s->stream, 0, true);
     if (st) return st;
     out = std::move(P);
+    return kOk;
+}
+
+// Launch helpers enqueue on s->stream; this points it at another stream of
+// the session for one scope.
+class OnStream {
+  public:
+    OnStream(sgn_hip_session *s, hipStream_t st) : s_(s), keep_(s->stream) { s->stream = st; }
+    ~OnStream() { s_->stream = keep_; }
+
+  private:
+    sgn_hip_session *s_;
+    hipStream_t keep_;
+};
+
+// Tiles per chunk of a pipelined batch decode (SGN_PIPELINE_TILES
+// overrides; 0 or a value >= n decodes the batch as one plan).
+size_t pipeline_tiles() {
+    const char *e = std::getenv("SGN_PIPELINE_TILES");
+    if (e && *e) return size_t(std::strtoull(e, nullptr, 10));
+    return 1024;
+}
+
+// Batch decode from host codestreams, pipelined in chunks: while chunk k's
+// codestreams and tables travel on the copy stream and chunk k-1 decodes
+// (chunks alternate between two decode streams, so neighbouring chunks
+// also overlap on the device), the host parses chunk k+1.  Each chunk is
+// its own plan in a device block of the session's cache.  Synchronous;
+// reports the first failing tile index like TileBatchError.
+Status decode_pipelined(sgn_hip_session *s, const sgn_tile_req *reqs, size_t n, size_t chunk,
+                        int32_t *tile_status, size_t *first_fail) {
+    HIP_TRY(hipSetDevice(s->device));
+    std::vector<std::unique_ptr<sgn_j2k_plan>> plans;
+    std::vector<size_t> first;
+    Status parse_st = kOk;
+    size_t parse_bad = 0;
+    std::string parse_msg;
+    hipEvent_t join = nullptr;
+    HIP_TRY(hipEventCreateWithFlags(&join, hipEventDisableTiming));
+    std::unique_ptr<std::remove_pointer_t<hipEvent_t>, hipError_t (*)(hipEvent_t)> join_guard(
+        join, hipEventDestroy);
+    for (size_t a = 0, c = 0; a < n; a += chunk, ++c) {
+        const size_t m = std::min(chunk, n - a);
+        auto P = std::make_unique<sgn_j2k_plan>();
+        P->blocks = s->blocks;
+        Status st = build_plan(reqs + a, m, *P, tile_status ? tile_status + a : nullptr);
+        if (st) {
+            parse_st = st;
+            parse_msg = last_error();
+            for (size_t i = 0; i < m; ++i)
+                if (!tile_status || tile_status[a + i]) {
+                    parse_bad = a + i;
+                    break;
+                }
+            break;
+        }
+        const int k = int(c & 1);
+        hipStream_t cs = k ? s->aux_stream : s->stream;
+        st = upload_plan(s, reqs + a, *P, s->copy_stream, k, false);
+        if (!st && hipStreamWaitEvent(cs, s->slot[k].done, 0) != hipSuccess) {
+            set_last_error("hipStreamWaitEvent failed");
+            st = kErrDevice;
+        }
+        if (!st) {
+            OnStream on(s, cs);
+            st = decode_plan(s, *P, SGN_MODE_BATCHED);
+        }
+        if (st) {
+            // let the chunks already enqueued finish before their plans go
+            for (hipStream_t q : {s->copy_stream, s->stream, s->aux_stream})
+                (void)hipStreamSynchronize(q);
+            return st;
+        }
+        plans.push_back(std::move(P));
+        first.push_back(a);
+    }
+    HIP_TRY(hipEventRecord(join, s->aux_stream));
+    HIP_TRY(hipStreamWaitEvent(s->stream, join, 0));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    for (size_t c = 0; c < plans.size(); ++c) {
+        size_t bad = 0;
+        const Status st = check_status(*plans[c], SGN_MODE_BATCHED, &bad);
+        if (st) {
+            if (first_fail) *first_fail = first[c] + bad;
+            set_last_error("code-block decode failure in tile " + std::to_string(first[c] + bad));
+            return st;
+        }
+    }
+    if (parse_st) {
+        if (first_fail) *first_fail = parse_bad;
+        set_last_error(parse_msg);
+        return parse_st;
+    }
     return kOk;
 }
 
@@ -881,6 +1100,9 @@ int sgn_hip_session_create(int device, sgn_hip_session **out) {
     s->device = device;
     HIP_TRY(hipSetDevice(device));
     HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    HIP_TRY(hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking));
+    HIP_TRY(hipStreamCreateWithFlags(&s->aux_stream, hipStreamNonBlocking));
+    s->blocks = new DeviceBlocks(uint64_t(16) << 30);
     const HtTables &T = ht_tables();
     std::vector<uint16_t> tab(2624);
     std::memcpy(tab.data(), T.vlc0, 2048);
@@ -897,10 +1119,16 @@ int sgn_hip_session_create(int device, sgn_hip_session **out) {
 void sgn_hip_session_destroy(sgn_hip_session *s) {
     if (!s) return;
     (void)hipSetDevice(s->device);
-    if (s->stream) {
-        (void)hipStreamSynchronize(s->stream);
-        (void)hipStreamDestroy(s->stream);
+    for (hipStream_t st : {s->stream, s->copy_stream, s->aux_stream})
+        if (st) {
+            (void)hipStreamSynchronize(st);
+            (void)hipStreamDestroy(st);
+        }
+    for (PinnedSlot &sl : s->slot) {
+        if (sl.p) (void)hipHostFree(sl.p);
+        if (sl.done) (void)hipEventDestroy(sl.done);
     }
+    delete s->blocks;
     if (s->d_tables) (void)hipFree(s->d_tables);
     delete s;
 }
@@ -983,6 +1211,17 @@ int sgn_device_download_2d(sgn_hip_session *s, void *host, size_t host_stride, c
                              hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     return SGN_OK;
+}
+
+int sgn_host_alloc(size_t bytes, void **out) {
+    if (!out) return SGN_E_ARG;
+    *out = nullptr;
+    HIP_TRY(hipHostMalloc(out, bytes ? bytes : 1, hipHostMallocDefault));
+    return SGN_OK;
+}
+
+void sgn_host_free(void *p) {
+    if (p) (void)hipHostFree(p);
 }
 
 int sgn_hip_event_create(void **ev) {
@@ -1167,12 +1406,16 @@ int sgn_j2k_plan_get_stats(const sgn_j2k_plan *P, sgn_j2k_plan_stats *st) {
     if (!P || !st) return SGN_E_ARG;
     st->tiles = P->ntiles;
     st->code_blocks = P->cb_lane.size() + P->cb_wave.size();
-    st->coded_bytes = P->arena_len;
+    st->coded_bytes = P->coded_bytes;
     st->output_bytes = P->out_bytes;
     st->output_pixels = P->out_pixels;
     st->coefficient_bytes = P->coef_elems * 4;
     st->launches_batched = P->launches_batched;
     st->narrow_samples = P->narrow ? 1u : 0u;
+    st->host_parse_ms = float(P->t_parse);
+    st->host_merge_ms = float(P->t_merge);
+    st->host_stage_ms = float(P->t_stage);
+    st->host_copy_ms = float(P->t_copy);
     return SGN_OK;
 }
 
@@ -1188,10 +1431,13 @@ int sgn_j2k_plan_tile_dims(const sgn_j2k_plan *P, size_t i, uint32_t *w, uint32_
 int sgn_j2k_decode_tiles_device(sgn_hip_session *s, const sgn_tile_job *jobs, size_t n, int fmt,
                                 uint32_t den, int mode, size_t *first_fail) {
     API_GUARD_BEGIN
-    if (!jobs && n) return SGN_E_ARG;
+    if (!s || (!jobs && n)) return SGN_E_ARG;
     std::vector<int32_t> ts(n, 0);
     std::unique_ptr<sgn_j2k_plan> P;
     const std::vector<sgn_tile_req> reqs = as_requests(jobs, n, fmt, den);
+    const size_t chunk = pipeline_tiles();
+    if (mode == SGN_MODE_BATCHED && chunk && n > chunk)
+        return decode_pipelined(s, reqs.data(), n, chunk, ts.data(), first_fail);
     Status st = make_plan(s, reqs.data(), n, P, ts.data());
     if (st) {
         if (first_fail)
@@ -1441,6 +1687,101 @@ int sgn_idwt_level(sgn_hip_session *s, int reversible, uint32_t x0, uint32_t y0,
             out[i] = static_cast<float>(static_cast<int32_t>(res[i]));
         else
             std::memcpy(&out[i], &res[i], 4);
+    }
+    return SGN_OK;
+    API_GUARD_END
+}
+
+int sgn_ht_decode_sub_band(sgn_hip_session *s, const uint8_t *data, size_t data_len,
+                           uint32_t width, uint32_t height, const sgn_ht_code_block_batch_job *jobs,
+                           size_t n, float *out, size_t out_len) {
+    API_GUARD_BEGIN
+    if (!s || (!jobs && n) || (!out && out_len)) return SGN_E_ARG;
+    const uint64_t band = uint64_t(width) * height;
+    if (out_len < band) {
+        set_last_error("sub-band output smaller than width * height");
+        return SGN_E_BUFFER;
+    }
+    // each block at (output_x, output_y) of the band, rows output_stride
+    // apart (the band width, decode.rs:1622); blocks must lie in the band
+    std::vector<sgn_ht_code_block_job> cbs(n);
+    for (size_t i = 0; i < n; ++i) {
+        const sgn_ht_code_block_batch_job &b = jobs[i];
+        sgn_ht_code_block_job j = b.code_block;
+        if (uint64_t(b.output_x) + j.width > width || uint64_t(b.output_y) + j.height > height ||
+            j.output_stride < j.width) {
+            set_last_error("code-block " + std::to_string(i) + " outside its sub-band");
+            return SGN_E_ARG;
+        }
+        const uint64_t at = uint64_t(b.output_y) * j.output_stride + b.output_x;
+        if (j.height && at + uint64_t(j.output_stride) * (j.height - 1) + j.width > out_len) {
+            set_last_error("code-block " + std::to_string(i) + " outside the output");
+            return SGN_E_BUFFER;
+        }
+        j.output_offset = static_cast<uint32_t>(at);
+        cbs[i] = j;
+    }
+    std::vector<float> tmp(out_len, 0.0f);
+    const int rc = sgn_ht_decode_code_blocks(s, data, data_len, cbs.data(), n, tmp.data(), out_len);
+    if (rc) return rc;
+    // samples of the blocks replace the caller's; the rest is left alone
+    for (const sgn_ht_code_block_job &j : cbs)
+        for (uint32_t y = 0; y < j.height; ++y)
+            std::memcpy(out + j.output_offset + size_t(y) * j.output_stride,
+                        tmp.data() + j.output_offset + size_t(y) * j.output_stride,
+                        size_t(j.width) * sizeof(float));
+    return SGN_OK;
+    API_GUARD_END
+}
+
+int sgn_store_component(sgn_hip_session *s, const float *input, size_t input_len,
+                        uint32_t input_width, uint32_t source_x, uint32_t source_y,
+                        uint32_t copy_width, uint32_t copy_height, float *output,
+                        size_t output_len, uint32_t output_width, uint32_t output_x,
+                        uint32_t output_y, float addend) {
+    API_GUARD_BEGIN
+    if (!s || (!input && input_len) || (!output && output_len)) return SGN_E_ARG;
+    if (!copy_width || !copy_height) return SGN_OK;
+    const uint64_t in_end = (uint64_t(source_y) + copy_height - 1) * input_width + source_x +
+                            copy_width;
+    const uint64_t out_end = (uint64_t(output_y) + copy_height - 1) * output_width + output_x +
+                             copy_width;
+    if (uint64_t(source_x) + copy_width > input_width || in_end > input_len) {
+        set_last_error("store source rectangle outside the input");
+        return SGN_E_ARG;
+    }
+    if (uint64_t(output_x) + copy_width > output_width || out_end > output_len) {
+        set_last_error("store destination rectangle outside the output plane");
+        return SGN_E_BUFFER;
+    }
+    HIP_TRY(hipSetDevice(s->device));
+    float *d_in = nullptr, *d_out = nullptr;
+    auto cleanup = [&]() {
+        if (d_in) (void)hipFree(d_in);
+        if (d_out) (void)hipFree(d_out);
+    };
+    if (hipMalloc(&d_in, input_len * 4) != hipSuccess ||
+        hipMalloc(&d_out, output_len * 4) != hipSuccess) {
+        cleanup();
+        set_last_error("hipMalloc failed");
+        return SGN_E_DEVICE;
+    }
+    hipError_t e = hipMemcpy(d_in, input, input_len * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_out, output, output_len * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        const dim3 grid((copy_width + 255) / 256, std::min<uint32_t>(copy_height, 65535));
+        hipLaunchKernelGGL(sgn_store_plane, grid, dim3(256), 0, s->stream, d_in, input_width,
+                           source_x, source_y, copy_width, copy_height, d_out, output_width,
+                           output_x, output_y, addend);
+        s->dispatches++;
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e == hipSuccess) e = hipMemcpy(output, d_out, output_len * 4, hipMemcpyDeviceToHost);
+    cleanup();
+    if (e != hipSuccess) {
+        set_last_error(std::string("HIP: ") + hipGetErrorString(e) + " in the store hook");
+        return SGN_E_DEVICE;
     }
     return SGN_OK;
     API_GUARD_END

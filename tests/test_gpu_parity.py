@@ -434,3 +434,67 @@ def test_inverse_mct_hook(session, oracle, rev):
     exp = O.inverse_mct(rev, *p, (128, 128, 128))
     for a, b in zip(got, exp):
         assert np.array_equal(a, b)
+
+
+def test_sub_band_hook_matches_oracle(session, oracle):
+    """decode_sub_band: every HT sub-band of a 9/7 codestream (32x32 blocks)
+    decoded as one batch per band equals the oracle's blocks, each
+    coefficient_to_i32 * step, placed at the block's offset in the band."""
+    hook = S.HtCodeBlockDecoder(session)
+    rng = np.random.default_rng(33)
+    cs = S.encode_j2k_ht(synth.texture_rgb(rng, 150, 110), reversible=0, base_step=1 / 256,
+                         cb_width_log2=5, cb_height_log2=5, levels=3)
+    jobs = S.code_block_jobs(cs)
+    ocb = O.code_blocks(cs)
+    assert len(jobs) == len(ocb)
+    bands = {}
+    for j, b in zip(jobs, ocb):
+        # both list blocks in (component, resolution, precinct, band) order
+        assert (j.width, j.height, j.number_of_coding_passes) == (b.x1 - b.x0, b.y1 - b.y0,
+                                                                    b.passes)
+        bands.setdefault((b.tile, b.comp, b.res, b.band), []).append((j, b))
+    assert len(bands) > 4
+    for key, members in bands.items():
+        bx0 = min(b.x0 for _, b in members)
+        by0 = min(b.y0 for _, b in members)
+        W = max(b.x1 for _, b in members) - bx0
+        H = max(b.y1 for _, b in members) - by0
+        batch = []
+        exp = np.zeros((H, W), np.float32)
+        for j, b in members:
+            j.output_stride = W
+            batch.append((b.x0 - bx0, b.y0 - by0, j))
+            if j.number_of_coding_passes == 0:
+                continue
+            data = cs[j.data_offset:j.data_offset + j.cleanup_length + j.refinement_length]
+            rc, raw = O.ht_decode_block(data, j.cleanup_length, j.refinement_length, j.width,
+                                        j.height, j.missing_bit_planes, j.number_of_coding_passes)
+            assert rc == 0
+            ints = np.vectorize(lambda v: O.coefficient_to_i32(int(v), j.num_bitplanes))(raw)
+            exp[b.y0 - by0:b.y1 - by0, b.x0 - bx0:b.x1 - bx0] = \
+                ints.astype(np.float32) * np.float32(j.dequantization_step)
+        got = hook.decode_sub_band(cs, W, H, batch)
+        assert np.array_equal(got, exp), key
+
+
+@pytest.mark.parametrize("case", [
+    # (input w, h, source x, y, copy w, h, output w, h, output x, y, addend)
+    (70, 45, 0, 0, 70, 45, 70, 45, 0, 0, 128.0),
+    (66, 40, 1, 3, 60, 30, 200, 90, 17, 55, 0.0),
+    (9, 9, 8, 8, 1, 1, 4, 4, 3, 3, 2048.0),
+])
+def test_store_component_hook(session, case):
+    """decode_store_component: the copy with the sign-shift addend of
+    decode.rs:1792-1810, in f32, leaving the rest of the plane alone."""
+    iw, ih, sx, sy, cw, ch, ow, oh, ox, oy, add = case
+    rng = np.random.default_rng(iw + ow)
+    inp = rng.normal(0, 300, (ih, iw)).astype(np.float32)
+    out = rng.normal(0, 300, (oh, ow)).astype(np.float32)
+    exp = out.copy()
+    exp[oy:oy + ch, ox:ox + cw] = inp[sy:sy + ch, sx:sx + cw] + np.float32(add)
+    got = S.HtCodeBlockDecoder(session).decode_store_component(
+        inp, iw, sx, sy, cw, ch, out, ow, ox, oy, add).reshape(oh, ow)
+    assert np.array_equal(got, exp)
+    with pytest.raises(S.BufferError):
+        S.HtCodeBlockDecoder(session).decode_store_component(
+            inp, iw, sx, sy, cw, ch, out, ow, ow - cw + 1, oy, add)

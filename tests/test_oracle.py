@@ -151,3 +151,43 @@ def test_batch_first_failing_index(oracle):
     streams[3] = bytes(bad)
     rc, idx, _outs = O.decode_batch(streams, 1, workers=2)
     assert rc != 0 and idx == 3
+
+
+def _gray8_closed_form(v, bit_depth):
+    """>8-bit sample to Gray8 as the reference's slow path computes it
+    (crates/signinum-j2k-native/src/lib.rs:2282-2293): f32 division by
+    2^bd - 1, times 255 in f32, rounded half away from zero, saturating."""
+    x = (v.astype(np.float32) / np.float32((1 << bit_depth) - 1)) * np.float32(255.0)
+    r = np.where(x >= 0, np.floor(x + np.float32(0.5)), np.ceil(x - np.float32(0.5)))
+    return np.clip(r, 0, 255).astype(np.uint8)
+
+
+@pytest.mark.parametrize("bit_depth", [10, 12, 16])
+def test_gray8_from_high_bit_depth_closed_form(oracle, bit_depth):
+    """Gray8 output of a >8-bit lossless codestream equals the closed form
+    applied to its Gray16 output, which itself equals the source samples."""
+    rng = np.random.default_rng(40 + bit_depth)
+    img = rng.integers(0, 1 << bit_depth, size=(45, 61), dtype=np.int64).astype(np.uint16)
+    img[0, :4] = [0, (1 << bit_depth) - 1, 1 << (bit_depth - 1), 8]
+    cs = S.encode_j2k_ht(img, bit_depth=bit_depth, use_mct=0, levels=3)
+    g16 = O.decode(cs, 3)[..., 0]
+    assert np.array_equal(g16, img)
+    assert np.array_equal(O.decode(cs, 0)[..., 0], _gray8_closed_form(img, bit_depth))
+
+
+def test_irreversible_multitile_odd_origins_close_to_openjpeg(oracle):
+    """9/7 + ICT over 100x70 tiles of a 300x200 image, 4 levels: the inner
+    tiles' coarse levels start at odd coordinates, where the first lifting
+    step and the K / 1/K scaling swap roles between even and odd samples.
+    OpenJPEG is independent of the oracle's geometry code."""
+    rng = np.random.default_rng(22)
+    img = synth.tissue_rgb(rng, 300, 200)
+    cs = S.encode_j2k_ht(img, tile_width=100, tile_height=70, levels=4, reversible=0,
+                         base_step=1.0 / 64)
+    out = O.decode(cs, 1).astype(np.int64)
+    diff = np.abs(out - _pillow(cs).astype(np.int64))
+    assert diff.max() <= 6 and diff.mean() < 1.5
+    p32 = O.decode_planes(cs)
+    p64 = O.decode_planes(cs, use_f64=True)
+    assert np.abs(p32 - p64).max() < 0.05
+    assert np.abs(out - img.astype(np.int64)).max() < 40
