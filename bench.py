@@ -429,8 +429,9 @@ def main():
                 raise SystemExit("oracle failed on the 9/7 sample")
             lsb, mx64, se64, cnt = 0, 0.0, 0.0, 0
             for j in range(k):
+                r32 = ref32[j].reshape(outs[j].shape)
                 lsb = max(lsb, int(np.abs(outs[j].astype(np.int32) -
-                                          ref32[j].astype(np.int32)).max()))
+                                          r32.astype(np.int32)).max()))
                 f64 = np.clip(O.decode_planes(lossy[j][0], use_f64=True), 0, 255)
                 d = outs[j].astype(np.float64) - f64.transpose(1, 2, 0)
                 mx64 = max(mx64, float(np.abs(d).max()))
