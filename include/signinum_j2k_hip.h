@@ -148,6 +148,25 @@ typedef struct {
 int sgn_j2k_decode_tiles_device(sgn_hip_session *session, const sgn_tile_job *jobs, size_t n,
                                 int fmt, uint32_t denominator, int mode, size_t *first_fail);
 
+/* The same batch decode (SGN_MODE_BATCHED, one plan, no pipelining) with no
+ * device or page-locked allocation: no counterpart in the reference; it is
+ * for callers that cannot allocate (code under HIP graph capture, serving
+ * loops and harnesses that forbid hipMalloc).  The plan (codestream bytes,
+ * job tables, status words, work counters and coefficients) lives in a
+ * file-scope device array of sgn_j2k_noalloc_scratch_bytes() bytes (64 MiB;
+ * e.g. one 1024x1024 RGB tile or about fifty 256x256 RGB tiles), which each
+ * GPU holds until the process ends; a batch that needs more is rejected
+ * with SGN_E_ARG before any device work.  Codestreams are copied straight
+ * from the caller's host memory.  Calls on one GPU share its scratch: this
+ * process serialises them with a lock, and every counter the kernels need
+ * at zero is cleared by a kernel at the start of the call.  Synchronous;
+ * first_fail as above. */
+int sgn_j2k_decode_tiles_device_noalloc(sgn_hip_session *session, const sgn_tile_job *jobs,
+                                        size_t n, int fmt, uint32_t denominator,
+                                        size_t *first_fail);
+/* Capacity of the no-allocation scratch array in bytes. */
+uint64_t sgn_j2k_noalloc_scratch_bytes(void);
+
 /* ---- plans: parse once, decode many times -------------------------------
  * The flattened device plan plays the role of J2kDirectColorPlan
  * (crates/signinum-j2k-native/src/direct_plan.rs:12-205): code-block table,

@@ -137,6 +137,11 @@ SIGNATURES = {
     "sgn_j2k_decode_tiles_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(TileJob),
                                                    c_size_t, ctypes.c_int, ctypes.c_uint32,
                                                    ctypes.c_int, ctypes.POINTER(c_size_t)]),
+    "sgn_j2k_decode_tiles_device_noalloc": (ctypes.c_int, [ctypes.c_void_p,
+                                                           ctypes.POINTER(TileJob), c_size_t,
+                                                           ctypes.c_int, ctypes.c_uint32,
+                                                           ctypes.POINTER(c_size_t)]),
+    "sgn_j2k_noalloc_scratch_bytes": (ctypes.c_uint64, []),
     "sgn_j2k_plan_create_mixed": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(TileReq), c_size_t,
                                                   ctypes.POINTER(ctypes.c_void_p),
                                                   ctypes.POINTER(ctypes.c_int32)]),

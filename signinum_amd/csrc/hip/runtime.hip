@@ -46,6 +46,16 @@ using namespace sgn;
 
 class DeviceBlocks;
 
+// Device memory of sgn_j2k_decode_tiles_device_noalloc: every plan that
+// entry point decodes (codestream arena, job tables, status words, work
+// counters, coefficient arena) is carved from this array, so the call
+// allocates nothing.  Each GPU that loads the library holds its own copy
+// until the process ends.  Host code must never pass the array's name to a
+// kernel or a copy (on the host it is only a stand-in): sessions resolve
+// each device's address with hipGetSymbolAddress (noalloc_scratch()).
+constexpr uint64_t kNoAllocScratchBytes = uint64_t(64) << 20;
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
 // Pinned host memory (hipHostMalloc) that plan uploads stage their tables
 // and non-pinned codestream bytes in, kept by the session and grown on
 // demand.  `done` is recorded after the last copy out of it was enqueued;
@@ -274,11 +284,12 @@ struct sgn_j2k_plan {
     void *d_block = nullptr;  // the one allocation every device pointer above lives in
     uint64_t device_bytes = 0;  // its size
     DeviceBlocks *blocks = nullptr;  // where d_block came from (else hipMalloc)
+    bool borrowed = false;           // d_block is the caller's memory: never freed here
 
     ~sgn_j2k_plan() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
-        if (d_block) {
+        if (d_block && !borrowed) {
             if (blocks)
                 blocks->put(d_block);
             else
@@ -600,8 +611,13 @@ Status slot_reserve(sgn_hip_session *s, int k, size_t bytes, uint8_t **out) {
 // the run layout allows.  Copies are enqueued on `copy`; the caller orders
 // the decode after them (same stream or an event).  With sync the call
 // returns after the copies completed.
+// With `fixed` (fixed_cap bytes of device memory the caller owns) nothing is
+// allocated: the plan is carved from it, every run and table is copied
+// straight from host memory (no pinned staging), and the call returns after
+// the copies completed.
 Status upload_plan(sgn_hip_session *s, const sgn_tile_req *jobs, sgn_j2k_plan &P,
-                   hipStream_t copy, int slot, bool sync) {
+                   hipStream_t copy, int slot, bool sync, void *fixed = nullptr,
+                   uint64_t fixed_cap = 0) {
     (void)jobs;
     const auto t0 = std::chrono::steady_clock::now();
     uint64_t off = 0;
@@ -625,93 +641,130 @@ Status upload_plan(sgn_hip_session *s, const sgn_tile_req *jobs, sgn_j2k_plan &P
     const uint64_t o_ff = take(P.fin_block_job[1].size() * sizeof(uint32_t));
     const uint64_t o_queue = take(16), o_coef = take(P.coef_elems * sizeof(uint32_t));
     P.device_bytes = off;
-    if ((P.blocks ? P.blocks->get(off, &P.d_block) : hipMalloc(&P.d_block, off)) != hipSuccess) {
+    if (fixed) {
+        if (off > fixed_cap) {
+            set_last_error("the batch needs " + std::to_string(off) +
+                           " bytes of device memory; the no-allocation scratch holds " +
+                           std::to_string(fixed_cap));
+            return kErrArg;
+        }
+        P.d_block = fixed;
+        P.borrowed = true;
+        uint8_t *d = static_cast<uint8_t *>(fixed);
+        for (const sgn_j2k_plan::CopyRun &run : P.runs)
+            HIP_TRY(hipMemcpyAsync(d + o_arena + run.arena_off, run.src, run.len,
+                                   hipMemcpyHostToDevice, copy));
+        auto put_dev = [&](uint64_t o, const auto &v) -> hipError_t {
+            if (v.empty()) return hipSuccess;
+            return hipMemcpyAsync(d + o, v.data(), bytes_of(v), hipMemcpyHostToDevice, copy);
+        };
+        HIP_TRY(put_dev(o_lane, P.cb_lane));
+        HIP_TRY(put_dev(o_plain, P.cb_lane_plain));
+        HIP_TRY(put_dev(o_wave, P.cb_wave));
+        HIP_TRY(put_dev(o_lv0, P.lv[0]));
+        HIP_TRY(put_dev(o_lv1, P.lv[1]));
+        HIP_TRY(put_dev(o_fin0, P.fin[0]));
+        HIP_TRY(put_dev(o_fin1, P.fin[1]));
+        HIP_TRY(put_dev(o_flv, P.final_lv));
+        HIP_TRY(put_dev(o_lbj0, P.lv_block_job[0]));
+        HIP_TRY(put_dev(o_lbj1, P.lv_block_job[1]));
+        HIP_TRY(put_dev(o_fbj0, P.fin_block_job[0]));
+        HIP_TRY(put_dev(o_fbj1, P.fin_block_job[1]));
+        HIP_TRY(hipStreamSynchronize(copy));
+        P.t_stage = 0;
+        P.t_copy = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() -
+                                                              t0).count();
+    } else if ((P.blocks ? P.blocks->get(off, &P.d_block) : hipMalloc(&P.d_block, off)) !=
+               hipSuccess) {
         P.d_block = nullptr;
         set_last_error("hipMalloc failed for the plan (" + std::to_string(off) + " bytes)");
         return kErrDevice;
     }
-    // which runs go up straight from the caller's memory
-    std::vector<uint8_t> direct(P.runs.size(), 0);
-    for (size_t r = 0; r < P.runs.size(); ++r)
-        direct[r] = P.runs[r].len >= (uint64_t(64) << 10) && host_pinned(P.runs[r].src);
-    // the staging slot mirrors [0, host_bytes) of the device block; arena
-    // bytes of direct runs are left unfilled there and never copied
-    uint8_t *h = nullptr;
-    {
-        const Status st = slot_reserve(s, slot, host_bytes, &h);
-        if (st) return st;
-    }
-    // staged codestream bytes, copied by the worker pool in 1 MB pieces
-    struct Piece {
-        const uint8_t *src;
-        uint64_t off, len;
-    };
-    std::vector<Piece> pieces;
-    constexpr uint64_t kPiece = uint64_t(1) << 20;
-    for (size_t r = 0; r < P.runs.size(); ++r) {
-        if (direct[r]) continue;
-        const sgn_j2k_plan::CopyRun &run = P.runs[r];
-        for (uint64_t q = 0; q < run.len; q += kPiece)
-            pieces.push_back({run.src + q, o_arena + run.arena_off + q,
-                              std::min(kPiece, run.len - q)});
-    }
-    WorkerPool::get().run(pieces.size(), [&](size_t i) {
-        std::memcpy(h + pieces[i].off, pieces[i].src, pieces[i].len);
-    });
-    auto put = [&](uint64_t o, const auto &v) {
-        if (!v.empty()) std::memcpy(h + o, v.data(), bytes_of(v));
-    };
-    put(o_lane, P.cb_lane);
-    put(o_plain, P.cb_lane_plain);
-    put(o_wave, P.cb_wave);
-    put(o_lv0, P.lv[0]);
-    put(o_lv1, P.lv[1]);
-    put(o_fin0, P.fin[0]);
-    put(o_fin1, P.fin[1]);
-    put(o_flv, P.final_lv);
-    put(o_lbj0, P.lv_block_job[0]);
-    put(o_lbj1, P.lv_block_job[1]);
-    put(o_fbj0, P.fin_block_job[0]);
-    put(o_fbj1, P.fin_block_job[1]);
-    const auto t1 = std::chrono::steady_clock::now();
-    uint8_t *d = static_cast<uint8_t *>(P.d_block);
-    // direct runs first, then the staged range: consecutive staged runs and
-    // the tables in one copy each
-    for (size_t r = 0; r < P.runs.size(); ++r)
-        if (direct[r])
-            HIP_TRY(hipMemcpyAsync(d + o_arena + P.runs[r].arena_off, P.runs[r].src,
-                                   P.runs[r].len, hipMemcpyHostToDevice, copy));
-    {
-        uint64_t a = UINT64_MAX, e = 0;
-        auto flush = [&]() -> hipError_t {
-            if (a == UINT64_MAX) return hipSuccess;
-            const hipError_t err =
-                hipMemcpyAsync(d + a, h + a, e - a, hipMemcpyHostToDevice, copy);
-            a = UINT64_MAX;
-            return err;
-        };
-        for (size_t r = 0; r < P.runs.size(); ++r) {
-            if (direct[r]) {
-                HIP_TRY(flush());
-                continue;
-            }
-            if (a == UINT64_MAX) a = o_arena + P.runs[r].arena_off;
-            e = o_arena + P.runs[r].arena_off + P.runs[r].len;
+    if (!fixed) {
+        // which runs go up straight from the caller's memory
+        std::vector<uint8_t> direct(P.runs.size(), 0);
+        for (size_t r = 0; r < P.runs.size(); ++r)
+            direct[r] = P.runs[r].len >= (uint64_t(64) << 10) && host_pinned(P.runs[r].src);
+        // the staging slot mirrors [0, host_bytes) of the device block; arena
+        // bytes of direct runs are left unfilled there and never copied
+        uint8_t *h = nullptr;
+        {
+            const Status st = slot_reserve(s, slot, host_bytes, &h);
+            if (st) return st;
         }
-        HIP_TRY(flush());
-        if (host_bytes > o_tables)
-            HIP_TRY(hipMemcpyAsync(d + o_tables, h + o_tables, host_bytes - o_tables,
-                                   hipMemcpyHostToDevice, copy));
+        // staged codestream bytes, copied by the worker pool in 1 MB pieces
+        struct Piece {
+            const uint8_t *src;
+            uint64_t off, len;
+        };
+        std::vector<Piece> pieces;
+        constexpr uint64_t kPiece = uint64_t(1) << 20;
+        for (size_t r = 0; r < P.runs.size(); ++r) {
+            if (direct[r]) continue;
+            const sgn_j2k_plan::CopyRun &run = P.runs[r];
+            for (uint64_t q = 0; q < run.len; q += kPiece)
+                pieces.push_back({run.src + q, o_arena + run.arena_off + q,
+                                  std::min(kPiece, run.len - q)});
+        }
+        WorkerPool::get().run(pieces.size(), [&](size_t i) {
+            std::memcpy(h + pieces[i].off, pieces[i].src, pieces[i].len);
+        });
+        auto put = [&](uint64_t o, const auto &v) {
+            if (!v.empty()) std::memcpy(h + o, v.data(), bytes_of(v));
+        };
+        put(o_lane, P.cb_lane);
+        put(o_plain, P.cb_lane_plain);
+        put(o_wave, P.cb_wave);
+        put(o_lv0, P.lv[0]);
+        put(o_lv1, P.lv[1]);
+        put(o_fin0, P.fin[0]);
+        put(o_fin1, P.fin[1]);
+        put(o_flv, P.final_lv);
+        put(o_lbj0, P.lv_block_job[0]);
+        put(o_lbj1, P.lv_block_job[1]);
+        put(o_fbj0, P.fin_block_job[0]);
+        put(o_fbj1, P.fin_block_job[1]);
+        const auto t1 = std::chrono::steady_clock::now();
+        uint8_t *d = static_cast<uint8_t *>(P.d_block);
+        // direct runs first, then the staged range: consecutive staged runs and
+        // the tables in one copy each
+        for (size_t r = 0; r < P.runs.size(); ++r)
+            if (direct[r])
+                HIP_TRY(hipMemcpyAsync(d + o_arena + P.runs[r].arena_off, P.runs[r].src,
+                                       P.runs[r].len, hipMemcpyHostToDevice, copy));
+        {
+            uint64_t a = UINT64_MAX, e = 0;
+            auto flush = [&]() -> hipError_t {
+                if (a == UINT64_MAX) return hipSuccess;
+                const hipError_t err =
+                    hipMemcpyAsync(d + a, h + a, e - a, hipMemcpyHostToDevice, copy);
+                a = UINT64_MAX;
+                return err;
+            };
+            for (size_t r = 0; r < P.runs.size(); ++r) {
+                if (direct[r]) {
+                    HIP_TRY(flush());
+                    continue;
+                }
+                if (a == UINT64_MAX) a = o_arena + P.runs[r].arena_off;
+                e = o_arena + P.runs[r].arena_off + P.runs[r].len;
+            }
+            HIP_TRY(flush());
+            if (host_bytes > o_tables)
+                HIP_TRY(hipMemcpyAsync(d + o_tables, h + o_tables, host_bytes - o_tables,
+                                       hipMemcpyHostToDevice, copy));
+        }
+        HIP_TRY(hipEventRecord(s->slot[slot].done, copy));
+        s->slot[slot].pending = true;
+        if (sync) {
+            HIP_TRY(hipEventSynchronize(s->slot[slot].done));
+            s->slot[slot].pending = false;
+        }
+        const auto t2 = std::chrono::steady_clock::now();
+        P.t_stage = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        P.t_copy = std::chrono::duration<double, std::milli>(t2 - t1).count();
     }
-    HIP_TRY(hipEventRecord(s->slot[slot].done, copy));
-    s->slot[slot].pending = true;
-    if (sync) {
-        HIP_TRY(hipEventSynchronize(s->slot[slot].done));
-        s->slot[slot].pending = false;
-    }
-    const auto t2 = std::chrono::steady_clock::now();
-    P.t_stage = std::chrono::duration<double, std::milli>(t1 - t0).count();
-    P.t_copy = std::chrono::duration<double, std::milli>(t2 - t1).count();
+    uint8_t *const d = static_cast<uint8_t *>(P.d_block);
     auto at = [&](uint64_t o, auto *&ptr, bool present) {
         ptr = present ? reinterpret_cast<std::remove_reference_t<decltype(ptr)>>(d + o) : nullptr;
     };
@@ -965,6 +1018,44 @@ Status make_plan(sgn_hip_session *s, const sgn_tile_req *jobs, size_t n,
     return kOk;
 }
 
+// Per-device address of `scratch`, resolved once per device with that
+// device current (sessions do it at creation), and a lock that serialises
+// the no-allocation calls of this process on that device: they share its
+// scratch.
+struct NoAllocSlot {
+    std::mutex lock;
+    void *ptr = nullptr;
+};
+
+constexpr int kMaxDevices = 64;
+
+NoAllocSlot &noalloc_slot(int device) {
+    static NoAllocSlot slots[kMaxDevices];
+    return slots[device];
+}
+
+Status noalloc_scratch(int device, void **out) {
+    static std::mutex resolve;
+    if (device < 0 || device >= kMaxDevices) {
+        set_last_error("device index beyond the no-allocation scratch table");
+        return kErrArg;
+    }
+    std::lock_guard<std::mutex> g(resolve);
+    NoAllocSlot &sl = noalloc_slot(device);
+    if (!sl.ptr) {
+        HIP_TRY(hipSetDevice(device));
+        void *p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) {
+            set_last_error("hipGetSymbolAddress returned no address for the scratch array");
+            return kErrDevice;
+        }
+        sl.ptr = p;
+    }
+    *out = sl.ptr;
+    return kOk;
+}
+
 // Launch helpers enqueue on s->stream; this points it at another stream of
 // the session for one scope.
 class OnStream {
@@ -1111,6 +1202,11 @@ int sgn_hip_session_create(int device, sgn_hip_session **out) {
     std::memcpy(tab.data() + 2368, T.uvlc1, 512);
     HIP_TRY(hipMalloc(&s->d_tables, tab.size() * 2));
     HIP_TRY(hipMemcpy(s->d_tables, tab.data(), tab.size() * 2, hipMemcpyHostToDevice));
+    {
+        void *scr = nullptr;
+        const Status st = noalloc_scratch(device, &scr);
+        if (st) return st;
+    }
     *out = s.release();
     return SGN_OK;
     API_GUARD_END
@@ -1454,6 +1550,45 @@ int sgn_j2k_decode_tiles_device(sgn_hip_session *s, const sgn_tile_job *jobs, si
     return check_status(*P, mode, first_fail);
     API_GUARD_END
 }
+
+int sgn_j2k_decode_tiles_device_noalloc(sgn_hip_session *s, const sgn_tile_job *jobs, size_t n,
+                                        int fmt, uint32_t den, size_t *first_fail) {
+    API_GUARD_BEGIN
+    if (!s || (!jobs && n)) return SGN_E_ARG;
+    void *scr = nullptr;
+    Status st = noalloc_scratch(s->device, &scr);
+    if (st) return st;
+    std::lock_guard<std::mutex> g(noalloc_slot(s->device).lock);
+    HIP_TRY(hipSetDevice(s->device));
+    const std::vector<sgn_tile_req> reqs = as_requests(jobs, n, fmt, den);
+    std::vector<int32_t> ts(n, 0);
+    sgn_j2k_plan P;
+    st = build_plan(reqs.data(), n, P, ts.data());
+    if (st) {
+        if (first_fail)
+            for (size_t i = 0; i < n; ++i)
+                if (ts[i]) {
+                    *first_fail = i;
+                    break;
+                }
+        return st;
+    }
+    st = upload_plan(s, reqs.data(), P, s->stream, 0, true, scr, kNoAllocScratchBytes);
+    if (st) {
+        (void)hipStreamSynchronize(s->stream);
+        return st;
+    }
+    st = decode_plan(s, P, SGN_MODE_BATCHED);
+    if (st) {
+        (void)hipStreamSynchronize(s->stream);
+        return st;
+    }
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return check_status(P, SGN_MODE_BATCHED, first_fail);
+    API_GUARD_END
+}
+
+uint64_t sgn_j2k_noalloc_scratch_bytes(void) { return kNoAllocScratchBytes; }
 
 int sgn_j2k_decode_device(sgn_hip_session *s, const uint8_t *bytes, size_t len, int fmt,
                           uint32_t den, void *dst, size_t pitch, size_t cap) {
