@@ -56,6 +56,12 @@ typedef struct {
     uint32_t tiles_x, tiles_y;
     uint32_t cb_width, cb_height;
     uint32_t progression;
+    /* Colorspace (crates/signinum-core/src/types.rs:9-20): 0 Grayscale,
+     * 1 YCbCr, 2 Rgb, 3 Cmyk, 4 Ycck, 5 SRgb, 6 SGray, 7 IccTagged, 8 Rct,
+     * 9 Ict; from a JP2 colr box, else inferred from the codestream
+     * (crates/signinum-j2k/src/parse/mod.rs:41-49). */
+    uint32_t colorspace;
+    uint32_t container; /* 0: raw codestream, 1: JP2 file (CompressedPayloadKind) */
 } sgn_j2k_info;
 
 const char *sgn_last_error_message(void);
@@ -67,6 +73,24 @@ int sgn_j2k_inspect(const uint8_t *bytes, size_t len, sgn_j2k_info *info);
  * (crates/signinum-j2k/src/decode.rs:12-33). */
 int sgn_j2k_scaled_dims(const uint8_t *bytes, size_t len, uint32_t denominator,
                         uint32_t *width, uint32_t *height);
+
+/* ---- containers ----------------------------------------------------------
+ * Every entry point that takes codestream bytes also accepts a JP2 file
+ * (crates/signinum-j2k/src/parse/boxes.rs:14-160) and decodes its jp2c
+ * codestream; JP2 files whose samples change after the codestream decode
+ * (palette, channel reordering, sYCC or CIELab) are refused with
+ * SGN_E_UNSUPPORTED by the decode entry points (the native decoder's direct
+ * device plane reuse rule, crates/signinum-j2k-native/src/lib.rs:1493-1509).
+ *
+ * DICOM encapsulated Pixel Data: extract_dicom_encapsulated_frames_with_limit
+ * (crates/signinum-j2k-metal/src/dicom.rs:28-89).  Frames (at most `limit`)
+ * are written back to back into `out`, frame k at frame_offsets[k] with
+ * frame_lengths[k] bytes.  With out == NULL only *frame_count and
+ * *total_bytes are set.  A Pixel Data element that is not encapsulated
+ * gives SGN_E_UNSUPPORTED; malformed or truncated Pixel Data SGN_E_MALFORMED. */
+int sgn_dicom_extract_frames(const uint8_t *bytes, size_t len, size_t limit, uint8_t *out,
+                             size_t out_capacity, size_t *frame_offsets, size_t *frame_lengths,
+                             size_t frames_capacity, size_t *frame_count, size_t *total_bytes);
 
 /* ---- sessions (CudaSession, crates/signinum-j2k-cuda/src/session.rs:11-49) */
 typedef struct sgn_hip_session sgn_hip_session;

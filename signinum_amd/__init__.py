@@ -24,7 +24,7 @@ __all__ = [
     "UnsupportedBackendError", "HipSession", "HipEvent", "J2kDecoder", "DeviceSurface", "TilePlan",
     "decode_tiles_into", "decode_tiles_to_device", "encode_j2k_lossless", "encode_j2k_ht",
     "HtCodeBlockDecoder", "code_block_jobs", "CostModel", "MODE_PLAIN", "MODE_BATCHED",
-    "MODE_GRAPH",
+    "MODE_GRAPH", "extract_dicom_encapsulated_frames",
 ]
 
 
@@ -327,6 +327,26 @@ def scaled_dims(data, scale=Downscale.NONE):
     h = ctypes.c_uint32()
     _check(_lib.load().sgn_j2k_scaled_dims(ptr, n, int(scale), ctypes.byref(w), ctypes.byref(h)))
     return w.value, h.value
+
+
+def extract_dicom_encapsulated_frames(data, limit=None):
+    """extract_dicom_encapsulated_frames(_with_limit)
+    (crates/signinum-j2k-metal/src/dicom.rs:22-89): the codestream bytes of
+    each frame of DICOM encapsulated Pixel Data, at most `limit` frames."""
+    L = _lib.load()
+    ptr, n, _keep = _buf(data)
+    lim = (1 << 63) if limit is None else int(limit)
+    count = ctypes.c_size_t()
+    total = ctypes.c_size_t()
+    _check(L.sgn_dicom_extract_frames(ptr, n, lim, None, 0, None, None, 0, ctypes.byref(count),
+                                      ctypes.byref(total)))
+    k = count.value
+    out = np.zeros(max(1, total.value), np.uint8)
+    offs = (ctypes.c_size_t * max(1, k))()
+    lens = (ctypes.c_size_t * max(1, k))()
+    _check(L.sgn_dicom_extract_frames(ptr, n, lim, out.ctypes.data, out.size, offs, lens, k,
+                                      ctypes.byref(count), ctypes.byref(total)))
+    return [out[offs[i]:offs[i] + lens[i]].tobytes() for i in range(k)]
 
 
 class J2kDecoder:

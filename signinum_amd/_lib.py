@@ -33,7 +33,8 @@ c_u8p = ctypes.POINTER(ctypes.c_uint8)
 class J2kInfo(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in (
         "width", "height", "num_components", "bit_depth", "num_levels", "reversible", "mct",
-        "high_throughput", "tiles_x", "tiles_y", "cb_width", "cb_height", "progression")]
+        "high_throughput", "tiles_x", "tiles_y", "cb_width", "cb_height", "progression",
+        "colorspace", "container")]
 
 
 class TileJob(ctypes.Structure):
@@ -142,6 +143,12 @@ SIGNATURES = {
                                                            ctypes.c_int, ctypes.c_uint32,
                                                            ctypes.POINTER(c_size_t)]),
     "sgn_j2k_noalloc_scratch_bytes": (ctypes.c_uint64, []),
+    "sgn_dicom_extract_frames": (ctypes.c_int, [ctypes.c_void_p, c_size_t, c_size_t,
+                                                ctypes.c_void_p, c_size_t,
+                                                ctypes.POINTER(c_size_t),
+                                                ctypes.POINTER(c_size_t), c_size_t,
+                                                ctypes.POINTER(c_size_t),
+                                                ctypes.POINTER(c_size_t)]),
     "sgn_j2k_plan_create_mixed": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(TileReq), c_size_t,
                                                   ctypes.POINTER(ctypes.c_void_p),
                                                   ctypes.POINTER(ctypes.c_int32)]),

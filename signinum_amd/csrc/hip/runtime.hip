@@ -391,16 +391,27 @@ void sort_by_cost(const std::vector<CbJob> &in, std::vector<CbJob> &out) {
     for (const CbJob &j : in) out[at[top - j.cost]++] = j;
 }
 
-Status build_plan(const sgn_tile_req *jobs, size_t n, sgn_j2k_plan &P, int32_t *tile_status) {
+Status build_plan(const sgn_tile_req *reqs, size_t n, sgn_j2k_plan &P, int32_t *tile_status) {
     const auto t0 = std::chrono::steady_clock::now();
     P.ntiles = n;
     std::vector<LocalPlan> lps(n);
     std::vector<int32_t> st(n, 0);
     std::vector<std::string> errs(n);
+    // each request's bytes narrowed to its codestream (a JP2 file's jp2c
+    // payload); only the codestream travels to the device
+    std::vector<sgn_tile_req> cs(reqs, reqs + n);
+    const sgn_tile_req *jobs = cs.data();
     WorkerPool::get().run(n, [&](size_t i) {
-        st[i] = build_local_plan(jobs[i].bytes, jobs[i].len, jobs[i].fmt, jobs[i].denominator,
-                                 reinterpret_cast<uint64_t>(jobs[i].dst), jobs[i].pitch,
-                                 jobs[i].capacity, lps[i]);
+        Container c;
+        st[i] = locate_codestream(reqs[i].bytes, reqs[i].len, c);
+        if (!st[i]) st[i] = device_decodable(c);
+        if (!st[i]) {
+            cs[i].bytes = c.cs;
+            cs[i].len = c.cs_len;
+            st[i] = build_local_plan(jobs[i].bytes, jobs[i].len, jobs[i].fmt, jobs[i].denominator,
+                                     reinterpret_cast<uint64_t>(jobs[i].dst), jobs[i].pitch,
+                                     jobs[i].capacity, lps[i]);
+        }
         if (st[i]) errs[i] = last_error();
     });
     int32_t first_bad = 0;
@@ -1601,8 +1612,11 @@ int sgn_j2k_decode_into(sgn_hip_session *s, const uint8_t *bytes, size_t len, in
                         uint32_t den, uint8_t *out, size_t out_len, size_t stride) {
     API_GUARD_BEGIN
     if (!s || !bytes || !out) return SGN_E_ARG;
+    Container c;
+    Status st = locate_codestream(bytes, len, c);
+    if (st) return st;
     MainHeader h;
-    Status st = read_main_header(bytes, len, h);
+    st = read_main_header(c.cs, c.cs_len, h);
     if (st) return st;
     OutputGeom og;
     st = output_geometry(h, fmt, den, og);

@@ -305,12 +305,14 @@ int sgn_server_submit(sgn_server *S, const sgn_serve_tile *tiles, size_t n, uint
     for (size_t t = 0; t < n; ++t) {
         const sgn_serve_tile &q = tiles[t];
         if (!q.bytes && q.len) return SGN_E_ARG;
+        Container c;
+        Status st = locate_codestream(q.bytes, q.len, c);
+        if (st || (st = device_decodable(c))) return st;
         MainHeader h;
-        Status st = read_main_header(q.bytes, q.len, h);
-        if (st) return st;
+        if ((st = read_main_header(c.cs, c.cs_len, h))) return st;
         OutputGeom og;
         if ((st = output_geometry(h, q.fmt, q.denominator, og))) return st;
-        r->bytes.emplace_back(q.bytes, q.bytes + q.len);
+        r->bytes.emplace_back(c.cs, c.cs + c.cs_len);
         r->fmt.push_back(q.fmt);
         r->den.push_back(q.denominator);
         r->width.push_back(og.width);
@@ -321,7 +323,7 @@ int sgn_server_submit(sgn_server *S, const sgn_serve_tile *tiles, size_t n, uint
         const uint64_t samples = uint64_t(h.xsiz - h.xo) * (h.ysiz - h.yo) * h.ncomp;
         r->out_bytes += out_b;
         // output + codestream + int32 coefficients (bands and levels) + tables
-        r->footprint += out_b + q.len + samples * 4 * 4 / 3 + samples / 8 + 256;
+        r->footprint += out_b + c.cs_len + samples * 4 * 4 / 3 + samples / 8 + 256;
     }
     if (r->footprint > S->cfg.hbm_budget) {
         set_last_error("request larger than the server's device-memory budget");

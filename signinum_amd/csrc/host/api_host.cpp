@@ -10,8 +10,11 @@ extern "C" {
 int sgn_j2k_inspect(const uint8_t *bytes, size_t len, sgn_j2k_info *info) {
     try {
         if (!bytes || !info) return SGN_E_ARG;
+        sgn::Container c;
+        sgn::Status st = sgn::locate_codestream(bytes, len, c);
+        if (st) return st;
         sgn::MainHeader h;
-        const sgn::Status st = sgn::read_main_header(bytes, len, h);
+        st = sgn::read_main_header(c.cs, c.cs_len, h);
         if (st) return st;
         *info = sgn_j2k_info{};
         info->width = h.xsiz - h.xo;
@@ -29,6 +32,16 @@ int sgn_j2k_inspect(const uint8_t *bytes, size_t len, sgn_j2k_info *info) {
         info->cb_width = 1u << h.comp[0].xcb;
         info->cb_height = 1u << h.comp[0].ycb;
         info->progression = h.prog;
+        // crates/signinum-j2k/src/parse/mod.rs:41-49,68-74: the JP2 colour
+        // specification, else inferred from components, MCT and transform
+        int cs = c.colorspace;
+        if (cs < 0)
+            cs = h.ncomp == 1 ? sgn::kCsSGray
+                 : h.ncomp == 3 ? (!h.mct ? sgn::kCsRgb
+                                   : h.comp[0].transform == 1 ? sgn::kCsRct : sgn::kCsIct)
+                                : sgn::kCsIccTagged;
+        info->colorspace = uint32_t(cs);
+        info->container = c.jp2 ? 1 : 0;
         return SGN_OK;
     } catch (const std::exception &e) {
         sgn::set_last_error(e.what());
@@ -39,8 +52,11 @@ int sgn_j2k_inspect(const uint8_t *bytes, size_t len, sgn_j2k_info *info) {
 int sgn_j2k_scaled_dims(const uint8_t *bytes, size_t len, uint32_t denominator, uint32_t *width,
                         uint32_t *height) {
     if (!bytes) return SGN_E_ARG;
+    sgn::Container c;
+    sgn::Status st = sgn::locate_codestream(bytes, len, c);
+    if (st) return st;
     sgn::MainHeader h;
-    sgn::Status st = sgn::read_main_header(bytes, len, h);
+    st = sgn::read_main_header(c.cs, c.cs_len, h);
     if (st) return st;
     if (denominator != 1 && denominator != 2 && denominator != 4 && denominator != 8) {
         sgn::set_last_error("downscale denominator must be 1, 2, 4 or 8");
@@ -58,8 +74,15 @@ extern "C" int sgn_j2k_code_blocks(const uint8_t *bytes, size_t len, sgn_ht_code
                                    size_t capacity, size_t *count) {
     try {
         if (!bytes || !count) return SGN_E_ARG;
+        sgn::Container c;
+        sgn::Status st = sgn::locate_codestream(bytes, len, c);
+        if (st) return st;
+        // data offsets stay relative to the caller's input
+        const uint64_t base = c.cs_offset;
+        bytes = c.cs;
+        len = c.cs_len;
         sgn::MainHeader h;
-        sgn::Status st = sgn::read_main_header(bytes, len, h);
+        st = sgn::read_main_header(bytes, len, h);
         if (st) return st;
         std::vector<sgn::TileBody> tiles;
         st = sgn::parse_tiles(bytes, len, h, tiles);
@@ -87,7 +110,7 @@ extern "C" int sgn_j2k_code_blocks(const uint8_t *bytes, size_t len, sgn_ht_code
                     st = sgn::band_quant(t.comp[c], bg.type, bg.res, K, delta);
                     if (st) return st;
                     sgn_ht_code_block_job j{};
-                    j.data_offset = sg.offset >= 0 ? uint64_t(sg.offset) : 0;
+                    j.data_offset = sg.offset >= 0 ? base + uint64_t(sg.offset) : 0;
                     j.cleanup_length = sg.offset >= 0 ? sg.lcup : 0;
                     j.refinement_length = sg.offset >= 0 ? sg.lref : 0;
                     j.width = cg.rect.w();

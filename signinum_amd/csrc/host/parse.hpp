@@ -22,6 +22,49 @@ struct CbSeg {
     uint32_t lcup = 0, lref = 0, zblk = 0, passes = 0;
 };
 
+// signinum-core Colorspace (crates/signinum-core/src/types.rs:9-20), as
+// sgn_j2k_info.colorspace reports it.
+enum ColorspaceCode : int {
+    kCsGrayscale = 0,
+    kCsYCbCr = 1,
+    kCsRgb = 2,
+    kCsCmyk = 3,
+    kCsYcck = 4,
+    kCsSRgb = 5,
+    kCsSGray = 6,
+    kCsIccTagged = 7,
+    kCsRct = 8,
+    kCsIct = 9,
+};
+
+// Where the codestream of an input lies: the input itself (raw codestream)
+// or the payload of a JP2 file's jp2c box, with what the JP2 header says.
+struct Container {
+    const uint8_t *cs = nullptr;
+    size_t cs_len = 0, cs_offset = 0;
+    bool jp2 = false;
+    int colorspace = -1;     // from colr (-1: none; inferred from the codestream)
+    uint32_t enumcs = 0;     // EnumCS of the first colr box (0: none or ICC)
+    bool saw_colr = false;
+    bool palette = false;    // pclr present
+    bool cdef = false, cdef_identity = false;
+};
+Status fail_status(Status s, const char *msg);
+bool looks_like_jp2(const uint8_t *data, size_t len);
+Status locate_codestream(const uint8_t *data, size_t len, Container &c);
+// kOk when the device decode produces the reference's samples for this
+// container (no palette, channel reordering or colour conversion after the
+// codestream decode), else kErrUnsupported.
+Status device_decodable(const Container &c);
+
+// Frames of DICOM encapsulated Pixel Data: each a list of (offset, length)
+// byte ranges of the input, concatenated in order.
+struct DicomFrame {
+    std::vector<std::pair<size_t, size_t>> pieces;
+    size_t bytes = 0;
+};
+Status dicom_frames(const uint8_t *data, size_t len, size_t limit, std::vector<DicomFrame> &frames);
+
 Status read_main_header(const uint8_t *data, size_t len, MainHeader &h);
 uint32_t skipped_levels(const MainHeader &h, uint32_t denominator);
 Status parse_tiles(const uint8_t *data, size_t len, const MainHeader &h,

@@ -42,7 +42,14 @@ def test_inspect_matches_oracle(oracle):
     cs = S.encode_j2k_ht(img, levels=3, cb_width_log2=5, cb_height_log2=5, progression=2)
     a = S.J2kDecoder.inspect(cs)
     b = O.inspect(cs)
-    assert a == b
+    assert {k: a[k] for k in b} == b
+    # infer_colorspace (crates/signinum-j2k/src/parse/mod.rs:41-49): RCT
+    # for three reversible components with MCT; raw codestream container
+    assert (a["colorspace"], a["container"]) == (8, 0)
+    ict = S.encode_j2k_ht(img, levels=3, reversible=0, base_step=1 / 64)
+    assert S.J2kDecoder.inspect(ict)["colorspace"] == 9
+    gray = S.encode_j2k_ht(img[..., 0].copy(), use_mct=0, levels=3)
+    assert S.J2kDecoder.inspect(gray)["colorspace"] == 6
 
 
 @pytest.mark.parametrize("size,den", [((240, 240), 4), ((1024, 256), 2), ((17, 37), 8),
@@ -73,7 +80,9 @@ def test_code_block_table_matches_oracle_blocks(oracle):
 def test_malformed_inputs_raise():
     with pytest.raises(S.J2kError):
         S.J2kDecoder.inspect(b"\xff\x4f\xff\x51\x00")
-    with pytest.raises(S.MalformedError):
+    # neither JP2 nor a codestream: J2kError::Unsupported
+    # (crates/signinum-j2k/src/parse/mod.rs:29-31)
+    with pytest.raises(S.UnsupportedError):
         S.J2kDecoder.inspect(b"not a codestream")
     rng = np.random.default_rng(2)
     cs = S.encode_j2k_lossless(synth.texture_rgb(rng, 16, 16))
