@@ -29,6 +29,7 @@
 #include "../host/parse.hpp"
 #include "../host/sgn_api.hpp"
 #include "ht_decode.hip"
+#include "ht_coop.hip"
 #include "ht_rows.hip"
 #include "idwt.hip"
 #include "cost_model.hip"
@@ -100,6 +101,22 @@ void rows_launch(hipStream_t st, uint32_t grid, const CbJob *jobs, uint32_t n,
                        0, st, jobs, n, arena, static_cast<C *>(coef), tables, status, queue);
 }
 
+// The cooperative kernel as a lane-list variant: one code-block per wave,
+// one wave per work-group (the grid is the block count).
+void coop_launch32(hipStream_t st, uint32_t, const CbJob *jobs, uint32_t n,
+                   const uint8_t *arena, void *coef, const uint16_t *tables, uint8_t *status,
+                   uint32_t *) {
+    hipLaunchKernelGGL((sgn_ht_coop<64, false, int32_t>), dim3(n), dim3(64), 0, st, jobs, n,
+                       arena, static_cast<int32_t *>(coef), tables, status, 0u);
+}
+
+void coop_launch16(hipStream_t st, uint32_t, const CbJob *jobs, uint32_t n,
+                   const uint8_t *arena, void *coef, const uint16_t *tables, uint8_t *status,
+                   uint32_t *) {
+    hipLaunchKernelGGL((sgn_ht_coop<64, false, int16_t>), dim3(n), dim3(64), 0, st, jobs, n,
+                       arena, static_cast<int16_t *>(coef), tables, status, 0u);
+}
+
 // launch[0] / kernel[0]: int32 samples; [1]: int16 samples (narrow plans)
 struct RowsVariant {
     const char *name;
@@ -107,6 +124,7 @@ struct RowsVariant {
     bool persist;
     RowsLaunch launch[2];
     const void *kernel[2];
+    bool coop = false;  // one code-block per wave (grid = block count)
 };
 
 // MINW: waves per SIMD the register allocation must allow (1: no bound)
@@ -128,12 +146,21 @@ const RowsVariant kRowsVariants[] = {
     SGN_ROWS_VARIANT("w4-seg4-small-queue", 4, 4, 4, 2, 1, true, 1),
     SGN_ROWS_VARIANT("w2-seg16-queue", 2, 16, 16, 4, 2, true, 1),
     SGN_ROWS_VARIANT("w1-seg8-queue", 1, 8, 8, 4, 2, true, 1),
+    {"wave-coop",
+     1,
+     false,
+     {coop_launch32, coop_launch16},
+     {reinterpret_cast<const void *>(sgn_ht_coop<64, false, int32_t>),
+      reinterpret_cast<const void *>(sgn_ht_coop<64, false, int16_t>)},
+     true},
 };
 #undef SGN_ROWS_VARIANT
 constexpr int kRowsVariantCount = int(sizeof(kRowsVariants) / sizeof(kRowsVariants[0]));
 // variant of a plan nobody autotuned: w4-seg4-small-queue, the autotuner's
 // choice on every batch measured (small rings, three work-groups per CU)
 constexpr int kDefaultLaneVariant = 3;
+// the cooperative kernel (one code-block per wave)
+constexpr int kCoopVariant = 6;
 
 }  // namespace
 
@@ -844,7 +871,7 @@ uint32_t launch_lane_variant(sgn_hip_session *s, sgn_j2k_plan &P, int v, const C
                              uint32_t count, uint8_t *status, bool clear = true) {
     if (!count) return 0;
     const RowsVariant &rv = kRowsVariants[v];
-    const uint32_t per_group = 64u * uint32_t(rv.waves);
+    const uint32_t per_group = rv.coop ? 1u : 64u * uint32_t(rv.waves);
     uint32_t grid = (count + per_group - 1) / per_group;
     uint32_t k = 1;
     if (rv.persist) {
@@ -864,9 +891,14 @@ uint32_t launch_lane(sgn_hip_session *s, sgn_j2k_plan &P, const CbJob *d_jobs, u
 uint32_t launch_wave(sgn_hip_session *s, sgn_j2k_plan &P, const CbJob *d_jobs, uint32_t count,
                      uint8_t *status) {
     if (!count) return 0;
-    hipLaunchKernelGGL(sgn_ht_block_wave, dim3(count), dim3(64), 0, s->stream, d_jobs, count,
-                       P.d_arena, reinterpret_cast<int32_t *>(P.d_coef), s->d_tables, status, 0u,
-                       P.narrow ? 1u : 0u);
+    if (P.narrow)
+        hipLaunchKernelGGL((sgn_ht_coop<1024, true, int16_t>), dim3(count), dim3(64), 0,
+                           s->stream, d_jobs, count, P.d_arena,
+                           reinterpret_cast<int16_t *>(P.d_coef), s->d_tables, status, 0u);
+    else
+        hipLaunchKernelGGL((sgn_ht_coop<1024, true, int32_t>), dim3(count), dim3(64), 0,
+                           s->stream, d_jobs, count, P.d_arena,
+                           reinterpret_cast<int32_t *>(P.d_coef), s->d_tables, status, 0u);
     return 1;
 }
 
@@ -1711,9 +1743,10 @@ static int hook_blocks(sgn_hip_session *s, const uint8_t *data, size_t data_len,
                                    uint32_t(lane.size()), d_data, d_out, s->d_tables, d_st,
                                    nullptr);
     if (!wave.empty())
-        hipLaunchKernelGGL(sgn_ht_block_wave, dim3(wave.size()), dim3(64), 0, s->stream, d_wave,
-                           uint32_t(wave.size()), d_data, reinterpret_cast<int32_t *>(d_out),
-                           s->d_tables, d_st + lane.size(), raw ? 1u : 0u, 0u);
+        hipLaunchKernelGGL((sgn_ht_coop<1024, true, int32_t>), dim3(wave.size()), dim3(64), 0,
+                           s->stream, d_wave, uint32_t(wave.size()), d_data,
+                           reinterpret_cast<int32_t *>(d_out), s->d_tables, d_st + lane.size(),
+                           raw ? 1u : 0u);
     s->dispatches += (lane.empty() ? 0 : 1) + (wave.empty() ? 0 : 1);
     if (hipStreamSynchronize(s->stream) != hipSuccess ||
         hipMemcpy(out, d_out, out_len * 4, hipMemcpyDeviceToHost) != hipSuccess) {
