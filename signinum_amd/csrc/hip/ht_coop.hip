@@ -58,18 +58,34 @@ struct CoopSrc {
     }
 };
 
+// DPP move of v from another lane (CTRL), 0 where the source lane is out
+// of range or the row is not in ROWS.
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ uint32_t dpp0(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWS, 0xF, false);
+}
+
 // Exclusive prefix sum over the 64 lanes (lane order); *total gets the sum.
-// Called by all lanes.
+// Inclusive scan inside each 16-lane row by row shifts, then the rows'
+// totals by row broadcasts (lane 15 into rows 1 and 3, lane 31 into rows 2
+// and 3).  Called by all lanes.
 __device__ __forceinline__ uint32_t wave_scan_excl(uint32_t v, uint32_t &total) {
-    const uint32_t lane = __lane_id();
     uint32_t s = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t t = __shfl_up(s, o);
-        if (lane >= (uint32_t)o) s += t;
-    }
-    total = __shfl(s, 63);
+    s += dpp0<0x111>(s);        // row_shr:1
+    s += dpp0<0x112>(s);        // row_shr:2
+    s += dpp0<0x114>(s);        // row_shr:4
+    s += dpp0<0x118>(s);        // row_shr:8
+    s += dpp0<0x142, 0xA>(s);   // row_bcast:15
+    s += dpp0<0x143, 0xC>(s);   // row_bcast:31
+    total = (uint32_t)__builtin_amdgcn_readlane((int)s, 63);
     return s - v;
+}
+
+// Lanes below this one whose flag is set (ballot + mbcnt).
+__device__ __forceinline__ uint32_t count_below(bool flag, uint64_t &mask) {
+    mask = __ballot(flag);
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
 // 32 bits of an unstuffed stream held in LDS words (LSB-first) from bit
@@ -100,10 +116,12 @@ __device__ __forceinline__ void ms_ring_fill(MsRing &r, uint32_t need, uint32_t 
     while (r.fill < need) {
         const uint32_t a = r.next + lane;
         const uint32_t b = a < r.end ? src.byte(a) : 0xFFu;
-        const uint32_t up = __shfl_up(b, 1);
-        const uint32_t pff = lane == 0 ? r.prev_ff : (up == 0xFFu ? 1u : 0u);
-        uint32_t tot;
-        const uint32_t off = wave_scan_excl(8u - pff, tot);
+        const uint32_t up = dpp0<0x138>(b);  // wave_shr:1, the previous byte
+        const bool pff = lane == 0 ? r.prev_ff != 0 : up == 0xFFu;
+        // each byte carries 8 bits, 7 after a 0xFF
+        uint64_t mask;
+        const uint32_t off = 8u * lane - count_below(pff, mask);
+        const uint32_t tot = 512u - (uint32_t)__builtin_popcountll(mask);
         // words past the partly written one that this round reaches start at zero
         const uint32_t w0 = r.fill >> 5, wl = (r.fill + tot) >> 5;
         if (lane >= 1 && w0 + lane <= wl) ring[(w0 + lane) & (kCoopRingWords - 1)] = 0;
@@ -112,7 +130,7 @@ __device__ __forceinline__ void ms_ring_fill(MsRing &r, uint32_t need, uint32_t 
         atomicOr(&ring[wi & (kCoopRingWords - 1)], b << sh);
         if (sh > 24) atomicOr(&ring[(wi + 1) & (kCoopRingWords - 1)], b >> (32 - sh));
         __syncthreads();
-        r.prev_ff = __shfl(b, 63) == 0xFFu ? 1u : 0u;
+        r.prev_ff = __builtin_amdgcn_readlane((int)b, 63) == 0xFF ? 1u : 0u;
         r.next += 64;
         r.fill += tot;
     }
@@ -326,17 +344,18 @@ __device__ uint32_t coop_refine(CoopLds<WMAX, true> &s, const uint8_t *arena, co
     for (uint32_t k0 = 0; k0 < lref; k0 += 64) {
         const uint32_t k = k0 + lane;
         const uint32_t b = k < lref ? (uint32_t)ref[lref - 1 - k] : 0u;
-        const uint32_t up = __shfl_up(b, 1);
-        const uint32_t unst = lane == 0 ? prev_big : (up > 0x8Fu ? 1u : 0u);
-        const uint32_t nb = 8u - ((unst && (b & 0x7F) == 0x7F) ? 1u : 0u);
-        uint32_t tot;
-        const uint32_t pos = fill + wave_scan_excl(nb, tot);
+        const uint32_t up = dpp0<0x138>(b);  // wave_shr:1, the byte read before
+        const bool unst = lane == 0 ? prev_big != 0 : up > 0x8Fu;
+        // 8 bits, 7 for a byte with its low 7 bits set after a byte > 0x8F
+        uint64_t mask;
+        const uint32_t pos = fill + 8u * lane - count_below(unst && (b & 0x7F) == 0x7F, mask);
+        const uint32_t tot = 512u - (uint32_t)__builtin_popcountll(mask);
         const uint32_t sh = pos & 31u, wi = pos >> 5;
         if (b) {
             atomicOr(&s.ref[wi], b << sh);
             if (sh > 24) atomicOr(&s.ref[wi + 1], b >> (32 - sh));
         }
-        prev_big = __shfl(b, 63) > 0x8Fu ? 1u : 0u;
+        prev_big = (uint32_t)__builtin_amdgcn_readlane((int)b, 63) > 0x8Fu ? 1u : 0u;
         fill += tot;
     }
     __syncthreads();
