@@ -161,6 +161,11 @@ constexpr int kRowsVariantCount = int(sizeof(kRowsVariants) / sizeof(kRowsVarian
 constexpr int kDefaultLaneVariant = 3;
 // the cooperative kernel (one code-block per wave)
 constexpr int kCoopVariant = 6;
+// Lane-list size up to which plans start with the cooperative kernel.  On
+// 256x256 RGB tiles (75 code-blocks each) it took 0.28 against 1.41 ms for
+// 16 tiles and 1.21 against 1.61 ms for 256 tiles (19,200 blocks); the lane
+// kernel wins once its waves fill the device.
+constexpr size_t kCoopMaxBlocks = 24576;
 
 }  // namespace
 
@@ -547,6 +552,10 @@ Status build_plan(const sgn_tile_req *reqs, size_t n, sgn_j2k_plan &P, int32_t *
     // code-blocks for the batched lane kernel, ordered by predicted cost so
     // that a wave's lanes finish together
     sort_by_cost(P.cb_lane_plain, P.cb_lane);
+    // kernel of a plan nobody autotuned: below kCoopMaxBlocks code-blocks the
+    // lane kernel cannot fill the device (each wave takes 64 blocks) and its
+    // time is one block's serial decode, so one block per wave finishes first
+    P.lane_variant = P.cb_lane.size() <= kCoopMaxBlocks ? kCoopVariant : kDefaultLaneVariant;
     // synthesis levels grouped by (transform, depth), tiles in order
     for (int rev = 0; rev < 2; ++rev) {
         P.lv_group[rev].assign(1, 0);
@@ -975,7 +984,9 @@ uint32_t enqueue_plain(sgn_hip_session *s, sgn_j2k_plan &P) {
     uint32_t k = 0;
     for (size_t t = 0; t < P.ntiles; ++t) {
         const uint32_t a0 = P.lane_plain_begin[t], a1 = P.lane_plain_begin[t + 1];
-        k += launch_lane_variant(s, P, 0, P.d_cb_lane_plain + a0, a1 - a0,
+        // one tile's code-blocks: the cooperative kernel (a tile has far
+        // fewer blocks than the lane kernel needs to fill the device)
+        k += launch_lane_variant(s, P, kCoopVariant, P.d_cb_lane_plain + a0, a1 - a0,
                                  P.d_status_lane + a0);
         const uint32_t b0 = P.wave_begin[t], b1 = P.wave_begin[t + 1];
         k += launch_wave(s, P, P.d_cb_wave + b0, b1 - b0, P.d_status_wave + b0);
@@ -1738,10 +1749,11 @@ static int hook_blocks(sgn_hip_session *s, const uint8_t *data, size_t data_len,
         cleanup();
         return SGN_E_DEVICE;
     }
+    // a hook call is one sub-band or a few: the cooperative kernel
     if (!lane.empty())
-        kRowsVariants[0].launch[0](s->stream, uint32_t((lane.size() + 255) / 256), d_lane,
-                                   uint32_t(lane.size()), d_data, d_out, s->d_tables, d_st,
-                                   nullptr);
+        kRowsVariants[kCoopVariant].launch[0](s->stream, uint32_t(lane.size()), d_lane,
+                                              uint32_t(lane.size()), d_data, d_out, s->d_tables,
+                                              d_st, nullptr);
     if (!wave.empty())
         hipLaunchKernelGGL((sgn_ht_coop<1024, true, int32_t>), dim3(wave.size()), dim3(64), 0,
                            s->stream, d_wave, uint32_t(wave.size()), d_data,
