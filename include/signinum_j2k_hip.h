@@ -477,6 +477,11 @@ typedef struct {
     uint32_t precinct_log2;   /* 0: maximal precincts; else PPx = PPy */
     uint32_t sop, eph;        /* SOP / EPH markers */
     uint32_t tile_width, tile_height; /* 0: one tile */
+    /* image origin (XOsiz, YOsiz) and tile-grid origin (XTOsiz <= XOsiz,
+     * YTOsiz <= YOsiz) on the reference grid; the image occupies
+     * [x0, x0 + width) x [y0, y0 + height) */
+    uint32_t image_x0, image_y0, tile_x0, tile_y0;
+    uint32_t packed_headers; /* 0: packet headers in the packets, 1: PPM, 2: PPT */
 } sgn_j2k_encode_options;
 
 void sgn_j2k_encode_default_options(sgn_j2k_encode_options *opts);

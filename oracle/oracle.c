@@ -16,9 +16,15 @@
  *                        crates/signinum-j2k/src/decode.rs:120-170,391-545
  *   tile batch           crates/signinum-j2k/src/batch.rs:110-147,197-226
  *
- * Scope: HTJ2K (Part 15) codestreams with one quality layer, components
- * without sub-sampling, image and tile origins at 0, no PPM/PPT.  Anything
- * else returns ORACLE_E_UNSUPPORTED.
+ * Scope: HTJ2K (Part 15) codestreams with one quality layer and components
+ * without sub-sampling; PPM and PPT packed packet headers
+ * (codestream.rs:761-780, tile.rs:251-300,697-706, segment.rs:54-133);
+ * signed components, whose flag the reference ignores (codestream.rs:
+ * 647-649); non-zero image and tile origins at full resolution.  A reduced-
+ * resolution decode with a non-zero image origin is refused: the
+ * reference's store subtracts the full-resolution image offset from
+ * reduced coordinates (decode.rs:1762-1790).  Anything else returns
+ * ORACLE_E_UNSUPPORTED.
  */
 #include <math.h>
 #include <pthread.h>
@@ -78,6 +84,14 @@ typedef struct {
     comp_t comp[MAXC];
     uint8_t skipped;
     size_t first_sot; /* offset of the first SOT marker */
+    /* PPM packets (codestream.rs:761-780, 164-177): every PPM segment is
+     * Zppm followed by (16-bit length, bytes) pairs; empty ones are dropped
+     * and the rest, in Zppm order, are the packet headers of tile-part 0,
+     * 1, ... of the codestream */
+    size_t nppm;
+    const uint8_t *ppm_data[4096];
+    uint32_t ppm_len[4096];
+    uint8_t ppm_z[4096];
 } header_t;
 
 typedef struct {
@@ -214,8 +228,13 @@ static int read_main_header(const uint8_t *data, size_t len, header_t *h, uint32
         h->comp[i].yr = (uint8_t)yr;
         if (xr == 0 || yr == 0 || h->comp[i].precision > 32) return ORACLE_E_MALFORMED;
         if (xr != 1 || yr != 1) return ORACLE_E_UNSUPPORTED;
+        /* the signed flag (ssiz bit 7) changes nothing (codestream.rs:647-649) */
     }
-    if (h->xo || h->yo || h->xto || h->yto) return ORACLE_E_UNSUPPORTED;
+    /* the first tile holds a reference grid point of the image (B-4,
+     * codestream.rs:582-596) */
+    if ((uint64_t)h->xto + h->xt <= h->xo || (uint64_t)h->yto + h->yt <= h->yo ||
+        h->xto > h->xo || h->yto > h->yo)
+        return ORACLE_E_MALFORMED;
 
     int have_cod = 0, have_qcd = 0;
     int coc_set[MAXC] = {0}, qcc_set[MAXC] = {0};
@@ -263,7 +282,22 @@ static int read_main_header(const uint8_t *data, size_t len, header_t *h, uint32
             if (read_q_params(&b, l - 2 - isz - 1, &qcc_c[ci])) return ORACLE_E_MALFORMED;
             qcc_set[ci] = 1;
         } else if (m == 0x60) { /* PPM */
-            return ORACLE_E_UNSUPPORTED;
+            uint32_t l, z, n;
+            if (rd16(&b, &l) || l < 3 || b.pos + (l - 2) > len) return ORACLE_E_MALFORMED;
+            size_t end = b.pos + (l - 2);
+            if (rd8(&b, &z)) return ORACLE_E_MALFORMED;
+            while (b.pos < end) {
+                if (rd16(&b, &n) || b.pos + n > end) return ORACLE_E_MALFORMED;
+                if (n) {
+                    if (h->nppm == sizeof(h->ppm_len) / sizeof(h->ppm_len[0]))
+                        return ORACLE_E_UNSUPPORTED;
+                    h->ppm_data[h->nppm] = data + b.pos;
+                    h->ppm_len[h->nppm] = n;
+                    h->ppm_z[h->nppm] = (uint8_t)z;
+                    h->nppm++;
+                }
+                b.pos += n;
+            }
         } else if (m == 0x50 || m == 0x59 || m == 0x5E || m == 0x55 || m == 0x64 ||
                    m == 0x63) {
             if (skip_segment(&b)) return ORACLE_E_MALFORMED;
@@ -275,6 +309,19 @@ static int read_main_header(const uint8_t *data, size_t len, header_t *h, uint32
     }
     if (!have_cod || !have_qcd) return ORACLE_E_MALFORMED;
     h->first_sot = b.pos;
+    /* PPM packets in Zppm order, stable (codestream.rs:169) */
+    for (size_t i = 1; i < h->nppm; ++i)
+        for (size_t j = i; j > 0 && h->ppm_z[j - 1] > h->ppm_z[j]; --j) {
+            const uint8_t *pd = h->ppm_data[j];
+            uint32_t pl = h->ppm_len[j];
+            uint8_t pz = h->ppm_z[j];
+            h->ppm_data[j] = h->ppm_data[j - 1];
+            h->ppm_len[j] = h->ppm_len[j - 1];
+            h->ppm_z[j] = h->ppm_z[j - 1];
+            h->ppm_data[j - 1] = pd;
+            h->ppm_len[j - 1] = pl;
+            h->ppm_z[j - 1] = pz;
+        }
     uint8_t min_nr = 255;
     for (uint32_t i = 0; i < csiz; ++i) {
         comp_t *c = &h->comp[i];
@@ -305,13 +352,14 @@ static int read_main_header(const uint8_t *data, size_t len, header_t *h, uint32
     /* reduced resolution (codestream.rs:137-162) */
     uint32_t skipped = 0;
     if (target_w && target_h) {
-        uint32_t wr = h->xsiz / target_w, hr = h->ysiz / target_h;
+        uint32_t wr = (h->xsiz - h->xo) / target_w, hr = (h->ysiz - h->yo) / target_h;
         uint32_t wl = wr ? 31 - (uint32_t)__builtin_clz(wr) : 0;
         uint32_t hl = hr ? 31 - (uint32_t)__builtin_clz(hr) : 0;
         skipped = wl < hl ? wl : hl;
     }
     if (skipped > (uint32_t)(min_nr - 1)) skipped = min_nr - 1;
     h->skipped = (uint8_t)skipped;
+    if (skipped && (h->xo || h->yo)) return ORACLE_E_UNSUPPORTED;
     return ORACLE_OK;
 }
 
@@ -594,28 +642,40 @@ static void free_tcomp(tcomp_t *tc)
 /* ------------------------------------------------------------------ */
 /* Tiles and tile parts (tile.rs:131-320)                              */
 
+/* One tile-part: its body and, when packet headers are packed (PPT of
+ * this tile-part in Zppt order, then the tile-part's PPM packet;
+ * tile.rs:291-309), the header pieces read one after another. */
+#define MAXHDR 64
+typedef struct {
+    const uint8_t *body;
+    size_t body_len;
+    int nhdr;
+    const uint8_t *hdr[MAXHDR];
+    size_t hdr_len[MAXHDR];
+} tpart_t;
+
 typedef struct {
     rect_t rect;
-    uint8_t *body; /* concatenated tile-part bodies */
-    size_t body_len, body_cap;
+    tpart_t *parts;
+    size_t nparts, cap;
     int seen;
     comp_t comp[MAXC];
     uint8_t prog, mct;
     uint16_t nlayers;
 } tile_t;
 
-static int append_body(tile_t *t, const uint8_t *p, size_t n)
+static tpart_t *add_part(tile_t *t)
 {
-    if (t->body_len + n > t->body_cap) {
-        size_t cap = (t->body_len + n) * 2 + 64;
-        uint8_t *nb = (uint8_t *)realloc(t->body, cap);
-        if (!nb) return ORACLE_E_NOMEM;
-        t->body = nb;
-        t->body_cap = cap;
+    if (t->nparts == t->cap) {
+        size_t cap = t->cap * 2 + 4;
+        tpart_t *np = (tpart_t *)realloc(t->parts, cap * sizeof(tpart_t));
+        if (!np) return NULL;
+        t->parts = np;
+        t->cap = cap;
     }
-    memcpy(t->body + t->body_len, p, n);
-    t->body_len += n;
-    return ORACLE_OK;
+    tpart_t *p = &t->parts[t->nparts++];
+    memset(p, 0, sizeof(*p));
+    return p;
 }
 
 static int parse_tiles(const uint8_t *data, size_t len, const header_t *h, tile_t *tiles,
@@ -636,8 +696,13 @@ static int parse_tiles(const uint8_t *data, size_t len, const header_t *h, tile_
         t->nlayers = h->nlayers;
     }
     br_t b = {data, len, h->first_sot};
+    size_t tp_index = 0; /* tile-parts seen, in codestream order */
     while (peek_marker(&b) == 0x90) {
         b.pos += 2;
+        int nppt = 0;
+        const uint8_t *ppt[MAXHDR];
+        size_t ppt_len[MAXHDR];
+        uint8_t ppt_z[MAXHDR];
         uint32_t lsot, isot, psot, tpsot, tnsot;
         if (rd16(&b, &lsot) || rd16(&b, &isot) || rd32(&b, &psot) || rd8(&b, &tpsot) ||
             rd8(&b, &tnsot))
@@ -722,8 +787,17 @@ static int parse_tiles(const uint8_t *data, size_t len, const header_t *h, tile_
                 }
             } else if (m == 0xD9) {
                 break;
-            } else if (m == 0x61) {
-                return ORACLE_E_UNSUPPORTED; /* PPT */
+            } else if (m == 0x61) { /* PPT (tile.rs:251-258,697-706) */
+                uint32_t l, z;
+                if (h->nppm) return ORACLE_E_MALFORMED; /* PPM and PPT together */
+                if (rd16(&b, &l) || l < 3 || b.pos + (l - 2) > len || rd8(&b, &z))
+                    return ORACLE_E_MALFORMED;
+                if (nppt == MAXHDR - 1) return ORACLE_E_UNSUPPORTED;
+                ppt[nppt] = data + b.pos;
+                ppt_len[nppt] = l - 3;
+                ppt_z[nppt] = (uint8_t)z;
+                nppt++;
+                b.pos += l - 3;
             } else if (m == 0x58 || m == 0x64) {
                 if (skip_segment(&b)) return ORACLE_E_MALFORMED;
             } else if (m >= 0x30 && m <= 0x3F) {
@@ -735,7 +809,34 @@ static int parse_tiles(const uint8_t *data, size_t len, const header_t *h, tile_
         if (used > data_len) return ORACLE_OK;
         size_t remain = data_len - used;
         if (b.pos + remain > len) return ORACLE_E_MALFORMED;
-        if (append_body(t, data + b.pos, remain)) return ORACLE_E_NOMEM;
+        tpart_t *tp = add_part(t);
+        if (!tp) return ORACLE_E_NOMEM;
+        tp->body = data + b.pos;
+        tp->body_len = remain;
+        /* PPT pieces in Zppt order (stable), then the PPM packet */
+        for (int i = 1; i < nppt; ++i)
+            for (int j = i; j > 0 && ppt_z[j - 1] > ppt_z[j]; --j) {
+                const uint8_t *pd = ppt[j];
+                size_t pl = ppt_len[j];
+                uint8_t pz = ppt_z[j];
+                ppt[j] = ppt[j - 1];
+                ppt_len[j] = ppt_len[j - 1];
+                ppt_z[j] = ppt_z[j - 1];
+                ppt[j - 1] = pd;
+                ppt_len[j - 1] = pl;
+                ppt_z[j - 1] = pz;
+            }
+        for (int i = 0; i < nppt; ++i) {
+            tp->hdr[tp->nhdr] = ppt[i];
+            tp->hdr_len[tp->nhdr] = ppt_len[i];
+            tp->nhdr++;
+        }
+        if (tp_index < h->nppm) {
+            tp->hdr[tp->nhdr] = h->ppm_data[tp_index];
+            tp->hdr_len[tp->nhdr] = h->ppm_len[tp_index];
+            tp->nhdr++;
+        }
+        tp_index++;
         t->seen = 1;
         b.pos += remain;
     }
@@ -896,22 +997,48 @@ static int read_ht_passes(pbr_t *r, uint32_t *n)
 
 static uint32_t ilog2u(uint32_t v) { return 31 - (uint32_t)__builtin_clz(v); }
 
-static int parse_packets(tile_t *t, int ncomp, tcomp_t *tcs, const prog_list_t *pl)
+/* Packets of one tile-part (segment.rs:54-133): while the header stream is
+ * not at its end, the next packet of the progression: an SOP marker in the
+ * body is skipped, the header is read from the header stream (the body
+ * itself when headers are not packed; with packed headers the next piece
+ * once the current one is exhausted, tile.rs:55-67), aligned, followed by
+ * an EPH marker when signalled, then the included blocks' bytes come from
+ * the body.  Returns -1 on a malformed packet (the caller stops, non-strict
+ * segment.rs:36-50). */
+static int parse_tile_part(const tpart_t *tp, size_t *next, int ncomp, tcomp_t *tcs,
+                           const comp_t *comps, const prog_list_t *pl)
 {
-    pbr_t r = {t->body, t->body_len, 0};
-    for (size_t i = 0; i < pl->n; ++i) {
-        const prog_t *pg = &pl->v[i];
-        if ((r.bitpos >> 3) >= r.len) break;
+    const int packed = tp->nhdr > 0;
+    int hi = 0;
+    pbr_t hb = {packed ? tp->hdr[0] : tp->body, packed ? tp->hdr_len[0] : tp->body_len, 0};
+    size_t body = 0; /* byte position in the body (packed headers) */
+    (void)ncomp;
+    for (;;) {
+        if (packed && (hb.bitpos >> 3) >= hb.len && hi + 1 < tp->nhdr) {
+            ++hi;
+            hb.d = tp->hdr[hi];
+            hb.len = tp->hdr_len[hi];
+            hb.bitpos = 0;
+        }
+        if ((hb.bitpos >> 3) >= hb.len) break;
+        if (*next >= pl->n) return -1;
+        const prog_t *pg = &pl->v[(*next)++];
         tcomp_t *tc = &tcs[pg->c];
         res_t *rs = &tc->res[pg->r];
-        const comp_t *cp = &t->comp[pg->c];
-        size_t byte = r.bitpos >> 3;
-        if ((cp->scod & 2) && byte + 1 < r.len && r.d[byte] == 0xFF && r.d[byte + 1] == 0x91) {
-            if (byte + 6 > r.len) return -1;
-            r.bitpos += 6 * 8;
+        const comp_t *cp = &comps[pg->c];
+        /* SOP in the body */
+        const uint8_t *bd = tp->body;
+        size_t bpos = packed ? body : (hb.bitpos >> 3);
+        if ((cp->scod & 2) && bpos + 1 < tp->body_len && bd[bpos] == 0xFF && bd[bpos + 1] == 0x91) {
+            if (bpos + 6 > tp->body_len) return -1;
+            if (packed)
+                body += 6;
+            else
+                hb.bitpos += 6 * 8;
         }
+        pbr_t *r = &hb;
         uint32_t nonzero;
-        if (pbr_bits(&r, 1, &nonzero)) return -1;
+        if (pbr_bits(r, 1, &nonzero)) return -1;
         if (nonzero) {
             for (int bi = 0; bi < rs->nbands; ++bi) {
                 prec_band_t *pb = &rs->prec[(size_t)pg->p * rs->nbands + bi];
@@ -920,9 +1047,9 @@ static int parse_packets(tile_t *t, int ncomp, tcomp_t *tcs, const prog_list_t *
                     int first = !cb->included;
                     uint32_t inc, v;
                     if (cb->included) {
-                        if (pbr_bits(&r, 1, &inc)) return -1;
+                        if (pbr_bits(r, 1, &inc)) return -1;
                     } else {
-                        int rc = tagtree_decode(&pb->incl, cb->xi, cb->yi, pg->l + 1u, &r, &v);
+                        int rc = tagtree_decode(&pb->incl, cb->xi, cb->yi, pg->l + 1u, r, &v);
                         if (rc < 0) return -1;
                         inc = (v <= pg->l);
                     }
@@ -931,7 +1058,7 @@ static int parse_packets(tile_t *t, int ncomp, tcomp_t *tcs, const prog_list_t *
                         /* zero bit-plane tag tree, read to completion */
                         uint32_t thr = 1;
                         for (;;) {
-                            int rc = tagtree_decode(&pb->zbp, cb->xi, cb->yi, thr, &r, &v);
+                            int rc = tagtree_decode(&pb->zbp, cb->xi, cb->yi, thr, r, &v);
                             if (rc < 0) return -1;
                             if (rc) break;
                             thr++;
@@ -942,7 +1069,7 @@ static int parse_packets(tile_t *t, int ncomp, tcomp_t *tcs, const prog_list_t *
                     cb->included = 1;
                     if (!first || cb->passes != 0) return -1;
                     uint32_t raw;
-                    if (read_ht_passes(&r, &raw) || raw > 94) return -1;
+                    if (read_ht_passes(r, &raw) || raw > 94) return -1;
                     uint32_t groups = raw > 0 ? (raw - 1) / 3 : 0;
                     cb->zblk += groups;
                     if (cb->zblk > 255) return -1;
@@ -950,19 +1077,19 @@ static int parse_packets(tile_t *t, int ncomp, tcomp_t *tcs, const prog_list_t *
                     uint32_t actual = raw - ph;
                     uint32_t bit;
                     for (;;) {
-                        if (pbr_bits(&r, 1, &bit)) return -1;
+                        if (pbr_bits(r, 1, &bit)) return -1;
                         if (!bit) break;
                         cb->lblock++;
                     }
                     uint32_t nb = cb->lblock + ilog2u(ph + 1);
                     if (nb > 32) return -1;
                     uint32_t lc;
-                    if (pbr_bits(&r, nb, &lc)) return -1;
+                    if (pbr_bits(r, nb, &lc)) return -1;
                     if (lc < 2 || lc >= 65535) return -1;
                     uint32_t lr = 0;
                     if (actual > 1) {
                         uint32_t nb2 = cb->lblock + (actual > 2 ? 1 : 0);
-                        if (nb2 > 32 || pbr_bits(&r, nb2, &lr)) return -1;
+                        if (nb2 > 32 || pbr_bits(r, nb2, &lr)) return -1;
                         if (lr >= 2047) return -1;
                     }
                     cb->passes = actual;
@@ -971,29 +1098,41 @@ static int parse_packets(tile_t *t, int ncomp, tcomp_t *tcs, const prog_list_t *
                 }
             }
         }
-        /* align to a byte boundary */
-        if (r.bitpos & 7) r.bitpos += 8 - (r.bitpos & 7);
+        /* align to a byte boundary; EPH follows the header */
+        if (r->bitpos & 7) r->bitpos += 8 - (r->bitpos & 7);
         if (cp->scod & 4) {
-            size_t bb = r.bitpos >> 3;
-            if (bb + 2 > r.len || r.d[bb] != 0xFF || r.d[bb + 1] != 0x92) return -1;
-            r.bitpos += 16;
+            size_t bb = r->bitpos >> 3;
+            if (bb + 2 > r->len || r->d[bb] != 0xFF || r->d[bb + 1] != 0x92) return -1;
+            r->bitpos += 16;
         }
+        if (!packed) body = hb.bitpos >> 3;
         if (nonzero) {
             for (int bi = 0; bi < rs->nbands; ++bi) {
                 prec_band_t *pb = &rs->prec[(size_t)pg->p * rs->nbands + bi];
                 for (uint32_t k = 0; k < pb->ncbx * pb->ncby; ++k) {
                     cblk_t *cb = &pb->cbs[k];
                     if (cb->passes == 0 || cb->data) continue;
-                    size_t bb = r.bitpos >> 3;
                     size_t need = (size_t)cb->lcup + cb->lref;
-                    if (bb + need > r.len) return -1;
-                    cb->data = r.d + bb;
-                    r.bitpos += need * 8;
+                    if (body + need > tp->body_len) return -1;
+                    cb->data = tp->body + body;
+                    body += need;
                 }
             }
         }
+        if (!packed) hb.bitpos = body * 8;
     }
     return 0;
+}
+
+static int parse_packets(tile_t *t, int ncomp, tcomp_t *tcs, const prog_list_t *pl)
+{
+    /* non-strict: a malformed tile-part ends its own packets only; the next
+     * tile-part continues the progression (segment.rs:36-50) */
+    size_t next = 0;
+    int rc = 0;
+    for (size_t i = 0; i < t->nparts; ++i)
+        if (parse_tile_part(&t->parts[i], &next, ncomp, tcs, t->comp, pl)) rc = -1;
+    return rc;
 }
 
 /* ------------------------------------------------------------------ */
@@ -1278,7 +1417,9 @@ static int decode_image(const uint8_t *data, size_t len, uint32_t denominator, i
     }
     rc = parse_tiles(data, len, h, tiles, ntiles);
     uint32_t sk = h->skipped;
-    uint32_t iw = cdiv(h->xsiz, 1u << sk), ih = cdiv(h->ysiz, 1u << sk);
+    /* image_width / image_height (codestream.rs:544-553); a non-zero image
+     * origin only at full resolution (read_main_header) */
+    uint32_t iw = cdiv(h->xsiz - h->xo, 1u << sk), ih = cdiv(h->ysiz - h->yo, 1u << sk);
     img->w = iw;
     img->h = ih;
     img->ncomp = h->ncomp;
@@ -1344,7 +1485,8 @@ static int decode_image(const uint8_t *data, size_t len, uint32_t denominator, i
             for (uint32_t y = 0; y < rh(rr); ++y)
                 for (uint32_t x = 0; x < rw(rr); ++x) {
                     size_t src = (size_t)y * rw(rr) + x;
-                    size_t dst = (size_t)c * plane + (size_t)(rr.y0 + y) * iw + rr.x0 + x;
+                    size_t dst = (size_t)c * plane + (size_t)(rr.y0 + y - h->yo) * iw +
+                                 (rr.x0 + x - h->xo);
                     if (use_f64)
                         img->planes64[dst] = ((double *)cur)[src] + (double)shift;
                     else
@@ -1404,7 +1546,7 @@ static int decode_image(const uint8_t *data, size_t len, uint32_t denominator, i
             }
         }
     }
-    for (uint32_t ti = 0; ti < ntiles; ++ti) free(tiles[ti].body);
+    for (uint32_t ti = 0; ti < ntiles; ++ti) free(tiles[ti].parts);
     free(tiles);
     free(tcs);
     free(h);
@@ -1535,8 +1677,8 @@ int oracle_scaled_dims(const uint8_t *data, size_t len, uint32_t denominator, ui
         rc = read_main_header(data, len, h, tw, th);
     }
     if (!rc) {
-        *out_w = cdiv(h->xsiz, 1u << h->skipped);
-        *out_h = cdiv(h->ysiz, 1u << h->skipped);
+        *out_w = cdiv(h->xsiz - h->xo, 1u << h->skipped);
+        *out_h = cdiv(h->ysiz - h->yo, 1u << h->skipped);
     }
     free(h);
     return rc;
@@ -1742,7 +1884,7 @@ int oracle_code_blocks(const uint8_t *data, size_t len, oracle_cb_t *out, size_t
                             o->y0 = cb->rect.y0;
                             o->x1 = cb->rect.x1;
                             o->y1 = cb->rect.y1;
-                            o->offset = cb->data ? (int64_t)(cb->data - t->body) : -1;
+                            o->offset = cb->data ? (int64_t)(cb->data - data) : -1;
                             o->lcup = cb->lcup;
                             o->lref = cb->lref;
                             o->zblk = cb->zblk;
@@ -1753,7 +1895,7 @@ int oracle_code_blocks(const uint8_t *data, size_t len, oracle_cb_t *out, size_t
             free_tcomp(&tcs[c]);
         }
     }
-    for (uint32_t ti = 0; tiles && ti < ntx * nty; ++ti) free(tiles[ti].body);
+    for (uint32_t ti = 0; tiles && ti < ntx * nty; ++ti) free(tiles[ti].parts);
     free(tiles);
     free(tcs);
     free(h);

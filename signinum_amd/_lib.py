@@ -93,7 +93,8 @@ class EncodeOptions(ctypes.Structure):
         "levels", "cb_width_log2", "cb_height_log2", "reversible", "use_mct", "progression",
         "guard_bits")] + [("base_step", ctypes.c_float)] +
         [(n, ctypes.c_uint32) for n in ("precinct_log2", "sop", "eph", "tile_width",
-                                         "tile_height")])
+                                         "tile_height", "image_x0", "image_y0", "tile_x0",
+                                         "tile_y0", "packed_headers")])
 
 
 class HtCodeBlockBatchJob(ctypes.Structure):

@@ -63,8 +63,12 @@ int sgn_j2k_scaled_dims(const uint8_t *bytes, size_t len, uint32_t denominator, 
         return SGN_E_ARG;
     }
     const uint32_t sk = sgn::skipped_levels(h, denominator);
-    if (width) *width = sgn::ceil_div(h.xsiz, 1u << sk);
-    if (height) *height = sgn::ceil_div(h.ysiz, 1u << sk);
+    if (sk && (h.xo || h.yo)) {
+        sgn::set_last_error("reduced resolution of an image with a non-zero origin");
+        return SGN_E_UNSUPPORTED;
+    }
+    if (width) *width = sgn::ceil_div(h.xsiz - h.xo, 1u << sk);
+    if (height) *height = sgn::ceil_div(h.ysiz - h.yo, 1u << sk);
     return SGN_OK;
 }
 

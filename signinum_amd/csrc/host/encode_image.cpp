@@ -198,11 +198,39 @@ Status encode_image(const void *pixels, uint32_t width, uint32_t height, uint32_
     }
     const bool rev = opt.reversible != 0;
     const bool mct = opt.use_mct && ncomp >= 3;
-    const uint32_t tw = opt.tile_width ? opt.tile_width : width;
-    const uint32_t th = opt.tile_height ? opt.tile_height : height;
+    const uint32_t xo = opt.image_x0, yo = opt.image_y0;
+    const uint32_t xto = opt.tile_x0, yto = opt.tile_y0;
+    const uint64_t xsiz64 = uint64_t(xo) + width, ysiz64 = uint64_t(yo) + height;
+    if (xsiz64 >= (1ull << 32) || ysiz64 >= (1ull << 32) || opt.packed_headers > 2) {
+        set_last_error("encode: invalid image origin or packed-header mode");
+        return kErrArg;
+    }
+    const uint32_t xsiz = uint32_t(xsiz64), ysiz = uint32_t(ysiz64);
+    const uint32_t tw = opt.tile_width ? opt.tile_width : xsiz - xto;
+    const uint32_t th = opt.tile_height ? opt.tile_height : ysiz - yto;
+    // B-4: the first tile holds a reference grid point of the image
+    if (xto > xo || yto > yo || uint64_t(xto) + tw <= xo || uint64_t(yto) + th <= yo) {
+        set_last_error("encode: the tile-grid origin must lie in the first tile's image area");
+        return kErrArg;
+    }
+    const uint32_t ntx = ceil_div(xsiz - xto, tw), nty = ceil_div(ysiz - yto, th);
+    auto tile_rect = [&](uint32_t ti) {
+        const uint32_t x0 = xto + (ti % ntx) * tw, y0 = yto + (ti / ntx) * th;
+        Rect r;
+        r.x0 = std::max(x0, xo);
+        r.y0 = std::max(y0, yo);
+        r.x1 = std::min(x0 + tw, xsiz);
+        r.y1 = std::min(y0 + th, ysiz);
+        return r;
+    };
     uint32_t levels = opt.levels;
     {
-        const uint32_t mn = std::min(tw, th);
+        // at most log2 of the smallest side of any tile
+        uint32_t mn = UINT32_MAX;
+        for (uint32_t ti = 0; ti < ntx * nty; ++ti) {
+            const Rect r = tile_rect(ti);
+            mn = std::min(mn, std::min(r.w(), r.h()));
+        }
         const uint32_t lim = mn ? 31 - __builtin_clz(mn) : 0;
         levels = std::min(levels, lim);
     }
@@ -264,10 +292,14 @@ Status encode_image(const void *pixels, uint32_t width, uint32_t height, uint32_
     }
 
     MainHeader mh;
-    mh.xsiz = width;
-    mh.ysiz = height;
+    mh.xsiz = xsiz;
+    mh.ysiz = ysiz;
+    mh.xo = xo;
+    mh.yo = yo;
     mh.xt = tw;
     mh.yt = th;
+    mh.xto = xto;
+    mh.yto = yto;
     mh.ncomp = static_cast<uint16_t>(ncomp);
     mh.prog = static_cast<uint8_t>(opt.progression);
     mh.layers = 1;
@@ -280,14 +312,14 @@ Status encode_image(const void *pixels, uint32_t width, uint32_t height, uint32_
     put16(cs, 0xFF51);
     put16(cs, 38 + 3 * ncomp);
     put16(cs, 0x4000);  // Rsiz: extended capabilities present (CAP)
-    put32(cs, width);
-    put32(cs, height);
-    put32(cs, 0);
-    put32(cs, 0);
+    put32(cs, xsiz);
+    put32(cs, ysiz);
+    put32(cs, xo);
+    put32(cs, yo);
     put32(cs, tw);
     put32(cs, th);
-    put32(cs, 0);
-    put32(cs, 0);
+    put32(cs, xto);
+    put32(cs, yto);
     put16(cs, ncomp);
     for (uint32_t c = 0; c < ncomp; ++c) {
         cs.push_back(static_cast<uint8_t>(bit_depth - 1));
@@ -328,14 +360,13 @@ Status encode_image(const void *pixels, uint32_t width, uint32_t height, uint32_
         for (uint32_t i = 0; i < cp.nsteps; ++i) put16(cs, (cp.eps[i] << 11) | cp.mu[i]);
     }
 
-    const uint32_t ntx = ceil_div(width, tw), nty = ceil_div(height, th);
     std::vector<uint8_t> blk;
+    // tile-parts in codestream order: header bytes (PPT segments) and body
+    std::vector<std::vector<uint8_t>> part_hdr, part_body;
+    // packet headers per tile-part (packed modes), split at packet boundaries
+    std::vector<std::vector<std::vector<uint8_t>>> packed(ntx * nty);
     for (uint32_t ti = 0; ti < ntx * nty; ++ti) {
-        Rect tr;
-        tr.x0 = (ti % ntx) * tw;
-        tr.y0 = (ti / ntx) * th;
-        tr.x1 = std::min(tr.x0 + tw, width);
-        tr.y1 = std::min(tr.y0 + th, height);
+        const Rect tr = tile_rect(ti);
         const uint32_t w = tr.w(), h = tr.h();
         const size_t plane = static_cast<size_t>(w) * h;
         // ---- sample extraction, DC shift, colour transform ----
@@ -343,10 +374,11 @@ Status encode_image(const void *pixels, uint32_t width, uint32_t height, uint32_
         std::vector<double> dp(rev ? 0 : plane * ncomp);
         const int32_t shift = 1 << (bit_depth - 1);
         for (uint32_t y = 0; y < h; ++y) {
-            const uint8_t *row = static_cast<const uint8_t *>(pixels) + (tr.y0 + y) * row_stride;
+            const uint8_t *row =
+                static_cast<const uint8_t *>(pixels) + size_t(tr.y0 + y - yo) * row_stride;
             for (uint32_t x = 0; x < w; ++x)
                 for (uint32_t c = 0; c < ncomp; ++c) {
-                    const size_t si = static_cast<size_t>(tr.x0 + x) * ncomp + c;
+                    const size_t si = static_cast<size_t>(tr.x0 + x - xo) * ncomp + c;
                     int32_t v = bit_depth <= 8 ? row[si]
                                                : reinterpret_cast<const uint16_t *>(row)[si];
                     v -= shift;
@@ -495,6 +527,7 @@ Status encode_image(const void *pixels, uint32_t width, uint32_t height, uint32_
         Status st = progression(mh.prog, 1, tr, tcs.data(), static_cast<int>(ncomp), order);
         if (st) return st;
         std::vector<uint8_t> body;
+        const bool pack = opt.packed_headers != 0;
         uint32_t packet_index = 0;
         for (const PacketId &pk : order) {
             const TileCompGeom &g = tcs[pk.comp];
@@ -511,7 +544,10 @@ Status encode_image(const void *pixels, uint32_t width, uint32_t height, uint32_
                 for (uint32_t k = 0; k < pb.ncbx * pb.ncby; ++k)
                     any |= !seg[pk.comp][pb.cb_base + k].empty();
             }
-            BitWriter bw(body);
+            // packed modes: the header (and its EPH) goes to its own buffer
+            std::vector<uint8_t> phdr;
+            std::vector<uint8_t> &hdr = pack ? phdr : body;
+            BitWriter bw(hdr);
             bw.bit(any ? 1 : 0);
             if (any) {
                 for (uint32_t b = 0; b < rs.nbands; ++b) {
@@ -552,7 +588,8 @@ Status encode_image(const void *pixels, uint32_t width, uint32_t height, uint32_
                 }
             }
             bw.flush();
-            if (opt.eph) put16(body, 0xFF92);
+            if (opt.eph) put16(hdr, 0xFF92);
+            if (pack) packed[ti].push_back(std::move(phdr));
             if (any)
                 for (uint32_t b = 0; b < rs.nbands; ++b) {
                     const PrecBand &pb = g.precs[rs.prec_base + pk.prec * rs.nbands + b];
@@ -562,13 +599,65 @@ Status encode_image(const void *pixels, uint32_t width, uint32_t height, uint32_
                     }
                 }
         }
-        // ---- tile-part ----
+        // ---- tile-part header extras: PPT segments (A.7.5), each holding
+        // whole packet headers, Zppt counting up ----
+        std::vector<uint8_t> th_extra;
+        if (opt.packed_headers == 2) {
+            std::vector<uint8_t> segd;
+            uint32_t z = 0;
+            auto flush_ppt = [&]() {
+                if (segd.empty()) return;
+                put16(th_extra, 0xFF61);
+                put16(th_extra, static_cast<uint32_t>(3 + segd.size()));
+                th_extra.push_back(static_cast<uint8_t>(z++));
+                th_extra.insert(th_extra.end(), segd.begin(), segd.end());
+                segd.clear();
+            };
+            for (const std::vector<uint8_t> &ph : packed[ti]) {
+                if (segd.size() + ph.size() > 65532) flush_ppt();
+                segd.insert(segd.end(), ph.begin(), ph.end());
+            }
+            flush_ppt();
+        }
+        part_hdr.push_back(std::move(th_extra));
+        part_body.push_back(std::move(body));
+    }
+    // ---- PPM segments (A.7.4) in the main header: per tile-part Nppm (four
+    // bytes) and its packet headers, whole entries per segment ----
+    if (opt.packed_headers == 1) {
+        std::vector<uint8_t> segd;
+        uint32_t z = 0;
+        auto flush_ppm = [&]() {
+            if (segd.empty()) return;
+            put16(cs, 0xFF60);
+            put16(cs, static_cast<uint32_t>(3 + segd.size()));
+            cs.push_back(static_cast<uint8_t>(z++));
+            cs.insert(cs.end(), segd.begin(), segd.end());
+            segd.clear();
+        };
+        for (uint32_t ti = 0; ti < ntx * nty; ++ti) {
+            std::vector<uint8_t> all;
+            for (const std::vector<uint8_t> &ph : packed[ti]) all.insert(all.end(), ph.begin(), ph.end());
+            if (all.size() + 4 > 65532) {
+                set_last_error("encode: a tile's packet headers exceed one PPM segment");
+                return kErrArg;
+            }
+            if (segd.size() + 4 + all.size() > 65532) flush_ppm();
+            put32(segd, static_cast<uint32_t>(all.size()));
+            segd.insert(segd.end(), all.begin(), all.end());
+        }
+        flush_ppm();
+    }
+    // ---- tile-parts ----
+    for (uint32_t ti = 0; ti < ntx * nty; ++ti) {
+        const std::vector<uint8_t> &hx = part_hdr[ti], &body = part_body[ti];
         put16(cs, 0xFF90);
         put16(cs, 10);
         put16(cs, ti);
-        put32(cs, static_cast<uint32_t>(12 + 2 + body.size()));
+        put32(cs, static_cast<uint32_t>(12 + hx.size() + 2 + body.size()));
         cs.push_back(0);
         cs.push_back(1);
+        cs.insert(cs.end(), hx.begin(), hx.end());
         put16(cs, 0xFF93);
         cs.insert(cs.end(), body.begin(), body.end());
     }

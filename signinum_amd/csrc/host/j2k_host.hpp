@@ -11,6 +11,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace sgn {
@@ -66,6 +67,9 @@ struct MainHeader {
     bool mct = false;
     CompParams comp[kMaxComps];
     size_t first_sot = 0;
+    // PPM packets (codestream.rs:761-780, 164-177): (offset, length) in the
+    // codestream, the packet headers of tile-part 0, 1, ... in Zppm order
+    std::vector<std::pair<size_t, size_t>> ppm;
     uint32_t tiles_x() const { return ceil_div(xsiz - xto, xt); }
     uint32_t tiles_y() const { return ceil_div(ysiz - yto, yt); }
 };

@@ -185,7 +185,7 @@ int tag_decode(TagTree &t, uint32_t x, uint32_t y, uint32_t threshold, PacketBit
 
 struct CbState {
     uint32_t zblk = 0, lblock = 3, passes = 0, lcup = 0, lref = 0;
-    int64_t data = -1;  // offset of the block data within the tile body
+    int64_t data = -1;  // offset of the block data in the codestream
     bool included = false;
 };
 
@@ -214,9 +214,15 @@ Status read_main_header(const uint8_t *data, size_t len, MainHeader &h) {
         prec[i] = uint8_t((s & 0x7F) + 1);
         if (xr == 0 || yr == 0 || prec[i] > 32) return fail(kErrMalformed, "invalid component");
         if (xr != 1 || yr != 1) return fail(kErrUnsupported, "sub-sampled components");
-        if (s & 0x80) return fail(kErrUnsupported, "signed components");
+        // the signed flag (s & 0x80) changes nothing: the reference decodes
+        // signed components exactly as unsigned ones (codestream.rs:647-649)
     }
-    if (h.xo || h.yo || h.xto || h.yto) return fail(kErrUnsupported, "non-zero image origin");
+    // the first tile holds a reference grid point of the image (B-4,
+    // codestream.rs:582-596)
+    if (uint64_t(h.xto) + h.xt <= h.xo || uint64_t(h.yto) + h.yt <= h.yo || h.xto > h.xo ||
+        h.yto > h.yo)
+        return fail(kErrMalformed, "invalid tile-grid origin");
+    std::vector<std::pair<uint8_t, std::pair<size_t, size_t>>> ppm;
     bool have_cod = false, have_qcd = false;
     CompParams cod;
     uint8_t cod_scod = 0;
@@ -263,7 +269,17 @@ Status read_main_header(const uint8_t *data, size_t len, MainHeader &h) {
                 return fail(kErrMalformed, "invalid QCC");
             qcc_set[ci] = true;
         } else if (m == 0x60) {
-            return fail(kErrUnsupported, "PPM marker");
+            // PPM (codestream.rs:761-780): Zppm, then (16-bit length,
+            // bytes) pairs; empty pairs are dropped
+            uint32_t l, z, nb;
+            if (!b.u16(l) || l < 3 || b.pos + (l - 2) > len) return fail(kErrMalformed, "invalid PPM");
+            const size_t end = b.pos + (l - 2);
+            if (!b.u8(z)) return fail(kErrMalformed, "invalid PPM");
+            while (b.pos < end) {
+                if (!b.u16(nb) || b.pos + nb > end) return fail(kErrMalformed, "invalid PPM");
+                if (nb) ppm.push_back({uint8_t(z), {b.pos, size_t(nb)}});
+                b.pos += nb;
+            }
         } else if (m == 0x50 || m == 0x59 || m == 0x5E || m == 0x55 || m == 0x64 || m == 0x63) {
             if (!b.skip_segment()) return fail(kErrMalformed, "truncated marker segment");
         } else if (m >= 0x30 && m <= 0x3F) {
@@ -273,6 +289,9 @@ Status read_main_header(const uint8_t *data, size_t len, MainHeader &h) {
     }
     if (!have_cod || !have_qcd) return fail(kErrMalformed, "missing COD or QCD");
     h.first_sot = b.pos;
+    std::stable_sort(ppm.begin(), ppm.end(),
+                     [](const auto &a, const auto &c) { return a.first < c.first; });
+    for (const auto &e : ppm) h.ppm.push_back(e.second);
     for (uint32_t i = 0; i < csiz; ++i) {
         CompParams &c = h.comp[i];
         if (coc_set[i]) {
@@ -295,7 +314,8 @@ uint32_t skipped_levels(const MainHeader &h, uint32_t denominator) {
     if (denominator <= 1) return 0;
     const uint32_t tw = ceil_div(h.xsiz - h.xo, denominator);
     const uint32_t th = ceil_div(h.ysiz - h.yo, denominator);
-    const uint32_t wr = h.xsiz / tw, hr = h.ysiz / th;
+    // image_width / target (codestream.rs:149-156)
+    const uint32_t wr = (h.xsiz - h.xo) / tw, hr = (h.ysiz - h.yo) / th;
     const uint32_t wl = wr ? 31 - __builtin_clz(wr) : 0, hl = hr ? 31 - __builtin_clz(hr) : 0;
     uint32_t s = std::min(wl, hl);
     uint32_t min_nr = 255;
@@ -318,8 +338,10 @@ Status parse_tiles(const uint8_t *data, size_t len, const MainHeader &h,
         t.layers = h.layers;
     }
     ByteReader b{data, len, h.first_sot};
+    size_t tp_index = 0;  // tile-parts seen, in codestream order
     while (b.marker() == 0x90) {
         b.pos += 2;
+        std::vector<std::pair<uint8_t, std::pair<size_t, size_t>>> ppt;
         uint32_t lsot, isot, psot, tp, tn;
         if (!b.u16(lsot) || !b.u16(isot) || !b.u32(psot) || !b.u8(tp) || !b.u8(tn))
             return fail(kErrMalformed, "truncated SOT");
@@ -388,7 +410,13 @@ Status parse_tiles(const uint8_t *data, size_t len, const MainHeader &h,
             } else if (m == 0xD9) {
                 break;
             } else if (m == 0x61) {
-                return fail(kErrUnsupported, "PPT marker");
+                // PPT (tile.rs:251-258,697-706); not beside PPM
+                uint32_t l, z;
+                if (!h.ppm.empty()) return fail(kErrMalformed, "PPM and PPT in one codestream");
+                if (!b.u16(l) || l < 3 || b.pos + (l - 2) > len || !b.u8(z))
+                    return fail(kErrMalformed, "invalid PPT");
+                ppt.push_back({uint8_t(z), {b.pos, size_t(l - 3)}});
+                b.pos += l - 3;
             } else if (m == 0x58 || m == 0x64) {
                 if (!b.skip_segment()) return fail(kErrMalformed, "truncated marker segment");
             } else if (m >= 0x30 && m <= 0x3F) {
@@ -401,7 +429,15 @@ Status parse_tiles(const uint8_t *data, size_t len, const MainHeader &h,
         if (used > dlen) return kOk;
         const size_t remain = dlen - used;
         if (b.pos + remain > len) return fail(kErrMalformed, "tile-part exceeds codestream");
-        t.parts.push_back({b.pos, remain});
+        TilePart part;
+        part.off = b.pos;
+        part.len = remain;
+        std::stable_sort(ppt.begin(), ppt.end(),
+                         [](const auto &a, const auto &c) { return a.first < c.first; });
+        for (const auto &e : ppt) part.hdr.push_back(e.second);
+        if (tp_index < h.ppm.size()) part.hdr.push_back(h.ppm[tp_index]);
+        ++tp_index;
+        t.parts.push_back(std::move(part));
         b.pos += remain;
     }
     return kOk;
@@ -409,33 +445,17 @@ Status parse_tiles(const uint8_t *data, size_t len, const MainHeader &h,
 
 // Packet headers of one tile (HT, single layer).  Fills the code-block
 // states in TileCompGeom::cbs order per component; offsets are relative to
-// the codestream start.  Non-strict: a malformed packet stops parsing and
-// leaves the remaining blocks empty (segment.rs:36-50).
+// the codestream start.  Tile-parts are read one after another with the
+// progression continuing (segment.rs:36-133): while the header stream is not
+// at its end, the next packet; an SOP marker in the body is skipped; the
+// header comes from the body, or with packed headers from the tile-part's
+// header pieces (the next piece once the current one is exhausted,
+// tile.rs:55-67); the header is aligned, followed by EPH when signalled,
+// and the included blocks' bytes come from the body.  Non-strict: a
+// malformed packet ends its tile-part; the next tile-part continues
+// (segment.rs:36-50).
 Status parse_packets(const uint8_t *cs, const TileBody &t, int ncomp, const TileCompGeom *tcs,
                      std::vector<std::vector<CbSeg>> &segs) {
-    // gather tile-part bodies into one contiguous view when needed
-    std::vector<uint8_t> joined;
-    const uint8_t *body;
-    size_t blen;
-    size_t single_off = 0;
-    if (t.parts.size() == 1) {
-        body = cs + t.parts[0].first;
-        blen = t.parts[0].second;
-        single_off = t.parts[0].first;
-    } else {
-        for (auto &p : t.parts) joined.insert(joined.end(), cs + p.first, cs + p.first + p.second);
-        body = joined.data();
-        blen = joined.size();
-    }
-    auto body_to_cs = [&](size_t off) -> int64_t {
-        if (t.parts.size() == 1) return int64_t(single_off + off);
-        size_t acc = 0;
-        for (auto &p : t.parts) {
-            if (off < acc + p.second) return int64_t(p.first + (off - acc));
-            acc += p.second;
-        }
-        return -1;
-    };
     std::vector<PacketId> order;
     Status st = progression(t.prog, t.layers, t.rect, tcs, ncomp, order);
     if (st) return st;
@@ -451,117 +471,138 @@ Status parse_packets(const uint8_t *cs, const TileBody &t, int ncomp, const Tile
             zbp[c][p].init(tcs[c].precs[p].ncbx, tcs[c].precs[p].ncby);
         }
     }
-    PacketBits r{body, blen, 0};
-    bool ok = true;
-    for (const PacketId &pk : order) {
-        if ((r.bit >> 3) >= blen) break;
-        const TileCompGeom &g = tcs[pk.comp];
-        const ResGeom &rs = g.res[pk.res];
-        const CompParams &cp = t.comp[pk.comp];
-        size_t byte = r.bit >> 3;
-        if ((cp.scod & 2) && byte + 1 < blen && body[byte] == 0xFF && body[byte + 1] == 0x91) {
-            if (byte + 6 > blen) { ok = false; break; }
-            r.bit += 48;
-        }
-        uint32_t nonzero;
-        if (!r.get(nonzero)) { ok = false; break; }
-        if (nonzero) {
-            for (uint32_t bi = 0; bi < rs.nbands && ok; ++bi) {
-                const uint32_t pi = rs.prec_base + pk.prec * rs.nbands + bi;
-                const PrecBand &pb = g.precs[pi];
-                for (uint32_t k = 0; k < pb.ncbx * pb.ncby && ok; ++k) {
-                    CbState &s = state[pk.comp][pb.cb_base + k];
-                    const CbGeom &cg = g.cbs[pb.cb_base + k];
-                    const bool first = !s.included;
-                    uint32_t inc, v;
-                    if (s.included) {
-                        if (!r.get(inc)) { ok = false; break; }
-                    } else {
-                        const int rc = tag_decode(incl[pk.comp][pi], cg.xi, cg.yi, pk.layer + 1u, r, v);
-                        if (rc < 0) { ok = false; break; }
-                        inc = v <= pk.layer;
-                    }
-                    if (!inc) continue;
-                    if (first) {
-                        uint32_t thr = 1;
-                        for (;;) {
-                            const int rc = tag_decode(zbp[pk.comp][pi], cg.xi, cg.yi, thr, r, v);
+    size_t next = 0;  // next packet of the progression
+    for (const TilePart &tp : t.parts) {
+        const uint8_t *body = cs + tp.off;
+        const size_t blen = tp.len;
+        const bool packed = !tp.hdr.empty();
+        size_t hi = 0;
+        PacketBits r = packed ? PacketBits{cs + tp.hdr[0].first, tp.hdr[0].second, 0}
+                              : PacketBits{body, blen, 0};
+        size_t bpos = 0;  // body byte position (packed headers)
+        for (;;) {
+            if (packed && (r.bit >> 3) >= r.len && hi + 1 < tp.hdr.size()) {
+                ++hi;
+                r = PacketBits{cs + tp.hdr[hi].first, tp.hdr[hi].second, 0};
+            }
+            if ((r.bit >> 3) >= r.len) break;
+            if (next >= order.size()) break;
+            const PacketId &pk = order[next++];
+            const TileCompGeom &g = tcs[pk.comp];
+            const ResGeom &rs = g.res[pk.res];
+            const CompParams &cp = t.comp[pk.comp];
+            bool ok = true;
+            {
+                const size_t at = packed ? bpos : (r.bit >> 3);
+                if ((cp.scod & 2) && at + 1 < blen && body[at] == 0xFF && body[at + 1] == 0x91) {
+                    if (at + 6 > blen) break;
+                    if (packed)
+                        bpos += 6;
+                    else
+                        r.bit += 48;
+                }
+            }
+            uint32_t nonzero;
+            if (!r.get(nonzero)) break;
+            if (nonzero) {
+                for (uint32_t bi = 0; bi < rs.nbands && ok; ++bi) {
+                    const uint32_t pi = rs.prec_base + pk.prec * rs.nbands + bi;
+                    const PrecBand &pb = g.precs[pi];
+                    for (uint32_t k = 0; k < pb.ncbx * pb.ncby && ok; ++k) {
+                        CbState &s = state[pk.comp][pb.cb_base + k];
+                        const CbGeom &cg = g.cbs[pb.cb_base + k];
+                        const bool first = !s.included;
+                        uint32_t inc, v;
+                        if (s.included) {
+                            if (!r.get(inc)) { ok = false; break; }
+                        } else {
+                            const int rc = tag_decode(incl[pk.comp][pi], cg.xi, cg.yi, pk.layer + 1u, r, v);
                             if (rc < 0) { ok = false; break; }
-                            if (rc) break;
-                            if (++thr > 255) { ok = false; break; }
+                            inc = v <= pk.layer;
                         }
-                        if (!ok) break;
-                        s.zblk = v;
-                    }
-                    s.included = true;
-                    if (!first || s.passes != 0) { ok = false; break; }
-                    // HT pass count codeword (segment.rs:508-530)
-                    uint32_t np = 1, b1;
-                    if (!r.get(b1)) { ok = false; break; }
-                    if (b1) {
-                        np = 2;
+                        if (!inc) continue;
+                        if (first) {
+                            uint32_t thr = 1;
+                            for (;;) {
+                                const int rc = tag_decode(zbp[pk.comp][pi], cg.xi, cg.yi, thr, r, v);
+                                if (rc < 0) { ok = false; break; }
+                                if (rc) break;
+                                if (++thr > 255) { ok = false; break; }
+                            }
+                            if (!ok) break;
+                            s.zblk = v;
+                        }
+                        s.included = true;
+                        if (!first || s.passes != 0) { ok = false; break; }
+                        // HT pass count codeword (segment.rs:508-530)
+                        uint32_t np = 1, b1;
                         if (!r.get(b1)) { ok = false; break; }
                         if (b1) {
-                            uint32_t e;
-                            if (!r.bits(2, e)) { ok = false; break; }
-                            np = 3 + e;
-                            if (e == 3) {
-                                if (!r.bits(5, e)) { ok = false; break; }
-                                np = 6 + e;
-                                if (e == 31) {
-                                    if (!r.bits(7, e)) { ok = false; break; }
-                                    np = 37 + e;
+                            np = 2;
+                            if (!r.get(b1)) { ok = false; break; }
+                            if (b1) {
+                                uint32_t e;
+                                if (!r.bits(2, e)) { ok = false; break; }
+                                np = 3 + e;
+                                if (e == 3) {
+                                    if (!r.bits(5, e)) { ok = false; break; }
+                                    np = 6 + e;
+                                    if (e == 31) {
+                                        if (!r.bits(7, e)) { ok = false; break; }
+                                        np = 37 + e;
+                                    }
                                 }
                             }
                         }
+                        if (np > 94) { ok = false; break; }
+                        // placeholder passes and lengths (segment.rs:542-581)
+                        const uint32_t groups = (np - 1) / 3;
+                        s.zblk += groups;
+                        const uint32_t ph = groups * 3, actual = np - ph;
+                        for (;;) {
+                            uint32_t bb;
+                            if (!r.get(bb)) { ok = false; break; }
+                            if (!bb) break;
+                            ++s.lblock;
+                        }
+                        if (!ok) break;
+                        const uint32_t nb = s.lblock + (31 - __builtin_clz(ph + 1));
+                        uint32_t lc = 0, lr = 0;
+                        if (nb > 32 || !r.bits(nb, lc) || lc < 2 || lc >= 65535) { ok = false; break; }
+                        if (actual > 1) {
+                            const uint32_t nb2 = s.lblock + (actual > 2 ? 1 : 0);
+                            if (nb2 > 32 || !r.bits(nb2, lr) || lr >= 2047) { ok = false; break; }
+                        }
+                        s.passes = actual;
+                        s.lcup = lc;
+                        s.lref = lr;
                     }
-                    if (np > 94) { ok = false; break; }
-                    // placeholder passes and lengths (segment.rs:542-581)
-                    const uint32_t groups = (np - 1) / 3;
-                    s.zblk += groups;
-                    const uint32_t ph = groups * 3, actual = np - ph;
-                    for (;;) {
-                        uint32_t bb;
-                        if (!r.get(bb)) { ok = false; break; }
-                        if (!bb) break;
-                        ++s.lblock;
-                    }
-                    if (!ok) break;
-                    const uint32_t nb = s.lblock + (31 - __builtin_clz(ph + 1));
-                    uint32_t lc = 0, lr = 0;
-                    if (nb > 32 || !r.bits(nb, lc) || lc < 2 || lc >= 65535) { ok = false; break; }
-                    if (actual > 1) {
-                        const uint32_t nb2 = s.lblock + (actual > 2 ? 1 : 0);
-                        if (nb2 > 32 || !r.bits(nb2, lr) || lr >= 2047) { ok = false; break; }
-                    }
-                    s.passes = actual;
-                    s.lcup = lc;
-                    s.lref = lr;
                 }
             }
-        }
-        if (!ok) break;
-        if (r.bit & 7) r.bit += 8 - (r.bit & 7);
-        if (cp.scod & 4) {
-            const size_t bb = r.bit >> 3;
-            if (bb + 2 > blen || body[bb] != 0xFF || body[bb + 1] != 0x92) { ok = false; break; }
-            r.bit += 16;
-        }
-        if (nonzero) {
-            for (uint32_t bi = 0; bi < rs.nbands && ok; ++bi) {
-                const PrecBand &pb = g.precs[rs.prec_base + pk.prec * rs.nbands + bi];
-                for (uint32_t k = 0; k < pb.ncbx * pb.ncby; ++k) {
-                    CbState &s = state[pk.comp][pb.cb_base + k];
-                    if (s.passes == 0 || s.data >= 0) continue;
-                    const size_t bb = r.bit >> 3;
-                    const size_t need = size_t(s.lcup) + s.lref;
-                    if (bb + need > blen) { ok = false; break; }
-                    s.data = int64_t(bb);
-                    r.bit += need * 8;
+            if (!ok) break;
+            if (r.bit & 7) r.bit += 8 - (r.bit & 7);
+            if (cp.scod & 4) {
+                const size_t bb = r.bit >> 3;
+                if (bb + 2 > r.len || r.d[bb] != 0xFF || r.d[bb + 1] != 0x92) break;
+                r.bit += 16;
+            }
+            if (!packed) bpos = r.bit >> 3;
+            if (nonzero) {
+                for (uint32_t bi = 0; bi < rs.nbands && ok; ++bi) {
+                    const PrecBand &pb = g.precs[rs.prec_base + pk.prec * rs.nbands + bi];
+                    for (uint32_t k = 0; k < pb.ncbx * pb.ncby; ++k) {
+                        CbState &s = state[pk.comp][pb.cb_base + k];
+                        if (s.passes == 0 || s.data >= 0) continue;
+                        const size_t need = size_t(s.lcup) + s.lref;
+                        if (bpos + need > blen) { ok = false; break; }
+                        s.data = int64_t(tp.off + bpos);
+                        bpos += need;
+                    }
                 }
             }
+            if (!ok) break;
+            if (!packed) r.bit = bpos * 8;
         }
-        if (!ok) break;
     }
     for (int c = 0; c < ncomp; ++c) {
         segs[c].resize(state[c].size());
@@ -572,14 +613,7 @@ Status parse_packets(const uint8_t *cs, const TileBody &t, int ncomp, const Tile
             o.zblk = s.zblk;
             o.lcup = s.lcup;
             o.lref = s.lref;
-            o.offset = -1;
-            if (s.data >= 0) {
-                // a block split across tile-parts is copied by the caller
-                const int64_t a = body_to_cs(size_t(s.data));
-                const int64_t e = body_to_cs(size_t(s.data) + s.lcup + s.lref - 1);
-                if (a >= 0 && e == a + int64_t(s.lcup + s.lref - 1)) o.offset = a;
-                else o.passes = 0;
-            }
+            o.offset = s.data >= 0 ? s.data : -1;
         }
     }
     return kOk;

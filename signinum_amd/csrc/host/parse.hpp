@@ -8,13 +8,21 @@
 
 namespace sgn {
 
+// One tile-part: the body and, when packet headers are packed, the header
+// pieces (PPT segments of the tile-part in Zppt order, then its PPM packet;
+// tile.rs:291-309), all as (offset, length) in the codestream.
+struct TilePart {
+    size_t off = 0, len = 0;
+    std::vector<std::pair<size_t, size_t>> hdr;
+};
+
 struct TileBody {
     Rect rect;
     CompParams comp[kMaxComps];
     uint8_t prog = 0;
     bool mct = false;
     uint16_t layers = 1;
-    std::vector<std::pair<size_t, size_t>> parts;  // (offset, length) of tile-part bodies
+    std::vector<TilePart> parts;
 };
 
 struct CbSeg {

@@ -42,8 +42,17 @@ Status output_geometry(const MainHeader &h, int fmt, uint32_t denominator, Outpu
         return kErrUnsupported;
     }
     const uint32_t sk = skipped_levels(h, denominator);
-    g.width = ceil_div(h.xsiz, 1u << sk);
-    g.height = ceil_div(h.ysiz, 1u << sk);
+    // The reference's store places reduced-resolution tiles by subtracting
+    // the full-resolution image offset from reduced coordinates
+    // (decode.rs:1762-1790): with a non-zero image origin only full
+    // resolution has a well-defined output.
+    if (sk && (h.xo || h.yo)) {
+        set_last_error("reduced resolution of an image with a non-zero origin");
+        return kErrUnsupported;
+    }
+    // image_width / image_height (codestream.rs:544-553)
+    g.width = ceil_div(h.xsiz - h.xo, 1u << sk);
+    g.height = ceil_div(h.ysiz - h.yo, 1u << sk);
     g.ncomp = h.ncomp;
     g.channels = uint32_t(ch);
     g.bytes_per_sample = (fmt == SGN_GRAY16 || fmt == SGN_RGB16) ? 2 : 1;
@@ -254,6 +263,12 @@ Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denomin
             }
             fj.precision[c < 4 ? c : 3] = cp.precision;
         }
+        // image coordinates: the reference grid minus the image origin
+        // (full resolution whenever the origin is not zero)
+        out_rect.x0 -= h.xo;
+        out_rect.x1 -= h.xo;
+        out_rect.y0 -= h.yo;
+        out_rect.y1 -= h.yo;
         if (out_rect.x1 > og.width || out_rect.y1 > og.height) {
             set_last_error("tile outside the output image");
             return kErrMalformed;
