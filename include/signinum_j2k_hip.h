@@ -31,6 +31,7 @@ extern "C" {
 #define SGN_E_BUFFER (-5)      /* output buffer too small */
 #define SGN_E_NOMEM (-6)
 #define SGN_E_DEVICE (-7)      /* HIP runtime failure / device unavailable */
+#define SGN_E_REGION (-8)      /* J2kError::InvalidRegion: region outside the image */
 
 /* PixelFormat (crates/signinum-core/src/pixel.rs:13-58). */
 typedef enum {
@@ -143,6 +144,27 @@ int sgn_j2k_decode_into(sgn_hip_session *session, const uint8_t *bytes, size_t l
 int sgn_j2k_decode_device(sgn_hip_session *session, const uint8_t *bytes, size_t len, int fmt,
                           uint32_t denominator, void *dst, size_t dst_pitch, size_t dst_capacity);
 
+/* ---- regions (J2kDecoder::decode_region_into / decode_region_scaled_into,
+ * crates/signinum-j2k/src/view.rs:215-311; decode.rs:36-71) --------------
+ * The region is in full-resolution image pixels; at 1/denominator the
+ * output is its covering rectangle, x / d .. ceil((x + w) / d).  A region
+ * outside the image gives SGN_E_REGION.  Only the code-blocks and synthesis
+ * blocks the region's pixels depend on are decoded, and tiles outside it
+ * are skipped. */
+int sgn_j2k_region_dims(const uint8_t *bytes, size_t len, uint32_t denominator, uint32_t x,
+                        uint32_t y, uint32_t w, uint32_t h, uint32_t *out_x, uint32_t *out_y,
+                        uint32_t *out_w, uint32_t *out_h);
+/* Decodes the region into host memory `out` (row stride in bytes). */
+int sgn_j2k_decode_region_into(sgn_hip_session *session, const uint8_t *bytes, size_t len,
+                               int fmt, uint32_t denominator, uint32_t x, uint32_t y, uint32_t w,
+                               uint32_t h, uint8_t *out, size_t out_len, size_t stride);
+/* Decodes the region into device memory `dst` (pitch in bytes) on the
+ * session stream; asynchronous. */
+int sgn_j2k_decode_region_device(sgn_hip_session *session, const uint8_t *bytes, size_t len,
+                                 int fmt, uint32_t denominator, uint32_t x, uint32_t y,
+                                 uint32_t w, uint32_t h, void *dst, size_t pitch,
+                                 size_t capacity);
+
 /* ---- tile batches --------------------------------------------------------
  * TileDecodeJob (crates/signinum-j2k/src/batch.rs:24-33) with a device
  * destination. */
@@ -202,9 +224,9 @@ typedef struct sgn_j2k_plan sgn_j2k_plan;
  * (optional, n entries) receives each tile's parse status. */
 int sgn_j2k_plan_create(sgn_hip_session *session, const sgn_tile_job *jobs, size_t n, int fmt,
                         uint32_t denominator, sgn_j2k_plan **out, int32_t *tile_status);
-/* One request of a heterogeneous batch: per-tile pixel format and scale
- * (TileRegionScaledDecodeJob, crates/signinum-j2k/src/batch.rs:35-47, over
- * the full tile; the request record of the device plan). */
+/* One request of a heterogeneous batch: per-tile pixel format, scale and
+ * region (TileRegionScaledDecodeJob, crates/signinum-j2k/src/batch.rs:34-46;
+ * the request record of the device plan). */
 typedef struct {
     const uint8_t *bytes;
     size_t len;
@@ -213,8 +235,20 @@ typedef struct {
     size_t capacity;      /* bytes available at dst */
     int32_t fmt;          /* sgn_pixel_format */
     uint32_t denominator; /* 1, 2, 4 or 8 */
+    /* has_roi != 0: decode only the region [roi_x, roi_x + roi_w) x [roi_y,
+     * roi_y + roi_h) of the full-resolution image; the output is the
+     * region's covering rectangle at the requested scale
+     * (Rect::scaled_covering), see sgn_j2k_region_dims */
+    uint32_t has_roi;
+    uint32_t roi_x, roi_y, roi_w, roi_h;
 } sgn_tile_req;
-/* As sgn_j2k_plan_create, with format and scale per tile. */
+/* decode_tiles_region_scaled_into (crates/signinum-j2k/src/batch.rs:
+ * 151-195): every request's region (or whole tile) at its scale and format,
+ * one batched plan; *first_fail as for sgn_j2k_decode_tiles_device.
+ * Synchronous. */
+int sgn_j2k_decode_tiles_region_device(sgn_hip_session *session, const sgn_tile_req *reqs,
+                                       size_t n, size_t *first_fail);
+/* As sgn_j2k_plan_create, with format, scale and region per tile. */
 int sgn_j2k_plan_create_mixed(sgn_hip_session *session, const sgn_tile_req *reqs, size_t n,
                               sgn_j2k_plan **out, int32_t *tile_status);
 void sgn_j2k_plan_destroy(sgn_j2k_plan *plan);
@@ -264,6 +298,9 @@ typedef struct {
      * merge into the batch tables, staging fill, copy submission (until the
      * copies completed, for a synchronous build) */
     float host_parse_ms, host_merge_ms, host_stage_ms, host_copy_ms;
+    /* code-blocks of the retained resolutions that region decodes skip
+     * (outside every region's dependencies, or in tiles outside it) */
+    uint64_t pruned_code_blocks;
 } sgn_j2k_plan_stats;
 int sgn_j2k_plan_get_stats(const sgn_j2k_plan *plan, sgn_j2k_plan_stats *stats);
 /* Output geometry of tile i: width, height, bytes per row (unpadded). */

@@ -24,7 +24,8 @@ __all__ = [
     "UnsupportedBackendError", "HipSession", "HipEvent", "J2kDecoder", "DeviceSurface", "TilePlan",
     "decode_tiles_into", "decode_tiles_to_device", "encode_j2k_lossless", "encode_j2k_ht",
     "HtCodeBlockDecoder", "code_block_jobs", "CostModel", "MODE_PLAIN", "MODE_BATCHED",
-    "MODE_GRAPH", "extract_dicom_encapsulated_frames",
+    "MODE_GRAPH", "extract_dicom_encapsulated_frames", "InvalidRegionError", "region_dims",
+    "decode_tiles_region_scaled_into",
 ]
 
 
@@ -101,6 +102,10 @@ class UnsupportedBackendError(J2kError):
     """Error::UnsupportedBackend (crates/signinum-j2k-cuda/src/error.rs:6-41)."""
 
 
+class InvalidRegionError(J2kError):
+    """J2kError::InvalidRegion (crates/signinum-j2k/src/error.rs:24-31)."""
+
+
 class TileBatchError(Exception):
     """TileBatchError (crates/signinum-j2k/src/batch.rs:47-67)."""
 
@@ -116,6 +121,7 @@ _ERRORS = {
     _lib.SGN_E_BLOCK: CodeBlockDecodeError,
     _lib.SGN_E_BUFFER: BufferError,
     _lib.SGN_E_DEVICE: DeviceError,
+    _lib.SGN_E_REGION: InvalidRegionError,
     _lib.SGN_E_NOMEM: J2kError,
     _lib.SGN_E_ARG: J2kError,
 }
@@ -349,6 +355,17 @@ def extract_dicom_encapsulated_frames(data, limit=None):
     return [out[offs[i]:offs[i] + lens[i]].tobytes() for i in range(k)]
 
 
+def region_dims(data, roi, scale=Downscale.NONE):
+    """(x, y, w, h) of the output of a region decode: Rect::scaled_covering
+    (crates/signinum-core/src/types.rs:69-83) of roi = (x, y, w, h)."""
+    ptr, n, _keep = _buf(data)
+    o = [ctypes.c_uint32() for _ in range(4)]
+    x, y, w, h = (int(v) for v in roi)
+    _check(_lib.load().sgn_j2k_region_dims(ptr, n, int(Downscale(scale)), x, y, w, h,
+                                           *[ctypes.byref(v) for v in o]))
+    return tuple(v.value for v in o)
+
+
 class J2kDecoder:
     """J2kDecoder (crates/signinum-j2k/src/view.rs:84-311; device adapter
     crates/signinum-j2k-cuda/src/decoder.rs:17-280)."""
@@ -413,6 +430,35 @@ class J2kDecoder:
                                                  surf.byte_len))
         return surf
 
+    def decode_region_into(self, pool, out, stride, fmt, roi):
+        """decode_region_into (crates/signinum-j2k/src/view.rs:224-256)."""
+        return self.decode_region_scaled_into(pool, out, stride, fmt, roi, Downscale.NONE)
+
+    def decode_region_scaled_into(self, pool, out, stride, fmt, roi, scale):
+        """decode_region_scaled_into (crates/signinum-j2k/src/view.rs:289-311):
+        the region's covering rectangle at 1/scale into host `out`."""
+        fmt = PixelFormat(fmt)
+        scale = Downscale(scale)
+        dst = np.asarray(out)
+        if not dst.flags["C_CONTIGUOUS"] or not dst.flags["WRITEABLE"]:
+            raise BufferError(_lib.SGN_E_BUFFER, "output must be a writable contiguous array")
+        s = self._sess()
+        ptr, n, _keep = _buf(self._data)
+        x, y, w, h = (int(v) for v in roi)
+        _check(_lib.load().sgn_j2k_decode_region_into(s.handle, ptr, n, int(fmt), int(scale),
+                                                      x, y, w, h, dst.ctypes.data, dst.nbytes,
+                                                      int(stride)))
+        return {"decoded": region_dims(self._data, roi, scale)}
+
+    def decode_region_to_numpy(self, fmt, roi, scale=Downscale.NONE):
+        fmt = PixelFormat(fmt)
+        _x, _y, w, h = region_dims(self._data, roi, scale)
+        out = np.zeros(max(1, h * w * fmt.bytes_per_pixel), np.uint8)
+        self.decode_region_scaled_into(None, out, w * fmt.bytes_per_pixel, fmt, roi, scale)
+        out = out[:h * w * fmt.bytes_per_pixel]
+        arr = out.view("<u2") if fmt.bytes_per_sample == 2 else out
+        return arr.reshape(h, w, fmt.channels)
+
     def decode_to_numpy(self, fmt, scale=Downscale.NONE):
         fmt = PixelFormat(fmt)
         w, h = scaled_dims(self._data, scale)
@@ -431,14 +477,15 @@ class TilePlan:
     `fmt` and `scale` are one value for the whole batch or one per tile
     (a heterogeneous batch of formats and reduced resolutions)."""
 
-    def __init__(self, session, inputs, fmt, scale=Downscale.NONE, surfaces=None):
+    def __init__(self, session, inputs, fmt, scale=Downscale.NONE, surfaces=None, rois=None):
         self.session = session
         self._inputs = [bytes(b) for b in inputs]
         n = len(self._inputs)
         fmts = list(fmt) if isinstance(fmt, (list, tuple)) else [fmt] * n
         scales = list(scale) if isinstance(scale, (list, tuple)) else [scale] * n
-        if len(fmts) != n or len(scales) != n:
-            raise ValueError("one format and one scale per tile")
+        self.rois = list(rois) if rois is not None else [None] * n
+        if len(fmts) != n or len(scales) != n or len(self.rois) != n:
+            raise ValueError("one format, scale and region per tile")
         self.formats = [PixelFormat(f) for f in fmts]
         self.scales = [Downscale(s) for s in scales]
         self.fmt = self.formats[0] if n else PixelFormat(fmt if not isinstance(fmt, (list, tuple)) else 0)
@@ -448,15 +495,21 @@ class TilePlan:
             dims = []
             for i, b in enumerate(self._inputs):
                 try:
-                    dims.append(scaled_dims(b, self.scales[i]))
+                    if self.rois[i] is None:
+                        dims.append(scaled_dims(b, self.scales[i]))
+                    else:
+                        dims.append(region_dims(b, self.rois[i], self.scales[i])[2:])
                 except J2kError as e:
                     raise TileBatchError(i, e)
             surfaces = _alloc_surfaces(session, dims, self.formats)
         self.surfaces = surfaces
         reqs = (_lib.TileReq * max(1, n))()
         for i, (k, s) in enumerate(zip(self._keep, surfaces)):
+            roi = self.rois[i]
             reqs[i] = _lib.TileReq(k.ctypes.data, k.size, s.device_ptr, s.pitch_bytes, s.byte_len,
-                                   int(self.formats[i]), int(self.scales[i]))
+                                   int(self.formats[i]), int(self.scales[i]),
+                                   0 if roi is None else 1,
+                                   *((0, 0, 0, 0) if roi is None else (int(v) for v in roi)))
         status = (ctypes.c_int32 * max(1, n))()
         h = ctypes.c_void_p()
         rc = _lib.load().sgn_j2k_plan_create_mixed(session.handle, reqs, n, ctypes.byref(h), status)
@@ -648,6 +701,43 @@ def decode_tiles_to_device(session, inputs, fmt, scale=Downscale.NONE, mode=MODE
     if rc != _lib.SGN_OK:
         raise TileBatchError(ff.value, _ERRORS.get(rc, J2kError)(rc, _lib.last_error()))
     return surfaces
+
+
+def decode_tiles_region_scaled_into(jobs, fmt, session=None):
+    """decode_tiles_region_scaled_into (crates/signinum-j2k/src/batch.rs:
+    151-195): jobs are (input_bytes, out_array, stride, roi, scale) with roi
+    = (x, y, w, h) in source pixels; every region is decoded in one batch and
+    written to its array.  Raises TileBatchError with the first failing
+    index."""
+    fmt = PixelFormat(fmt)
+    own = session is None
+    if own:
+        session = HipSession()
+    try:
+        keep, dims = [], []
+        for i, (inp, _out, _stride, roi, scale) in enumerate(jobs):
+            keep.append(np.frombuffer(bytes(inp), np.uint8))
+            try:
+                dims.append(region_dims(keep[-1], roi, scale)[2:])
+            except J2kError as e:
+                raise TileBatchError(i, e)
+        surfs = _alloc_surfaces(session, dims, [fmt] * len(jobs))
+        n = len(jobs)
+        reqs = (_lib.TileReq * max(1, n))()
+        for i, ((_inp, _out, _stride, roi, scale), k, s) in enumerate(zip(jobs, keep, surfs)):
+            reqs[i] = _lib.TileReq(k.ctypes.data, k.size, s.device_ptr, s.pitch_bytes, s.byte_len,
+                                   int(fmt), int(Downscale(scale)), 1, *(int(v) for v in roi))
+        ff = ctypes.c_size_t(0)
+        rc = _lib.load().sgn_j2k_decode_tiles_region_device(session.handle, reqs, n,
+                                                            ctypes.byref(ff))
+        if rc != _lib.SGN_OK:
+            raise TileBatchError(ff.value, _ERRORS.get(rc, J2kError)(rc, _lib.last_error()))
+        for (_inp, out, stride, _roi, _scale), s in zip(jobs, surfs):
+            if s.byte_len:
+                s.download_into(out, stride)
+    finally:
+        if own:
+            session.close()
 
 
 def decode_tiles_into(jobs, fmt, session=None, scale=Downscale.NONE, mode=MODE_BATCHED):

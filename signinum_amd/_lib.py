@@ -17,6 +17,7 @@ SGN_E_BLOCK = -4
 SGN_E_BUFFER = -5
 SGN_E_NOMEM = -6
 SGN_E_DEVICE = -7
+SGN_E_REGION = -8
 
 MODE_PLAIN = 0
 MODE_BATCHED = 1
@@ -45,7 +46,9 @@ class TileJob(ctypes.Structure):
 class TileReq(ctypes.Structure):
     _fields_ = [("bytes", ctypes.c_void_p), ("len", c_size_t), ("dst", ctypes.c_void_p),
                 ("pitch", c_size_t), ("capacity", c_size_t), ("fmt", ctypes.c_int32),
-                ("denominator", ctypes.c_uint32)]
+                ("denominator", ctypes.c_uint32), ("has_roi", ctypes.c_uint32),
+                ("roi_x", ctypes.c_uint32), ("roi_y", ctypes.c_uint32),
+                ("roi_w", ctypes.c_uint32), ("roi_h", ctypes.c_uint32)]
 
 
 class ServerConfig(ctypes.Structure):
@@ -76,7 +79,8 @@ class PlanStats(ctypes.Structure):
                 ("output_pixels", ctypes.c_uint64), ("coefficient_bytes", ctypes.c_uint64),
                 ("launches_batched", ctypes.c_uint32), ("narrow_samples", ctypes.c_uint32),
                 ("host_parse_ms", ctypes.c_float), ("host_merge_ms", ctypes.c_float),
-                ("host_stage_ms", ctypes.c_float), ("host_copy_ms", ctypes.c_float)]
+                ("host_stage_ms", ctypes.c_float), ("host_copy_ms", ctypes.c_float),
+                ("pruned_code_blocks", ctypes.c_uint64)]
 
 
 class HtCodeBlockJob(ctypes.Structure):
@@ -144,6 +148,17 @@ SIGNATURES = {
                                                            ctypes.c_int, ctypes.c_uint32,
                                                            ctypes.POINTER(c_size_t)]),
     "sgn_j2k_noalloc_scratch_bytes": (ctypes.c_uint64, []),
+    "sgn_j2k_region_dims": (ctypes.c_int, [ctypes.c_void_p, c_size_t] + [ctypes.c_uint32] * 5 +
+                            [ctypes.POINTER(ctypes.c_uint32)] * 4),
+    "sgn_j2k_decode_region_into": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, c_size_t,
+                                                  ctypes.c_int] + [ctypes.c_uint32] * 5 +
+                                   [ctypes.c_void_p, c_size_t, c_size_t]),
+    "sgn_j2k_decode_region_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, c_size_t,
+                                                    ctypes.c_int] + [ctypes.c_uint32] * 5 +
+                                     [ctypes.c_void_p, c_size_t, c_size_t]),
+    "sgn_j2k_decode_tiles_region_device": (ctypes.c_int, [ctypes.c_void_p,
+                                                          ctypes.POINTER(TileReq), c_size_t,
+                                                          ctypes.POINTER(c_size_t)]),
     "sgn_dicom_extract_frames": (ctypes.c_int, [ctypes.c_void_p, c_size_t, c_size_t,
                                                 ctypes.c_void_p, c_size_t,
                                                 ctypes.POINTER(c_size_t),

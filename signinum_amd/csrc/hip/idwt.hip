@@ -771,7 +771,7 @@ __device__ __forceinline__ void level_body(const LevelJob *jobs, uint32_t njobs,
     const LevelView L = level_view(lj, coef, &sb);
     const uint32_t local = blk - lj->block_base;
     const uint32_t nbx = lj->nbx, out = lj->out;
-    const int bx = (int)(local % nbx), by = (int)(local / nbx);
+    const int bx = (int)(lj->bx0 + local % nbx), by = (int)(lj->by0 + local / nbx);
     int bw, bh;
     const uint32_t opitch = coef_pitch((uint32_t)L.w);
     const int bh_blk = min(kBlk, L.h - by * kBlk);
@@ -864,7 +864,12 @@ __device__ __forceinline__ void final_body(const FinalJob *jobs, uint32_t njobs,
     const FinalJob *fj = job_of(jobs, njobs, block_job, blk);
     const uint32_t local = blk - fj->block_base;
     const uint32_t nbx = fj->nbx;
-    const int bx = (int)(local % nbx), by = (int)(local / nbx);
+    const int bx = (int)(fj->bx0 + local % nbx), by = (int)(fj->by0 + local / nbx);
+    // pixels kept, relative to the block: a region decode writes only
+    // [cx0, cx1) x [cy0, cy1) of the tile, pixel by pixel
+    const int kx0 = (int)fj->cx0 - bx * kBlk, kx1 = (int)fj->cx1 - bx * kBlk;
+    const int ky0 = (int)fj->cy0 - by * kBlk, ky1 = (int)fj->cy1 - by * kBlk;
+    const bool cropped = kx0 > 0 || ky0 > 0 || kx1 < kBlk || ky1 < kBlk;
     FinalView F;
     F.nc = fj->ncomp > 3 ? 3 : fj->ncomp;
     F.fmt = fj->fmt;
@@ -908,7 +913,8 @@ __device__ __forceinline__ void final_body(const FinalJob *jobs, uint32_t njobs,
         __syncthreads();
     }
     // Each thread converts and stores four horizontally adjacent pixels.
-    const bool words = F.bps == 1 && ((reinterpret_cast<uintptr_t>(F.dst) | F.pitch) & 3) == 0;
+    const bool words =
+        !cropped && F.bps == 1 && ((reinterpret_cast<uintptr_t>(F.dst) | F.pitch) & 3) == 0;
     const bool int8_out = REV && F.bps == 1 && F.same8;
     // 5/3 with RCT into 8-bit RGB (the common case), integers throughout:
     // R = Cr + I, G = I, B = Cb + I with I = Y - ((Cb + Cr) >> 2), each
@@ -958,6 +964,7 @@ __device__ __forceinline__ void final_body(const FinalJob *jobs, uint32_t njobs,
     for (int k = threadIdx.x; k < (kBlk / 4) * bh; k += kIdwtThreads) {
         const int y = k / (kBlk / 4), x4 = (k % (kBlk / 4)) * 4;
         if (x4 >= bw || (rct_rgb8 && x4 + 4 <= bw)) continue;
+        if (cropped && (y < ky0 || y >= ky1 || x4 + 4 <= kx0 || x4 >= kx1)) continue;
         const int np = min(4, bw - x4);
         uint32_t o[4][4];
         // the four pixels of each LDS plane in one 16-byte (8-byte) read
@@ -1055,6 +1062,7 @@ __device__ __forceinline__ void final_body(const FinalJob *jobs, uint32_t njobs,
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
                     if (q >= np || c >= F.ch) continue;
+                    if (cropped && (x4 + q < kx0 || x4 + q >= kx1)) continue;
                     if (F.bps == 1) {
                         px[q * F.ch + c] = (uint8_t)o[q][c];
                     } else {

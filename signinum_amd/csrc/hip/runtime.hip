@@ -281,6 +281,7 @@ struct sgn_j2k_plan {
     uint32_t depth_max = 0;
     uint64_t coef_elems = 0, arena_len = 0, out_bytes = 0, out_pixels = 0;
     uint64_t coded_bytes = 0;  // sum of the codestream lengths
+    uint64_t pruned_cbs = 0;   // code-blocks region decodes skip
     // Codestream bytes reach the device arena as runs: tiles whose bytes
     // follow each other in host memory (gaps under 4 KB) form one run and
     // keep their relative offsets, so a batch laid out in one pinned buffer
@@ -440,9 +441,15 @@ Status build_plan(const sgn_tile_req *reqs, size_t n, sgn_j2k_plan &P, int32_t *
         if (!st[i]) {
             cs[i].bytes = c.cs;
             cs[i].len = c.cs_len;
+            Region roi;
+            roi.on = jobs[i].has_roi != 0;
+            roi.x = jobs[i].roi_x;
+            roi.y = jobs[i].roi_y;
+            roi.w = jobs[i].roi_w;
+            roi.h = jobs[i].roi_h;
             st[i] = build_local_plan(jobs[i].bytes, jobs[i].len, jobs[i].fmt, jobs[i].denominator,
                                      reinterpret_cast<uint64_t>(jobs[i].dst), jobs[i].pitch,
-                                     jobs[i].capacity, lps[i]);
+                                     jobs[i].capacity, lps[i], roi);
         }
         if (st[i]) errs[i] = last_error();
     });
@@ -521,6 +528,7 @@ Status build_plan(const sgn_tile_req *reqs, size_t n, sgn_j2k_plan &P, int32_t *
         P.tile_row_bytes[i] = lp.row_bytes;
         P.out_bytes += lp.out_bytes;
         P.out_pixels += uint64_t(lp.width) * lp.height;
+        P.pruned_cbs += lp.pruned_cbs;
         const uint32_t cbase = static_cast<uint32_t>(coef_base[i]);
         for (size_t k = 0; k < lp.cbs.size(); ++k) {
             CbJob j = lp.cbs[k];
@@ -1566,6 +1574,7 @@ int sgn_j2k_plan_get_stats(const sgn_j2k_plan *P, sgn_j2k_plan_stats *st) {
     st->host_merge_ms = float(P->t_merge);
     st->host_stage_ms = float(P->t_stage);
     st->host_copy_ms = float(P->t_copy);
+    st->pruned_code_blocks = P->pruned_cbs;
     return SGN_OK;
 }
 
@@ -1643,6 +1652,72 @@ int sgn_j2k_decode_tiles_device_noalloc(sgn_hip_session *s, const sgn_tile_job *
 }
 
 uint64_t sgn_j2k_noalloc_scratch_bytes(void) { return kNoAllocScratchBytes; }
+
+int sgn_j2k_decode_tiles_region_device(sgn_hip_session *s, const sgn_tile_req *reqs, size_t n,
+                                       size_t *first_fail) {
+    API_GUARD_BEGIN
+    if (!s || (!reqs && n)) return SGN_E_ARG;
+    std::vector<int32_t> ts(n, 0);
+    std::unique_ptr<sgn_j2k_plan> P;
+    Status st = make_plan(s, reqs, n, P, ts.data());
+    if (st) {
+        if (first_fail)
+            for (size_t i = 0; i < n; ++i)
+                if (ts[i]) {
+                    *first_fail = i;
+                    break;
+                }
+        return st;
+    }
+    st = decode_plan(s, *P, SGN_MODE_BATCHED);
+    if (st) return st;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return check_status(*P, SGN_MODE_BATCHED, first_fail);
+    API_GUARD_END
+}
+
+int sgn_j2k_decode_region_device(sgn_hip_session *s, const uint8_t *bytes, size_t len, int fmt,
+                                 uint32_t den, uint32_t x, uint32_t y, uint32_t w, uint32_t h,
+                                 void *dst, size_t pitch, size_t cap) {
+    sgn_tile_req r{bytes, len, dst, pitch, cap, fmt, den, 1u, x, y, w, h};
+    size_t ff = 0;
+    return sgn_j2k_decode_tiles_region_device(s, &r, 1, &ff);
+}
+
+int sgn_j2k_decode_region_into(sgn_hip_session *s, const uint8_t *bytes, size_t len, int fmt,
+                               uint32_t den, uint32_t x, uint32_t y, uint32_t w, uint32_t h,
+                               uint8_t *out, size_t out_len, size_t stride) {
+    API_GUARD_BEGIN
+    if (!s || !bytes || !out) return SGN_E_ARG;
+    uint32_t rx, ry, rw, rh;
+    const int rc0 = sgn_j2k_region_dims(bytes, len, den, x, y, w, h, &rx, &ry, &rw, &rh);
+    if (rc0) return rc0;
+    Container c;
+    Status st = locate_codestream(bytes, len, c);
+    if (st) return st;
+    MainHeader hd;
+    if ((st = read_main_header(c.cs, c.cs_len, hd))) return st;
+    OutputGeom og;
+    if ((st = output_geometry(hd, fmt, den, og))) return st;
+    const size_t row = size_t(rw) * og.channels * og.bytes_per_sample;
+    if (stride < row || (rh && out_len < stride * (rh - 1) + row)) {
+        set_last_error("output buffer too small");
+        return SGN_E_BUFFER;
+    }
+    if (!rw || !rh) return SGN_OK;
+    HIP_TRY(hipSetDevice(s->device));
+    void *d = nullptr;
+    HIP_TRY(hipMalloc(&d, row * rh));
+    int rc = sgn_j2k_decode_region_device(s, bytes, len, fmt, den, x, y, w, h, d, row, row * rh);
+    if (rc == SGN_OK &&
+        hipMemcpy2D(out, stride, d, row, row, rh, hipMemcpyDeviceToHost) != hipSuccess) {
+        set_last_error("download failed");
+        rc = SGN_E_DEVICE;
+    }
+    (void)hipFree(d);
+    return rc;
+    API_GUARD_END
+}
 
 int sgn_j2k_decode_device(sgn_hip_session *s, const uint8_t *bytes, size_t len, int fmt,
                           uint32_t den, void *dst, size_t pitch, size_t cap) {

@@ -72,6 +72,41 @@ int sgn_j2k_scaled_dims(const uint8_t *bytes, size_t len, uint32_t denominator, 
     return SGN_OK;
 }
 
+int sgn_j2k_region_dims(const uint8_t *bytes, size_t len, uint32_t denominator, uint32_t x,
+                        uint32_t y, uint32_t w, uint32_t h, uint32_t *out_x, uint32_t *out_y,
+                        uint32_t *out_w, uint32_t *out_h) {
+    if (!bytes) return SGN_E_ARG;
+    sgn::Container c;
+    sgn::Status st = sgn::locate_codestream(bytes, len, c);
+    if (st) return st;
+    sgn::MainHeader hd;
+    st = sgn::read_main_header(c.cs, c.cs_len, hd);
+    if (st) return st;
+    // the format only sets the byte layout: any one the components map to
+    sgn::OutputGeom og;
+    st = sgn::output_geometry(hd, hd.ncomp == 1 ? SGN_GRAY8 : SGN_RGB8, denominator, og);
+    if (st == sgn::kErrUnsupported && hd.ncomp != 1 && hd.ncomp != 3) {
+        sgn::MainHeader g = hd;
+        g.ncomp = 1;
+        st = sgn::output_geometry(g, SGN_GRAY8, denominator, og);
+    }
+    if (st) return st;
+    sgn::Region roi;
+    roi.on = true;
+    roi.x = x;
+    roi.y = y;
+    roi.w = w;
+    roi.h = h;
+    sgn::Rect r;
+    st = sgn::region_output(hd, og, denominator, roi, r);
+    if (st) return st;
+    if (out_x) *out_x = r.x0;
+    if (out_y) *out_y = r.y0;
+    if (out_w) *out_w = r.w();
+    if (out_h) *out_h = r.h();
+    return SGN_OK;
+}
+
 }  // extern "C"
 
 extern "C" int sgn_j2k_code_blocks(const uint8_t *bytes, size_t len, sgn_ht_code_block_job *jobs,

@@ -25,6 +25,7 @@ enum Status : int {
     kErrBuffer = -5,
     kErrNoMem = -6,
     kErrDevice = -7,
+    kErrRegion = -8,
 };
 
 void set_last_error(const std::string &msg);

@@ -101,11 +101,29 @@ struct LocalPlan {
     uint32_t width = 0, height = 0;
     size_t row_bytes = 0;
     uint64_t out_bytes = 0;
+    uint64_t pruned_cbs = 0;         // code-blocks a region decode does not need
 };
 
+// Region of interest in full-resolution image coordinates (Rect of
+// TileRegionScaledDecodeJob, crates/signinum-j2k/src/batch.rs:34-46).
+struct Region {
+    bool on = false;
+    uint32_t x = 0, y = 0, w = 0, h = 0;
+};
+
+// Output rectangle of a region decode at 1/denominator: the region scaled
+// to cover it (Rect::scaled_covering, crates/signinum-core/src/types.rs:
+// 69-83), checked against the full and the decoded image
+// (crates/signinum-j2k/src/decode.rs:36-71,377-389).
+Status region_output(const MainHeader &h, const OutputGeom &og, uint32_t denominator,
+                     const Region &roi, Rect &out);
+
 // Parses one codestream into plan records; the output of tile t starts at
-// dst + y0 * pitch + x0 * bytes_per_pixel.
+// dst + y0 * pitch + x0 * bytes_per_pixel.  With a region, only the region
+// is decoded and written (from dst): tiles outside it are dropped and the
+// code-blocks and synthesis blocks its pixels do not depend on are pruned.
 Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denominator,
-                        uint64_t dst, uint64_t pitch, uint64_t capacity, LocalPlan &lp);
+                        uint64_t dst, uint64_t pitch, uint64_t capacity, LocalPlan &lp,
+                        const Region &roi = Region());
 
 }  // namespace sgn

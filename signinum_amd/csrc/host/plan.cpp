@@ -60,8 +60,60 @@ Status output_geometry(const MainHeader &h, int fmt, uint32_t denominator, Outpu
     return kOk;
 }
 
+Status region_output(const MainHeader &h, const OutputGeom &og, uint32_t denominator,
+                     const Region &roi, Rect &out) {
+    if (!roi.on) {
+        out = {0, 0, og.width, og.height};
+        return kOk;
+    }
+    const uint64_t fw = h.xsiz - h.xo, fh = h.ysiz - h.yo;
+    if (uint64_t(roi.x) + roi.w > fw || uint64_t(roi.y) + roi.h > fh) {
+        set_last_error("region outside the image");
+        return kErrRegion;
+    }
+    const uint32_t d = denominator ? denominator : 1;
+    out.x0 = roi.x / d;
+    out.y0 = roi.y / d;
+    out.x1 = ceil_div(roi.x + roi.w, d);
+    out.y1 = ceil_div(roi.y + roi.h, d);
+    if (out.x1 > og.width || out.y1 > og.height) {
+        set_last_error("scaled region outside the decoded image");
+        return kErrRegion;
+    }
+    return kOk;
+}
+
+namespace {
+
+// Synthesis inputs of output samples [a, b) of one level, as sample
+// indices of its low- or high-pass band: index k sits at 2k (low) or 2k + 1
+// (high), and the 9/7 synthesis reaches two coefficients to either side of
+// a sample (5/3: one); a margin of four covers both with room to spare.
+constexpr uint32_t kRegionMargin = 4;
+
+void band_need(uint32_t a, uint32_t b, uint32_t lo, uint32_t hi, uint32_t &na, uint32_t &nb) {
+    const uint32_t x0 = a / 2, x1 = (b + 1) / 2 + kRegionMargin;
+    na = std::max(lo, x0 > kRegionMargin ? x0 - kRegionMargin : 0u);
+    nb = std::min(hi, x1);
+    if (nb < na) nb = na;
+}
+
+Rect need_of(const Rect &n, const Rect &band) {
+    Rect r;
+    band_need(n.x0, n.x1, band.x0, band.x1, r.x0, r.x1);
+    band_need(n.y0, n.y1, band.y0, band.y1, r.y0, r.y1);
+    return r;
+}
+
+bool meets(const Rect &a, const Rect &b) {
+    return a.x0 < b.x1 && b.x0 < a.x1 && a.y0 < b.y1 && b.y0 < a.y1;
+}
+
+}  // namespace
+
 Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denominator,
-                        uint64_t dst, uint64_t pitch, uint64_t capacity, LocalPlan &lp) {
+                        uint64_t dst, uint64_t pitch, uint64_t capacity, LocalPlan &lp,
+                        const Region &roi) {
     lp = LocalPlan();
     MainHeader h;
     Status st = read_main_header(cs, len, h);
@@ -69,15 +121,23 @@ Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denomin
     OutputGeom og;
     st = output_geometry(h, fmt, denominator, og);
     if (st) return st;
-    lp.width = og.width;
-    lp.height = og.height;
-    lp.row_bytes = og.row_bytes;
-    lp.out_bytes = uint64_t(og.row_bytes) * og.height;
-    if (pitch < og.row_bytes ||
-        (og.height && capacity < pitch * (og.height - 1) + og.row_bytes)) {
+    // the output: the whole decoded image or the region's covering rectangle
+    Rect reg;
+    st = region_output(h, og, denominator, roi, reg);
+    if (st) return st;
+    const size_t row_bytes = size_t(reg.w()) * og.channels * og.bytes_per_sample;
+    lp.width = reg.w();
+    lp.height = reg.h();
+    lp.row_bytes = row_bytes;
+    lp.out_bytes = uint64_t(row_bytes) * reg.h();
+    if (pitch < row_bytes || (reg.h() && capacity < pitch * (reg.h() - 1) + row_bytes)) {
         set_last_error("output surface too small for the decoded image");
         return kErrBuffer;
     }
+    if (roi.on && (reg.w() == 0 || reg.h() == 0)) return kOk;  // an empty region
+    // the region on the reference grid at the decoded resolution (a
+    // non-zero image origin only at full resolution)
+    const Rect regA{reg.x0 + h.xo, reg.y0 + h.yo, reg.x1 + h.xo, reg.y1 + h.yo};
     const uint32_t sk = skipped_levels(h, denominator);
     std::vector<TileBody> tiles;
     st = parse_tiles(cs, len, h, tiles);
@@ -89,6 +149,9 @@ Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denomin
 
     std::vector<TileCompGeom> tcs(static_cast<size_t>(nc));
     std::vector<std::vector<CbSeg>> segs;
+    // per component: the samples each retained resolution must deliver
+    // (region decodes), absolute coordinates of that resolution
+    std::vector<std::vector<Rect>> need(static_cast<size_t>(nc));
     for (const TileBody &t : tiles) {
         if (t.layers > 1) {
             set_last_error("multi-layer HTJ2K packet assembly");
@@ -104,6 +167,24 @@ Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denomin
             if (tcs[c].nres <= sk) {
                 set_last_error("resolution levels below the requested scale");
                 return kErrMalformed;
+            }
+        }
+        if (roi.on) {
+            // a tile outside the region is not decoded at all
+            const Rect F = tcs[0].res[tcs[0].nres - sk - 1].rect;
+            if (!meets(F, regA)) {
+                for (int c = 0; c < nc; ++c) lp.pruned_cbs += tcs[c].cbs.size();
+                continue;
+            }
+            for (int c = 0; c < nc; ++c) {
+                const TileCompGeom &g = tcs[c];
+                const uint32_t R = g.nres - sk;
+                need[c].assign(R, Rect());
+                const Rect fr = g.res[R - 1].rect;
+                need[c][R - 1] = {std::max(fr.x0, regA.x0), std::max(fr.y0, regA.y0),
+                                  std::min(fr.x1, regA.x1), std::min(fr.y1, regA.y1)};
+                for (uint32_t r = R - 1; r >= 1; --r)
+                    need[c][r - 1] = need_of(need[c][r], g.res[r - 1].rect);
             }
         }
         st = parse_packets(cs, t, nc, tcs.data(), segs);
@@ -177,6 +258,16 @@ Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denomin
                     j.zblk = static_cast<uint8_t>(std::min<uint32_t>(sg.zblk, 255));
                     j.passes = static_cast<uint8_t>(sg.passes);
                 }
+                if (roi.on) {
+                    // a block none of the region's samples depends on is not
+                    // decoded: the synthesis of the samples the region needs
+                    // never reads its part of the band
+                    const Rect bn = bg.res == 0 ? need[c][0] : need_of(need[c][bg.res], bg.rect);
+                    if (!meets(cg.rect, bn)) {
+                        ++lp.pruned_cbs;
+                        continue;
+                    }
+                }
                 j.cost = (j.flags & kCbZero) ? 0 : sg.lcup + sg.lref + (uint32_t(j.w) * j.h) / 4;
                 lp.cbs.push_back(j);
                 lp.cb_band.push_back(band_off[cg.band]);
@@ -233,6 +324,17 @@ Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denomin
                 lj.levels = 1;
                 lj.nbx = static_cast<uint16_t>(ceil_div(rs.rect.w(), 64));
                 lj.nby = static_cast<uint16_t>(ceil_div(rs.rect.h(), 64));
+                if (roi.on) {
+                    // only the 64x64 blocks holding samples the region needs
+                    const Rect &n = need[c][r];
+                    const uint32_t bx0 = (n.x0 - rs.rect.x0) / 64, by0 = (n.y0 - rs.rect.y0) / 64;
+                    const uint32_t bx1 = ceil_div(n.x1 - rs.rect.x0, 64);
+                    const uint32_t by1 = ceil_div(n.y1 - rs.rect.y0, 64);
+                    lj.bx0 = static_cast<uint16_t>(bx0);
+                    lj.by0 = static_cast<uint16_t>(by0);
+                    lj.nbx = static_cast<uint16_t>(bx1 > bx0 ? bx1 - bx0 : 0);
+                    lj.nby = static_cast<uint16_t>(by1 > by0 ? by1 - by0 : 0);
+                }
                 if (r + 1 < R) {
                     lj.out = static_cast<uint32_t>(lp.coef_elems);
                     lp.coef_elems += uint64_t(coef_pitch(rs.rect.w())) * rs.rect.h();
@@ -273,20 +375,30 @@ Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denomin
             set_last_error("tile outside the output image");
             return kErrMalformed;
         }
-        fj.dst = dst + uint64_t(out_rect.y0) * pitch + uint64_t(out_rect.x0) * bpp;
+        // the tile's pixel (0, 0) in the output, which starts at the
+        // region's corner (possibly outside it: the kernel writes only the
+        // crop rectangle)
+        fj.dst = dst + uint64_t(int64_t(out_rect.y0) - int64_t(reg.y0)) * pitch +
+                 uint64_t(int64_t(out_rect.x0) - int64_t(reg.x0)) * bpp;
         fj.pitch = pitch;
         fj.out_x = out_rect.x0;
         fj.out_y = out_rect.y0;
         fj.width = out_rect.w();
         fj.height = out_rect.h();
-        fj.nbx = static_cast<uint16_t>(ceil_div(fj.width, 64));
-        fj.nby = static_cast<uint16_t>(ceil_div(fj.height, 64));
+        fj.cx0 = std::max(out_rect.x0, reg.x0) - out_rect.x0;
+        fj.cy0 = std::max(out_rect.y0, reg.y0) - out_rect.y0;
+        fj.cx1 = std::min(out_rect.x1, reg.x1) - out_rect.x0;
+        fj.cy1 = std::min(out_rect.y1, reg.y1) - out_rect.y0;
+        fj.bx0 = static_cast<uint16_t>(fj.cx0 / 64);
+        fj.by0 = static_cast<uint16_t>(fj.cy0 / 64);
+        fj.nbx = static_cast<uint16_t>(ceil_div(fj.cx1, 64) - fj.bx0);
+        fj.nby = static_cast<uint16_t>(ceil_div(fj.cy1, 64) - fj.by0);
         fj.ncomp = static_cast<uint8_t>(nc);
         fj.fmt = static_cast<uint8_t>(fmt);
         fj.mct = mct ? 1 : 0;
         fj.reversible = t.comp[0].transform == 1;
         fj.same8 = same8 ? 1 : 0;
-        if (fj.width && fj.height) lp.finals.push_back(fj);
+        if (fj.cx1 > fj.cx0 && fj.cy1 > fj.cy0) lp.finals.push_back(fj);
     }
     return kOk;
 }

@@ -58,6 +58,7 @@ struct LevelJob {
     uint8_t float_in;         // bit b set: band b holds float samples, not integers
     uint8_t levels;           // 1: one synthesis level, 0: copy LL only
     uint8_t narrow;           // bit b set: band b holds int16 code-block samples
+    uint16_t bx0, by0;        // first 64x64 block of the nbx x nby computed (region decodes)
 };
 
 // Final stage of one tile: last synthesis level of every component fused
@@ -74,6 +75,12 @@ struct FinalJob {
     uint8_t same8;            // all components 8-bit
     uint8_t pad[3];
     uint32_t out_x, out_y;    // position of this J2K tile inside the output image
+    uint16_t bx0, by0;        // first 64x64 block of the nbx x nby decoded
+    // Pixels written, relative to the tile's rectangle: [cx0, cx1) x [cy0,
+    // cy1) (the whole rectangle unless the decode is of a region); dst is
+    // the address the tile's pixel (0, 0) would have, which lies outside the
+    // output when the region starts inside the tile.
+    uint32_t cx0, cy0, cx1, cy1;
 };
 
 }  // namespace sgn
