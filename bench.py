@@ -294,6 +294,10 @@ def main():
         log("autotune (HT lane kernel, ms): " +
             ", ".join(f"{k} {v:.3f}" for k, v in tune.items()) +
             f" -> {S.TilePlan.lane_variants()[plan.lane_variant]}")
+        dwt = plan.autotune_dwt(reps=5)
+        log("autotune (synthesis-level shape, ms): " +
+            ", ".join(f"{k} {v:.3f}" for k, v in dwt.items()) +
+            f" -> {S.TilePlan.dwt_shapes()[plan.dwt_shape]}")
 
     # ---- batched, graph-replayed, codestreams resident in HBM ----
     for _ in range(args.warmup):
@@ -503,6 +507,7 @@ def main():
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
         "lane_kernel": S.TilePlan.lane_variants()[plan.lane_variant],
         "autotune_ms": None if tune is None else {k: round(v, 4) for k, v in tune.items()},
+        "dwt_shape": S.TilePlan.dwt_shapes()[plan.dwt_shape],
         "code_block_order": order_name,
         "order_ms": None if order_ms is None else {k: round(v, 4) for k, v in order_ms.items()},
         "roofline": roofline,

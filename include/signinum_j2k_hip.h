@@ -271,8 +271,9 @@ int sgn_j2k_plan_decode_stage(sgn_hip_session *session, sgn_j2k_plan *plan, int 
 /* ---- autotuning of the HT lane kernel ------------------------------------
  * No counterpart in the reference (its CPU decoder has no launch shapes):
  * the lane kernel comes in variants of work-group size, quad pairs decoded
- * between LDS refills, and one-shot grid versus persistent waves pulling
- * 64-block chunks from a device work counter. */
+ * between LDS refills, one-shot grid versus persistent waves pulling chunks
+ * of 64, 128 or 256 code-blocks from a device work counter, and the
+ * cooperative one-block-per-wave kernel. */
 int sgn_ht_lane_variant_count(void);
 const char *sgn_ht_lane_variant_name(int variant);
 int sgn_j2k_plan_set_lane_variant(sgn_j2k_plan *plan, int variant);
@@ -283,6 +284,23 @@ int sgn_j2k_plan_get_lane_variant(const sgn_j2k_plan *plan);
  * variant's mean time in milliseconds.  Synchronous. */
 int sgn_j2k_plan_autotune(sgn_hip_session *session, sgn_j2k_plan *plan, int reps, float *ms,
                           size_t count, int *chosen);
+
+/* ---- autotuning of the synthesis-level tile shape -------------------------
+ * No counterpart in the reference.  The level stage's launches come in
+ * shapes: 64x64 output blocks computed by four or by two waves per
+ * work-group, each with or without the one-wave 32x32 kernel for launches
+ * whose levels are all at most 32x32.  Every shape computes the same
+ * coefficients. */
+int sgn_dwt_shape_count(void);
+const char *sgn_dwt_shape_name(int shape);
+int sgn_j2k_plan_set_dwt_shape(sgn_j2k_plan *plan, int shape);
+int sgn_j2k_plan_get_dwt_shape(const sgn_j2k_plan *plan);
+/* Times the level stage `reps` times in every shape (HIP events on the
+ * session stream; the code-block stage runs once first so the levels read
+ * decoded bands), keeps the fastest and writes it to *chosen; ms (optional,
+ * `count` entries) receives each shape's mean time.  Synchronous. */
+int sgn_j2k_plan_autotune_dwt(sgn_hip_session *session, sgn_j2k_plan *plan, int reps, float *ms,
+                              size_t count, int *chosen);
 
 typedef struct {
     uint64_t tiles;

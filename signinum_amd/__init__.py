@@ -551,6 +551,33 @@ class TilePlan:
         names = self.lane_variants()
         return {names[i]: float(ms[i]) for i in range(n)}
 
+    @staticmethod
+    def dwt_shapes():
+        """Names of the synthesis-level launch shapes the autotuner chooses
+        from."""
+        L = _lib.load()
+        return [L.sgn_dwt_shape_name(i).decode() for i in range(L.sgn_dwt_shape_count())]
+
+    @property
+    def dwt_shape(self):
+        return int(_lib.load().sgn_j2k_plan_get_dwt_shape(self.handle))
+
+    @dwt_shape.setter
+    def dwt_shape(self, v):
+        _check(_lib.load().sgn_j2k_plan_set_dwt_shape(self.handle, int(v)))
+
+    def autotune_dwt(self, reps=5):
+        """Times the level stage in every synthesis-level shape, keeps the
+        fastest and returns {shape name: mean ms}."""
+        L = _lib.load()
+        n = L.sgn_dwt_shape_count()
+        ms = (ctypes.c_float * n)()
+        chosen = ctypes.c_int()
+        _check(L.sgn_j2k_plan_autotune_dwt(self.session.handle, self.handle, int(reps), ms, n,
+                                           ctypes.byref(chosen)))
+        names = self.dwt_shapes()
+        return {names[i]: float(ms[i]) for i in range(n)}
+
     ORDER_LENGTH, ORDER_INPUT, ORDER_MODEL = 0, 1, 2
     ORDER_GEOMETRY, ORDER_GEOMETRY_COST, ORDER_GEOMETRY_MODEL = 3, 4, 5
 

@@ -175,6 +175,13 @@ SIGNATURES = {
     "sgn_j2k_plan_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                              ctypes.POINTER(ctypes.c_float), c_size_t,
                                              ctypes.POINTER(ctypes.c_int)]),
+    "sgn_dwt_shape_count": (ctypes.c_int, []),
+    "sgn_dwt_shape_name": (ctypes.c_char_p, [ctypes.c_int]),
+    "sgn_j2k_plan_set_dwt_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "sgn_j2k_plan_get_dwt_shape": (ctypes.c_int, [ctypes.c_void_p]),
+    "sgn_j2k_plan_autotune_dwt": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                                 ctypes.POINTER(ctypes.c_float), c_size_t,
+                                                 ctypes.POINTER(ctypes.c_int)]),
     "sgn_cost_model_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64,
                                              ctypes.POINTER(ctypes.c_void_p)]),
     "sgn_cost_model_destroy": (None, [ctypes.c_void_p]),

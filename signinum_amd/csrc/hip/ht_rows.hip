@@ -480,10 +480,12 @@ __device__ __forceinline__ void rows_wave(const CbJob *__restrict__ jobs, uint32
 }
 
 // The lane kernel.  PERSIST: a grid sized to the resident capacity whose
-// waves take the next 64 code-blocks (in the host's cost order) from the
-// device work counter *queue until none are left; otherwise one wave per 64
+// waves take the next CHUNK code-blocks (in the host's cost order, a
+// multiple of 64: one block per lane, CHUNK / 64 rounds) from the device
+// work counter *queue until none are left; otherwise one wave per 64
 // code-blocks.  *queue must be zero at launch.
-template <int WAVES, int SEG, int MS, int VS, int LS, bool PERSIST, typename C, int MINW = 1>
+template <int WAVES, int SEG, int MS, int VS, int LS, bool PERSIST, typename C, int MINW = 1,
+          int CHUNK = 64>
 __global__ void __launch_bounds__(WAVES * 64, MINW)
 sgn_ht_rows(const CbJob *__restrict__ jobs, uint32_t n, const uint8_t *__restrict__ arena,
             C *__restrict__ coef, const uint16_t *__restrict__ tables,
@@ -500,13 +502,15 @@ sgn_ht_rows(const CbJob *__restrict__ jobs, uint32_t n, const uint8_t *__restric
         rows_wave<WAVES, SEG, MS, VS, LS, C>(jobs, n, first, arena, coef, status, s_tab, s_q, s_ms,
                                   s_vlc, s_mel);
     } else {
+        static_assert(CHUNK % 64 == 0, "a chunk is whole waves of code-blocks");
         for (;;) {
             uint32_t first = 0;
-            if ((threadIdx.x & 63) == 0) first = atomicAdd(queue, 64u);
+            if ((threadIdx.x & 63) == 0) first = atomicAdd(queue, (uint32_t)CHUNK);
             first = __shfl(first, 0);
             if (first >= n) break;
-            rows_wave<WAVES, SEG, MS, VS, LS, C>(jobs, n, first, arena, coef, status, s_tab, s_q,
-                                      s_ms, s_vlc, s_mel);
+            for (uint32_t c = 0; c < (uint32_t)CHUNK && first + c < n; c += 64)
+                rows_wave<WAVES, SEG, MS, VS, LS, C>(jobs, n, first + c, arena, coef, status, s_tab,
+                                          s_q, s_ms, s_vlc, s_mel);
         }
     }
 }

@@ -35,6 +35,9 @@ struct Geo {
 };
 using Big = Geo<kBlk, kIdwtThreads>;
 using Small = Geo<32, 64>;
+// 64x64 blocks with two waves: half the threads per block, twice the
+// work-groups per CU (a level tile shape the plan's autotuner can pick)
+using Mid = Geo<kBlk, 128>;
 constexpr float kKappa = 1.2301741f;
 constexpr float kNegAlpha = 1.5861343f;
 constexpr float kNegBeta = 0.052980117f;
@@ -503,9 +506,10 @@ __device__ __forceinline__ void fetch_window(const LevelView &L, int bx, int by,
 // walking the chunks of one row write 8 words apart, three to a bank; lanes
 // walking the rows of one chunk read a separate 8 bytes per row.)  Window
 // rows past the block's last (yy >= nh) repeat the last row of their parity.
-// Positions left of the level (block column 0) and right of it (last block
-// column) are then mirrored from inside (whole-sample symmetric extension,
-// F.3.7); chunks outside the band rows load the level's first word instead.
+// Positions left of the level (block column 0) and right of it (any block
+// whose right halo passes the level's last column) are then mirrored from
+// inside (whole-sample symmetric extension, F.3.7); chunks outside the band
+// rows load the level's first word instead.
 // No barrier at the end.
 template <bool REV, class G = Big>
 __device__ __forceinline__ void fetch_chunks(const LevelView &L, int bx, int by,
@@ -589,7 +593,10 @@ __device__ __forceinline__ void fetch_chunks(const LevelView &L, int bx, int by,
             }
         }
     }
-    const bool left = ox == 0, right = ox + bw == L.w;
+    // the right halo (H columns past the block) crosses the level's end in
+    // the last block column and also in the one before it when the level
+    // ends fewer than H columns past that block (widths 64k + 1 .. 64k + H - 1)
+    const bool left = ox == 0, right = L.w - ox - bw < H;
     if (left || right) {
         __syncthreads();
         const int xl = 8 + (L.w - 1 - ox);  // row position of the level's last column
@@ -806,6 +813,20 @@ extern "C" __global__ void __launch_bounds__(Small::kThreads)
 sgn_idwt_level_irr_small(const LevelJob *jobs, uint32_t njobs, uint32_t block_offset,
                          const uint32_t *block_job, uint32_t *coef) {
     level_body<false, Small>(jobs, njobs, block_offset, block_job, coef);
+}
+
+// 64x64 blocks with two waves (Mid): the same blocks, each work-group's
+// window walked by half as many threads.
+extern "C" __global__ void __launch_bounds__(Mid::kThreads)
+sgn_idwt_level_rev_w2(const LevelJob *jobs, uint32_t njobs, uint32_t block_offset,
+                      const uint32_t *block_job, uint32_t *coef) {
+    level_body<true, Mid>(jobs, njobs, block_offset, block_job, coef);
+}
+
+extern "C" __global__ void __launch_bounds__(Mid::kThreads)
+sgn_idwt_level_irr_w2(const LevelJob *jobs, uint32_t njobs, uint32_t block_offset,
+                      const uint32_t *block_job, uint32_t *coef) {
+    level_body<false, Mid>(jobs, njobs, block_offset, block_job, coef);
 }
 
 // ---- final stage -------------------------------------------------------

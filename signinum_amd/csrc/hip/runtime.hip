@@ -93,12 +93,13 @@ namespace {
 using RowsLaunch = void (*)(hipStream_t, uint32_t, const CbJob *, uint32_t, const uint8_t *,
                             void *, const uint16_t *, uint8_t *, uint32_t *);
 
-template <int W, int S, int M, int VS, int LS, bool PERSIST, typename C, int MINW>
+template <int W, int S, int M, int VS, int LS, bool PERSIST, typename C, int MINW, int CHUNK>
 void rows_launch(hipStream_t st, uint32_t grid, const CbJob *jobs, uint32_t n,
                  const uint8_t *arena, void *coef, const uint16_t *tables, uint8_t *status,
                  uint32_t *queue) {
-    hipLaunchKernelGGL((sgn_ht_rows<W, S, M, VS, LS, PERSIST, C, MINW>), dim3(grid), dim3(W * 64),
-                       0, st, jobs, n, arena, static_cast<C *>(coef), tables, status, queue);
+    hipLaunchKernelGGL((sgn_ht_rows<W, S, M, VS, LS, PERSIST, C, MINW, CHUNK>), dim3(grid),
+                       dim3(W * 64), 0, st, jobs, n, arena, static_cast<C *>(coef), tables, status,
+                       queue);
 }
 
 // The cooperative kernel as a lane-list variant: one code-block per wave,
@@ -127,25 +128,26 @@ struct RowsVariant {
     bool coop = false;  // one code-block per wave (grid = block count)
 };
 
-// MINW: waves per SIMD the register allocation must allow (1: no bound)
-#define SGN_ROWS_VARIANT(name, W, S, M, VS, LS, Q, MINW)                                     \
+// MINW: waves per SIMD the register allocation must allow (1: no bound);
+// CHUNK: code-blocks a persistent wave takes per work-counter grab
+#define SGN_ROWS_VARIANT(name, W, S, M, VS, LS, Q, MINW, CHUNK)                              \
     {name,                                                                                 \
      W,                                                                                    \
      Q,                                                                                    \
-     {rows_launch<W, S, M, VS, LS, Q, int32_t, MINW>,                                      \
-      rows_launch<W, S, M, VS, LS, Q, int16_t, MINW>},                                     \
-     {reinterpret_cast<const void *>(sgn_ht_rows<W, S, M, VS, LS, Q, int32_t, MINW>),      \
-      reinterpret_cast<const void *>(sgn_ht_rows<W, S, M, VS, LS, Q, int16_t, MINW>)}}
+     {rows_launch<W, S, M, VS, LS, Q, int32_t, MINW, CHUNK>,                               \
+      rows_launch<W, S, M, VS, LS, Q, int16_t, MINW, CHUNK>},                              \
+     {reinterpret_cast<const void *>(sgn_ht_rows<W, S, M, VS, LS, Q, int32_t, MINW, CHUNK>), \
+      reinterpret_cast<const void *>(sgn_ht_rows<W, S, M, VS, LS, Q, int16_t, MINW, CHUNK>)}}
 
 // name: waves per work-group, quad pairs per segment, small rings (3 work-
-// groups per CU instead of 2), persistent queue
+// groups per CU instead of 2), persistent queue and its chunk
 const RowsVariant kRowsVariants[] = {
-    SGN_ROWS_VARIANT("w4-seg8", 4, 8, 8, 4, 2, false, 1),
-    SGN_ROWS_VARIANT("w4-seg8-queue", 4, 8, 8, 4, 2, true, 1),
-    SGN_ROWS_VARIANT("w4-seg4-queue", 4, 4, 4, 4, 2, true, 1),
-    SGN_ROWS_VARIANT("w4-seg4-small-queue", 4, 4, 4, 2, 1, true, 1),
-    SGN_ROWS_VARIANT("w2-seg16-queue", 2, 16, 16, 4, 2, true, 1),
-    SGN_ROWS_VARIANT("w1-seg8-queue", 1, 8, 8, 4, 2, true, 1),
+    SGN_ROWS_VARIANT("w4-seg8", 4, 8, 8, 4, 2, false, 1, 64),
+    SGN_ROWS_VARIANT("w4-seg8-queue", 4, 8, 8, 4, 2, true, 1, 64),
+    SGN_ROWS_VARIANT("w4-seg4-queue", 4, 4, 4, 4, 2, true, 1, 64),
+    SGN_ROWS_VARIANT("w4-seg4-small-queue", 4, 4, 4, 2, 1, true, 1, 64),
+    SGN_ROWS_VARIANT("w2-seg16-queue", 2, 16, 16, 4, 2, true, 1, 64),
+    SGN_ROWS_VARIANT("w1-seg8-queue", 1, 8, 8, 4, 2, true, 1, 64),
     {"wave-coop",
      1,
      false,
@@ -153,6 +155,8 @@ const RowsVariant kRowsVariants[] = {
      {reinterpret_cast<const void *>(sgn_ht_coop<64, false, int32_t>),
       reinterpret_cast<const void *>(sgn_ht_coop<64, false, int16_t>)},
      true},
+    SGN_ROWS_VARIANT("w4-seg4-small-queue128", 4, 4, 4, 2, 1, true, 1, 128),
+    SGN_ROWS_VARIANT("w4-seg4-small-queue256", 4, 4, 4, 2, 1, true, 1, 256),
 };
 #undef SGN_ROWS_VARIANT
 constexpr int kRowsVariantCount = int(sizeof(kRowsVariants) / sizeof(kRowsVariants[0]));
@@ -161,6 +165,25 @@ constexpr int kRowsVariantCount = int(sizeof(kRowsVariants) / sizeof(kRowsVarian
 constexpr int kDefaultLaneVariant = 3;
 // the cooperative kernel (one code-block per wave)
 constexpr int kCoopVariant = 6;
+
+// Work-group shapes of the synthesis-level launches the autotuner chooses
+// from: the 64x64-block kernel (four or two waves per block) and whether
+// launches whose levels are all at most 32x32 take the one-wave 32x32
+// kernel instead.  Every shape computes the same blocks.
+using LevelKernel = void (*)(const LevelJob *, uint32_t, uint32_t, const uint32_t *, uint32_t *);
+struct DwtShape {
+    const char *name;
+    LevelKernel kernel[2];  // [irreversible, reversible]
+    uint32_t threads;
+    bool small;
+};
+const DwtShape kDwtShapes[] = {
+    {"64x64-w4+32x32-w1", {sgn_idwt_level_irr, sgn_idwt_level_rev}, kIdwtThreads, true},
+    {"64x64-w4", {sgn_idwt_level_irr, sgn_idwt_level_rev}, kIdwtThreads, false},
+    {"64x64-w2+32x32-w1", {sgn_idwt_level_irr_w2, sgn_idwt_level_rev_w2}, Mid::kThreads, true},
+    {"64x64-w2", {sgn_idwt_level_irr_w2, sgn_idwt_level_rev_w2}, Mid::kThreads, false},
+};
+constexpr int kDwtShapeCount = int(sizeof(kDwtShapes) / sizeof(kDwtShapes[0]));
 // Lane-list size up to which plans start with the cooperative kernel.  On
 // 256x256 RGB tiles (75 code-blocks each) it took 0.28 against 1.41 ms for
 // 16 tiles and 1.21 against 1.61 ms for 256 tiles (19,200 blocks); the lane
@@ -310,6 +333,7 @@ struct sgn_j2k_plan {
     uint8_t *d_status_lane = nullptr, *d_status_wave = nullptr;
     uint32_t *d_queue = nullptr;  // work counter of the persistent lane kernel
     int lane_variant = kDefaultLaneVariant;  // index into kRowsVariants
+    int dwt_shape = 0;                       // index into kDwtShapes
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
     uint32_t launches_batched = 0;
@@ -931,12 +955,13 @@ uint32_t launch_levels(sgn_hip_session *s, sgn_j2k_plan &P, int rev, uint32_t a,
     const uint32_t blocks = group_blocks(P.lv[rev], a, b);
     if (!blocks) return 0;
     const uint32_t off = P.lv[rev][a].block_base;
-    // levels of at most 32x32 take the one-wave kernel (same block count:
-    // each is one block for either kernel)
-    const bool small = small_levels(P.lv[rev], a, b);
-    const auto kernel = rev ? (small ? sgn_idwt_level_rev_small : sgn_idwt_level_rev)
-                            : (small ? sgn_idwt_level_irr_small : sgn_idwt_level_irr);
-    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(small ? Small::kThreads : kIdwtThreads), 0,
+    const DwtShape &g = kDwtShapes[P.dwt_shape];
+    // levels of at most 32x32 take the one-wave kernel where the shape says
+    // so (same block count: each is one block for either kernel)
+    const bool small = g.small && small_levels(P.lv[rev], a, b);
+    const auto kernel = small ? (rev ? sgn_idwt_level_rev_small : sgn_idwt_level_irr_small)
+                              : g.kernel[rev];
+    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(small ? Small::kThreads : g.threads), 0,
                        s->stream, P.d_lv[rev], uint32_t(P.lv[rev].size()), off,
                        P.d_lv_block_job[rev], P.d_coef);
     return 1;
@@ -1520,6 +1545,71 @@ int sgn_j2k_plan_autotune(sgn_hip_session *s, sgn_j2k_plan *P, int reps, float *
     if (st) return st;
     if (P->lane_variant != best) drop_graph(*P);
     P->lane_variant = best;
+    if (chosen) *chosen = best;
+    return SGN_OK;
+    API_GUARD_END
+}
+
+int sgn_dwt_shape_count(void) { return kDwtShapeCount; }
+
+const char *sgn_dwt_shape_name(int v) {
+    return (v >= 0 && v < kDwtShapeCount) ? kDwtShapes[v].name : nullptr;
+}
+
+int sgn_j2k_plan_set_dwt_shape(sgn_j2k_plan *P, int v) {
+    if (!P || v < 0 || v >= kDwtShapeCount) return SGN_E_ARG;
+    if (P->dwt_shape != v) drop_graph(*P);
+    P->dwt_shape = v;
+    return SGN_OK;
+}
+
+int sgn_j2k_plan_get_dwt_shape(const sgn_j2k_plan *P) { return P ? P->dwt_shape : -1; }
+
+int sgn_j2k_plan_autotune_dwt(sgn_hip_session *s, sgn_j2k_plan *P, int reps, float *ms, size_t cap,
+                              int *chosen) {
+    API_GUARD_BEGIN
+    if (!s || !P || reps < 1) return SGN_E_ARG;
+    HIP_TRY(hipSetDevice(s->device));
+    const int keep = P->dwt_shape;
+    int rc = sgn_j2k_plan_decode_stage(s, P, SGN_STAGE_BLOCKS);
+    if (rc) return rc;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        set_last_error("hipEventCreate failed");
+        return SGN_E_DEVICE;
+    }
+    int best = keep;
+    float best_ms = 0.0f;
+    for (int v = 0; v < kDwtShapeCount && !rc; ++v) {
+        P->dwt_shape = v;
+        float t = 0.0f;
+        rc = sgn_j2k_plan_decode_stage(s, P, SGN_STAGE_LEVELS);  // warm-up
+        bool ok = !rc && hipEventRecord(e0, s->stream) == hipSuccess;
+        for (int r = 0; r < reps && ok; ++r) ok = !sgn_j2k_plan_decode_stage(s, P, SGN_STAGE_LEVELS);
+        ok = ok && hipEventRecord(e1, s->stream) == hipSuccess &&
+             hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&t, e0, e1) == hipSuccess;
+        if (!ok) {
+            if (!rc) {
+                set_last_error("synthesis-level shape failed to run");
+                rc = SGN_E_DEVICE;
+            }
+            break;
+        }
+        t /= float(reps);
+        if (ms && size_t(v) < cap) ms[v] = t;
+        if (v == 0 || t < best_ms) {
+            best = v;
+            best_ms = t;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    P->dwt_shape = keep;
+    if (rc) return rc;
+    if (P->dwt_shape != best) drop_graph(*P);
+    P->dwt_shape = best;
     if (chosen) *chosen = best;
     return SGN_OK;
     API_GUARD_END

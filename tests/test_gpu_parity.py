@@ -498,3 +498,64 @@ def test_store_component_hook(session, case):
     with pytest.raises(S.BufferError):
         S.HtCodeBlockDecoder(session).decode_store_component(
             inp, iw, sx, sy, cw, ch, out, ow, ow - cw + 1, oy, add)
+
+
+def test_every_dwt_shape_and_autotune(session, oracle):
+    # every synthesis-level launch shape (64x64 blocks by four or two waves,
+    # with or without the 32x32 one-wave kernel) decodes 5/3 and 9/7 tiles
+    # of several sizes to the same pixels (the lifting arithmetic of each
+    # sample does not depend on the shape): the oracle's for 5/3, within
+    # 1 LSB of it for 9/7; the shape autotuner keeps one
+    rng = np.random.default_rng(31)
+    streams, want = [], []
+    for k, (w, h) in enumerate(((96, 96), (300, 170), (40, 24), (513, 257))):
+        img = synth.tissue_rgb(rng, w, h)
+        cs = (S.encode_j2k_lossless(img) if k % 2 == 0 else
+              S.encode_j2k_ht(img, reversible=0, base_step=1 / 64, levels=5))
+        streams.append(cs)
+        want.append(O.decode(cs, 1))
+    plan = S.TilePlan(session, streams, S.PixelFormat.RGB8)
+    names = S.TilePlan.dwt_shapes()
+    assert len(names) == 4 and plan.dwt_shape == 0
+    first = None
+    for v in range(len(names)):
+        plan.dwt_shape = v
+        for mode in (S.MODE_GRAPH, S.MODE_BATCHED):
+            for s in plan.surfaces:
+                s.fill(0)
+            plan.decode(mode)
+            session.synchronize()
+            got = [s.to_numpy() for s in plan.surfaces]
+            if first is None:
+                first = got
+            for k in range(len(got)):
+                assert np.array_equal(got[k], first[k]), (names[v], k)
+    for k in range(len(want)):
+        d = np.abs(first[k].astype(np.int64) - want[k].astype(np.int64)).max()
+        assert d == 0 if k % 2 == 0 else d <= 1, k   # 9/7 tolerance: 1 LSB of 8-bit
+    times = plan.autotune_dwt(reps=2)
+    assert set(times) == set(names) and all(t > 0 for t in times.values())
+    assert times[names[plan.dwt_shape]] == min(times.values())
+    plan.decode(S.MODE_GRAPH)
+    session.synchronize()
+    assert all(np.array_equal(s.to_numpy(), w) for s, w in zip(plan.surfaces, first))
+    plan.close()
+
+
+@pytest.mark.parametrize("reversible", [1, 0])
+def test_widths_just_past_a_block_column(session, oracle, reversible):
+    # level widths 64k + 1 .. 64k + 3: the right halo of the block column
+    # before the last passes the level's end and must be mirrored from
+    # inside (symmetric extension), not read from past the band rows
+    rng = np.random.default_rng(41)
+    src = synth.tissue_rgb(rng, 515, 195)
+    for w, h in ((513, 195), (514, 130), (515, 67), (129, 65), (66, 190), (65, 65)):
+        img = src[:h, :w].copy()
+        for levels in (1, 2, 5):
+            cs = (S.encode_j2k_ht(img, levels=levels) if reversible else
+                  S.encode_j2k_ht(img, reversible=0, base_step=1 / 64, levels=levels))
+            got = _gpu(session, cs, S.PixelFormat.RGB8).astype(np.int64)
+            d = np.abs(got - O.decode(cs, 1).astype(np.int64)).max()
+            assert d == 0 if reversible else d <= 1, (w, h, levels, d)  # 9/7: 1 LSB of 8-bit
+            if reversible:
+                assert np.array_equal(got, img), (w, h, levels)
