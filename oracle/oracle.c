@@ -377,13 +377,23 @@ typedef struct {
     tagnode_t *nodes;
 } tagtree_t;
 
+/* A classic code-block's contribution from one packet: `len` bytes of
+ * codeword segment `seg` carrying `passes` coding passes. */
+typedef struct {
+    const uint8_t *data;
+    uint32_t len, seg, passes;
+} contrib_t;
+
 typedef struct {
     rect_t rect;
     uint32_t xi, yi;
     int included;
     uint32_t zblk, lblock, passes;
-    const uint8_t *data; /* cleanup followed by refinement */
+    const uint8_t *data; /* HT: cleanup followed by refinement */
     uint32_t lcup, lref;
+    /* classic: contributions in codestream order, layers with any */
+    contrib_t *contrib;
+    uint32_t ncontrib, capcontrib, layers_in;
 } cblk_t;
 
 typedef struct {
@@ -623,6 +633,8 @@ static void free_tcomp(tcomp_t *tc)
         if (rs->prec) {
             uint32_t n = rs->npx * rs->npy * (uint32_t)rs->nbands;
             for (uint32_t i = 0; i < n; ++i) {
+                for (uint32_t k = 0; rs->prec[i].cbs && k < rs->prec[i].ncbx * rs->prec[i].ncby; ++k)
+                    free(rs->prec[i].cbs[k].contrib);
                 free(rs->prec[i].cbs);
                 free(rs->prec[i].incl.nodes);
                 free(rs->prec[i].zbp.nodes);
@@ -997,6 +1009,88 @@ static int read_ht_passes(pbr_t *r, uint32_t *n)
 
 static uint32_t ilog2u(uint32_t v) { return 31 - (uint32_t)__builtin_clz(v); }
 
+/* Table B.4: number of coding passes (segment.rs:481-506). */
+static int read_classic_passes(pbr_t *r, uint32_t *n)
+{
+    uint32_t b;
+    if (pbr_bits(r, 1, &b)) return -1;
+    if (!b) {
+        *n = 1;
+        return 0;
+    }
+    if (pbr_bits(r, 1, &b)) return -1;
+    if (!b) {
+        *n = 2;
+        return 0;
+    }
+    uint32_t v;
+    if (pbr_bits(r, 2, &v)) return -1;
+    if (v < 3) {
+        *n = 3 + v;
+        return 0;
+    }
+    if (pbr_bits(r, 5, &v)) return -1;
+    if (v < 31) {
+        *n = 6 + v;
+        return 0;
+    }
+    if (pbr_bits(r, 7, &v)) return -1;
+    *n = 37 + v;
+    return 0;
+}
+
+/* Codeword segment of coding pass p (segment.rs:223-239): every pass with
+ * termination on each pass, Table D.9 with selective bypass, otherwise one
+ * per layer that contributes. */
+static uint32_t classic_segment(const cblk_t *cb, int style, uint32_t p)
+{
+    if (style & ORACLE_CB_TERMALL) return p;
+    if (style & ORACLE_CB_BYPASS) return oracle_bypass_segment(p);
+    return cb->layers_in;
+}
+
+/* Classic code-block header after inclusion (segment.rs:168-301): passes,
+ * Lblock increment, then one length per codeword segment the new passes
+ * touch, of Lblock + floor(log2(passes in it)) bits. */
+static int read_classic_block(pbr_t *r, cblk_t *cb, int style)
+{
+    uint32_t added, bit;
+    if (read_classic_passes(r, &added)) return -1;
+    for (;;) {
+        if (pbr_bits(r, 1, &bit)) return -1;
+        if (!bit) break;
+        cb->lblock++;
+    }
+    const uint32_t prev = cb->passes, cum = prev + added;
+    if (cum > 1 + 3 * 31) return -1;
+    uint32_t last = classic_segment(cb, style, prev), count = 0;
+    for (uint32_t p = prev; p <= cum; ++p) {
+        const uint32_t sg = p < cum ? classic_segment(cb, style, p) : last + 1;
+        if (p < cum && sg == last) {
+            count++;
+            continue;
+        }
+        if (count) {
+            const uint32_t nb = cb->lblock + ilog2u(count);
+            uint32_t len;
+            if (nb > 32 || pbr_bits(r, nb, &len)) return -1;
+            if (cb->ncontrib == cb->capcontrib) {
+                uint32_t cap = cb->capcontrib ? 2 * cb->capcontrib : 4;
+                contrib_t *v = (contrib_t *)realloc(cb->contrib, cap * sizeof(contrib_t));
+                if (!v) return -1;
+                cb->contrib = v;
+                cb->capcontrib = cap;
+            }
+            cb->contrib[cb->ncontrib++] = (contrib_t){NULL, len, last, count};
+        }
+        last = sg;
+        count = 1;
+    }
+    cb->passes = cum;
+    cb->layers_in++;
+    return 0;
+}
+
 /* Packets of one tile-part (segment.rs:54-133): while the header stream is
  * not at its end, the next packet of the progression: an SOP marker in the
  * body is skipped, the header is read from the header stream (the body
@@ -1067,6 +1161,10 @@ static int parse_tile_part(const tpart_t *tp, size_t *next, int ncomp, tcomp_t *
                         cb->zblk = v;
                     }
                     cb->included = 1;
+                    if (!(cp->cbstyle & ORACLE_CB_HT)) {
+                        if (read_classic_block(r, cb, cp->cbstyle)) return -1;
+                        continue;
+                    }
                     if (!first || cb->passes != 0) return -1;
                     uint32_t raw;
                     if (read_ht_passes(r, &raw) || raw > 94) return -1;
@@ -1111,6 +1209,17 @@ static int parse_tile_part(const tpart_t *tp, size_t *next, int ncomp, tcomp_t *
                 prec_band_t *pb = &rs->prec[(size_t)pg->p * rs->nbands + bi];
                 for (uint32_t k = 0; k < pb->ncbx * pb->ncby; ++k) {
                     cblk_t *cb = &pb->cbs[k];
+                    if (!(cp->cbstyle & ORACLE_CB_HT)) {
+                        /* this packet's contributions: those without data yet */
+                        for (uint32_t q = 0; q < cb->ncontrib; ++q) {
+                            contrib_t *ct = &cb->contrib[q];
+                            if (ct->data) continue;
+                            if (body + ct->len > tp->body_len) return -1;
+                            ct->data = tp->body + body;
+                            body += ct->len;
+                        }
+                        continue;
+                    }
                     if (cb->passes == 0 || cb->data) continue;
                     size_t need = (size_t)cb->lcup + cb->lref;
                     if (body + need > tp->body_len) return -1;
@@ -1162,6 +1271,82 @@ static float pow2i_f32(int e)
     return 1.0f / (float)(1u << (-e));
 }
 
+/* Classic code-blocks of one band (decode.rs:1189-1343, bitplane.rs:90-186):
+ * each block's contributions joined in codestream order; with selective
+ * bypass or termination on each pass the joined bytes split into codeword
+ * segments where the segment index changes, the passes of a segment
+ * following on from the previous one; a raw segment is one that starts
+ * after pass 9 on a significance-propagation or refinement pass. */
+static int decode_band_classic(const comp_t *c, res_t *rs, int bi, uint32_t K, float delta,
+                               double delta64, int use_f64)
+{
+    band_t *bd = &rs->bands[bi];
+    const uint32_t w = rw(bd->rect);
+    const int style = c->cbstyle;
+    const uint32_t np = rs->npx * rs->npy;
+    int rc = ORACLE_OK;
+    if (K > 32) return ORACLE_E_MALFORMED;
+    for (uint32_t p = 0; p < np && !rc; ++p) {
+        prec_band_t *pb = &rs->prec[(size_t)p * rs->nbands + bi];
+        for (uint32_t k = 0; k < pb->ncbx * pb->ncby && !rc; ++k) {
+            cblk_t *cb = &pb->cbs[k];
+            const uint32_t cw = rw(cb->rect), ch = rh(cb->rect);
+            const uint32_t planes = K > cb->zblk ? K - cb->zblk : 0;
+            if (cw == 0 || ch == 0 || cb->passes == 0 || planes == 0) continue;
+            size_t total = 0;
+            for (uint32_t q = 0; q < cb->ncontrib; ++q) total += cb->contrib[q].data ? cb->contrib[q].len : 0;
+            uint8_t *joined = (uint8_t *)malloc(total + 1);
+            oracle_seg_t *segs = (oracle_seg_t *)calloc(cb->ncontrib + 1, sizeof(oracle_seg_t));
+            int32_t *v = (int32_t *)malloc((size_t)cw * ch * sizeof(int32_t));
+            if (!joined || !segs || !v) {
+                free(joined);
+                free(segs);
+                free(v);
+                return ORACLE_E_NOMEM;
+            }
+            size_t at = 0, nseg = 0;
+            uint32_t passes = 0;
+            for (uint32_t q = 0; q < cb->ncontrib; ++q) {
+                const contrib_t *ct = &cb->contrib[q];
+                if (q == 0 || ct->seg != cb->contrib[q - 1].seg) {
+                    oracle_seg_t *sg = &segs[nseg++];
+                    sg->off = (uint32_t)at;
+                    sg->start = (uint8_t)passes;
+                    sg->arith = !(style & ORACLE_CB_BYPASS) || passes <= 9 || passes % 3 == 0;
+                }
+                oracle_seg_t *sg = &segs[nseg - 1];
+                if (ct->data) {
+                    memcpy(joined + at, ct->data, ct->len);
+                    at += ct->len;
+                    sg->len += ct->len;
+                }
+                passes += ct->passes;
+                sg->end = (uint8_t)passes;
+            }
+            st_blocks++;
+            if (cb->passes > st_max_passes) st_max_passes = cb->passes;
+            rc = oracle_classic_decode_block(joined, at, segs, nseg, cw, ch, planes, cb->passes,
+                                             bd->type, style, v, cw);
+            if (!rc) {
+                const uint32_t ox = cb->rect.x0 - bd->rect.x0, oy = cb->rect.y0 - bd->rect.y0;
+                for (uint32_t y = 0; y < ch; ++y)
+                    for (uint32_t x = 0; x < cw; ++x) {
+                        const int32_t q = v[(size_t)y * cw + x];
+                        const size_t o = (size_t)(oy + y) * w + ox + x;
+                        if (use_f64)
+                            bd->coef64[o] = (double)q * delta64;
+                        else
+                            bd->coef[o] = (float)q * delta;
+                    }
+            }
+            free(joined);
+            free(segs);
+            free(v);
+        }
+    }
+    return rc;
+}
+
 static int decode_band(const comp_t *c, res_t *rs, int bi, uint32_t r, int use_f64)
 {
     band_t *bd = &rs->bands[bi];
@@ -1191,6 +1376,7 @@ static int decode_band(const comp_t *c, res_t *rs, int bi, uint32_t r, int use_f
     uint32_t np = rs->npx * rs->npy;
     uint32_t *buf = NULL;
     size_t bufcap = 0;
+    if (!(c->cbstyle & ORACLE_CB_HT)) return decode_band_classic(c, rs, bi, K, delta, delta64, use_f64);
     for (uint32_t p = 0; p < np; ++p) {
         prec_band_t *pb = &rs->prec[(size_t)p * rs->nbands + bi];
         for (uint32_t k = 0; k < pb->ncbx * pb->ncby; ++k) {
@@ -1436,13 +1622,11 @@ static int decode_image(const uint8_t *data, size_t len, uint32_t denominator, i
     }
     for (uint32_t ti = 0; ti < ntiles && !rc; ++ti) {
         tile_t *t = &tiles[ti];
+        /* several quality layers: classic code-blocks only (an HT block
+         * included twice is malformed in the packet parser) */
         for (int c = 0; c < h->ncomp; ++c)
-            if (!(t->comp[c].cbstyle & 0x40)) rc = ORACLE_E_UNSUPPORTED; /* classic */
+            if ((t->comp[c].cbstyle & ORACLE_CB_HT) && t->nlayers > 1) rc = ORACLE_E_UNSUPPORTED;
         if (rc) break;
-        if (t->nlayers > 1) {
-            rc = ORACLE_E_UNSUPPORTED;
-            break;
-        }
         for (int c = 0; c < h->ncomp && !rc; ++c) rc = build_tcomp(h, &t->comp[c], t->rect, &tcs[c]);
         if (!rc) {
             for (int c = 0; c < h->ncomp; ++c)

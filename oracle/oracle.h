@@ -84,6 +84,37 @@ int oracle_ht_decode_block(const uint8_t *data, uint32_t lcup, uint32_t lref, ui
 
 int32_t oracle_coefficient_to_i32(uint32_t value, uint32_t k_max);
 
+/* Code-block style bits of SPcod/SPcoc (Table A.19). */
+enum {
+    ORACLE_CB_BYPASS = 0x01,
+    ORACLE_CB_RESET = 0x02,
+    ORACLE_CB_TERMALL = 0x04,
+    ORACLE_CB_CAUSAL = 0x08,
+    ORACLE_CB_PTERM = 0x10,
+    ORACLE_CB_SEGSYM = 0x20,
+    ORACLE_CB_HT = 0x40
+};
+
+/* One codeword segment of a classic code-block: bytes [off, off + len) of
+ * the block's combined data carry passes [start, end); arith 0 marks a raw
+ * (bypass) segment. */
+typedef struct {
+    uint32_t off, len;
+    uint8_t start, end, arith, pad;
+} oracle_seg_t;
+
+/* Segment index of coding pass p under selective bypass (Table D.9). */
+uint32_t oracle_bypass_segment(uint32_t p);
+
+/* One classic (EBCOT) code-block: `bitplanes` = the sub-band's bit-planes
+ * less the missing ones, `passes` coding passes; band 0 LL, 1 HL, 2 LH,
+ * 3 HH; style the code-block style bits.  Without bypass or termination on
+ * each pass the whole data is one arithmetic segment and segs is unused.
+ * Writes signed integer coefficients (bit-plane 0 at bit 0). */
+int oracle_classic_decode_block(const uint8_t *data, size_t len, const oracle_seg_t *segs,
+                                size_t nseg, uint32_t w, uint32_t h, uint32_t bitplanes,
+                                uint32_t passes, int band, int style, int32_t *out, size_t stride);
+
 /* One 2D synthesis level over [x0,x1) x [y0,y1) from four f32 bands
  * (idwt.rs:535-1143); out holds the level row-major. */
 int oracle_idwt_level(int reversible, uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1,

@@ -32,6 +32,7 @@
 #include "ht_coop.hip"
 #include "ht_rows.hip"
 #include "idwt.hip"
+#include "t1_classic.hip"
 #include "cost_model.hip"
 
 using namespace sgn;
@@ -332,6 +333,15 @@ struct sgn_j2k_plan {
     uint32_t *d_fin_redo = nullptr;  // 5/3 final blocks left to the int32-plane pass
     uint8_t *d_status_lane = nullptr, *d_status_wave = nullptr;
     uint32_t *d_queue = nullptr;  // work counter of the persistent lane kernel
+    // classic (EBCOT) code-blocks, by kernel shape (small, big, wide:
+    // [classic_begin[v], classic_begin[v + 1])), costliest first; their
+    // segment table; bytes of blocks joined from several packets
+    std::vector<ClassicJob> classic;
+    uint32_t classic_begin[4] = {0, 0, 0, 0};
+    std::vector<ClassicSeg> cseg;
+    std::vector<uint8_t> joined;
+    ClassicJob *d_classic = nullptr;
+    ClassicSeg *d_cseg = nullptr;
     int lane_variant = kDefaultLaneVariant;  // index into kRowsVariants
     int dwt_shape = 0;                       // index into kDwtShapes
     hipGraphExec_t graph_exec = nullptr;
@@ -521,8 +531,27 @@ Status build_plan(const sgn_tile_req *reqs, size_t n, sgn_j2k_plan &P, int32_t *
         coef_base[i] = coef;
         coef += lps[i].coef_elems;
     }
-    // the HT kernels read whole aligned 16-byte units: pad the arena so the
-    // unit holding its last byte lies inside the allocation
+    // classic code-blocks whose contributions were joined on the host: one
+    // more run after the codestreams
+    std::vector<uint64_t> joined_base(n, 0);
+    {
+        uint64_t total = 0;
+        for (const LocalPlan &lp : lps) total += lp.joined.size();
+        P.joined.clear();
+        if (total) {
+            P.joined.reserve(total);
+            arena = (arena + 15) & ~uint64_t(15);
+            for (size_t i = 0; i < n; ++i) {
+                joined_base[i] = arena + P.joined.size();
+                P.joined.insert(P.joined.end(), lps[i].joined.begin(), lps[i].joined.end());
+            }
+            P.runs.push_back({P.joined.data(), arena, total});
+            arena += total;
+        }
+    }
+    // the HT kernels read whole aligned 16-byte units and the classic kernel
+    // a window of three dwords: pad the arena so that both stay inside the
+    // allocation
     arena = (arena + 31) & ~uint64_t(15);
     if (arena >= (1ull << 32) || coef >= (1ull << 32)) {
         set_last_error("batch too large for 32-bit plan offsets; split it");
@@ -535,7 +564,10 @@ Status build_plan(const sgn_tile_req *reqs, size_t n, sgn_j2k_plan &P, int32_t *
         for (const LocalPlan &lp : lps) kmax = std::max(kmax, lp.max_k);
         const char *env = std::getenv("SGN_COEF_NARROW");
         const bool allow = !(env && env[0] == '0');
-        P.narrow = allow && kmax <= 15 && 2 * coef < (1ull << 32);
+        bool classic = false;
+        for (const LocalPlan &lp : lps) classic |= !lp.classic.empty();
+        // classic blocks OR magnitude bits into int32 samples
+        P.narrow = allow && !classic && kmax <= 15 && 2 * coef < (1ull << 32);
     }
     P.lane_plain_begin.assign(1, 0);
     P.wave_begin.assign(1, 0);
@@ -566,6 +598,15 @@ Status build_plan(const sgn_tile_req *reqs, size_t n, sgn_j2k_plan &P, int32_t *
             else
                 P.cb_wave.push_back(j);
         }
+        const uint32_t seg_base = static_cast<uint32_t>(P.cseg.size());
+        P.cseg.insert(P.cseg.end(), lp.cseg.begin(), lp.cseg.end());
+        for (ClassicJob j : lp.classic) {
+            j.src += static_cast<uint32_t>((j.style & kClassicJoined) ? joined_base[i] : arena_base[i]);
+            j.dst += cbase;
+            j.seg += seg_base;
+            j.tile = static_cast<uint32_t>(i);
+            P.classic.push_back(j);
+        }
         P.lane_plain_begin.push_back(static_cast<uint32_t>(P.cb_lane_plain.size()));
         P.wave_begin.push_back(static_cast<uint32_t>(P.cb_wave.size()));
         auto rebase = [&](LevelJob &l) {
@@ -584,6 +625,24 @@ Status build_plan(const sgn_tile_req *reqs, size_t n, sgn_j2k_plan &P, int32_t *
     // code-blocks for the batched lane kernel, ordered by predicted cost so
     // that a wave's lanes finish together
     sort_by_cost(P.cb_lane_plain, P.cb_lane);
+    // classic blocks: by kernel shape, costliest first within each
+    {
+        auto shape = [](const ClassicJob &j) {
+            const uint32_t w = classic_words(j.w, j.h);
+            return w <= uint32_t(ClassicSmall::kWords) ? 0 : (w <= uint32_t(ClassicBig::kWords) ? 1 : 2);
+        };
+        std::stable_sort(P.classic.begin(), P.classic.end(),
+                         [&](const ClassicJob &a, const ClassicJob &b) {
+                             const int sa = shape(a), sb = shape(b);
+                             return sa != sb ? sa < sb : a.cost > b.cost;
+                         });
+        P.classic_begin[0] = 0;
+        for (int v = 0; v < 3; ++v) {
+            uint32_t e = P.classic_begin[v];
+            while (e < P.classic.size() && shape(P.classic[e]) == v) ++e;
+            P.classic_begin[v + 1] = e;
+        }
+    }
     // kernel of a plan nobody autotuned: below kCoopMaxBlocks code-blocks the
     // lane kernel cannot fill the device (each wave takes 64 blocks) and its
     // time is one block's serial decode, so one block per wave finishes first
@@ -715,6 +774,7 @@ Status upload_plan(sgn_hip_session *s, const sgn_tile_req *jobs, sgn_j2k_plan &P
     const uint64_t o_flv = take(bytes_of(P.final_lv));
     const uint64_t o_lbj0 = take(bytes_of(P.lv_block_job[0])), o_lbj1 = take(bytes_of(P.lv_block_job[1]));
     const uint64_t o_fbj0 = take(bytes_of(P.fin_block_job[0])), o_fbj1 = take(bytes_of(P.fin_block_job[1]));
+    const uint64_t o_cls = take(bytes_of(P.classic)), o_cseg = take(bytes_of(P.cseg));
     const uint64_t host_bytes = off;
     const uint64_t o_stl = take(P.cb_lane.size()), o_stw = take(P.cb_wave.size());
     const uint64_t o_ff = take(P.fin_block_job[1].size() * sizeof(uint32_t));
@@ -749,6 +809,8 @@ Status upload_plan(sgn_hip_session *s, const sgn_tile_req *jobs, sgn_j2k_plan &P
         HIP_TRY(put_dev(o_lbj1, P.lv_block_job[1]));
         HIP_TRY(put_dev(o_fbj0, P.fin_block_job[0]));
         HIP_TRY(put_dev(o_fbj1, P.fin_block_job[1]));
+        HIP_TRY(put_dev(o_cls, P.classic));
+        HIP_TRY(put_dev(o_cseg, P.cseg));
         HIP_TRY(hipStreamSynchronize(copy));
         P.t_stage = 0;
         P.t_copy = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() -
@@ -803,6 +865,8 @@ Status upload_plan(sgn_hip_session *s, const sgn_tile_req *jobs, sgn_j2k_plan &P
         put(o_lbj1, P.lv_block_job[1]);
         put(o_fbj0, P.fin_block_job[0]);
         put(o_fbj1, P.fin_block_job[1]);
+        put(o_cls, P.classic);
+        put(o_cseg, P.cseg);
         const auto t1 = std::chrono::steady_clock::now();
         uint8_t *d = static_cast<uint8_t *>(P.d_block);
         // direct runs first, then the staged range: consecutive staged runs and
@@ -860,6 +924,8 @@ Status upload_plan(sgn_hip_session *s, const sgn_tile_req *jobs, sgn_j2k_plan &P
     at(o_lbj1, P.d_lv_block_job[1], !P.lv_block_job[1].empty());
     at(o_fbj0, P.d_fin_block_job[0], !P.fin_block_job[0].empty());
     at(o_fbj1, P.d_fin_block_job[1], !P.fin_block_job[1].empty());
+    at(o_cls, P.d_classic, !P.classic.empty());
+    at(o_cseg, P.d_cseg, !P.cseg.empty());
     at(o_stl, P.d_status_lane, !P.cb_lane.empty());
     at(o_stw, P.d_status_wave, !P.cb_wave.empty());
     at(o_ff, P.d_fin_redo, !P.fin_block_job[1].empty());
@@ -943,6 +1009,31 @@ uint32_t launch_wave(sgn_hip_session *s, sgn_j2k_plan &P, const CbJob *d_jobs, u
     return 1;
 }
 
+// Every classic code-block of the plan: one launch per kernel shape (one
+// block per lane).
+uint32_t launch_classic(sgn_hip_session *s, sgn_j2k_plan &P) {
+    uint32_t k = 0;
+    for (int v = 0; v < 3; ++v) {
+        const uint32_t a = P.classic_begin[v], b = P.classic_begin[v + 1];
+        if (a >= b) continue;
+        const uint32_t n = b - a;
+        const ClassicJob *jobs = P.d_classic + a;
+        int32_t *coef = reinterpret_cast<int32_t *>(P.d_coef);
+        if (v == 0)
+            hipLaunchKernelGGL(sgn_t1_classic<ClassicSmall>, dim3((n + 63) / 64), dim3(64), 0,
+                               s->stream, jobs, n, P.d_arena, P.d_cseg, coef);
+        else if (v == 1)
+            hipLaunchKernelGGL(sgn_t1_classic<ClassicBig>, dim3((n + 63) / 64), dim3(64), 0,
+                               s->stream, jobs, n, P.d_arena, P.d_cseg, coef);
+        else
+            hipLaunchKernelGGL(sgn_t1_classic<ClassicWide>,
+                               dim3((n + ClassicWide::kLanes - 1) / ClassicWide::kLanes), dim3(64),
+                               0, s->stream, jobs, n, P.d_arena, P.d_cseg, coef);
+        ++k;
+    }
+    return k;
+}
+
 // True when every level of [a, b) is at most 32x32 (one Small block).
 bool small_levels(const std::vector<LevelJob> &v, uint32_t a, uint32_t b) {
     for (uint32_t j = a; j < b; ++j)
@@ -1002,6 +1093,7 @@ uint32_t launch_finals(sgn_hip_session *s, sgn_j2k_plan &P, int rev, uint32_t a,
 
 uint32_t enqueue_batched(sgn_hip_session *s, sgn_j2k_plan &P) {
     uint32_t k = clear_counters(s, P, 2);  // one node for both counters
+    k += launch_classic(s, P);
     k += launch_lane(s, P, P.d_cb_lane, static_cast<uint32_t>(P.cb_lane.size()), P.d_status_lane,
                      false);
     k += launch_wave(s, P, P.d_cb_wave, static_cast<uint32_t>(P.cb_wave.size()), P.d_status_wave);
@@ -1014,7 +1106,8 @@ uint32_t enqueue_batched(sgn_hip_session *s, sgn_j2k_plan &P) {
 }
 
 uint32_t enqueue_plain(sgn_hip_session *s, sgn_j2k_plan &P) {
-    uint32_t k = 0;
+    // classic code-blocks of every tile first (one launch per kernel shape)
+    uint32_t k = launch_classic(s, P);
     for (size_t t = 0; t < P.ntiles; ++t) {
         const uint32_t a0 = P.lane_plain_begin[t], a1 = P.lane_plain_begin[t + 1];
         // one tile's code-blocks: the cooperative kernel (a tile has far
@@ -1629,6 +1722,7 @@ int sgn_j2k_plan_decode_stage(sgn_hip_session *s, sgn_j2k_plan *P, int stage) {
     HIP_TRY(hipSetDevice(s->device));
     uint32_t k = 0;
     if (stage == SGN_STAGE_BLOCKS) {
+        k += launch_classic(s, *P);
         k += launch_lane(s, *P, P->d_cb_lane, static_cast<uint32_t>(P->cb_lane.size()),
                          P->d_status_lane);
         k += launch_wave(s, *P, P->d_cb_wave, static_cast<uint32_t>(P->cb_wave.size()),
@@ -1653,7 +1747,7 @@ int sgn_j2k_plan_decode_stage(sgn_hip_session *s, sgn_j2k_plan *P, int stage) {
 int sgn_j2k_plan_get_stats(const sgn_j2k_plan *P, sgn_j2k_plan_stats *st) {
     if (!P || !st) return SGN_E_ARG;
     st->tiles = P->ntiles;
-    st->code_blocks = P->cb_lane.size() + P->cb_wave.size();
+    st->code_blocks = P->cb_lane.size() + P->cb_wave.size() + P->classic.size();
     st->coded_bytes = P->coded_bytes;
     st->output_bytes = P->out_bytes;
     st->output_pixels = P->out_pixels;

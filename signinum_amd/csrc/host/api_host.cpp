@@ -132,6 +132,10 @@ extern "C" int sgn_j2k_code_blocks(const uint8_t *bytes, size_t len, sgn_ht_code
         uint64_t packed = 0;
         for (const sgn::TileBody &t : tiles) {
             for (int c = 0; c < h.ncomp; ++c) {
+                if (!(t.comp[c].cbstyle & sgn::kCbsHt)) {
+                    sgn::set_last_error("classic (EBCOT) code-blocks have no HT block jobs");
+                    return SGN_E_UNSUPPORTED;
+                }
                 st = sgn::build_tile_comp(t.comp[c], t.rect, tcs[c]);
                 if (st) return st;
             }

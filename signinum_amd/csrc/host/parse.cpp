@@ -187,7 +187,69 @@ struct CbState {
     uint32_t zblk = 0, lblock = 3, passes = 0, lcup = 0, lref = 0;
     int64_t data = -1;  // offset of the block data in the codestream
     bool included = false;
+    uint32_t layers_in = 0;            // classic: layers that contributed
+    std::vector<ClassicPiece> pieces;  // classic: contributions, offsets set from the body
 };
+
+// Table B.4: number of coding passes (segment.rs:481-506).
+bool classic_passes(PacketBits &r, uint32_t &n) {
+    uint32_t b, v;
+    if (!r.get(b)) return false;
+    if (!b) return n = 1, true;
+    if (!r.get(b)) return false;
+    if (!b) return n = 2, true;
+    if (!r.bits(2, v)) return false;
+    if (v < 3) return n = 3 + v, true;
+    if (!r.bits(5, v)) return false;
+    if (v < 31) return n = 6 + v, true;
+    if (!r.bits(7, v)) return false;
+    return n = 37 + v, true;
+}
+
+// Codeword segment of coding pass p (segment.rs:223-239, Table D.9).
+uint32_t classic_segment(const CbState &s, uint8_t style, uint32_t p) {
+    if (style & kCbsTermAll) return p;
+    if (style & kCbsBypass) return p < 10 ? 0 : 1 + 2 * ((p - 10) / 3) + ((p - 10) % 3 == 2);
+    return s.layers_in;
+}
+
+// A classic block's header after inclusion (segment.rs:168-301): passes,
+// Lblock increment, one length per codeword segment the new passes touch
+// (Lblock + floor(log2(its passes)) bits).
+bool classic_block_header(PacketBits &r, CbState &s, uint8_t style) {
+    uint32_t added, b;
+    if (!classic_passes(r, added)) return false;
+    for (;;) {
+        if (!r.get(b)) return false;
+        if (!b) break;
+        ++s.lblock;
+    }
+    const uint32_t prev = s.passes, cum = prev + added;
+    if (cum > 1 + 3 * 31) return false;
+    uint32_t last = classic_segment(s, style, prev), count = 0;
+    for (uint32_t p = prev; p <= cum; ++p) {
+        const uint32_t sg = p < cum ? classic_segment(s, style, p) : last + 1;
+        if (p < cum && sg == last) {
+            ++count;
+            continue;
+        }
+        if (count) {
+            const uint32_t nb = s.lblock + (31 - __builtin_clz(count));
+            uint32_t len;
+            if (nb > 32 || !r.bits(nb, len)) return false;
+            ClassicPiece pc;
+            pc.len = len;
+            pc.seg = last;
+            pc.passes = count;
+            s.pieces.push_back(pc);
+        }
+        last = sg;
+        count = 1;
+    }
+    s.passes = cum;
+    ++s.layers_in;
+    return true;
+}
 
 }  // namespace
 
@@ -533,6 +595,10 @@ Status parse_packets(const uint8_t *cs, const TileBody &t, int ncomp, const Tile
                             s.zblk = v;
                         }
                         s.included = true;
+                        if (!(cp.cbstyle & kCbsHt)) {
+                            if (!classic_block_header(r, s, cp.cbstyle)) { ok = false; break; }
+                            continue;
+                        }
                         if (!first || s.passes != 0) { ok = false; break; }
                         // HT pass count codeword (segment.rs:508-530)
                         uint32_t np = 1, b1;
@@ -592,6 +658,17 @@ Status parse_packets(const uint8_t *cs, const TileBody &t, int ncomp, const Tile
                     const PrecBand &pb = g.precs[rs.prec_base + pk.prec * rs.nbands + bi];
                     for (uint32_t k = 0; k < pb.ncbx * pb.ncby; ++k) {
                         CbState &s = state[pk.comp][pb.cb_base + k];
+                        if (!(cp.cbstyle & kCbsHt)) {
+                            // this packet's contributions: the ones without an offset yet
+                            for (ClassicPiece &pc : s.pieces) {
+                                if (pc.offset >= 0) continue;
+                                if (bpos + pc.len > blen) { ok = false; break; }
+                                pc.offset = int64_t(tp.off + bpos);
+                                bpos += pc.len;
+                            }
+                            if (!ok) break;
+                            continue;
+                        }
                         if (s.passes == 0 || s.data >= 0) continue;
                         const size_t need = size_t(s.lcup) + s.lref;
                         if (bpos + need > blen) { ok = false; break; }
@@ -610,6 +687,11 @@ Status parse_packets(const uint8_t *cs, const TileBody &t, int ncomp, const Tile
             const CbState &s = state[c][k];
             CbSeg &o = segs[c][k];
             o.passes = s.data >= 0 ? s.passes : 0;
+            if (!(t.comp[c].cbstyle & kCbsHt)) {
+                // contributions whose packet body was never read have no bytes
+                o.passes = s.passes;
+                o.pieces = s.pieces;
+            }
             o.zblk = s.zblk;
             o.lcup = s.lcup;
             o.lref = s.lref;

@@ -25,9 +25,18 @@ struct TileBody {
     std::vector<TilePart> parts;
 };
 
+// One packet's contribution to a classic code-block: `len` bytes at
+// codestream offset `offset` of codeword segment `seg`, carrying `passes`
+// coding passes.
+struct ClassicPiece {
+    int64_t offset = -1;
+    uint32_t len = 0, seg = 0, passes = 0;
+};
+
 struct CbSeg {
     int64_t offset = -1;  // byte offset of cleanup||refinement in the codestream
     uint32_t lcup = 0, lref = 0, zblk = 0, passes = 0;
+    std::vector<ClassicPiece> pieces;  // classic code-blocks: contributions in order
 };
 
 // signinum-core Colorspace (crates/signinum-core/src/types.rs:9-20), as
@@ -102,6 +111,13 @@ struct LocalPlan {
     size_t row_bytes = 0;
     uint64_t out_bytes = 0;
     uint64_t pruned_cbs = 0;         // code-blocks a region decode does not need
+    // classic (EBCOT) code-blocks: jobs with `src` in the codestream, or, for
+    // blocks whose contributions are not contiguous there (several layers),
+    // in `joined` (flag kClassicJoined); segment table of the blocks with
+    // selective bypass or termination on each pass
+    std::vector<ClassicJob> classic;
+    std::vector<ClassicSeg> cseg;
+    std::vector<uint8_t> joined;
 };
 
 // Region of interest in full-resolution image coordinates (Rect of

@@ -111,6 +111,90 @@ bool meets(const Rect &a, const Rect &b) {
 
 }  // namespace
 
+// A classic code-block's job (decode.rs:1189-1343; bitplane.rs:90-186):
+// its contributions joined in codestream order (in place when they are
+// contiguous there), and with selective bypass or termination on each pass
+// the codeword segments, split where the segment index changes; a segment
+// is raw when it starts after pass 9 on a significance-propagation or
+// refinement pass.
+static Status classic_job(const uint8_t *cs, const CompParams &cp, const BandGeom &bg, const CbGeom &cg,
+                   const CbSeg &sg, uint32_t K, uint32_t band_off, LocalPlan &lp) {
+    ClassicJob j{};
+    j.stride = coef_pitch(bg.rect.w());
+    j.dst = band_off + (cg.rect.y0 - bg.rect.y0) * j.stride + (cg.rect.x0 - bg.rect.x0);
+    j.w = static_cast<uint16_t>(cg.rect.w());
+    j.h = static_cast<uint16_t>(cg.rect.h());
+    if (j.w == 0 || j.h == 0) return kOk;
+    // the decoder's per-block state: (w + 2) x (stripes + 2) words, at most
+    // 3080 (a 1024x4 block)
+    if ((uint32_t(j.w) + 2) * (((uint32_t(j.h) + 3) >> 2) + 2) > 3080) {
+        set_last_error("classic code-block shape too large");
+        return kErrUnsupported;
+    }
+    if (K > 32) {
+        set_last_error("too many bit-planes");
+        return kErrMalformed;
+    }
+    const uint32_t planes = K > sg.zblk ? K - sg.zblk : 0;  // non-strict: saturating
+    if (planes > 31) {
+        set_last_error("32 magnitude bit-planes in a classic code-block");
+        return kErrUnsupported;
+    }
+    j.planes = static_cast<uint8_t>(planes);
+    j.passes = static_cast<uint8_t>(sg.passes);
+    j.band = static_cast<uint8_t>(bg.type & 3u);
+    j.style = static_cast<uint8_t>(cp.cbstyle & 0x3F);
+    // data: in place when the contributions follow each other
+    int64_t first = -1, end = -1;
+    bool contiguous = true;
+    uint64_t total = 0;
+    for (const ClassicPiece &pc : sg.pieces) {
+        if (pc.offset < 0 || pc.len == 0) continue;
+        if (first < 0) first = pc.offset;
+        else if (pc.offset != end) contiguous = false;
+        end = pc.offset + pc.len;
+        total += pc.len;
+    }
+    if (total >= (1ull << 31)) {
+        set_last_error("classic code-block data too long");
+        return kErrMalformed;
+    }
+    j.len = static_cast<uint32_t>(total);
+    if (contiguous || total == 0) {
+        j.src = first < 0 ? 0u : static_cast<uint32_t>(first);
+    } else {
+        j.src = static_cast<uint32_t>(lp.joined.size());
+        j.style |= kClassicJoined;
+        for (const ClassicPiece &pc : sg.pieces)
+            if (pc.offset >= 0) lp.joined.insert(lp.joined.end(), cs + pc.offset, cs + pc.offset + pc.len);
+    }
+    if (j.style & (kCbsBypass | kCbsTermAll)) {
+        j.seg = static_cast<uint32_t>(lp.cseg.size());
+        uint32_t at = 0, passes = 0;
+        for (size_t q = 0; q < sg.pieces.size(); ++q) {
+            const ClassicPiece &pc = sg.pieces[q];
+            if (q == 0 || pc.seg != sg.pieces[q - 1].seg) {
+                ClassicSeg s{};
+                s.off = at;
+                s.start = static_cast<uint8_t>(passes);
+                s.raw = (j.style & kCbsBypass) && passes > 9 && passes % 3 != 0;
+                lp.cseg.push_back(s);
+            }
+            ClassicSeg &s = lp.cseg.back();
+            if (pc.offset >= 0) {
+                at += pc.len;
+                s.len += pc.len;
+            }
+            passes += pc.passes;
+            s.end = static_cast<uint8_t>(passes);
+        }
+        j.nseg = static_cast<uint32_t>(lp.cseg.size()) - j.seg;
+    }
+    j.cost = j.planes && j.passes ? j.len + uint32_t(j.w) * j.h * j.passes / 8 : 0;
+    lp.classic.push_back(j);
+    return kOk;
+}
+
 Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denominator,
                         uint64_t dst, uint64_t pitch, uint64_t capacity, LocalPlan &lp,
                         const Region &roi) {
@@ -153,13 +237,11 @@ Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denomin
     // (region decodes), absolute coordinates of that resolution
     std::vector<std::vector<Rect>> need(static_cast<size_t>(nc));
     for (const TileBody &t : tiles) {
-        if (t.layers > 1) {
-            set_last_error("multi-layer HTJ2K packet assembly");
-            return kErrUnsupported;
-        }
         for (int c = 0; c < nc; ++c) {
-            if (!(t.comp[c].cbstyle & 0x40)) {
-                set_last_error("classic (EBCOT) code-blocks are not on the HT path");
+            // several quality layers: classic code-blocks only (an HT block
+            // included twice is malformed in the packet parser)
+            if (t.layers > 1 && (t.comp[c].cbstyle & kCbsHt)) {
+                set_last_error("multi-layer HTJ2K packet assembly");
                 return kErrUnsupported;
             }
             st = build_tile_comp(t.comp[c], t.rect, tcs[c]);
@@ -234,6 +316,18 @@ Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denomin
                 if (bg.res >= R) continue;
                 const CbSeg &sg = segs[c][k];
                 const uint32_t K = band_k[cg.band];
+                if (!(cp.cbstyle & kCbsHt)) {
+                    if (roi.on) {
+                        const Rect bn = bg.res == 0 ? need[c][0] : need_of(need[c][bg.res], bg.rect);
+                        if (!meets(cg.rect, bn)) {
+                            ++lp.pruned_cbs;
+                            continue;
+                        }
+                    }
+                    st = classic_job(cs, cp, bg, cg, sg, K, band_off[cg.band], lp);
+                    if (st) return st;
+                    continue;
+                }
                 CbJob j{};
                 j.stride = coef_pitch(bg.rect.w());
                 j.dst = band_off[cg.band] + (cg.rect.y0 - bg.rect.y0) * j.stride +

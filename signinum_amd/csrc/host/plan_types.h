@@ -25,6 +25,40 @@ static_assert(sizeof(CbJob) == 32, "CbJob layout");
 // (0 LL, 1 HL, 2 LH, 3 HH; a feature of the decode-cost model)
 enum : uint8_t { kCbStripeCausal = 1, kCbZero = 2, kCbBandShift = 2 };
 
+// One classic (EBCOT) code-block (the device form of J2kCodeBlockDecodeJob,
+// crates/signinum-j2k-native/src/lib.rs:268-296): MQ / raw codeword
+// segments decoded into int32 coefficients.
+struct ClassicJob {
+    uint32_t src;     // byte offset of the block's joined segments in the arena
+    uint32_t dst;     // int32 index of the block's first sample in the coefficient arena
+    uint32_t stride;  // band row pitch in elements
+    uint32_t len;     // bytes of joined segments
+    uint16_t w, h;
+    uint8_t planes;   // bit-planes: the band's, less the block's missing ones
+    uint8_t passes;   // coding passes present
+    uint8_t band;     // 0 LL, 1 HL, 2 LH, 3 HH
+    uint8_t style;    // code-block style bits (Table A.19) | kClassicJoined
+    uint32_t seg;     // first entry of the block's segments in the segment table
+    uint32_t nseg;    // 0: one arithmetic segment over all of the data
+    uint32_t tile;    // owning tile job
+    uint32_t cost;    // ordering key
+};
+static_assert(sizeof(ClassicJob) == 40, "ClassicJob layout");
+
+// A codeword segment: bytes [off, off + len) of the block's data carry
+// coding passes [start, end); raw: bypass (no arithmetic coding).
+struct ClassicSeg {
+    uint32_t off, len;
+    uint8_t start, end, raw, pad;
+};
+static_assert(sizeof(ClassicSeg) == 12, "ClassicSeg layout");
+
+enum : uint8_t {
+    kCbsBypass = 0x01, kCbsReset = 0x02, kCbsTermAll = 0x04, kCbsCausal = 0x08,
+    kCbsSegSym = 0x20, kCbsHt = 0x40,
+    kClassicJoined = 0x80  // src indexes the plan's joined-data run, not the codestream
+};
+
 // Input features of the code-block decode-cost model (5 values, each
 // roughly in [0, 1]): coded bytes, coding passes, missing MSBs (Zblk),
 // sub-band and sample count.
