@@ -537,6 +537,10 @@ typedef struct {
      * [x0, x0 + width) x [y0, y0 + height) */
     uint32_t image_x0, image_y0, tile_x0, tile_y0;
     uint32_t packed_headers; /* 0: packet headers in the packets, 1: PPM, 2: PPT */
+    uint32_t block_coder;    /* 0: HT (Part 15), 1: classic EBCOT / MQ (Part 1) */
+    uint32_t cb_style;       /* classic code-block style bits: 0x01 selective bypass,
+                                0x02 context reset, 0x04 termination on each pass,
+                                0x08 vertically causal context, 0x20 segmentation symbols */
 } sgn_j2k_encode_options;
 
 void sgn_j2k_encode_default_options(sgn_j2k_encode_options *opts);

@@ -23,6 +23,7 @@ __all__ = [
     "MalformedError", "CodeBlockDecodeError", "BufferError", "DeviceError", "TileBatchError",
     "UnsupportedBackendError", "HipSession", "HipEvent", "J2kDecoder", "DeviceSurface", "TilePlan",
     "decode_tiles_into", "decode_tiles_to_device", "encode_j2k_lossless", "encode_j2k_ht",
+    "encode_j2k_classic",
     "HtCodeBlockDecoder", "code_block_jobs", "CostModel", "MODE_PLAIN", "MODE_BATCHED",
     "MODE_GRAPH", "extract_dicom_encapsulated_frames", "InvalidRegionError", "region_dims",
     "decode_tiles_region_scaled_into",
@@ -819,6 +820,15 @@ def encode_j2k_ht(samples, bit_depth=None, **options):
                                    ctypes.byref(o), out.ctypes.data, n.value, ctypes.byref(n))
     _check(rc)
     return out[:n.value].tobytes()
+
+
+def encode_j2k_classic(samples, bit_depth=None, cb_style=0, **options):
+    """Classic (EBCOT / MQ, Part 1) encode, every coding pass; cb_style is
+    the code-block style (0x01 selective bypass, 0x02 context reset, 0x04
+    termination on each pass, 0x08 vertically causal context, 0x20
+    segmentation symbols).  Other options as encode_j2k_ht."""
+    return encode_j2k_ht(samples, bit_depth=bit_depth, block_coder=1, cb_style=cb_style,
+                         **options)
 
 
 def encode_j2k_lossless(samples, bit_depth=None, levels=5, cb_log2=6, progression=0):

@@ -98,7 +98,8 @@ class EncodeOptions(ctypes.Structure):
         "guard_bits")] + [("base_step", ctypes.c_float)] +
         [(n, ctypes.c_uint32) for n in ("precinct_log2", "sop", "eph", "tile_width",
                                          "tile_height", "image_x0", "image_y0", "tile_x0",
-                                         "tile_y0", "packed_headers")])
+                                         "tile_y0", "packed_headers", "block_coder",
+                                         "cb_style")])
 
 
 class HtCodeBlockBatchJob(ctypes.Structure):

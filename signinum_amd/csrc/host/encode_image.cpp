@@ -10,6 +10,7 @@
 #include <cstring>
 #include <vector>
 
+#include "encode_classic.hpp"
 #include "j2k_host.hpp"
 #include "sgn_api.hpp"
 
@@ -177,6 +178,36 @@ void tag_set_leaves(TagTree &t, const std::vector<uint32_t> &leaves) {
             }
 }
 
+// A classic block's header after its zero bit-plane code (B.10.6-B.10.7):
+// the pass count (Table B.4), Lblock raised until every segment length
+// fits in Lblock + floor(log2(its passes)) bits, the lengths.
+void classic_block_header(BitWriter &bw, const ClassicBlockCode &bc) {
+    const uint32_t n = bc.passes;
+    if (n == 1) {
+        bw.bit(0);
+    } else if (n == 2) {
+        bw.bits(2, 2);
+    } else if (n <= 5) {
+        bw.bits(0xC | (n - 3), 4);
+    } else if (n <= 36) {
+        bw.bits(0xF, 4);
+        bw.bits(n - 6, 5);
+    } else {
+        bw.bits(0x1FF, 9);
+        bw.bits(n - 37, 7);
+    }
+    uint32_t lblock = 3;
+    for (const ClassicBlockCode::Segment &sg : bc.segments) {
+        const uint32_t lg = 31 - __builtin_clz(sg.passes);
+        while (sg.bytes.size() >= (uint64_t(1) << (lblock + lg))) ++lblock;
+    }
+    for (uint32_t k = 3; k < lblock; ++k) bw.bit(1);
+    bw.bit(0);
+    for (const ClassicBlockCode::Segment &sg : bc.segments)
+        bw.bits(static_cast<uint32_t>(sg.bytes.size()),
+                static_cast<int>(lblock + (31 - __builtin_clz(sg.passes))));
+}
+
 void put16(std::vector<uint8_t> &o, uint32_t v) {
     o.push_back(static_cast<uint8_t>(v >> 8));
     o.push_back(static_cast<uint8_t>(v));
@@ -248,7 +279,14 @@ Status encode_image(const void *pixels, uint32_t width, uint32_t height, uint32_
     cp.levels = static_cast<uint8_t>(levels);
     cp.xcb = static_cast<uint8_t>(opt.cb_width_log2);
     cp.ycb = static_cast<uint8_t>(opt.cb_height_log2);
-    cp.cbstyle = 0x40;
+    const bool classic = opt.block_coder == 1;
+    if (opt.block_coder > 1 || (opt.cb_style & ~0x3Fu) || (!classic && opt.cb_style)) {
+        set_last_error("encode: invalid block coder or code-block style");
+        return kErrArg;
+    }
+    // classic: the requested style, less predictable termination (the
+    // encoder's flush is the plain C.2.9 one)
+    cp.cbstyle = classic ? static_cast<uint8_t>(opt.cb_style & 0x2F) : 0x40;
     cp.transform = rev ? 1 : 0;
     cp.scod = static_cast<uint8_t>((opt.precinct_log2 ? 1 : 0) | (opt.sop ? 2 : 0) |
                                    (opt.eph ? 4 : 0));
@@ -311,7 +349,7 @@ Status encode_image(const void *pixels, uint32_t width, uint32_t height, uint32_
     put16(cs, 0xFF4F);
     put16(cs, 0xFF51);
     put16(cs, 38 + 3 * ncomp);
-    put16(cs, 0x4000);  // Rsiz: extended capabilities present (CAP)
+    put16(cs, classic ? 0 : 0x4000);  // Rsiz: extended capabilities (CAP) for HT
     put32(cs, xsiz);
     put32(cs, ysiz);
     put32(cs, xo);
@@ -326,7 +364,7 @@ Status encode_image(const void *pixels, uint32_t width, uint32_t height, uint32_
         cs.push_back(1);
         cs.push_back(1);
     }
-    {
+    if (!classic) {
         // CAP: Part 15 capabilities, Ccap15 as in codestream_write.rs:166-180
         put16(cs, 0xFF50);
         put16(cs, 8);
@@ -344,7 +382,7 @@ Status encode_image(const void *pixels, uint32_t width, uint32_t height, uint32_
     cs.push_back(static_cast<uint8_t>(levels));
     cs.push_back(static_cast<uint8_t>(cp.xcb - 2));
     cs.push_back(static_cast<uint8_t>(cp.ycb - 2));
-    cs.push_back(0x40);
+    cs.push_back(cp.cbstyle);
     cs.push_back(rev ? 1 : 0);
     if (opt.precinct_log2)
         for (uint32_t r = 0; r <= levels; ++r)
@@ -505,9 +543,11 @@ Status encode_image(const void *pixels, uint32_t width, uint32_t height, uint32_
         }
         // ---- code-block coding ----
         std::vector<std::vector<std::vector<uint8_t>>> seg(ncomp);
+        std::vector<std::vector<ClassicBlockCode>> cls(ncomp);
         for (uint32_t c = 0; c < ncomp; ++c) {
             TileCompGeom &g = tcs[c];
             seg[c].resize(g.cbs.size());
+            if (classic) cls[c].resize(g.cbs.size());
             for (size_t k = 0; k < g.cbs.size(); ++k) {
                 const CbGeom &cb = g.cbs[k];
                 const BandGeom &bg = g.bands[cb.band];
@@ -518,6 +558,19 @@ Status encode_image(const void *pixels, uint32_t width, uint32_t height, uint32_
                 const int32_t *base = qcoef[c].data() + band_off[c][cb.band] +
                                       static_cast<size_t>(cb.rect.y0 - bg.rect.y0) * bg.rect.w() +
                                       (cb.rect.x0 - bg.rect.x0);
+                if (classic) {
+                    ClassicBlockCode &bc = cls[c][k];
+                    if (!classic_encode_block(base, cb.rect.w(), cb.rect.h(), bg.rect.w(), K,
+                                              bg.type, cp.cbstyle, bc)) {
+                        set_last_error("encode: coefficient beyond the band's bit-planes");
+                        return kErrArg;
+                    }
+                    // the body bytes; a block with passes is included (its
+                    // first segment, arithmetically coded, is never empty)
+                    for (const ClassicBlockCode::Segment &sg : bc.segments)
+                        seg[c][k].insert(seg[c][k].end(), sg.bytes.begin(), sg.bytes.end());
+                    continue;
+                }
                 st = ht_encode_block(base, cb.rect.w(), cb.rect.h(), bg.rect.w(), K, seg[c][k]);
                 if (st) return st;
             }
@@ -565,7 +618,7 @@ Status encode_image(const void *pixels, uint32_t width, uint32_t height, uint32_
                     for (uint32_t k = 0; k < n; ++k) {
                         const bool inc = !seg[pk.comp][pb.cb_base + k].empty();
                         iv[k] = inc ? 0 : 1;
-                        zv[k] = K - 1;
+                        zv[k] = classic ? (inc ? cls[pk.comp][pb.cb_base + k].missing_msbs : 0) : K - 1;
                     }
                     tag_set_leaves(incl, iv);
                     tag_set_leaves(zbp, zv);
@@ -574,6 +627,12 @@ Status encode_image(const void *pixels, uint32_t width, uint32_t height, uint32_
                         const uint32_t xi = k % pb.ncbx, yi = k / pb.ncbx;
                         tag_encode(incl, xi, yi, 1, bw);
                         if (s.empty()) continue;
+                        if (classic) {
+                            const ClassicBlockCode &bc = cls[pk.comp][pb.cb_base + k];
+                            tag_encode(zbp, xi, yi, bc.missing_msbs + 1, bw);  // fully determined
+                            classic_block_header(bw, bc);
+                            continue;
+                        }
                         tag_encode(zbp, xi, yi, K, bw);  // value K-1 < K: fully determined
                         bw.bit(0);  // one coding pass
                         const uint32_t len = static_cast<uint32_t>(s.size());

@@ -89,3 +89,32 @@ def test_classic_has_no_ht_block_jobs(oracle):
     cs = pil_encode(synth.tissue_rgb(np.random.default_rng(5), 64, 64))
     with pytest.raises(S.UnsupportedError):
         S.code_block_jobs(cs)
+
+
+STYLES = [0, 0x01, 0x02, 0x04, 0x08, 0x20, 0x05, 0x0D, 0x2F]
+
+
+@pytest.mark.parametrize("style", STYLES)
+def test_classic_encoder_styles_read_by_openjpeg_and_oracle(oracle, style):
+    """The package's classic encoder with every code-block style: OpenJPEG
+    decodes it to the source (pins the encoder's MQ coder, raw segments,
+    terminations and segment lengths), and so does the oracle (pins its
+    bypass, reset, termination, causal and segmentation-symbol handling)."""
+    img = synth.tissue_rgb(np.random.default_rng(40 + style), 150, 100)
+    cs = S.encode_j2k_classic(img, cb_style=style)
+    assert np.array_equal(pil_decode(cs), img)
+    assert np.array_equal(O.decode(cs, 1), img)
+
+
+@pytest.mark.parametrize("style", [0, 0x05, 0x2F])
+def test_classic_encoder_irreversible_and_gray12(oracle, style):
+    img = synth.tissue_rgb(np.random.default_rng(50 + style), 130, 90)
+    cs = S.encode_j2k_classic(img, cb_style=style, reversible=0, base_step=1 / 256)
+    out = O.decode(cs, 1).astype(int)
+    assert np.abs(out - img).max() <= 3
+    # OpenJPEG reconstructs at the mid-point, the reference does not
+    assert np.abs(out - pil_decode(cs).astype(int)).max() <= 2
+    g = synth.spots_gray12(np.random.default_rng(60 + style), 100, 70)
+    cs = S.encode_j2k_classic(g, bit_depth=12, use_mct=0, cb_style=style, cb_width_log2=5,
+                              cb_height_log2=5, levels=4)
+    assert np.array_equal(O.decode(cs, int(S.PixelFormat.GRAY16)).view("<u2").reshape(70, 100), g)

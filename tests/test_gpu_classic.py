@@ -69,3 +69,26 @@ def test_classic_gray16(session, oracle):
     cs = pil_encode(g)
     got = S.J2kDecoder(cs, session).decode_to_numpy(S.PixelFormat.GRAY16)
     assert np.array_equal(got[..., 0], g)
+
+
+def test_classic_styles_on_device(session, oracle):
+    """Every code-block style of the package's classic encoder (raw bypass
+    segments, context reset, termination on each pass, vertically causal
+    context, segmentation symbols) in one batch, 5/3 and 9/7."""
+    from tests.test_classic import STYLES
+    rng = np.random.default_rng(35)
+    streams, want = [], []
+    for k, style in enumerate(STYLES):
+        img = synth.tissue_rgb(rng, 120 + 4 * k, 90)
+        rev = k % 2
+        cs = S.encode_j2k_classic(img, cb_style=style, reversible=rev, base_step=1 / 256,
+                                  cb_width_log2=5 + (k % 2), cb_height_log2=6 - (k % 2))
+        streams.append(cs)
+        want.append((O.decode(cs, 1).astype(np.int64), rev))
+    plan = S.TilePlan(session, streams, S.PixelFormat.RGB8)
+    plan.decode(S.MODE_GRAPH)
+    session.synchronize()
+    for k, (ref, rev) in enumerate(want):
+        d = np.abs(plan.surfaces[k].to_numpy().astype(np.int64) - ref).max()
+        assert d <= (0 if rev else 1), (STYLES[k], d)  # 9/7: 1 LSB of 8-bit
+    plan.close()
