@@ -78,6 +78,7 @@ struct sgn_hip_session {
     // per lane-kernel variant and sample type (2 v + narrow): resident
     // work-groups (0: unknown)
     std::vector<uint32_t> resident;
+    uint32_t classic_resident[4] = {0, 0, 0, 0};  // resident work-groups per classic kernel shape
     // pipelined batch decode: host-to-device copies run on copy_stream
     // while the previous chunk decodes on `stream`; two staging slots
     hipStream_t copy_stream = nullptr;
@@ -185,6 +186,20 @@ const DwtShape kDwtShapes[] = {
     {"64x64-w2", {sgn_idwt_level_irr_w2, sgn_idwt_level_rev_w2}, Mid::kThreads, false},
 };
 constexpr int kDwtShapeCount = int(sizeof(kDwtShapes) / sizeof(kDwtShapes[0]));
+
+// Classic code-block kernels by block width (32, 64, 256, 1024).
+using ClassicKernel = void (*)(const ClassicJob *, uint32_t, const uint8_t *, const ClassicSeg *,
+                               int32_t *, uint32_t *);
+const ClassicKernel kClassicKernels[4] = {sgn_t1_classic<ClassicW32>, sgn_t1_classic<ClassicW64>,
+                                          sgn_t1_classic<ClassicW256>,
+                                          sgn_t1_classic<ClassicW1024>};
+int classic_shape(uint32_t w) {
+    return w <= uint32_t(ClassicW32::kMaxW) ? 0
+           : w <= uint32_t(ClassicW64::kMaxW) ? 1
+           : w <= uint32_t(ClassicW256::kMaxW) ? 2
+                                                : 3;
+}
+uint32_t classic_lanes(int v) { return v == 3 ? uint32_t(ClassicW1024::kLanes) : 64u; }
 // Lane-list size up to which plans start with the cooperative kernel.  On
 // 256x256 RGB tiles (75 code-blocks each) it took 0.28 against 1.41 ms for
 // 16 tiles and 1.21 against 1.61 ms for 256 tiles (19,200 blocks); the lane
@@ -337,7 +352,9 @@ struct sgn_j2k_plan {
     // [classic_begin[v], classic_begin[v + 1])), costliest first; their
     // segment table; bytes of blocks joined from several packets
     std::vector<ClassicJob> classic;
-    uint32_t classic_begin[4] = {0, 0, 0, 0};
+    uint32_t classic_begin[5] = {0, 0, 0, 0, 0};
+    uint32_t classic_grid[4] = {0, 0, 0, 0};  // work-groups launched per shape
+    uint32_t *d_crows = nullptr;              // the lanes' row stores
     std::vector<ClassicSeg> cseg;
     std::vector<uint8_t> joined;
     ClassicJob *d_classic = nullptr;
@@ -627,17 +644,14 @@ Status build_plan(const sgn_tile_req *reqs, size_t n, sgn_j2k_plan &P, int32_t *
     sort_by_cost(P.cb_lane_plain, P.cb_lane);
     // classic blocks: by kernel shape, costliest first within each
     {
-        auto shape = [](const ClassicJob &j) {
-            const uint32_t w = classic_words(j.w, j.h);
-            return w <= uint32_t(ClassicSmall::kWords) ? 0 : (w <= uint32_t(ClassicBig::kWords) ? 1 : 2);
-        };
+        auto shape = [](const ClassicJob &j) { return classic_shape(j.w); };
         std::stable_sort(P.classic.begin(), P.classic.end(),
                          [&](const ClassicJob &a, const ClassicJob &b) {
                              const int sa = shape(a), sb = shape(b);
                              return sa != sb ? sa < sb : a.cost > b.cost;
                          });
         P.classic_begin[0] = 0;
-        for (int v = 0; v < 3; ++v) {
+        for (int v = 0; v < 4; ++v) {
             uint32_t e = P.classic_begin[v];
             while (e < P.classic.size() && shape(P.classic[e]) == v) ++e;
             P.classic_begin[v + 1] = e;
@@ -753,6 +767,8 @@ Status slot_reserve(sgn_hip_session *s, int k, size_t bytes, uint8_t **out) {
 // allocated: the plan is carved from it, every run and table is copied
 // straight from host memory (no pinned staging), and the call returns after
 // the copies completed.
+uint32_t classic_resident(sgn_hip_session *s, int v);
+
 Status upload_plan(sgn_hip_session *s, const sgn_tile_req *jobs, sgn_j2k_plan &P,
                    hipStream_t copy, int slot, bool sync, void *fixed = nullptr,
                    uint64_t fixed_cap = 0) {
@@ -778,6 +794,15 @@ Status upload_plan(sgn_hip_session *s, const sgn_tile_req *jobs, sgn_j2k_plan &P
     const uint64_t host_bytes = off;
     const uint64_t o_stl = take(P.cb_lane.size()), o_stw = take(P.cb_wave.size());
     const uint64_t o_ff = take(P.fin_block_job[1].size() * sizeof(uint32_t));
+    uint64_t crows_bytes = 0;
+    for (int v = 0; v < 4; ++v) {
+        const uint32_t n = P.classic_begin[v + 1] - P.classic_begin[v];
+        const uint32_t lanes = classic_lanes(v);
+        P.classic_grid[v] = n ? std::min((n + lanes - 1) / lanes, classic_resident(s, v)) : 0;
+        crows_bytes = std::max<uint64_t>(crows_bytes, uint64_t(P.classic_grid[v]) * lanes *
+                                                          kClassicStateDwords * sizeof(uint32_t));
+    }
+    const uint64_t o_crows = take(crows_bytes);
     const uint64_t o_queue = take(16), o_coef = take(P.coef_elems * sizeof(uint32_t));
     P.device_bytes = off;
     if (fixed) {
@@ -929,6 +954,7 @@ Status upload_plan(sgn_hip_session *s, const sgn_tile_req *jobs, sgn_j2k_plan &P
     at(o_stl, P.d_status_lane, !P.cb_lane.empty());
     at(o_stw, P.d_status_wave, !P.cb_wave.empty());
     at(o_ff, P.d_fin_redo, !P.fin_block_job[1].empty());
+    at(o_crows, P.d_crows, crows_bytes != 0);
     at(o_queue, P.d_queue, true);
     at(o_coef, P.d_coef, P.coef_elems != 0);
     return kOk;
@@ -963,6 +989,22 @@ uint32_t resident_groups(sgn_hip_session *s, int v, bool narrow) {
         s->resident[slot] = uint32_t(per_cu) * uint32_t(s->cus);
     }
     return s->resident[slot];
+}
+
+uint32_t classic_resident(sgn_hip_session *s, int v) {
+    if (!s->classic_resident[v]) {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, reinterpret_cast<const void *>(kClassicKernels[v]), 64, 0) != hipSuccess ||
+            per_cu < 1)
+            per_cu = 1;
+        if (!s->cus && (hipDeviceGetAttribute(&s->cus, hipDeviceAttributeMultiprocessorCount,
+                                              s->device) != hipSuccess ||
+                        s->cus < 1))
+            s->cus = 1;
+        s->classic_resident[v] = uint32_t(per_cu) * uint32_t(s->cus);
+    }
+    return s->classic_resident[v];
 }
 
 // Clears the plan's device counters (word 0: lane work counter, word 1:
@@ -1010,25 +1052,18 @@ uint32_t launch_wave(sgn_hip_session *s, sgn_j2k_plan &P, const CbJob *d_jobs, u
 }
 
 // Every classic code-block of the plan: one launch per kernel shape (one
-// block per lane).
+// block per lane; the grid at most the resident work-groups, which then
+// walk the block groups).
 uint32_t launch_classic(sgn_hip_session *s, sgn_j2k_plan &P) {
     uint32_t k = 0;
-    for (int v = 0; v < 3; ++v) {
+    for (int v = 0; v < 4; ++v) {
         const uint32_t a = P.classic_begin[v], b = P.classic_begin[v + 1];
-        if (a >= b) continue;
+        if (a >= b || !P.classic_grid[v]) continue;
         const uint32_t n = b - a;
         const ClassicJob *jobs = P.d_classic + a;
         int32_t *coef = reinterpret_cast<int32_t *>(P.d_coef);
-        if (v == 0)
-            hipLaunchKernelGGL(sgn_t1_classic<ClassicSmall>, dim3((n + 63) / 64), dim3(64), 0,
-                               s->stream, jobs, n, P.d_arena, P.d_cseg, coef);
-        else if (v == 1)
-            hipLaunchKernelGGL(sgn_t1_classic<ClassicBig>, dim3((n + 63) / 64), dim3(64), 0,
-                               s->stream, jobs, n, P.d_arena, P.d_cseg, coef);
-        else
-            hipLaunchKernelGGL(sgn_t1_classic<ClassicWide>,
-                               dim3((n + ClassicWide::kLanes - 1) / ClassicWide::kLanes), dim3(64),
-                               0, s->stream, jobs, n, P.d_arena, P.d_cseg, coef);
+        hipLaunchKernelGGL(kClassicKernels[v], dim3(P.classic_grid[v]), dim3(64), 0, s->stream,
+                           jobs, n, P.d_arena, P.d_cseg, coef, P.d_crows);
         ++k;
     }
     return k;

@@ -8,16 +8,21 @@
 //
 // Decomposition: one code-block per lane.  The MQ decoder is one serial
 // chain per block (every decision updates the interval and the context
-// the next decision reads), so the parallelism is across blocks.  A lane
-// keeps its block's coding state in LDS as one 16-bit word per stripe
-// column (4 samples: significance, sign, visited-in-this-bit-plane and
-// refined-before bits), with a border of zero words around the block, and
-// walks the stripe columns with the 3x3 words around the current one in
-// registers: three LDS loads per column step, one store when it changed.
-// Magnitude bits go to the block's int32 coefficients in HBM by atomic OR
-// (no read in the decode chain); the block is zeroed first and the signs
-// are applied in a final sweep.  The 19 context states and the Qe table sit
-// in LDS.
+// the next decision reads), so the parallelism is across blocks.  A block's
+// coding state is one 16-bit word per stripe column (4 samples:
+// significance, sign, visited-in-this-bit-plane and refined-before bits),
+// kept in a row store in HBM (one row of w + 2 words per stripe, zero
+// columns at both ends).  A pass walks the stripes with three rows in LDS
+// (the stripe above, this one, the one below: a ring, the row below loaded
+// when a stripe starts and the current row written back when it ends), so
+// a lane needs a few hundred bytes of LDS and several work-groups share a
+// CU; within a stripe the 3x3 words around the current column sit in
+// registers (three LDS loads per column step, one store).  Magnitude bits
+// go to the block's int32 coefficients in HBM by atomic OR (no read in the
+// decode chain); the block is zeroed first and the signs are applied in a
+// final sweep.  Each context state is its packed Qe-table entry plus the
+// MPS bit, so the common decision (MPS, no renormalisation) is one LDS
+// read.
 #include <hip/hip_runtime.h>
 
 #include "../host/plan_types.h"
@@ -38,18 +43,24 @@ __constant__ uint32_t kMqTable[47] = {
 
 constexpr int kClassicContexts = 19;
 constexpr int kCxRunLength = 17, kCxUniform = 18;
+constexpr uint32_t kMpsBit = 1u << 15;  // MPS bit of a context word (bits 14-15 of an entry are 0)
 
-// LDS words a block of w x h needs: (w + 2) columns x (stripes + 2).
-__host__ __device__ inline uint32_t classic_words(uint32_t w, uint32_t h) {
-    return (w + 2) * (((h + 3) >> 2) + 2);
+// Dwords of one stripe row of a block w wide: w + 2 words, padded to even.
+__host__ __device__ inline uint32_t classic_row_dwords(uint32_t w) { return (w + 3) >> 1; }
+// Dwords of a block's row store: one row per stripe.
+__host__ __device__ inline uint32_t classic_state_dwords(uint32_t w, uint32_t h) {
+    return classic_row_dwords(w) * ((h + 3) >> 2);
 }
+// Row-store dwords reserved per lane: the largest block of any shape
+// (4 x 1024: 3 x 256 dwords).
+constexpr uint32_t kClassicStateDwords = 768;
 
-// Kernel shapes: per-lane LDS words and active lanes per 64-thread
-// work-group.  Small fits 32x32 blocks (three work-groups per CU), Big
-// 64x64 (one per CU), Wide the elongated shapes up to 1024x4 (16 lanes).
-struct ClassicSmall { static constexpr int kWords = 340, kLanes = 64; };
-struct ClassicBig { static constexpr int kWords = 1188, kLanes = 64; };
-struct ClassicWide { static constexpr int kWords = 3080, kLanes = 16; };
+// Kernel shapes by block width: LDS ring of three rows per lane, active
+// lanes per 64-thread work-group.
+struct ClassicW32 { static constexpr int kRowDw = 17, kLanes = 64, kMaxW = 32; };
+struct ClassicW64 { static constexpr int kRowDw = 33, kLanes = 64, kMaxW = 64; };
+struct ClassicW256 { static constexpr int kRowDw = 129, kLanes = 64, kMaxW = 256; };
+struct ClassicW1024 { static constexpr int kRowDw = 513, kLanes = 16, kMaxW = 1024; };
 
 // Reads of one codeword segment: absolute arena positions [p, end), bytes
 // past `end` read as 0xFF (the MQ decoder's fill, arithmetic_decoder.rs:
@@ -91,8 +102,8 @@ struct Mq {
 
     __device__ void bytein() {
         const uint32_t b = in.at(p);
+        const uint32_t b1 = in.at(p + 1);
         if (b == 0xFFu) {
-            const uint32_t b1 = in.at(p + 1);
             if (b1 > 0x8Fu) {
                 ct = 8;  // a marker or the end: 1 bits from here on
             } else {
@@ -101,7 +112,6 @@ struct Mq {
                 ct = 7;
             }
         } else {
-            const uint32_t b1 = in.at(p + 1);
             ++p;
             c += 0xFF00u - (b1 << 8);
             ct = 8;
@@ -116,12 +126,11 @@ struct Mq {
         ct -= 7;
         a = 0x8000u;
     }
-    // decision with context state *cx (bit 7 MPS, bits 0-6 Qe index)
-    __device__ uint32_t decode(uint8_t *cx, const uint32_t *qtab) {
-        uint32_t s = *cx;
-        const uint32_t e = qtab[s & 0x7Fu];
+    // decision with context word *cx (its Qe-table entry | MPS bit)
+    __device__ uint32_t decode(uint32_t *cx, const uint32_t *qtab) {
+        const uint32_t e = *cx;
         const uint32_t qe = e >> 16;
-        uint32_t mps = s >> 7;
+        uint32_t mps = (e >> 15) & 1u;
         a -= qe;
         if ((c >> 16) < a && (a & 0x8000u)) return mps;
         const bool lps = (c >> 16) >= a;
@@ -133,8 +142,8 @@ struct Mq {
         const uint32_t d = took_lps ? 1u - mps : mps;
         if (lps) a = qe;
         if (took_lps && (e & 1u)) mps ^= 1u;
-        const uint32_t next = took_lps ? (e >> 1) & 0x7Fu : (e >> 8) & 0xFFu;
-        *cx = uint8_t(mps << 7 | next);
+        const uint32_t next = took_lps ? (e >> 1) & 0x7Fu : (e >> 8) & 0x3Fu;
+        *cx = qtab[next] | (mps << 15);
         do {
             if (ct == 0) bytein();
             const uint32_t sh = min(ct, uint32_t(__builtin_clz(a) - 16));
@@ -209,24 +218,55 @@ __device__ inline uint32_t col6(uint32_t up, uint32_t mid, uint32_t dn, uint32_t
 
 template <class G>
 struct ClassicLane {
-    uint16_t *st;       // this lane's words (lane-interleaved: stride kLanes)
-    uint8_t *cx;        // this lane's 19 context states (stride kLanes)
+    uint32_t *ring;     // this lane's three LDS rows (lane-interleaved dwords: stride kLanes)
+    uint32_t *cx;       // this lane's 19 context words (stride kLanes)
+    uint32_t *rows;     // this lane's row store in HBM (ndw dwords per stripe)
     const uint32_t *qtab;
     int32_t *coef;
-    uint32_t dst, stride, w, h, pw, band;
+    uint32_t dst, stride, w, h, ns, ndw, band;
     bool causal, raw;
     uint32_t bit;
     Mq mq;
     RawBits rb;
 
-    __device__ uint16_t &word(uint32_t s1, uint32_t x1) { return st[(s1 * pw + x1) * G::kLanes]; }
-    __device__ uint8_t &ctx(uint32_t label) { return cx[label * G::kLanes]; }
+    __device__ uint16_t *word_at(uint32_t slot, uint32_t x1) {
+        return reinterpret_cast<uint16_t *>(ring + (slot * G::kRowDw + (x1 >> 1)) * G::kLanes) +
+               (x1 & 1u);
+    }
+    __device__ uint32_t word(uint32_t slot, uint32_t x1) { return *word_at(slot, x1); }
+    __device__ uint32_t &ctx(uint32_t label) { return cx[label * G::kLanes]; }
+
+    // ring slot of stripe s (-1 .. ns)
+    __device__ static uint32_t slot_of(int s) { return uint32_t(s + 3) % 3u; }
+    __device__ void load_row(int s) {
+        uint32_t *r = ring + slot_of(s) * G::kRowDw * G::kLanes;
+        if (s < 0 || uint32_t(s) >= ns) {
+            for (uint32_t k = 0; k < ndw; ++k) r[k * G::kLanes] = 0;
+            return;
+        }
+        const uint32_t *src = rows + uint32_t(s) * ndw;
+        uint32_t k = 0;
+        for (; k + 8 <= ndw; k += 8) {
+            uint32_t v[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[q] = src[k + q];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) r[(k + q) * G::kLanes] = v[q];
+        }
+        for (; k < ndw; ++k) r[k * G::kLanes] = src[k];
+    }
+    __device__ void store_row(int s) {
+        const uint32_t *r = ring + slot_of(s) * G::kRowDw * G::kLanes;
+        uint32_t *d = rows + uint32_t(s) * ndw;
+        for (uint32_t k = 0; k < ndw; ++k) d[k] = r[k * G::kLanes];
+    }
 
     __device__ void reset_contexts() {
-        for (int k = 0; k < kClassicContexts; ++k) ctx(k) = 0;
-        ctx(0) = 4;
-        ctx(kCxRunLength) = 3;
-        ctx(kCxUniform) = 46;
+        const uint32_t z = qtab[0];
+        for (int k = 0; k < kClassicContexts; ++k) ctx(k) = z;
+        ctx(0) = qtab[4];
+        ctx(kCxRunLength) = qtab[3];
+        ctx(kCxUniform) = qtab[46];
     }
     __device__ uint32_t decide(uint32_t label) {
         return raw ? rb.get() : mq.decode(&ctx(label), qtab);
@@ -257,26 +297,30 @@ struct ClassicLane {
         return raw ? s : (s ^ flip);  // a raw sign bit is the sign itself
     }
 
-    // One coding pass over the block.  KIND 0 cleanup, 1 significance
-    // propagation, 2 magnitude refinement.
+    // One coding pass over the block.  KIND 0 cleanup (also clears the
+    // visited bits once a stripe is done), 1 significance propagation,
+    // 2 magnitude refinement.
     template <int KIND>
     __device__ void pass() {
-        const uint32_t nstripes = (h + 3) >> 2;
         const bool diag = band == 3, swap = band == 1;
-        for (uint32_t s = 0; s < nstripes; ++s) {
-            const uint32_t rows = min(4u, h - 4 * s);
+        load_row(-1);
+        load_row(0);
+        for (uint32_t s = 0; s < ns; ++s) {
+            load_row(int(s) + 1);
+            const uint32_t su = slot_of(int(s) - 1), sm = slot_of(int(s)), sd = slot_of(int(s) + 1);
+            const uint32_t rows4 = min(4u, h - 4 * s);
             const uint32_t y0 = 4 * s;
             // words of the previous (L), current (C) and next (R) column in
             // the stripes above, this and below
             uint32_t uL = 0, mL = 0, dL = 0;
-            uint32_t uC = word(s, 1), mC = word(s + 1, 1), dC = word(s + 2, 1);
+            uint32_t uC = word(su, 1), mC = word(sm, 1), dC = word(sd, 1);
             for (uint32_t x = 0; x < w; ++x) {
-                const uint32_t uR = word(s, x + 2), mR = word(s + 1, x + 2), dR = word(s + 2, x + 2);
+                const uint32_t uR = word(su, x + 2), mR = word(sm, x + 2), dR = word(sd, x + 2);
                 const uint32_t dnL = causal ? 0u : dL, dnC = causal ? 0u : dC, dnR = causal ? 0u : dR;
                 const uint32_t sL = col6(uL, mL, dnL, kSig), sR = col6(uR, mR, dnR, kSig);
                 const uint32_t before = mC;
                 uint32_t r = 0;
-                if (KIND == 0 && rows == 4 && ((mC >> kSig) & 0xFu) == 0 &&
+                if (KIND == 0 && rows4 == 4 && ((mC >> kSig) & 0xFu) == 0 &&
                     ((mC >> kVisit) & 0xFu) == 0 && sL == 0 && sR == 0 &&
                     col6(uC, 0, dnC, kSig) == 0) {
                     // run-length mode (D.3.4)
@@ -294,7 +338,7 @@ struct ClassicLane {
                         ++r;
                     }
                 }
-                for (; r < rows; ++r) {
+                for (; r < rows4; ++r) {
                     const uint32_t sC = col6(uC, mC, dnC, kSig);
                     const uint32_t sig = (mC >> (kSig + r)) & 1u, vis = (mC >> (kVisit + r)) & 1u;
                     const uint32_t hc = ((sL >> (r + 1)) & 1u) + ((sR >> (r + 1)) & 1u);
@@ -323,7 +367,9 @@ struct ClassicLane {
                                     col6(uR, mR, dnR, kNeg), r);
                     mC |= (1u << (kSig + r)) | (neg << (kNeg + r));
                 }
-                if (mC != before) word(s + 1, x + 1) = uint16_t(mC);
+                // the word as the next column sees it; the visited bits end
+                // with the cleanup pass (only this sample reads them)
+                if (mC != before) *word_at(sm, x + 1) = uint16_t(mC);
                 uL = uC;
                 mL = mC;
                 dL = dC;
@@ -331,6 +377,9 @@ struct ClassicLane {
                 mC = mR;
                 dC = dR;
             }
+            if (KIND == 0)
+                for (uint32_t k = 0; k < ndw; ++k) ring[(sm * G::kRowDw + k) * G::kLanes] &= ~0x0F000F00u;
+            store_row(int(s));
         }
     }
 
@@ -339,16 +388,12 @@ struct ClassicLane {
         for (uint32_t p = p0; p < p1; ++p) {
             bit = planes - 1 - (p + 2) / 3;
             const uint32_t kind = p % 3;
+            // the previous pass's row stores land before this pass reads them
+            __threadfence();
             if (kind == 0) {
                 pass<0>();
                 if (style & kCbsSegSym)  // four UNIFORM symbols, not checked (non-strict)
                     for (int k = 0; k < 4; ++k) (void)decide(kCxUniform);
-                const uint32_t nstripes = (h + 3) >> 2;
-                for (uint32_t s = 0; s < nstripes; ++s)
-                    for (uint32_t x = 0; x < w; ++x) {
-                        uint16_t &v = word(s + 1, x + 1);
-                        if (v & (0xFu << kVisit)) v = uint16_t(v & ~(0xFu << kVisit));
-                    }
             } else if (kind == 1) {
                 pass<1>();
             } else {
@@ -361,7 +406,8 @@ struct ClassicLane {
 
 template <class G>
 __device__ void classic_block(const ClassicJob &j, const uint8_t *arena, const ClassicSeg *segs,
-                              int32_t *coef, uint16_t *st, uint8_t *cx, const uint32_t *qtab) {
+                              int32_t *coef, uint32_t *ring, uint32_t *cx, uint32_t *rows,
+                              const uint32_t *qtab) {
     const uint32_t w = j.w, h = j.h;
     // zero the block's coefficients (magnitude bits arrive by OR)
     for (uint32_t y = 0; y < h; ++y) {
@@ -369,22 +415,22 @@ __device__ void classic_block(const ClassicJob &j, const uint8_t *arena, const C
         for (uint32_t x = 0; x < w; ++x) row[x] = 0;
     }
     if (!j.planes || !j.passes) return;
-    __threadfence();  // the zeros land before the first OR
     ClassicLane<G> L;
-    L.st = st;
+    L.ring = ring;
     L.cx = cx;
+    L.rows = rows;
     L.qtab = qtab;
     L.coef = coef;
     L.dst = j.dst;
     L.stride = j.stride;
     L.w = w;
     L.h = h;
-    L.pw = w + 2;
+    L.ns = (h + 3) >> 2;
+    L.ndw = classic_row_dwords(w);
     L.band = j.band;
     L.causal = (j.style & kCbsCausal) != 0;
     L.raw = false;
-    const uint32_t nwords = classic_words(w, h);
-    for (uint32_t k = 0; k < nwords; ++k) st[k * G::kLanes] = 0;
+    for (uint32_t k = 0; k < L.ns * L.ndw; ++k) rows[k] = 0;
     L.reset_contexts();
     const uint32_t maxp = 1 + 3 * (uint32_t(j.planes) - 1);
     if (j.nseg == 0) {
@@ -401,31 +447,43 @@ __device__ void classic_block(const ClassicJob &j, const uint8_t *arena, const C
             L.run(j.planes, sg.start, min(uint32_t(sg.end), maxp), j.style);
         }
     }
-    // signs: the magnitudes' OR results must be visible to this lane's loads
+    // signs: the magnitudes' OR results and the last row stores must be
+    // visible to this lane's loads
     __threadfence();
     for (uint32_t y = 0; y < h; ++y) {
-        const uint32_t s1 = (y >> 2) + 1, r = y & 3u;
+        const uint32_t *srow = rows + (y >> 2) * L.ndw;
+        const uint32_t r = y & 3u;
         int32_t *row = coef + j.dst + y * j.stride;
-        for (uint32_t x = 0; x < w; ++x)
-            if ((L.word(s1, x + 1) >> (kNeg + r)) & 1u) row[x] = -row[x];
+        for (uint32_t x = 0; x < w; ++x) {
+            const uint32_t wd = (srow[(x + 1) >> 1] >> (((x + 1) & 1u) * 16u)) & 0xFFFFu;
+            if ((wd >> (kNeg + r)) & 1u) row[x] = -row[x];
+        }
     }
 }
 
+// One code-block per lane; a grid of at most the resident work-groups
+// walks the job groups (one group of kLanes blocks per work-group turn),
+// each lane keeping its row store at rows + (blockIdx.x * kLanes + lane) *
+// kClassicStateDwords.
 template <class G>
 __global__ void __launch_bounds__(64)
 sgn_t1_classic(const ClassicJob *__restrict__ jobs, uint32_t n, const uint8_t *__restrict__ arena,
-               const ClassicSeg *__restrict__ segs, int32_t *__restrict__ coef) {
-    __shared__ uint16_t st[G::kWords * G::kLanes];
-    __shared__ uint8_t cx[kClassicContexts * G::kLanes];
+               const ClassicSeg *__restrict__ segs, int32_t *__restrict__ coef,
+               uint32_t *__restrict__ rows) {
+    __shared__ uint32_t ring[3 * G::kRowDw * G::kLanes];
+    __shared__ uint32_t cx[kClassicContexts * G::kLanes];
     __shared__ uint32_t qtab[48];
     const uint32_t lane = threadIdx.x;
     if (lane < 47) qtab[lane] = kMqTable[lane];
     __syncthreads();
     if (lane >= uint32_t(G::kLanes)) return;
-    const uint32_t i = blockIdx.x * G::kLanes + lane;
-    if (i >= n) return;
-    const ClassicJob j = jobs[i];
-    classic_block<G>(j, arena, segs, coef, st + lane, cx + lane, qtab);
+    uint32_t *my_rows = rows + (size_t(blockIdx.x) * G::kLanes + lane) * kClassicStateDwords;
+    for (uint32_t g = blockIdx.x; g * G::kLanes < n; g += gridDim.x) {
+        const uint32_t i = g * G::kLanes + lane;
+        if (i >= n) break;
+        const ClassicJob j = jobs[i];
+        classic_block<G>(j, arena, segs, coef, ring + lane, cx + lane, my_rows, qtab);
+    }
 }
 
 }  // namespace sgn

@@ -125,9 +125,9 @@ static Status classic_job(const uint8_t *cs, const CompParams &cp, const BandGeo
     j.w = static_cast<uint16_t>(cg.rect.w());
     j.h = static_cast<uint16_t>(cg.rect.h());
     if (j.w == 0 || j.h == 0) return kOk;
-    // the decoder's per-block state: (w + 2) x (stripes + 2) words, at most
-    // 3080 (a 1024x4 block)
-    if ((uint32_t(j.w) + 2) * (((uint32_t(j.h) + 3) >> 2) + 2) > 3080) {
+    // the decoder's per-block row store: (w + 3) / 2 dwords per stripe, at
+    // most 768 (a 4x1024 block)
+    if (((uint32_t(j.w) + 3) >> 1) * ((uint32_t(j.h) + 3) >> 2) > 768 || j.w > 1024) {
         set_last_error("classic code-block shape too large");
         return kErrUnsupported;
     }
