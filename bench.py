@@ -218,6 +218,67 @@ def lossy_config(S, synth, sess, n, sample):
     return res, lossy, outs
 
 
+def classic_config(S, sess, sources, n, sample):
+    """Classic (EBCOT / MQ) code-blocks: the first n tiles of the headline
+    batch re-encoded by the package's classic encoder (one layer, every
+    coding pass), decoded as one graph-replayed plan; every tile checked
+    against its source; a sample timed through the oracle on one thread."""
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(16) as ex:
+        streams = list(ex.map(S.encode_j2k_classic, sources[:n]))
+    plan = S.TilePlan(sess, streams, S.PixelFormat.RGB8)
+    ms = graph_ms(S, sess, plan)
+    for k in range(n):
+        if not np.array_equal(plan.surfaces[k].to_numpy(), sources[k]):
+            raise SystemExit(f"classic batch tile {k} differs from its source")
+    st = plan.stats()
+    plan.close()
+    res = {"workload": f"{n} x 256x256 RGB8 classic EBCOT/MQ lossless (5/3 + RCT, 5 levels, "
+                       "64x64 code-blocks, one layer), the headline tiles re-encoded",
+           "mpix_s": round(st["output_pixels"] / (ms * 1e-3) / 1e6, 1),
+           "ms_per_step": round(ms, 3), "coded_mb": round(st["coded_bytes"] / 1e6, 2),
+           "code_blocks": st["code_blocks"], "tiles_checked_vs_source": n}
+    if sample:
+        from tests import oracle_ref as O
+        t0 = time.perf_counter()
+        rc, _i, _o = O.decode_batch(streams[:sample], 1, workers=1)
+        dt = time.perf_counter() - t0
+        if rc != 0:
+            raise SystemExit("oracle failed on the classic sample")
+        res["oracle_single_thread_mpix_s"] = round(sample * 65536 / dt / 1e6, 2)
+    return res
+
+
+def refine_config(S, sess, sources, n):
+    """HT code-blocks with refinement passes: the first n headline tiles
+    re-encoded with a cleanup of the bits above bit 0 and SigProp / MagRef
+    passes for bit 0 in every code-block (ht_refine=2), against the same
+    tiles cleanup-only; the decode is checked against the oracle on a
+    sample (the refined blocks drop the samples SigProp does not reach)."""
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(16) as ex:
+        refined = list(ex.map(lambda t: S.encode_j2k_ht(t, ht_refine=2), sources[:n]))
+        plain = list(ex.map(S.encode_j2k_lossless, sources[:n]))
+    out = {}
+    for name, streams in (("refined", refined), ("cleanup_only", plain)):
+        plan = S.TilePlan(sess, streams, S.PixelFormat.RGB8)
+        plan.autotune(reps=2)
+        ms = graph_ms(S, sess, plan)
+        st = plan.stats()
+        if name == "refined":
+            from tests import oracle_ref as O
+            for k in range(0, n, max(1, n // 16)):
+                if not np.array_equal(plan.surfaces[k].to_numpy(), O.decode(streams[k], 1)):
+                    raise SystemExit(f"refined tile {k} differs from the oracle")
+        out[name] = {"mpix_s": round(st["output_pixels"] / (ms * 1e-3) / 1e6, 1),
+                     "ms_per_step": round(ms, 3), "coded_mb": round(st["coded_bytes"] / 1e6, 2),
+                     "lane_kernel": S.TilePlan.lane_variants()[plan.lane_variant]}
+        plan.close()
+    return {"workload": f"{n} x 256x256 RGB8 HTJ2K 5/3, every code-block with cleanup + "
+                        "SigProp + MagRef passes, beside the same tiles cleanup-only",
+            **out}
+
+
 def serving(S, sess, pool, rates, seconds):
     from signinum_amd.serving import Server, run_poisson
     out = []
@@ -254,6 +315,8 @@ def main():
     ap.add_argument("--mixed-tiles", type=int, default=2048)
     ap.add_argument("--lossy-tiles", type=int, default=1024)
     ap.add_argument("--lossy-oracle-sample", type=int, default=16)
+    ap.add_argument("--classic-tiles", type=int, default=1024)
+    ap.add_argument("--classic-oracle-sample", type=int, default=32)
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--serve-rates", default="150,600,2000,6000")
     args = ap.parse_args()
@@ -451,6 +514,15 @@ def main():
             if lsb > 1:
                 raise SystemExit(f"9/7 output differs from the oracle by {lsb} LSB")
         log(f"irreversible batch: {extras['irreversible_batch']} ({time.time() - t0:.1f}s)")
+        t0 = time.time()
+        extras["classic_batch"] = classic_config(
+            S, sess, sources, min(args.classic_tiles, len(sources)),
+            0 if args.no_cpu_baseline else args.classic_oracle_sample)
+        log(f"classic batch: {extras['classic_batch']} ({time.time() - t0:.1f}s)")
+        t0 = time.time()
+        extras["refinement_batch"] = refine_config(S, sess, sources,
+                                                   min(args.classic_tiles, len(sources)))
+        log(f"refinement batch: {extras['refinement_batch']} ({time.time() - t0:.1f}s)")
         t0 = time.time()
         pool = [(streams[i], S.PixelFormat.RGB8, S.Downscale.NONE, 256 * 256)
                 for i in range(0, len(streams), max(1, len(streams) // 64))]

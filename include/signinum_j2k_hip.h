@@ -541,6 +541,10 @@ typedef struct {
     uint32_t cb_style;       /* classic code-block style bits: 0x01 selective bypass,
                                 0x02 context reset, 0x04 termination on each pass,
                                 0x08 vertically causal context, 0x20 segmentation symbols */
+    uint32_t ht_refine;      /* HT: code bit 0 in SigProp / MagRef passes after a cleanup
+                                of the bits above; 1 only in code-blocks where SigProp
+                                reaches every sample of magnitude 1 (lossless), 2 in every
+                                code-block (the unreached samples are dropped) */
 } sgn_j2k_encode_options;
 
 void sgn_j2k_encode_default_options(sgn_j2k_encode_options *opts);

@@ -16,7 +16,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsigninum_j2k_hip.so")
 ARCH = os.environ.get("SGN_OFFLOAD_ARCH", "gfx950")
 
-HOST_SOURCES = ["container.cpp", "geometry.cpp", "parse.cpp", "plan.cpp", "encode.cpp", "encode_classic.cpp", "encode_image.cpp",
+HOST_SOURCES = ["container.cpp", "geometry.cpp", "parse.cpp", "plan.cpp", "encode.cpp", "encode_ht_refine.cpp", "encode_classic.cpp", "encode_image.cpp",
                 "api_encode.cpp", "api_host.cpp"]
 HIP_SOURCES = ["runtime.hip"]
 

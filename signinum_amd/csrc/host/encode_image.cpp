@@ -16,6 +16,9 @@
 
 namespace sgn {
 
+bool ht_encode_block_refined(const int32_t *coef, uint32_t w, uint32_t h, size_t stride,
+                             uint32_t K, bool lossless_only, std::vector<uint8_t> &cleanup,
+                             std::vector<uint8_t> &refine);
 Status ht_encode_block(const int32_t *coef, uint32_t w, uint32_t h, size_t stride,
                        uint32_t k_max, std::vector<uint8_t> &out);
 
@@ -544,9 +547,13 @@ Status encode_image(const void *pixels, uint32_t width, uint32_t height, uint32_
         // ---- code-block coding ----
         std::vector<std::vector<std::vector<uint8_t>>> seg(ncomp);
         std::vector<std::vector<ClassicBlockCode>> cls(ncomp);
+        // HT blocks with refinement passes: the refinement segment's length
+        // (0: cleanup only)
+        std::vector<std::vector<uint32_t>> ref_len(ncomp);
         for (uint32_t c = 0; c < ncomp; ++c) {
             TileCompGeom &g = tcs[c];
             seg[c].resize(g.cbs.size());
+            ref_len[c].assign(g.cbs.size(), 0);
             if (classic) cls[c].resize(g.cbs.size());
             for (size_t k = 0; k < g.cbs.size(); ++k) {
                 const CbGeom &cb = g.cbs[k];
@@ -570,6 +577,17 @@ Status encode_image(const void *pixels, uint32_t width, uint32_t height, uint32_
                     for (const ClassicBlockCode::Segment &sg : bc.segments)
                         seg[c][k].insert(seg[c][k].end(), sg.bytes.begin(), sg.bytes.end());
                     continue;
+                }
+                if (opt.ht_refine) {
+                    std::vector<uint8_t> cup, ref;
+                    if (ht_encode_block_refined(base, cb.rect.w(), cb.rect.h(), bg.rect.w(), K,
+                                                opt.ht_refine == 1, cup, ref) &&
+                        ref.size() < 2047) {
+                        seg[c][k] = std::move(cup);
+                        ref_len[c][k] = static_cast<uint32_t>(ref.size());
+                        seg[c][k].insert(seg[c][k].end(), ref.begin(), ref.end());
+                        continue;
+                    }
                 }
                 st = ht_encode_block(base, cb.rect.w(), cb.rect.h(), bg.rect.w(), K, seg[c][k]);
                 if (st) return st;
@@ -618,7 +636,8 @@ Status encode_image(const void *pixels, uint32_t width, uint32_t height, uint32_
                     for (uint32_t k = 0; k < n; ++k) {
                         const bool inc = !seg[pk.comp][pb.cb_base + k].empty();
                         iv[k] = inc ? 0 : 1;
-                        zv[k] = classic ? (inc ? cls[pk.comp][pb.cb_base + k].missing_msbs : 0) : K - 1;
+                        zv[k] = classic ? (inc ? cls[pk.comp][pb.cb_base + k].missing_msbs : 0)
+                                        : (ref_len[pk.comp][pb.cb_base + k] ? K - 2 : K - 1);
                     }
                     tag_set_leaves(incl, iv);
                     tag_set_leaves(zbp, zv);
@@ -633,7 +652,23 @@ Status encode_image(const void *pixels, uint32_t width, uint32_t height, uint32_
                             classic_block_header(bw, bc);
                             continue;
                         }
-                        tag_encode(zbp, xi, yi, K, bw);  // value K-1 < K: fully determined
+                        tag_encode(zbp, xi, yi, K, bw);  // value below K: fully determined
+                        const uint32_t lref = ref_len[pk.comp][pb.cb_base + k];
+                        if (lref) {
+                            // cleanup, SigProp and MagRef: three passes ('11' '00'),
+                            // lengths of Lblock and Lblock + 1 bits
+                            bw.bits(0xC, 4);
+                            const uint32_t lcup = static_cast<uint32_t>(s.size()) - lref;
+                            uint32_t lblock = 3;
+                            while (lcup >= (1u << lblock) || lref >= (2u << lblock)) {
+                                bw.bit(1);
+                                ++lblock;
+                            }
+                            bw.bit(0);
+                            bw.bits(lcup, static_cast<int>(lblock));
+                            bw.bits(lref, static_cast<int>(lblock + 1));
+                            continue;
+                        }
                         bw.bit(0);  // one coding pass
                         const uint32_t len = static_cast<uint32_t>(s.size());
                         uint32_t lblock = 3;
