@@ -545,6 +545,9 @@ typedef struct {
                                 of the bits above; 1 only in code-blocks where SigProp
                                 reaches every sample of magnitude 1 (lossless), 2 in every
                                 code-block (the unreached samples are dropped) */
+    uint32_t subsampling;    /* byte c: component c's XRsiz (low nibble) and YRsiz (high
+                                nibble), 0 read as 1; the colour transform is skipped
+                                unless the first three components share one */
 } sgn_j2k_encode_options;
 
 void sgn_j2k_encode_default_options(sgn_j2k_encode_options *opts);

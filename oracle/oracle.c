@@ -227,7 +227,6 @@ static int read_main_header(const uint8_t *data, size_t len, header_t *h, uint32
         h->comp[i].xr = (uint8_t)xr;
         h->comp[i].yr = (uint8_t)yr;
         if (xr == 0 || yr == 0 || h->comp[i].precision > 32) return ORACLE_E_MALFORMED;
-        if (xr != 1 || yr != 1) return ORACLE_E_UNSUPPORTED;
         /* the signed flag (ssiz bit 7) changes nothing (codestream.rs:647-649) */
     }
     /* the first tile holds a reference grid point of the image (B-4,
@@ -419,7 +418,9 @@ typedef struct {
 } res_t;
 
 typedef struct {
-    rect_t rect;
+    rect_t rect;  /* the tile-component, component coordinates (B-12) */
+    rect_t trect; /* the tile on the reference grid */
+    uint32_t xr, yr;
     int nres;
     res_t res[MAXR];
 } tcomp_t;
@@ -513,10 +514,16 @@ static int tagtree_decode(tagtree_t *t, uint32_t x, uint32_t y, uint32_t thresho
     return path[0]->value < threshold ? 1 : 0;
 }
 
-static int build_tcomp(const header_t *h, const comp_t *c, rect_t trect, tcomp_t *tc)
+static int build_tcomp(const header_t *h, const comp_t *c, rect_t tile, tcomp_t *tc)
 {
     memset(tc, 0, sizeof(*tc));
+    /* tile-component rectangle (B-12, tile.rs:327-357) */
+    rect_t trect = {cdiv(tile.x0, c->xr), cdiv(tile.y0, c->yr), cdiv(tile.x1, c->xr),
+                    cdiv(tile.y1, c->yr)};
     tc->rect = trect;
+    tc->trect = tile;
+    tc->xr = c->xr;
+    tc->yr = c->yr;
     tc->nres = c->nlevels + 1;
     uint32_t nl = c->nlevels;
     for (int r = 0; r < tc->nres; ++r) {
@@ -885,24 +892,45 @@ static int prog_push(prog_list_t *pl, uint32_t l, uint32_t r, uint32_t c, uint32
     return 0;
 }
 
-/* Precinct index for a reference-grid position (B.12.1.3), or -1. */
-static long pos_precinct(const tile_t *t, const tcomp_t *tc, int r, uint32_t nl, uint32_t x,
-                         uint32_t y)
+/* Reference-grid position of precinct rows / columns (tile.rs:544-645):
+ * the first at the tile origin, or at the next multiple of the step
+ * XRsiz * 2^(PP + NL - r) when the resolution's origin times 2^(NL - r) is a
+ * multiple of it; each next one at the next multiple of the step. */
+static uint64_t first_position(uint64_t origin, uint64_t res_origin, uint32_t s, uint64_t step)
 {
-    const res_t *rs = &tc->res[r];
-    if (rs->npx * rs->npy == 0) return -1;
-    uint32_t s = nl - (uint32_t)r;
-    uint64_t stepx = 1ull << (rs->ppx + s), stepy = 1ull << (rs->ppy + s);
-    int okx = (x % stepx) == 0 ||
-              (x == t->rect.x0 && (((uint64_t)rs->rect.x0 << s) % stepx) != 0);
-    int oky = (y % stepy) == 0 ||
-              (y == t->rect.y0 && (((uint64_t)rs->rect.y0 << s) % stepy) != 0);
-    if (!okx || !oky) return -1;
-    uint64_t d = 1ull << s;
-    uint32_t px = (uint32_t)((((uint64_t)x + d - 1) / d) >> rs->ppx) - (rs->rect.x0 >> rs->ppx);
-    uint32_t py = (uint32_t)((((uint64_t)y + d - 1) / d) >> rs->ppy) - (rs->rect.y0 >> rs->ppy);
-    if (px >= rs->npx || py >= rs->npy) return -1;
-    return (long)(py * rs->npx + px);
+    if (origin % step != 0 && ((res_origin << s) % step) == 0) return (origin / step + 1) * step;
+    return origin;
+}
+
+typedef struct {
+    uint64_t y, x;
+    uint32_t c, r, idx;
+} ppos_t;
+
+static int prog_order;
+static int ppos_cmp(const void *pa, const void *pb)
+{
+    const ppos_t *a = (const ppos_t *)pa, *b = (const ppos_t *)pb;
+    uint64_t ka[5], kb[5];
+    if (prog_order == 2) { /* RPCL: resolution, y, x, component */
+        uint64_t t1[5] = {a->r, a->y, a->x, a->c, a->idx}, t2[5] = {b->r, b->y, b->x, b->c, b->idx};
+        memcpy(ka, t1, sizeof(ka));
+        memcpy(kb, t2, sizeof(kb));
+    } else if (prog_order == 3) { /* PCRL: y, x, component, resolution */
+        uint64_t t1[5] = {a->y, a->x, a->c, a->r, a->idx}, t2[5] = {b->y, b->x, b->c, b->r, b->idx};
+        memcpy(ka, t1, sizeof(ka));
+        memcpy(kb, t2, sizeof(kb));
+    } else { /* CPRL: component, y, x, resolution */
+        uint64_t t1[5] = {a->c, a->y, a->x, a->r, a->idx}, t2[5] = {b->c, b->y, b->x, b->r, b->idx};
+        memcpy(ka, t1, sizeof(ka));
+        memcpy(kb, t2, sizeof(kb));
+    }
+    for (int i = 0; i < 5; ++i)
+        if (ka[i] != kb[i]) return ka[i] < kb[i] ? -1 : 1;
+    /* equal keys: keep the generation order (component, resolution, index) */
+    if (a->c != b->c) return a->c < b->c ? -1 : 1;
+    if (a->r != b->r) return a->r < b->r ? -1 : 1;
+    return 0;
 }
 
 static int build_progression(const tile_t *t, int ncomp, const tcomp_t *tcs, prog_list_t *pl)
@@ -932,49 +960,42 @@ static int build_progression(const tile_t *t, int ncomp, const tcomp_t *tcs, pro
         }
         return ORACLE_OK;
     }
-    /* position-driven orders */
-    uint64_t dx = ~0ull, dy = ~0ull;
+    /* position-driven orders: every precinct of every component and
+     * resolution sorted by its reference-grid position (progression.rs:
+     * 257-345) */
+    size_t total = 0;
+    for (int c = 0; c < ncomp; ++c)
+        for (int r = 0; r < tcs[c].nres; ++r) total += (size_t)tcs[c].res[r].npx * tcs[c].res[r].npy;
+    ppos_t *v = (ppos_t *)malloc((total + 1) * sizeof(ppos_t));
+    if (!v) return ORACLE_E_NOMEM;
+    size_t n = 0;
     for (int c = 0; c < ncomp; ++c)
         for (int r = 0; r < tcs[c].nres; ++r) {
+            const res_t *rs = &tcs[c].res[r];
+            if (rs->npx * rs->npy == 0) continue;
             uint32_t s = (uint32_t)(tcs[c].nres - 1 - r);
-            uint64_t sx = 1ull << (tcs[c].res[r].ppx + s), sy = 1ull << (tcs[c].res[r].ppy + s);
-            if (sx < dx) dx = sx;
-            if (sy < dy) dy = sy;
-        }
-    rect_t tr = t->rect;
-#define YLOOP for (uint64_t y = tr.y0; y < tr.y1; y += dy - (y % dy))
-#define XLOOP for (uint64_t x = tr.x0; x < tr.x1; x += dx - (x % dx))
-    if (t->prog == 2) { /* RPCL */
-        for (uint32_t r = 0; r < maxr; ++r)
-            YLOOP XLOOP for (int c = 0; c < ncomp; ++c) {
-                if ((int)r >= tcs[c].nres) continue;
-                long k = pos_precinct(t, &tcs[c], (int)r, (uint32_t)tcs[c].nres - 1, (uint32_t)x,
-                                      (uint32_t)y);
-                if (k < 0) continue;
-                for (uint32_t l = 0; l < L; ++l)
-                    if (prog_push(pl, l, r, (uint32_t)c, (uint32_t)k)) return ORACLE_E_NOMEM;
+            uint64_t stepx = (uint64_t)tcs[c].xr << (rs->ppx + s);
+            uint64_t stepy = (uint64_t)tcs[c].yr << (rs->ppy + s);
+            uint64_t ry = first_position(tcs[c].trect.y0, rs->rect.y0, s, stepy);
+            for (uint32_t py = 0; py < rs->npy; ++py) {
+                uint64_t rx = first_position(tcs[c].trect.x0, rs->rect.x0, s, stepx);
+                for (uint32_t px = 0; px < rs->npx; ++px) {
+                    v[n++] = (ppos_t){ry, rx, (uint32_t)c, (uint32_t)r, py * rs->npx + px};
+                    rx = (rx / stepx + 1) * stepx;
+                }
+                ry = (ry / stepy + 1) * stepy;
             }
-    } else if (t->prog == 3) { /* PCRL */
-        YLOOP XLOOP for (int c = 0; c < ncomp; ++c) for (int r = 0; r < tcs[c].nres; ++r) {
-            long k = pos_precinct(t, &tcs[c], r, (uint32_t)tcs[c].nres - 1, (uint32_t)x,
-                                  (uint32_t)y);
-            if (k < 0) continue;
-            for (uint32_t l = 0; l < L; ++l)
-                if (prog_push(pl, l, (uint32_t)r, (uint32_t)c, (uint32_t)k)) return ORACLE_E_NOMEM;
         }
-    } else { /* CPRL */
-        for (int c = 0; c < ncomp; ++c)
-            YLOOP XLOOP for (int r = 0; r < tcs[c].nres; ++r) {
-                long k = pos_precinct(t, &tcs[c], r, (uint32_t)tcs[c].nres - 1, (uint32_t)x,
-                                      (uint32_t)y);
-                if (k < 0) continue;
-                for (uint32_t l = 0; l < L; ++l)
-                    if (prog_push(pl, l, (uint32_t)r, (uint32_t)c, (uint32_t)k))
-                        return ORACLE_E_NOMEM;
+    prog_order = t->prog;
+    /* merge sort is stable; qsort with the generation order as last key */
+    qsort(v, n, sizeof(ppos_t), ppos_cmp);
+    for (size_t i = 0; i < n; ++i)
+        for (uint32_t l = 0; l < L; ++l)
+            if (prog_push(pl, l, v[i].r, v[i].c, v[i].idx)) {
+                free(v);
+                return ORACLE_E_NOMEM;
             }
-    }
-#undef YLOOP
-#undef XLOOP
+    free(v);
     return ORACLE_OK;
 }
 
@@ -1563,6 +1584,24 @@ typedef struct {
     int mct, reversible;
 } image_t;
 
+/* Shrink factors of the output (codestream.rs:663-683): the components'
+ * sub-sampling when every component has the same, else 1. */
+static void shrink_of(const header_t *h, uint32_t *sx, uint32_t *sy)
+{
+    int same = 1;
+    for (int c = 1; c < h->ncomp; ++c)
+        same &= h->comp[c].xr == h->comp[0].xr && h->comp[c].yr == h->comp[0].yr;
+    *sx = same ? h->comp[0].xr : 1;
+    *sy = same ? h->comp[0].yr : 1;
+}
+
+static int any_subsampled(const header_t *h)
+{
+    for (int c = 0; c < h->ncomp; ++c)
+        if (h->comp[c].xr != 1 || h->comp[c].yr != 1) return 1;
+    return 0;
+}
+
 static int decode_image(const uint8_t *data, size_t len, uint32_t denominator, int use_f64,
                         image_t *img)
 {
@@ -1582,8 +1621,10 @@ static int decode_image(const uint8_t *data, size_t len, uint32_t denominator, i
             return rc;
         }
         if (denominator > 1) {
-            tw = cdiv(probe.xsiz - probe.xo, denominator);
-            th = cdiv(probe.ysiz - probe.yo, denominator);
+            uint32_t sx, sy;
+            shrink_of(&probe, &sx, &sy);
+            tw = cdiv(cdiv(probe.xsiz - probe.xo, sx), denominator);
+            th = cdiv(cdiv(probe.ysiz - probe.yo, sy), denominator);
         }
     }
     rc = read_main_header(data, len, h, tw, th);
@@ -1591,6 +1632,15 @@ static int decode_image(const uint8_t *data, size_t len, uint32_t denominator, i
         free(h);
         return rc;
     }
+    /* the store of a reduced resolution indexes sub-sampled components
+     * with full-resolution offsets (decode.rs:1858-1898): full resolution
+     * only */
+    if (h->skipped && any_subsampled(h)) {
+        free(h);
+        return ORACLE_E_UNSUPPORTED;
+    }
+    uint32_t shx, shy;
+    shrink_of(h, &shx, &shy);
     uint32_t ntx = cdiv(h->xsiz - h->xto, h->xt), nty = cdiv(h->ysiz - h->yto, h->yt);
     uint32_t ntiles = ntx * nty;
     tile_t *tiles = (tile_t *)calloc(ntiles, sizeof(tile_t));
@@ -1605,7 +1655,7 @@ static int decode_image(const uint8_t *data, size_t len, uint32_t denominator, i
     uint32_t sk = h->skipped;
     /* image_width / image_height (codestream.rs:544-553); a non-zero image
      * origin only at full resolution (read_main_header) */
-    uint32_t iw = cdiv(h->xsiz - h->xo, 1u << sk), ih = cdiv(h->ysiz - h->yo, 1u << sk);
+    uint32_t iw = cdiv(h->xsiz - h->xo, shx << sk), ih = cdiv(h->ysiz - h->yo, shy << sk);
     img->w = iw;
     img->h = ih;
     img->ncomp = h->ncomp;
@@ -1666,6 +1716,35 @@ static int decode_image(const uint8_t *data, size_t len, uint32_t denominator, i
              * the image plane; add the sign shift unless MCT follows. */
             rect_t rr = tc->res[nr - 1].rect;
             float shift = t->mct ? 0.0f : (float)(1u << (t->comp[c].precision - 1));
+            if (tc->xr != 1 || tc->yr != 1 || shx != 1 || shy != 1) {
+                /* sample by sample (decode.rs:1858-1898): sample x covers
+                 * [XRsiz x / shrink, + XRsiz) of the output grid, later
+                 * samples overwriting earlier ones */
+                uint32_t xoff = cdiv(h->xo, shx), yoff = cdiv(h->yo, shy);
+                for (uint32_t y = rr.y0; y < rr.y1; ++y) {
+                    uint32_t ry = (tc->yr * y) / shy;
+                    for (uint32_t x = rr.x0; x < rr.x1; ++x) {
+                        uint32_t rx = (tc->xr * x) / shx;
+                        size_t src = (size_t)(y - rr.y0) * rw(rr) + (x - rr.x0);
+                        double v = use_f64 ? ((double *)cur)[src] + (double)shift
+                                           : (double)(((float *)cur)[src] + shift);
+                        uint32_t xa = rx > xoff ? rx : xoff, xb = rx + tc->xr;
+                        uint32_t ya = ry > yoff ? ry : yoff, yb = ry + tc->yr;
+                        if (xb > iw + xoff) xb = iw + xoff;
+                        if (yb > ih + yoff) yb = ih + yoff;
+                        for (uint32_t xp = xa; xp < xb; ++xp)
+                            for (uint32_t yp = ya; yp < yb; ++yp) {
+                                size_t dst = (size_t)c * plane + (size_t)(yp - yoff) * iw + (xp - xoff);
+                                if (use_f64)
+                                    img->planes64[dst] = v;
+                                else
+                                    img->planes[dst] = (float)v;
+                            }
+                    }
+                }
+                if (own) free(cur);
+                continue;
+            }
             for (uint32_t y = 0; y < rh(rr); ++y)
                 for (uint32_t x = 0; x < rw(rr); ++x) {
                     size_t src = (size_t)y * rw(rr) + x;
@@ -1680,7 +1759,9 @@ static int decode_image(const uint8_t *data, size_t len, uint32_t denominator, i
         }
         for (int c = 0; c < h->ncomp; ++c) free_tcomp(&tcs[c]);
     }
-    /* MCT + sign shift (decode.rs:114-119; mct.rs:11-156) */
+    /* MCT + sign shift (decode.rs:114-119; mct.rs:11-156); components of
+     * different sub-sampling are not transformed together here */
+    if (!rc && tiles[0].mct && any_subsampled(h) && (shx == 1 && shy == 1)) rc = ORACLE_E_UNSUPPORTED;
     if (!rc && tiles[0].mct) {
         if (h->ncomp >= 3) {
             int tr = h->comp[0].transform;
@@ -1829,8 +1910,10 @@ int oracle_inspect(const uint8_t *data, size_t len, oracle_info_t *info)
     int rc = read_main_header(data, len, h, 0, 0);
     if (!rc) {
         memset(info, 0, sizeof(*info));
-        info->width = h->xsiz - h->xo;
-        info->height = h->ysiz - h->yo;
+        uint32_t sx, sy;
+        shrink_of(h, &sx, &sy);
+        info->width = cdiv(h->xsiz - h->xo, sx);
+        info->height = cdiv(h->ysiz - h->yo, sy);
         info->num_components = h->ncomp;
         info->bit_depth = h->comp[0].precision;
         uint32_t nl = 255;
@@ -1857,12 +1940,18 @@ int oracle_scaled_dims(const uint8_t *data, size_t len, uint32_t denominator, ui
     if (!h) return ORACLE_E_NOMEM;
     int rc = read_main_header(data, len, h, 0, 0);
     if (!rc && denominator > 1) {
-        uint32_t tw = cdiv(h->xsiz - h->xo, denominator), th = cdiv(h->ysiz - h->yo, denominator);
+        uint32_t sx, sy;
+        shrink_of(h, &sx, &sy);
+        uint32_t tw = cdiv(cdiv(h->xsiz - h->xo, sx), denominator);
+        uint32_t th = cdiv(cdiv(h->ysiz - h->yo, sy), denominator);
         rc = read_main_header(data, len, h, tw, th);
     }
+    if (!rc && h->skipped && any_subsampled(h)) rc = ORACLE_E_UNSUPPORTED;
     if (!rc) {
-        *out_w = cdiv(h->xsiz - h->xo, 1u << h->skipped);
-        *out_h = cdiv(h->ysiz - h->yo, 1u << h->skipped);
+        uint32_t sx, sy;
+        shrink_of(h, &sx, &sy);
+        *out_w = cdiv(h->xsiz - h->xo, sx << h->skipped);
+        *out_h = cdiv(h->ysiz - h->yo, sy << h->skipped);
     }
     free(h);
     return rc;

@@ -804,6 +804,11 @@ def encode_j2k_ht(samples, bit_depth=None, **options):
     if arr.dtype not in (np.uint8, np.uint16) or (bit_depth > 8) != (arr.dtype == np.uint16):
         raise J2kError(_lib.SGN_E_ARG, "samples must be uint8 (<= 8 bits) or uint16")
     o = default_encode_options()
+    if "subsampling" in options and not isinstance(options["subsampling"], int):
+        # [(dx, dy), ...] per component
+        options = dict(options)
+        options["subsampling"] = sum(((dx & 0xF) | (dy & 0xF) << 4) << (8 * c)
+                                     for c, (dx, dy) in enumerate(options["subsampling"]))
     for k, v in options.items():
         if not hasattr(o, k):
             raise TypeError(f"unknown encode option {k}")

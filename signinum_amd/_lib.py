@@ -99,7 +99,7 @@ class EncodeOptions(ctypes.Structure):
         [(n, ctypes.c_uint32) for n in ("precinct_log2", "sop", "eph", "tile_width",
                                          "tile_height", "image_x0", "image_y0", "tile_x0",
                                          "tile_y0", "packed_headers", "block_coder",
-                                         "cb_style", "ht_refine")])
+                                         "cb_style", "ht_refine", "subsampling")])
 
 
 class HtCodeBlockBatchJob(ctypes.Structure):

@@ -1132,6 +1132,41 @@ sgn_final_irr(const FinalJob *jobs, uint32_t njobs, uint32_t block_offset,
 }
 
 // Byte fill used instead of memset nodes (safe under graph capture).
+// Output of tile-components on different sampling grids (StoreJob): one
+// work-group per 64x64 block of a job's output rectangle; each pixel takes
+// the component sample its grid position falls on (decode.rs:1858-1898),
+// shifted, converted like the fused final stage and written as its channel
+// (with alpha from channel 0 in RGBA8).
+extern "C" __global__ void __launch_bounds__(kIdwtThreads)
+sgn_store_sampled(const StoreJob *jobs, uint32_t njobs, uint32_t block_offset,
+                  const uint32_t *block_job, const uint32_t *coef) {
+    const uint32_t blk = blockIdx.x + block_offset;
+    const StoreJob *sj = job_of(jobs, njobs, block_job, blk);
+    const uint32_t local = blk - sj->block_base;
+    const uint32_t x0 = sj->ox0 + (local % sj->nbx) * kBlk, y0 = sj->oy0 + (local / sj->nbx) * kBlk;
+    const uint32_t x1 = min(x0 + (uint32_t)kBlk, sj->ox1), y1 = min(y0 + (uint32_t)kBlk, sj->oy1);
+    const bool rev = (sj->flags & 1u) != 0, same8 = (sj->flags & 2u) != 0;
+    const bool alpha = (sj->flags & 4u) != 0;
+    const float shift = (float)(1u << (sj->precision - 1));
+    uint8_t *const out = reinterpret_cast<uint8_t *>(sj->dst);
+    for (uint32_t k = threadIdx.x; k < (uint32_t)(kBlk * kBlk); k += kIdwtThreads) {
+        const uint32_t X = x0 + k % kBlk, Y = y0 + k / kBlk;
+        if (X >= x1 || Y >= y1) continue;
+        const uint32_t u = (X + sj->xo) / sj->xr - sj->cx0, v = (Y + sj->yo) / sj->yr - sj->cy0;
+        const uint32_t w = coef[sj->plane + v * sj->pitch + u];
+        const float f = (rev ? (float)(int32_t)w : __uint_as_float(w)) + shift;
+        uint8_t *px = out + (size_t)Y * sj->opitch + (size_t)X * sj->nchan * sj->bps;
+        if (sj->bps == 1) {
+            px[sj->chan] = (uint8_t)out_u8(f, same8, sj->precision);
+            if (alpha) px[3] = 255;
+        } else {
+            const uint32_t o = out_u16(f, sj->prec0);
+            px[2 * sj->chan] = (uint8_t)(o & 0xFFu);
+            px[2 * sj->chan + 1] = (uint8_t)(o >> 8);
+        }
+    }
+}
+
 extern "C" __global__ void sgn_fill_bytes(uint8_t *p, size_t bytes, size_t words, uint32_t v4) {
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     uint32_t *w = reinterpret_cast<uint32_t *>(p);

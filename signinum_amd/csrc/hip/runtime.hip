@@ -359,6 +359,13 @@ struct sgn_j2k_plan {
     std::vector<uint8_t> joined;
     ClassicJob *d_classic = nullptr;
     ClassicSeg *d_cseg = nullptr;
+    // output jobs of tiles whose components are on different sampling
+    // grids, their 64x64-block table, and each tile's range
+    std::vector<StoreJob> store;
+    std::vector<uint32_t> store_block_job;
+    std::vector<std::pair<uint32_t, uint32_t>> tile_store;  // (first, count)
+    StoreJob *d_store = nullptr;
+    uint32_t *d_store_block_job = nullptr;
     int lane_variant = kDefaultLaneVariant;  // index into kRowsVariants
     int dwt_shape = 0;                       // index into kDwtShapes
     hipGraphExec_t graph_exec = nullptr;
@@ -615,6 +622,15 @@ Status build_plan(const sgn_tile_req *reqs, size_t n, sgn_j2k_plan &P, int32_t *
             else
                 P.cb_wave.push_back(j);
         }
+        P.tile_store.push_back({static_cast<uint32_t>(P.store.size()),
+                                static_cast<uint32_t>(lp.stores.size())});
+        for (StoreJob sj : lp.stores) {
+            sj.plane += cbase;
+            sj.block_base = static_cast<uint32_t>(P.store_block_job.size());
+            for (uint32_t b = 0; b < uint32_t(sj.nbx) * sj.nby; ++b)
+                P.store_block_job.push_back(static_cast<uint32_t>(P.store.size()));
+            P.store.push_back(sj);
+        }
         const uint32_t seg_base = static_cast<uint32_t>(P.cseg.size());
         P.cseg.insert(P.cseg.end(), lp.cseg.begin(), lp.cseg.end());
         for (ClassicJob j : lp.classic) {
@@ -791,6 +807,7 @@ Status upload_plan(sgn_hip_session *s, const sgn_tile_req *jobs, sgn_j2k_plan &P
     const uint64_t o_lbj0 = take(bytes_of(P.lv_block_job[0])), o_lbj1 = take(bytes_of(P.lv_block_job[1]));
     const uint64_t o_fbj0 = take(bytes_of(P.fin_block_job[0])), o_fbj1 = take(bytes_of(P.fin_block_job[1]));
     const uint64_t o_cls = take(bytes_of(P.classic)), o_cseg = take(bytes_of(P.cseg));
+    const uint64_t o_sto = take(bytes_of(P.store)), o_sbj = take(bytes_of(P.store_block_job));
     const uint64_t host_bytes = off;
     const uint64_t o_stl = take(P.cb_lane.size()), o_stw = take(P.cb_wave.size());
     const uint64_t o_ff = take(P.fin_block_job[1].size() * sizeof(uint32_t));
@@ -836,6 +853,8 @@ Status upload_plan(sgn_hip_session *s, const sgn_tile_req *jobs, sgn_j2k_plan &P
         HIP_TRY(put_dev(o_fbj1, P.fin_block_job[1]));
         HIP_TRY(put_dev(o_cls, P.classic));
         HIP_TRY(put_dev(o_cseg, P.cseg));
+        HIP_TRY(put_dev(o_sto, P.store));
+        HIP_TRY(put_dev(o_sbj, P.store_block_job));
         HIP_TRY(hipStreamSynchronize(copy));
         P.t_stage = 0;
         P.t_copy = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() -
@@ -892,6 +911,8 @@ Status upload_plan(sgn_hip_session *s, const sgn_tile_req *jobs, sgn_j2k_plan &P
         put(o_fbj1, P.fin_block_job[1]);
         put(o_cls, P.classic);
         put(o_cseg, P.cseg);
+        put(o_sto, P.store);
+        put(o_sbj, P.store_block_job);
         const auto t1 = std::chrono::steady_clock::now();
         uint8_t *d = static_cast<uint8_t *>(P.d_block);
         // direct runs first, then the staged range: consecutive staged runs and
@@ -951,6 +972,8 @@ Status upload_plan(sgn_hip_session *s, const sgn_tile_req *jobs, sgn_j2k_plan &P
     at(o_fbj1, P.d_fin_block_job[1], !P.fin_block_job[1].empty());
     at(o_cls, P.d_classic, !P.classic.empty());
     at(o_cseg, P.d_cseg, !P.cseg.empty());
+    at(o_sto, P.d_store, !P.store.empty());
+    at(o_sbj, P.d_store_block_job, !P.store_block_job.empty());
     at(o_stl, P.d_status_lane, !P.cb_lane.empty());
     at(o_stw, P.d_status_wave, !P.cb_wave.empty());
     at(o_ff, P.d_fin_redo, !P.fin_block_job[1].empty());
@@ -1126,6 +1149,19 @@ uint32_t launch_finals(sgn_hip_session *s, sgn_j2k_plan &P, int rev, uint32_t a,
     return 1;
 }
 
+// Store jobs [a, b) (sub-sampled components): one work-group per 64x64
+// output block.
+uint32_t launch_stores(sgn_hip_session *s, sgn_j2k_plan &P, uint32_t a, uint32_t b) {
+    if (a >= b) return 0;
+    const StoreJob &first = P.store[a], &last = P.store[b - 1];
+    const uint32_t blocks = last.block_base + uint32_t(last.nbx) * last.nby - first.block_base;
+    if (!blocks) return 0;
+    hipLaunchKernelGGL(sgn_store_sampled, dim3(blocks), dim3(kIdwtThreads), 0, s->stream,
+                       P.d_store, uint32_t(P.store.size()), first.block_base, P.d_store_block_job,
+                       P.d_coef);
+    return 1;
+}
+
 uint32_t enqueue_batched(sgn_hip_session *s, sgn_j2k_plan &P) {
     uint32_t k = clear_counters(s, P, 2);  // one node for both counters
     k += launch_classic(s, P);
@@ -1137,6 +1173,7 @@ uint32_t enqueue_batched(sgn_hip_session *s, sgn_j2k_plan &P) {
             k += launch_levels(s, P, rev, P.lv_group[rev][d - 1], P.lv_group[rev][d]);
     for (int rev = 0; rev < 2; ++rev)
         k += launch_finals(s, P, rev, 0, static_cast<uint32_t>(P.fin[rev].size()), false);
+    k += launch_stores(s, P, 0, static_cast<uint32_t>(P.store.size()));
     return k;
 }
 
@@ -1157,6 +1194,8 @@ uint32_t enqueue_plain(sgn_hip_session *s, sgn_j2k_plan &P) {
             const auto ff = P.tile_final_first[t];
             k += launch_finals(s, P, ff.first, ff.second, ff.second + P.tile_final_count[t]);
         }
+        k += launch_stores(s, P, P.tile_store[t].first,
+                           P.tile_store[t].first + P.tile_store[t].second);
     }
     return k;
 }
@@ -1769,6 +1808,7 @@ int sgn_j2k_plan_decode_stage(sgn_hip_session *s, sgn_j2k_plan *P, int stage) {
     } else if (stage == SGN_STAGE_FINAL) {
         for (int rev = 0; rev < 2; ++rev)
             k += launch_finals(s, *P, rev, 0, static_cast<uint32_t>(P->fin[rev].size()));
+        k += launch_stores(s, *P, 0, static_cast<uint32_t>(P->store.size()));
     } else {
         set_last_error("unknown stage");
         return SGN_E_ARG;

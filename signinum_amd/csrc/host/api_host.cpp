@@ -17,8 +17,9 @@ int sgn_j2k_inspect(const uint8_t *bytes, size_t len, sgn_j2k_info *info) {
         st = sgn::read_main_header(c.cs, c.cs_len, h);
         if (st) return st;
         *info = sgn_j2k_info{};
-        info->width = h.xsiz - h.xo;
-        info->height = h.ysiz - h.yo;
+        const sgn::Sampling smp = sgn::sampling_of(h);
+        info->width = sgn::ceil_div(h.xsiz - h.xo, smp.sx);
+        info->height = sgn::ceil_div(h.ysiz - h.yo, smp.sy);
         info->num_components = h.ncomp;
         info->bit_depth = h.comp[0].precision;
         uint32_t nl = 255;
@@ -63,12 +64,17 @@ int sgn_j2k_scaled_dims(const uint8_t *bytes, size_t len, uint32_t denominator, 
         return SGN_E_ARG;
     }
     const uint32_t sk = sgn::skipped_levels(h, denominator);
+    const sgn::Sampling smp = sgn::sampling_of(h);
     if (sk && (h.xo || h.yo)) {
         sgn::set_last_error("reduced resolution of an image with a non-zero origin");
         return SGN_E_UNSUPPORTED;
     }
-    if (width) *width = sgn::ceil_div(h.xsiz - h.xo, 1u << sk);
-    if (height) *height = sgn::ceil_div(h.ysiz - h.yo, 1u << sk);
+    if (sk && smp.any) {
+        sgn::set_last_error("reduced resolution of sub-sampled components");
+        return SGN_E_UNSUPPORTED;
+    }
+    if (width) *width = sgn::ceil_div(h.xsiz - h.xo, smp.sx << sk);
+    if (height) *height = sgn::ceil_div(h.ysiz - h.yo, smp.sy << sk);
     return SGN_OK;
 }
 

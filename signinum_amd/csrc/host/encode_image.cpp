@@ -231,7 +231,17 @@ Status encode_image(const void *pixels, uint32_t width, uint32_t height, uint32_
         return kErrArg;
     }
     const bool rev = opt.reversible != 0;
-    const bool mct = opt.use_mct && ncomp >= 3;
+    // sub-sampling of component c: nibbles of byte c (0 reads as 1)
+    uint32_t sub_x[4], sub_y[4];
+    for (uint32_t c = 0; c < 4; ++c) {
+        sub_x[c] = (opt.subsampling >> (8 * c)) & 0xF;
+        sub_y[c] = (opt.subsampling >> (8 * c + 4)) & 0xF;
+        if (!sub_x[c]) sub_x[c] = 1;
+        if (!sub_y[c]) sub_y[c] = 1;
+    }
+    // the colour transform needs the first three components on one grid
+    const bool mct = opt.use_mct && ncomp >= 3 && sub_x[0] == sub_x[1] && sub_x[1] == sub_x[2] &&
+                     sub_y[0] == sub_y[1] && sub_y[1] == sub_y[2];
     const uint32_t xo = opt.image_x0, yo = opt.image_y0;
     const uint32_t xto = opt.tile_x0, yto = opt.tile_y0;
     const uint64_t xsiz64 = uint64_t(xo) + width, ysiz64 = uint64_t(yo) + height;
@@ -259,11 +269,15 @@ Status encode_image(const void *pixels, uint32_t width, uint32_t height, uint32_
     };
     uint32_t levels = opt.levels;
     {
-        // at most log2 of the smallest side of any tile
+        // at most log2 of the smallest side of any tile-component
         uint32_t mn = UINT32_MAX;
         for (uint32_t ti = 0; ti < ntx * nty; ++ti) {
             const Rect r = tile_rect(ti);
-            mn = std::min(mn, std::min(r.w(), r.h()));
+            for (uint32_t c = 0; c < ncomp; ++c) {
+                const uint32_t cw = ceil_div(r.x1, sub_x[c]) - ceil_div(r.x0, sub_x[c]);
+                const uint32_t ch = ceil_div(r.y1, sub_y[c]) - ceil_div(r.y0, sub_y[c]);
+                mn = std::min(mn, std::min(cw, ch));
+            }
         }
         const uint32_t lim = mn ? 31 - __builtin_clz(mn) : 0;
         levels = std::min(levels, lim);
@@ -345,7 +359,11 @@ Status encode_image(const void *pixels, uint32_t width, uint32_t height, uint32_
     mh.prog = static_cast<uint8_t>(opt.progression);
     mh.layers = 1;
     mh.mct = mct;
-    for (uint32_t c = 0; c < ncomp; ++c) mh.comp[c] = cp;
+    for (uint32_t c = 0; c < ncomp; ++c) {
+        mh.comp[c] = cp;
+        mh.comp[c].xr = static_cast<uint8_t>(sub_x[c]);
+        mh.comp[c].yr = static_cast<uint8_t>(sub_y[c]);
+    }
 
     // ---- main header ----
     cs.clear();
@@ -364,8 +382,8 @@ Status encode_image(const void *pixels, uint32_t width, uint32_t height, uint32_
     put16(cs, ncomp);
     for (uint32_t c = 0; c < ncomp; ++c) {
         cs.push_back(static_cast<uint8_t>(bit_depth - 1));
-        cs.push_back(1);
-        cs.push_back(1);
+        cs.push_back(static_cast<uint8_t>(sub_x[c]));
+        cs.push_back(static_cast<uint8_t>(sub_y[c]));
     }
     if (!classic) {
         // CAP: Part 15 capabilities, Ccap15 as in codestream_write.rs:166-180
@@ -414,20 +432,29 @@ Status encode_image(const void *pixels, uint32_t width, uint32_t height, uint32_
         std::vector<int32_t> ip(rev ? plane * ncomp : 0);
         std::vector<double> dp(rev ? 0 : plane * ncomp);
         const int32_t shift = 1 << (bit_depth - 1);
-        for (uint32_t y = 0; y < h; ++y) {
-            const uint8_t *row =
-                static_cast<const uint8_t *>(pixels) + size_t(tr.y0 + y - yo) * row_stride;
-            for (uint32_t x = 0; x < w; ++x)
-                for (uint32_t c = 0; c < ncomp; ++c) {
-                    const size_t si = static_cast<size_t>(tr.x0 + x - xo) * ncomp + c;
-                    int32_t v = bit_depth <= 8 ? row[si]
+        // component c's samples (B-12: its tile rectangle divided by the
+        // sub-sampling), sample (u, v) taken from the image pixel at
+        // reference-grid position (u XRsiz, v YRsiz), row pitch the
+        // component's width, at the start of the component's plane
+        for (uint32_t c = 0; c < ncomp; ++c) {
+            const uint32_t xr = sub_x[c], yr = sub_y[c];
+            const Rect cr{ceil_div(tr.x0, xr), ceil_div(tr.y0, yr), ceil_div(tr.x1, xr),
+                          ceil_div(tr.y1, yr)};
+            for (uint32_t v = cr.y0; v < cr.y1; ++v) {
+                const uint8_t *row =
+                    static_cast<const uint8_t *>(pixels) + size_t(v * yr - yo) * row_stride;
+                for (uint32_t u = cr.x0; u < cr.x1; ++u) {
+                    const size_t si = static_cast<size_t>(u * xr - xo) * ncomp + c;
+                    int32_t s = bit_depth <= 8 ? row[si]
                                                : reinterpret_cast<const uint16_t *>(row)[si];
-                    v -= shift;
+                    s -= shift;
+                    const size_t di = c * plane + size_t(v - cr.y0) * cr.w() + (u - cr.x0);
                     if (rev)
-                        ip[c * plane + static_cast<size_t>(y) * w + x] = v;
+                        ip[di] = s;
                     else
-                        dp[c * plane + static_cast<size_t>(y) * w + x] = v;
+                        dp[di] = s;
                 }
+            }
         }
         if (mct) {
             for (size_t i = 0; i < plane; ++i) {
@@ -449,7 +476,7 @@ Status encode_image(const void *pixels, uint32_t width, uint32_t height, uint32_
         std::vector<std::vector<int32_t>> qcoef(ncomp);  // per component: bands concatenated
         std::vector<std::vector<size_t>> band_off(ncomp);
         for (uint32_t c = 0; c < ncomp; ++c) {
-            Status st = build_tile_comp(cp, tr, tcs[c]);
+            Status st = build_tile_comp(mh.comp[c], tr, tcs[c]);
             if (st) return st;
             TileCompGeom &g = tcs[c];
             // work buffer in tile coordinates; each level analyses the

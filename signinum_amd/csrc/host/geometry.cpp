@@ -1,6 +1,9 @@
 // Tile-component geometry, progression orders, quantisation and HT tables.
+#include <algorithm>
 #include <cmath>
 #include <mutex>
+#include <tuple>
+#include <vector>
 
 #include "ht_vlc_rows.hpp"
 #include "j2k_host.hpp"
@@ -17,8 +20,13 @@ const char *last_error() { return g_last_error.c_str(); }
 // The precinct origin in sub-band coordinates follows the reference
 // (tile.rs:542-570): the resolution-domain anchor floor(x0 / 2^PP) * 2^PP,
 // halved together with the exponent for r > 0.
-Status build_tile_comp(const CompParams &cp, const Rect &trect, TileCompGeom &g) {
+Status build_tile_comp(const CompParams &cp, const Rect &tile, TileCompGeom &g) {
     g = TileCompGeom();
+    g.trect = tile;
+    g.xr = cp.xr;
+    g.yr = cp.yr;
+    const Rect trect{ceil_div(tile.x0, cp.xr), ceil_div(tile.y0, cp.yr), ceil_div(tile.x1, cp.xr),
+                     ceil_div(tile.y1, cp.yr)};
     g.rect = trect;
     g.nres = static_cast<uint8_t>(cp.levels + 1);
     const uint32_t nl = cp.levels;
@@ -126,30 +134,17 @@ Status build_tile_comp(const CompParams &cp, const Rect &trect, TileCompGeom &g)
     return kOk;
 }
 
-// Precinct of a reference-grid position for the position-driven orders
-// (B.12.1.3-B.12.1.5), or -1 when no packet starts there.
-static long position_precinct(const Rect &trect, const TileCompGeom &tc, uint32_t r, uint32_t x,
-                              uint32_t y) {
-    const ResGeom &rs = tc.res[r];
-    if (rs.npx * rs.npy == 0) return -1;
-    const uint32_t s = tc.nres - 1 - r;
-    const uint64_t stepx = 1ull << (rs.ppx + s), stepy = 1ull << (rs.ppy + s);
-    const bool okx = (x % stepx) == 0 ||
-                     (x == trect.x0 && ((static_cast<uint64_t>(rs.rect.x0) << s) % stepx) != 0);
-    const bool oky = (y % stepy) == 0 ||
-                     (y == trect.y0 && ((static_cast<uint64_t>(rs.rect.y0) << s) % stepy) != 0);
-    if (!okx || !oky) return -1;
-    const uint64_t d = 1ull << s;
-    const uint32_t px =
-        static_cast<uint32_t>(((x + d - 1) / d) >> rs.ppx) - (rs.rect.x0 >> rs.ppx);
-    const uint32_t py =
-        static_cast<uint32_t>(((y + d - 1) / d) >> rs.ppy) - (rs.rect.y0 >> rs.ppy);
-    if (px >= rs.npx || py >= rs.npy) return -1;
-    return static_cast<long>(py * rs.npx + px);
-}
-
+// Packet order of one tile (B.12).  Layer- and resolution-major orders
+// walk (layer, resolution, component, precinct) directly; the position
+// orders sort every precinct of every component and resolution by its
+// position on the reference grid (progression.rs:257-345): a precinct row
+// or column starts at the tile origin, or at the next multiple of the step
+// XRsiz * 2^(PP + NL - r) when the resolution's origin times 2^(NL - r) is
+// a multiple of it, and each next one at the next multiple of the step
+// (tile.rs:544-645).
 Status progression(uint8_t order, uint16_t layers, const Rect &trect, const TileCompGeom *tcs,
                    int ncomp, std::vector<PacketId> &out) {
+    (void)trect;
     out.clear();
     uint32_t maxr = 0;
     for (int c = 0; c < ncomp; ++c) maxr = tcs[c].nres > maxr ? tcs[c].nres : maxr;
@@ -173,40 +168,47 @@ Status progression(uint8_t order, uint16_t layers, const Rect &trect, const Tile
         return kOk;
     }
     if (order > 4) return kErrMalformed;
-    uint64_t dx = ~0ull, dy = ~0ull;
-    for (int c = 0; c < ncomp; ++c)
-        for (uint32_t r = 0; r < tcs[c].nres; ++r) {
-            const uint32_t s = tcs[c].nres - 1 - r;
-            const uint64_t sx = 1ull << (tcs[c].res[r].ppx + s);
-            const uint64_t sy = 1ull << (tcs[c].res[r].ppy + s);
-            dx = sx < dx ? sx : dx;
-            dy = sy < dy ? sy : dy;
-        }
-    auto emit_pos = [&](uint32_t r, int c, uint64_t x, uint64_t y) {
-        if (r >= tcs[c].nres) return;
-        const long k = position_precinct(trect, tcs[c], r, static_cast<uint32_t>(x),
-                                         static_cast<uint32_t>(y));
-        if (k < 0) return;
-        for (uint32_t l = 0; l < layers; ++l)
-            out.push_back({static_cast<uint16_t>(l), static_cast<uint16_t>(r),
-                           static_cast<uint16_t>(c), static_cast<uint32_t>(k)});
+    struct Pos {
+        uint64_t y, x;
+        uint16_t c, r;
+        uint32_t idx;
     };
-    if (order == 2) {  // RPCL
-        for (uint32_t r = 0; r < maxr; ++r)
-            for (uint64_t y = trect.y0; y < trect.y1; y += dy - (y % dy))
-                for (uint64_t x = trect.x0; x < trect.x1; x += dx - (x % dx))
-                    for (int c = 0; c < ncomp; ++c) emit_pos(r, c, x, y);
-    } else if (order == 3) {  // PCRL
-        for (uint64_t y = trect.y0; y < trect.y1; y += dy - (y % dy))
-            for (uint64_t x = trect.x0; x < trect.x1; x += dx - (x % dx))
-                for (int c = 0; c < ncomp; ++c)
-                    for (uint32_t r = 0; r < tcs[c].nres; ++r) emit_pos(r, c, x, y);
-    } else {  // CPRL
-        for (int c = 0; c < ncomp; ++c)
-            for (uint64_t y = trect.y0; y < trect.y1; y += dy - (y % dy))
-                for (uint64_t x = trect.x0; x < trect.x1; x += dx - (x % dx))
-                    for (uint32_t r = 0; r < tcs[c].nres; ++r) emit_pos(r, c, x, y);
+    std::vector<Pos> v;
+    auto first_at = [](uint64_t origin, uint64_t res_origin, uint32_t s, uint64_t step) {
+        if (origin % step != 0 && ((res_origin << s) % step) == 0)
+            return (origin / step + 1) * step;
+        return origin;
+    };
+    for (int c = 0; c < ncomp; ++c) {
+        const TileCompGeom &g = tcs[c];
+        for (uint32_t r = 0; r < g.nres; ++r) {
+            const ResGeom &rs = g.res[r];
+            if (rs.npx * rs.npy == 0) continue;
+            const uint32_t s = g.nres - 1 - r;
+            const uint64_t stepx = uint64_t(g.xr) << (rs.ppx + s);
+            const uint64_t stepy = uint64_t(g.yr) << (rs.ppy + s);
+            uint64_t ry = first_at(g.trect.y0, rs.rect.y0, s, stepy);
+            for (uint32_t py = 0; py < rs.npy; ++py) {
+                uint64_t rx = first_at(g.trect.x0, rs.rect.x0, s, stepx);
+                for (uint32_t px = 0; px < rs.npx; ++px) {
+                    v.push_back({ry, rx, static_cast<uint16_t>(c), static_cast<uint16_t>(r),
+                                 py * rs.npx + px});
+                    rx = (rx / stepx + 1) * stepx;
+                }
+                ry = (ry / stepy + 1) * stepy;
+            }
+        }
     }
+    auto key = [&](const Pos &p) {
+        // (major .. minor) per order; the precinct index breaks ties
+        if (order == 2) return std::make_tuple(uint64_t(p.r), p.y, p.x, uint64_t(p.c), p.idx);
+        if (order == 3) return std::make_tuple(p.y, p.x, uint64_t(p.c), uint64_t(p.r), p.idx);
+        return std::make_tuple(uint64_t(p.c), p.y, p.x, uint64_t(p.r), p.idx);
+    };
+    std::stable_sort(v.begin(), v.end(), [&](const Pos &a, const Pos &b) { return key(a) < key(b); });
+    for (const Pos &p : v)
+        for (uint32_t l = 0; l < layers; ++l)
+            out.push_back({static_cast<uint16_t>(l), p.r, p.c, p.idx});
     return kOk;
 }
 

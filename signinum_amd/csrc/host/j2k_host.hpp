@@ -58,6 +58,7 @@ struct CompParams {
     uint16_t nsteps = 0;
     uint16_t eps[3 * kMaxRes + 1] = {};
     uint16_t mu[3 * kMaxRes + 1] = {};
+    uint8_t xr = 1, yr = 1;  // sub-sampling on the reference grid (XRsiz, YRsiz)
 };
 
 struct MainHeader {
@@ -105,7 +106,9 @@ struct ResGeom {
 };
 
 struct TileCompGeom {
-    Rect rect;
+    Rect rect;         // the tile-component, in component coordinates (B-12)
+    Rect trect;        // the tile on the reference grid
+    uint8_t xr = 1, yr = 1;
     uint8_t nres = 0;
     ResGeom res[kMaxRes];
     std::vector<BandGeom> bands;
@@ -114,7 +117,8 @@ struct TileCompGeom {
 };
 
 // Builds the resolution / sub-band / precinct / code-block partition of one
-// tile-component.
+// tile-component; tile_rect is the tile on the reference grid, divided by
+// the component's sub-sampling (B-12).
 Status build_tile_comp(const CompParams &cp, const Rect &tile_rect, TileCompGeom &g);
 
 struct PacketId {

@@ -269,13 +269,14 @@ Status read_main_header(const uint8_t *data, size_t len, MainHeader &h) {
         return fail(kErrMalformed, "invalid SIZ");
     if (csiz > kMaxComps) return fail(kErrUnsupported, "too many components");
     h.ncomp = uint16_t(csiz);
-    uint8_t prec[kMaxComps];
+    uint8_t prec[kMaxComps], sub_x[kMaxComps], sub_y[kMaxComps];
     for (uint32_t i = 0; i < csiz; ++i) {
         uint32_t s, xr, yr;
         if (!b.u8(s) || !b.u8(xr) || !b.u8(yr)) return fail(kErrMalformed, "truncated SIZ");
         prec[i] = uint8_t((s & 0x7F) + 1);
         if (xr == 0 || yr == 0 || prec[i] > 32) return fail(kErrMalformed, "invalid component");
-        if (xr != 1 || yr != 1) return fail(kErrUnsupported, "sub-sampled components");
+        sub_x[i] = uint8_t(xr);
+        sub_y[i] = uint8_t(yr);
         // the signed flag (s & 0x80) changes nothing: the reference decodes
         // signed components exactly as unsigned ones (codestream.rs:647-649)
     }
@@ -365,6 +366,8 @@ Status read_main_header(const uint8_t *data, size_t len, MainHeader &h) {
         }
         apply_quant(c, qcc_set[i] ? qcc[i] : qcd);
         c.precision = prec[i];
+        c.xr = sub_x[i];
+        c.yr = sub_y[i];
         if (c.nsteps == 0) return fail(kErrMalformed, "missing step sizes");
         if (c.qstyle != 1 && c.levels > 0 && 1 + (c.levels - 1) * 3 + 2 >= c.nsteps)
             return fail(kErrMalformed, "insufficient step sizes");
@@ -374,10 +377,12 @@ Status read_main_header(const uint8_t *data, size_t len, MainHeader &h) {
 
 uint32_t skipped_levels(const MainHeader &h, uint32_t denominator) {
     if (denominator <= 1) return 0;
-    const uint32_t tw = ceil_div(h.xsiz - h.xo, denominator);
-    const uint32_t th = ceil_div(h.ysiz - h.yo, denominator);
+    const Sampling smp = sampling_of(h);
+    const uint32_t iw = ceil_div(h.xsiz - h.xo, smp.sx), ih = ceil_div(h.ysiz - h.yo, smp.sy);
+    const uint32_t tw = ceil_div(iw, denominator);
+    const uint32_t th = ceil_div(ih, denominator);
     // image_width / target (codestream.rs:149-156)
-    const uint32_t wr = (h.xsiz - h.xo) / tw, hr = (h.ysiz - h.yo) / th;
+    const uint32_t wr = iw / tw, hr = ih / th;
     const uint32_t wl = wr ? 31 - __builtin_clz(wr) : 0, hl = hr ? 31 - __builtin_clz(hr) : 0;
     uint32_t s = std::min(wl, hl);
     uint32_t min_nr = 255;

@@ -96,6 +96,14 @@ struct OutputGeom {
 };
 Status output_geometry(const MainHeader &h, int fmt, uint32_t denominator, OutputGeom &g);
 
+// Component sampling: any sub-sampled, mixed (not all alike), and the
+// output grid's factors (the common sub-sampling, else 1).
+struct Sampling {
+    bool any = false, mixed = false;
+    uint32_t sx = 1, sy = 1;
+};
+Sampling sampling_of(const MainHeader &h);
+
 // Plan records of one codestream with offsets local to it; merged into a
 // batch plan by rebasing (arena, coefficient and job offsets).
 struct LocalPlan {
@@ -118,6 +126,9 @@ struct LocalPlan {
     std::vector<ClassicJob> classic;
     std::vector<ClassicSeg> cseg;
     std::vector<uint8_t> joined;
+    // components on different sampling grids: per tile-component output
+    // jobs in place of the fused final stage
+    std::vector<StoreJob> stores;
 };
 
 // Region of interest in full-resolution image coordinates (Rect of

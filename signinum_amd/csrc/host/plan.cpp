@@ -27,6 +27,21 @@ static int fmt_channels(int fmt) {
     }
 }
 
+Sampling sampling_of(const MainHeader &h) {
+    Sampling s;
+    bool same = true;
+    for (int c = 0; c < h.ncomp; ++c) {
+        s.any |= h.comp[c].xr != 1 || h.comp[c].yr != 1;
+        same &= h.comp[c].xr == h.comp[0].xr && h.comp[c].yr == h.comp[0].yr;
+    }
+    s.mixed = s.any && !same;
+    // the output grid: the components' own when they share one
+    // (codestream.rs:663-683)
+    s.sx = same ? h.comp[0].xr : 1;
+    s.sy = same ? h.comp[0].yr : 1;
+    return s;
+}
+
 Status output_geometry(const MainHeader &h, int fmt, uint32_t denominator, OutputGeom &g) {
     const int ch = fmt_channels(fmt);
     if (!ch) {
@@ -42,6 +57,13 @@ Status output_geometry(const MainHeader &h, int fmt, uint32_t denominator, Outpu
         return kErrUnsupported;
     }
     const uint32_t sk = skipped_levels(h, denominator);
+    const Sampling smp = sampling_of(h);
+    // a reduced resolution of sub-sampled components is stored with
+    // full-resolution component offsets (decode.rs:1858-1898)
+    if (sk && smp.any) {
+        set_last_error("reduced resolution of sub-sampled components");
+        return kErrUnsupported;
+    }
     // The reference's store places reduced-resolution tiles by subtracting
     // the full-resolution image offset from reduced coordinates
     // (decode.rs:1762-1790): with a non-zero image origin only full
@@ -51,8 +73,8 @@ Status output_geometry(const MainHeader &h, int fmt, uint32_t denominator, Outpu
         return kErrUnsupported;
     }
     // image_width / image_height (codestream.rs:544-553)
-    g.width = ceil_div(h.xsiz - h.xo, 1u << sk);
-    g.height = ceil_div(h.ysiz - h.yo, 1u << sk);
+    g.width = ceil_div(h.xsiz - h.xo, smp.sx << sk);
+    g.height = ceil_div(h.ysiz - h.yo, smp.sy << sk);
     g.ncomp = h.ncomp;
     g.channels = uint32_t(ch);
     g.bytes_per_sample = (fmt == SGN_GRAY16 || fmt == SGN_RGB16) ? 2 : 1;
@@ -66,7 +88,8 @@ Status region_output(const MainHeader &h, const OutputGeom &og, uint32_t denomin
         out = {0, 0, og.width, og.height};
         return kOk;
     }
-    const uint64_t fw = h.xsiz - h.xo, fh = h.ysiz - h.yo;
+    const Sampling smp = sampling_of(h);
+    const uint64_t fw = ceil_div(h.xsiz - h.xo, smp.sx), fh = ceil_div(h.ysiz - h.yo, smp.sy);
     if (uint64_t(roi.x) + roi.w > fw || uint64_t(roi.y) + roi.h > fh) {
         set_last_error("region outside the image");
         return kErrRegion;
@@ -219,9 +242,25 @@ Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denomin
         return kErrBuffer;
     }
     if (roi.on && (reg.w() == 0 || reg.h() == 0)) return kOk;  // an empty region
-    // the region on the reference grid at the decoded resolution (a
+    const Sampling smp = sampling_of(h);
+    // the image origin on the output grid; with components sharing a
+    // sub-sampling it must fall on that grid (else the reference's store
+    // replicates an edge sample, decode.rs:1858-1898)
+    const uint32_t xoff = ceil_div(h.xo, smp.sx), yoff = ceil_div(h.yo, smp.sy);
+    // an image origin off a component's sampling grid leaves the first
+    // columns / rows without a sample of it (the reference writes zeros
+    // there, decode.rs:1858-1898): not decoded here
+    for (int c = 0; c < h.ncomp; ++c)
+        if (h.xo % h.comp[c].xr || h.yo % h.comp[c].yr) {
+            set_last_error("image origin off a component's sampling grid");
+            return kErrUnsupported;
+        }
+    // the region on the (component) grid at the decoded resolution (a
     // non-zero image origin only at full resolution)
-    const Rect regA{reg.x0 + h.xo, reg.y0 + h.yo, reg.x1 + h.xo, reg.y1 + h.yo};
+    const Rect regA{reg.x0 + xoff, reg.y0 + yoff, reg.x1 + xoff, reg.y1 + yoff};
+    // region decodes skip code-blocks and synthesis blocks, except with
+    // sub-sampled components (roi.rs:24-30: decoded whole, then cropped)
+    const bool prune = roi.on && !smp.any;
     const uint32_t sk = skipped_levels(h, denominator);
     std::vector<TileBody> tiles;
     st = parse_tiles(cs, len, h, tiles);
@@ -251,7 +290,7 @@ Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denomin
                 return kErrMalformed;
             }
         }
-        if (roi.on) {
+        if (prune) {
             // a tile outside the region is not decoded at all
             const Rect F = tcs[0].res[tcs[0].nres - sk - 1].rect;
             if (!meets(F, regA)) {
@@ -272,6 +311,10 @@ Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denomin
         st = parse_packets(cs, t, nc, tcs.data(), segs);
         if (st) return st;
         const bool mct = t.mct && nc >= 3;
+        if (mct && smp.mixed) {
+            set_last_error("colour transform over components on different sampling grids");
+            return kErrUnsupported;
+        }
         if (mct && (t.comp[0].transform != t.comp[1].transform ||
                     t.comp[1].transform != t.comp[2].transform)) {
             set_last_error("inverse MCT over components with different transforms");
@@ -317,7 +360,7 @@ Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denomin
                 const CbSeg &sg = segs[c][k];
                 const uint32_t K = band_k[cg.band];
                 if (!(cp.cbstyle & kCbsHt)) {
-                    if (roi.on) {
+                    if (prune) {
                         const Rect bn = bg.res == 0 ? need[c][0] : need_of(need[c][bg.res], bg.rect);
                         if (!meets(cg.rect, bn)) {
                             ++lp.pruned_cbs;
@@ -352,7 +395,7 @@ Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denomin
                     j.zblk = static_cast<uint8_t>(std::min<uint32_t>(sg.zblk, 255));
                     j.passes = static_cast<uint8_t>(sg.passes);
                 }
-                if (roi.on) {
+                if (prune) {
                     // a block none of the region's samples depends on is not
                     // decoded: the synthesis of the samples the region needs
                     // never reads its part of the band
@@ -371,6 +414,19 @@ Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denomin
             uint32_t prev_w = coef_pitch(g.bands[g.res[0].band[0]].rect.w());
             float prev_delta = band_delta[g.res[0].band[0]];
             bool prev_float = false;
+            // with components on different grids the last level goes to a
+            // plane of its own, read by the component's store job
+            uint32_t plane = 0;
+            auto to_plane = [&](LevelJob &lj, const Rect &rr, uint32_t depth) {
+                lj.nbx = static_cast<uint16_t>(ceil_div(rr.w(), 64));
+                lj.nby = static_cast<uint16_t>(ceil_div(rr.h(), 64));
+                lj.bx0 = lj.by0 = 0;
+                lj.out = static_cast<uint32_t>(lp.coef_elems);
+                plane = lj.out;
+                lp.coef_elems += uint64_t(coef_pitch(rr.w())) * rr.h();
+                lp.levels.push_back(lj);
+                lp.level_depth.push_back(static_cast<uint8_t>(depth));
+            };
             if (R == 1) {
                 LevelJob lj{};
                 lj.band[0] = prev_out;
@@ -384,8 +440,12 @@ Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denomin
                 lj.reversible = rev;
                 lj.levels = 0;
                 lj.narrow = 1;  // slot 0 is the code-block band itself
-                fj.level[c] = static_cast<uint32_t>(lp.finals_lv.size());
-                lp.finals_lv.push_back(lj);
+                if (smp.mixed) {
+                    to_plane(lj, rr, 1);
+                } else {
+                    fj.level[c] = static_cast<uint32_t>(lp.finals_lv.size());
+                    lp.finals_lv.push_back(lj);
+                }
             }
             for (uint32_t r = 1; r < R; ++r) {
                 const ResGeom &rs = g.res[r];
@@ -418,7 +478,7 @@ Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denomin
                 lj.levels = 1;
                 lj.nbx = static_cast<uint16_t>(ceil_div(rs.rect.w(), 64));
                 lj.nby = static_cast<uint16_t>(ceil_div(rs.rect.h(), 64));
-                if (roi.on) {
+                if (prune) {
                     // only the 64x64 blocks holding samples the region needs
                     const Rect &n = need[c][r];
                     const uint32_t bx0 = (n.x0 - rs.rect.x0) / 64, by0 = (n.y0 - rs.rect.y0) / 64;
@@ -438,6 +498,8 @@ Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denomin
                     prev_w = coef_pitch(rs.rect.w());
                     prev_delta = 1.0f;
                     prev_float = !rev;
+                } else if (smp.mixed) {
+                    to_plane(lj, rs.rect, r);
                 } else {
                     fj.level[c] = static_cast<uint32_t>(lp.finals_lv.size());
                     lp.finals_lv.push_back(lj);
@@ -450,6 +512,41 @@ Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denomin
                 return kErrUnsupported;
             }
             const Rect fr = g.res[R - 1].rect;
+            if (smp.mixed) {
+                // output pixels whose grid position falls on this tile's
+                // samples: [xr cx0, xr cx1) of the reference grid, inside
+                // the image and the region
+                StoreJob sj{};
+                sj.plane = plane;
+                sj.pitch = coef_pitch(fr.w());
+                sj.cx0 = fr.x0;
+                sj.cy0 = fr.y0;
+                const uint64_t gx0 = std::max<uint64_t>(uint64_t(g.xr) * fr.x0, h.xo);
+                const uint64_t gy0 = std::max<uint64_t>(uint64_t(g.yr) * fr.y0, h.yo);
+                const uint64_t gx1 = std::min<uint64_t>(uint64_t(g.xr) * fr.x1, h.xsiz);
+                const uint64_t gy1 = std::min<uint64_t>(uint64_t(g.yr) * fr.y1, h.ysiz);
+                sj.ox0 = std::max<uint32_t>(uint32_t(gx0 - h.xo), reg.x0);
+                sj.oy0 = std::max<uint32_t>(uint32_t(gy0 - h.yo), reg.y0);
+                sj.ox1 = gx1 > h.xo ? std::min<uint32_t>(uint32_t(gx1 - h.xo), reg.x1) : 0;
+                sj.oy1 = gy1 > h.yo ? std::min<uint32_t>(uint32_t(gy1 - h.yo), reg.y1) : 0;
+                sj.xo = h.xo;
+                sj.yo = h.yo;
+                sj.dst = dst - uint64_t(reg.y0) * pitch - uint64_t(reg.x0) * bpp;
+                sj.opitch = pitch;
+                sj.nbx = static_cast<uint16_t>(sj.ox1 > sj.ox0 ? ceil_div(sj.ox1 - sj.ox0, 64) : 0);
+                sj.nby = static_cast<uint16_t>(sj.oy1 > sj.oy0 ? ceil_div(sj.oy1 - sj.oy0, 64) : 0);
+                sj.xr = g.xr;
+                sj.yr = g.yr;
+                sj.chan = static_cast<uint8_t>(c);
+                sj.nchan = static_cast<uint8_t>(og.channels);
+                sj.bps = static_cast<uint8_t>(og.bytes_per_sample);
+                sj.precision = cp.precision;
+                sj.prec0 = t.comp[0].precision;
+                sj.flags = static_cast<uint8_t>((rev ? 1 : 0) | (same8 ? 2 : 0) |
+                                                (c == 0 && og.channels == 4 ? 4 : 0));
+                if (sj.nbx && sj.nby) lp.stores.push_back(sj);
+                continue;
+            }
             if (c == 0)
                 out_rect = fr;
             else if (fr.x0 != out_rect.x0 || fr.y0 != out_rect.y0 || fr.x1 != out_rect.x1 ||
@@ -459,12 +556,13 @@ Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denomin
             }
             fj.precision[c < 4 ? c : 3] = cp.precision;
         }
-        // image coordinates: the reference grid minus the image origin
+        if (smp.mixed) continue;  // the store jobs write this tile
+        // image coordinates: the (component) grid minus the image origin
         // (full resolution whenever the origin is not zero)
-        out_rect.x0 -= h.xo;
-        out_rect.x1 -= h.xo;
-        out_rect.y0 -= h.yo;
-        out_rect.y1 -= h.yo;
+        out_rect.x0 -= xoff;
+        out_rect.x1 -= xoff;
+        out_rect.y0 -= yoff;
+        out_rect.y1 -= yoff;
         if (out_rect.x1 > og.width || out_rect.y1 > og.height) {
             set_last_error("tile outside the output image");
             return kErrMalformed;

@@ -117,4 +117,24 @@ struct FinalJob {
     uint32_t cx0, cy0, cx1, cy1;
 };
 
+// Output of one tile-component of an image whose components are sampled
+// on different grids: every output pixel (image coordinates X, Y) of
+// [ox0, ox1) x [oy0, oy1) takes the component sample at ((X + xo) / xr,
+// (Y + yo) / yr) from the component's plane (the last synthesis level's
+// output), shifted, converted and written as channel `chan`
+// (decode.rs:1858-1898: each sample covers xr x yr pixels of the grid).
+struct StoreJob {
+    uint32_t plane;               // element offset of the plane (int32 5/3, f32 9/7)
+    uint32_t pitch;               // its row pitch in elements
+    uint32_t cx0, cy0;            // component coordinates of the plane's (0, 0)
+    uint32_t ox0, oy0, ox1, oy1;  // output pixels written
+    uint32_t xo, yo;              // image origin on the reference grid
+    uint64_t dst;                 // device address of output pixel (0, 0)
+    uint64_t opitch;              // output row bytes
+    uint32_t block_base;          // first work-group (64x64 output blocks)
+    uint16_t nbx, nby;
+    uint8_t xr, yr, chan, nchan;  // sub-sampling, channel written, channels per pixel
+    uint8_t bps, precision, prec0, flags;  // flags: 1 reversible, 2 all 8-bit, 4 alpha
+};
+
 }  // namespace sgn
