@@ -319,6 +319,9 @@ def main():
     ap.add_argument("--classic-oracle-sample", type=int, default=32)
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--serve-rates", default="150,600,2000,6000")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy a MIB MiB device buffer three times (the plain copy "
+                         "kernel whose FETCH_SIZE / WRITE_SIZE calibrate the HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build decodes on a single MI355X; "
@@ -328,6 +331,20 @@ def main():
 
     import signinum_amd as S
     from signinum_amd import synth
+
+    if args.copy_calibration:
+        sess = S.HipSession(0)
+        side = 1024
+        rows = args.copy_calibration * (1 << 20) // (side * 4)
+        a = S.DeviceSurface(sess, side, rows, S.PixelFormat.RGBA8)
+        b = S.DeviceSurface(sess, side, rows, S.PixelFormat.RGBA8)
+        a.fill(1)
+        for _ in range(3):
+            b.copy_from(a)
+        sess.synchronize()
+        log(f"copied {args.copy_calibration} MiB three times")
+        sess.close()
+        return
 
     t0 = time.time()
     streams, sources, cls = synth.lossless_batch_with_pixels(args.tiles, size=256)
