@@ -1,8 +1,10 @@
 // HT cleanup decode, one code-block per lane, with the compressed streams
 // staged through LDS.
 //
-// Each wave decodes 64 cleanup-only code-blocks (at most 64x64; the host
-// sorts blocks by predicted cost so the lanes of a wave carry similar work).
+// Each wave decodes 64 code-blocks (at most 64x64; the host sorts blocks by
+// predicted cost so the lanes of a wave carry similar work).  Of a block
+// with SigProp / MagRef passes this kernel decodes the cleanup; sgn_ht_refine
+// runs the passes after it over the samples the cleanup wrote.
 // The decode walks every block in segments of kSegPairs quad pairs of one
 // quad row.  Between segments the wave stops at a uniform point and refills,
 // for all 64 lanes at once, three per-lane LDS rings (MagSgn, VLC, MEL) with
@@ -21,6 +23,7 @@
 #include "../host/plan_types.h"
 #include "ht_common.hpp"
 #include "ht_readers4.hpp"
+#include "ht_refine.hpp"
 
 namespace sgn {
 
@@ -121,6 +124,7 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
     for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor(v, o));
     return v;
 }
+
 
 // Decodes the 64 code-blocks jobs[first .. first + 64) (lane l takes
 // first + l) with the calling wave.  WAVES waves share the work-group's
@@ -513,6 +517,29 @@ sgn_ht_rows(const CbJob *__restrict__ jobs, uint32_t n, const uint8_t *__restric
                                           s_q, s_ms, s_vlc, s_mel);
         }
     }
+}
+
+// SigProp / MagRef passes of the code-blocks jobs[0 .. n) (each with more
+// than one pass, at most 64x64), one block per lane, after the lane kernel
+// wrote their cleanup samples.  A block whose cleanup failed holds zeros or
+// partial samples; the passes keep to its rectangle and its refinement
+// segment whatever the samples, and its status already reports it.
+template <typename C>
+__global__ void __launch_bounds__(256)
+sgn_ht_refine(const CbJob *__restrict__ jobs, uint32_t n, const uint8_t *__restrict__ arena,
+              C *__restrict__ coef) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const CbJob j = jobs[i];
+    // pass count rules of decode_impl (ht_block_decode.rs:1578-1592); more
+    // than three passes never reach this kernel
+    if ((j.flags & kCbZero) || j.passes < 2 || j.passes > 3 || j.lref == 0 || j.zblk >= 29 ||
+        j.lcup < 2 || j.w > 64 || j.h > 64)
+        return;
+    const uint32_t shift = j.kmax >= 31 ? 0u : 31u - j.kmax;
+    rows_refine<C>(coef + j.dst, j.w, j.h, j.stride, ((j.dst | j.stride) & 3u) == 0, arena,
+                   j.src + j.lcup, j.lref, j.passes, (j.flags & kCbStripeCausal) != 0,
+                   30u - j.zblk, shift);
 }
 
 // Timing run for the decode-cost model: of each wave's 64 jobs only lane

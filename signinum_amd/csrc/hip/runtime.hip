@@ -92,13 +92,16 @@ namespace {
 // Lane-kernel variants the autotuner chooses from: work-group shape, quad
 // pairs per segment (and MagSgn ring slots), and one-shot grid versus
 // persistent waves pulling 64-block chunks from a device work counter.
+// refine: the list holds blocks with SigProp / MagRef passes (the lane
+// kernel handles them in any case; the cooperative one needs its refining
+// instance)
 using RowsLaunch = void (*)(hipStream_t, uint32_t, const CbJob *, uint32_t, const uint8_t *,
-                            void *, const uint16_t *, uint8_t *, uint32_t *);
+                            void *, const uint16_t *, uint8_t *, uint32_t *, bool);
 
 template <int W, int S, int M, int VS, int LS, bool PERSIST, typename C, int MINW, int CHUNK>
 void rows_launch(hipStream_t st, uint32_t grid, const CbJob *jobs, uint32_t n,
                  const uint8_t *arena, void *coef, const uint16_t *tables, uint8_t *status,
-                 uint32_t *queue) {
+                 uint32_t *queue, bool) {
     hipLaunchKernelGGL((sgn_ht_rows<W, S, M, VS, LS, PERSIST, C, MINW, CHUNK>), dim3(grid),
                        dim3(W * 64), 0, st, jobs, n, arena, static_cast<C *>(coef), tables, status,
                        queue);
@@ -106,18 +109,15 @@ void rows_launch(hipStream_t st, uint32_t grid, const CbJob *jobs, uint32_t n,
 
 // The cooperative kernel as a lane-list variant: one code-block per wave,
 // one wave per work-group (the grid is the block count).
-void coop_launch32(hipStream_t st, uint32_t, const CbJob *jobs, uint32_t n,
-                   const uint8_t *arena, void *coef, const uint16_t *tables, uint8_t *status,
-                   uint32_t *) {
-    hipLaunchKernelGGL((sgn_ht_coop<64, false, int32_t>), dim3(n), dim3(64), 0, st, jobs, n,
-                       arena, static_cast<int32_t *>(coef), tables, status, 0u);
-}
-
-void coop_launch16(hipStream_t st, uint32_t, const CbJob *jobs, uint32_t n,
-                   const uint8_t *arena, void *coef, const uint16_t *tables, uint8_t *status,
-                   uint32_t *) {
-    hipLaunchKernelGGL((sgn_ht_coop<64, false, int16_t>), dim3(n), dim3(64), 0, st, jobs, n,
-                       arena, static_cast<int16_t *>(coef), tables, status, 0u);
+template <typename C>
+void coop_launch(hipStream_t st, uint32_t, const CbJob *jobs, uint32_t n, const uint8_t *arena,
+                 void *coef, const uint16_t *tables, uint8_t *status, uint32_t *, bool refine) {
+    if (refine)
+        hipLaunchKernelGGL((sgn_ht_coop<64, true, C>), dim3(n), dim3(64), 0, st, jobs, n, arena,
+                           static_cast<C *>(coef), tables, status, 0u);
+    else
+        hipLaunchKernelGGL((sgn_ht_coop<64, false, C>), dim3(n), dim3(64), 0, st, jobs, n, arena,
+                           static_cast<C *>(coef), tables, status, 0u);
 }
 
 // launch[0] / kernel[0]: int32 samples; [1]: int16 samples (narrow plans)
@@ -153,7 +153,7 @@ const RowsVariant kRowsVariants[] = {
     {"wave-coop",
      1,
      false,
-     {coop_launch32, coop_launch16},
+     {coop_launch<int32_t>, coop_launch<int16_t>},
      {reinterpret_cast<const void *>(sgn_ht_coop<64, false, int32_t>),
       reinterpret_cast<const void *>(sgn_ht_coop<64, false, int16_t>)},
      true},
@@ -212,9 +212,13 @@ namespace {
 
 constexpr uint32_t kLaneMaxDim = 64;
 
+// Blocks of the lane-list kernels: at most 64x64, cleanup with or without
+// SigProp / MagRef passes.
 bool lane_eligible(const CbJob &j) {
-    return j.w <= kLaneMaxDim && j.h <= kLaneMaxDim && ((j.flags & kCbZero) || j.passes == 1);
+    return j.w <= kLaneMaxDim && j.h <= kLaneMaxDim && ((j.flags & kCbZero) || j.passes <= 3);
 }
+
+bool lane_refines(const CbJob &j) { return !(j.flags & kCbZero) && j.passes > 1; }
 
 struct LevelRange {
     uint8_t rev, depth;
@@ -305,6 +309,10 @@ struct sgn_j2k_plan {
     size_t ntiles = 0;
     // host tables
     std::vector<CbJob> cb_lane, cb_lane_plain, cb_wave;
+    // lane-list blocks with SigProp / MagRef passes, costliest first: the
+    // refinement kernel runs them after a lane-kernel variant (the
+    // cooperative kernel runs them itself)
+    std::vector<CbJob> cb_refine;
     std::vector<uint32_t> lane_plain_begin, wave_begin;  // per tile, size ntiles+1
     std::vector<LevelJob> lv[2];                          // [rev] sorted by (depth, tile)
     std::vector<uint32_t> lv_group[2];                    // start index of each depth, size D+1
@@ -337,10 +345,12 @@ struct sgn_j2k_plan {
     // magnitude bit-planes, so every sample fits); intermediate level
     // outputs stay int32.  Each band keeps its int32-sized slot.
     bool narrow = false;
+    bool lane_refined = false;  // a lane-list block has SigProp / MagRef passes
     // device
     uint8_t *d_arena = nullptr;
     uint32_t *d_coef = nullptr;
     CbJob *d_cb_lane = nullptr, *d_cb_lane_plain = nullptr, *d_cb_wave = nullptr;
+    CbJob *d_cb_refine = nullptr;
     LevelJob *d_lv[2] = {nullptr, nullptr};
     LevelJob *d_final_lv = nullptr;
     FinalJob *d_fin[2] = {nullptr, nullptr};
@@ -617,10 +627,12 @@ Status build_plan(const sgn_tile_req *reqs, size_t n, sgn_j2k_plan &P, int32_t *
             // plus the offset inside the band
             j.dst = P.narrow ? j.dst + lp.cb_band[k] + 2 * cbase : j.dst + cbase;
             j.tile = static_cast<uint32_t>(i);
-            if (lane_eligible(j))
+            if (lane_eligible(j)) {
                 P.cb_lane_plain.push_back(j);
-            else
+                if (lane_refines(j)) P.cb_refine.push_back(j);
+            } else {
                 P.cb_wave.push_back(j);
+            }
         }
         P.tile_store.push_back({static_cast<uint32_t>(P.store.size()),
                                 static_cast<uint32_t>(lp.stores.size())});
@@ -658,6 +670,10 @@ Status build_plan(const sgn_tile_req *reqs, size_t n, sgn_j2k_plan &P, int32_t *
     // code-blocks for the batched lane kernel, ordered by predicted cost so
     // that a wave's lanes finish together
     sort_by_cost(P.cb_lane_plain, P.cb_lane);
+    P.lane_refined = !P.cb_refine.empty();
+    std::stable_sort(P.cb_refine.begin(), P.cb_refine.end(), [](const CbJob &a, const CbJob &b) {
+        return uint32_t(a.lref) + a.lcup > uint32_t(b.lref) + b.lcup;
+    });
     // classic blocks: by kernel shape, costliest first within each
     {
         auto shape = [](const ClassicJob &j) { return classic_shape(j.w); };
@@ -800,7 +816,7 @@ Status upload_plan(sgn_hip_session *s, const sgn_tile_req *jobs, sgn_j2k_plan &P
     const uint64_t o_arena = take(P.arena_len);
     const uint64_t o_tables = off;
     const uint64_t o_lane = take(bytes_of(P.cb_lane)), o_plain = take(bytes_of(P.cb_lane_plain));
-    const uint64_t o_wave = take(bytes_of(P.cb_wave));
+    const uint64_t o_wave = take(bytes_of(P.cb_wave)), o_ref = take(bytes_of(P.cb_refine));
     const uint64_t o_lv0 = take(bytes_of(P.lv[0])), o_lv1 = take(bytes_of(P.lv[1]));
     const uint64_t o_fin0 = take(bytes_of(P.fin[0])), o_fin1 = take(bytes_of(P.fin[1]));
     const uint64_t o_flv = take(bytes_of(P.final_lv));
@@ -842,6 +858,7 @@ Status upload_plan(sgn_hip_session *s, const sgn_tile_req *jobs, sgn_j2k_plan &P
         HIP_TRY(put_dev(o_lane, P.cb_lane));
         HIP_TRY(put_dev(o_plain, P.cb_lane_plain));
         HIP_TRY(put_dev(o_wave, P.cb_wave));
+        HIP_TRY(put_dev(o_ref, P.cb_refine));
         HIP_TRY(put_dev(o_lv0, P.lv[0]));
         HIP_TRY(put_dev(o_lv1, P.lv[1]));
         HIP_TRY(put_dev(o_fin0, P.fin[0]));
@@ -900,6 +917,7 @@ Status upload_plan(sgn_hip_session *s, const sgn_tile_req *jobs, sgn_j2k_plan &P
         put(o_lane, P.cb_lane);
         put(o_plain, P.cb_lane_plain);
         put(o_wave, P.cb_wave);
+        put(o_ref, P.cb_refine);
         put(o_lv0, P.lv[0]);
         put(o_lv1, P.lv[1]);
         put(o_fin0, P.fin[0]);
@@ -961,6 +979,7 @@ Status upload_plan(sgn_hip_session *s, const sgn_tile_req *jobs, sgn_j2k_plan &P
     at(o_lane, P.d_cb_lane, !P.cb_lane.empty());
     at(o_plain, P.d_cb_lane_plain, !P.cb_lane_plain.empty());
     at(o_wave, P.d_cb_wave, !P.cb_wave.empty());
+    at(o_ref, P.d_cb_refine, !P.cb_refine.empty());
     at(o_lv0, P.d_lv[0], !P.lv[0].empty());
     at(o_lv1, P.d_lv[1], !P.lv[1].empty());
     at(o_fin0, P.d_fin[0], !P.fin[0].empty());
@@ -1051,7 +1070,20 @@ uint32_t launch_lane_variant(sgn_hip_session *s, sgn_j2k_plan &P, int v, const C
         grid = std::min(grid, resident_groups(s, v, P.narrow));
     }
     rv.launch[P.narrow ? 1 : 0](s->stream, grid, d_jobs, count, P.d_arena, P.d_coef, s->d_tables,
-                                status, P.d_queue);
+                                status, P.d_queue, P.lane_refined);
+    if (!rv.coop && d_jobs == P.d_cb_lane && !P.cb_refine.empty()) {
+        // refinement passes of the batch's lane blocks, after their cleanup
+        const uint32_t nr = static_cast<uint32_t>(P.cb_refine.size());
+        if (P.narrow)
+            hipLaunchKernelGGL((sgn_ht_refine<int16_t>), dim3((nr + 255) / 256), dim3(256), 0,
+                               s->stream, P.d_cb_refine, nr, P.d_arena,
+                               reinterpret_cast<int16_t *>(P.d_coef));
+        else
+            hipLaunchKernelGGL((sgn_ht_refine<int32_t>), dim3((nr + 255) / 256), dim3(256), 0,
+                               s->stream, P.d_cb_refine, nr, P.d_arena,
+                               reinterpret_cast<int32_t *>(P.d_coef));
+        ++k;
+    }
     return k;
 }
 
@@ -2027,6 +2059,7 @@ static int hook_blocks(sgn_hip_session *s, const uint8_t *data, size_t data_len,
     if (!s || (!data && data_len) || (!jobs && n) || (!out && out_len)) return SGN_E_ARG;
     HIP_TRY(hipSetDevice(s->device));
     std::vector<CbJob> lane, wave;
+    bool refined = false;
     for (size_t i = 0; i < n; ++i) {
         const sgn_ht_code_block_job &q = jobs[i];
         CbJob j{};
@@ -2054,10 +2087,12 @@ static int hook_blocks(sgn_hip_session *s, const uint8_t *data, size_t data_len,
         if (q.num_bitplanes == 0 || q.number_of_coding_passes == 0 || actual == 0)
             j.flags |= kCbZero;
         j.tile = static_cast<uint32_t>(i);
-        if (!raw && lane_eligible(j))
+        if (!raw && lane_eligible(j)) {
             lane.push_back(j);
-        else
+            refined |= lane_refines(j);
+        } else {
             wave.push_back(j);
+        }
     }
     uint8_t *d_data = nullptr, *d_st = nullptr;
     uint32_t *d_out = nullptr;
@@ -2087,7 +2122,7 @@ static int hook_blocks(sgn_hip_session *s, const uint8_t *data, size_t data_len,
     if (!lane.empty())
         kRowsVariants[kCoopVariant].launch[0](s->stream, uint32_t(lane.size()), d_lane,
                                               uint32_t(lane.size()), d_data, d_out, s->d_tables,
-                                              d_st, nullptr);
+                                              d_st, nullptr, refined);
     if (!wave.empty())
         hipLaunchKernelGGL((sgn_ht_coop<1024, true, int32_t>), dim3(wave.size()), dim3(64), 0,
                            s->stream, d_wave, uint32_t(wave.size()), d_data,

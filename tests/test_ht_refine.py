@@ -64,3 +64,49 @@ def test_refined_irreversible_close_to_openjpeg(oracle):
     ours = O.decode(cs, 1).astype(int)
     assert np.abs(ours - img).max() <= 6
     assert np.abs(ours - _pil(cs).astype(int)).max() <= 6
+
+
+def test_lane_refinement_passes_match_oracle(tmp_path):
+    """The device's per-lane SigProp / MagRef passes (ht_refine.hpp, compiled
+    for the host by refine_check.cpp) on the cleanup samples of every block
+    with refinement passes, against the oracle's full block decode."""
+    import os
+    import shutil
+    import struct
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("needs g++")
+    here = os.path.dirname(os.path.abspath(__file__))
+    root = os.path.dirname(here)
+    exe = tmp_path / "refine_check"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-w", "-I", os.path.join(root, "oracle"),
+                    "-o", str(exe), os.path.join(here, "refine_check.cpp")], check=True)
+    cases = [(synth.tissue_rgb, dict(ht_refine=1), 160, 120),
+             (synth.texture_rgb, dict(ht_refine=2), 133, 77),
+             (synth.tissue_rgb, dict(ht_refine=2, reversible=0, base_step=1 / 256), 150, 100),
+             (synth.texture_rgb, dict(ht_refine=1, cb_width_log2=5, cb_height_log2=6), 140, 90),
+             (synth.tissue_rgb, dict(ht_refine=2, cb_width_log2=6, cb_height_log2=4), 120, 100)]
+    total = 0
+    for k, (gen, kw, w, h) in enumerate(cases):
+        img = gen(np.random.default_rng(90 + k), w, h)
+        streams = [S.encode_j2k_ht(img, **kw)]
+        if k == 0:
+            g = synth.spots_gray12(np.random.default_rng(99), 100, 70).astype(np.uint16) * 16
+            streams.append(S.encode_j2k_ht(g, bit_depth=16, use_mct=0, ht_refine=1))
+        for cs in streams:
+            jobs = [j for j in S.code_block_jobs(cs) if j.number_of_coding_passes > 1]
+            assert jobs
+            path = tmp_path / f"blocks{k}.bin"
+            with open(path, "wb") as f:
+                f.write(struct.pack("<QQ", len(cs), len(jobs)))
+                f.write(cs)
+                for j in jobs:
+                    f.write(struct.pack("<Q8I", j.data_offset, j.cleanup_length,
+                                        j.refinement_length, j.width, j.height,
+                                        j.missing_bit_planes, j.number_of_coding_passes,
+                                        j.num_bitplanes, j.stripe_causal))
+            out = subprocess.run([str(exe), str(path)], capture_output=True, text=True,
+                                 timeout=120)
+            assert out.returncode == 0 and out.stdout.startswith("ok"), out.stdout + out.stderr
+            total += len(jobs)
+    assert total > 100
