@@ -78,7 +78,7 @@ struct sgn_hip_session {
     // per lane-kernel variant and sample type (2 v + narrow): resident
     // work-groups (0: unknown)
     std::vector<uint32_t> resident;
-    uint32_t classic_resident[4] = {0, 0, 0, 0};  // resident work-groups per classic kernel shape
+    uint32_t classic_resident[4][3] = {};  // resident work-groups per classic kernel (shape, lanes)
     // pipelined batch decode: host-to-device copies run on copy_stream
     // while the previous chunk decodes on `stream`; two staging slots
     hipStream_t copy_stream = nullptr;
@@ -187,19 +187,29 @@ const DwtShape kDwtShapes[] = {
 };
 constexpr int kDwtShapeCount = int(sizeof(kDwtShapes) / sizeof(kDwtShapes[0]));
 
-// Classic code-block kernels by block width (32, 64, 256, 1024).
+// Classic code-block kernels by block width (32, 64, 256, 1024) and active
+// lanes per wave (64, 32, 16; the 1024-wide shape always takes 16).
 using ClassicKernel = void (*)(const ClassicJob *, uint32_t, const uint8_t *, const ClassicSeg *,
                                int32_t *, uint32_t *);
-const ClassicKernel kClassicKernels[4] = {sgn_t1_classic<ClassicW32>, sgn_t1_classic<ClassicW64>,
-                                          sgn_t1_classic<ClassicW256>,
-                                          sgn_t1_classic<ClassicW1024>};
+constexpr int kClassicLaneOpts = 3;
+constexpr uint32_t kClassicLaneCount[kClassicLaneOpts] = {64, 32, 16};
+const ClassicKernel kClassicKernels[4][kClassicLaneOpts] = {
+    {sgn_t1_classic<ClassicW32<64>>, sgn_t1_classic<ClassicW32<32>>,
+     sgn_t1_classic<ClassicW32<16>>},
+    {sgn_t1_classic<ClassicW64<64>>, sgn_t1_classic<ClassicW64<32>>,
+     sgn_t1_classic<ClassicW64<16>>},
+    {sgn_t1_classic<ClassicW256<64>>, sgn_t1_classic<ClassicW256<32>>,
+     sgn_t1_classic<ClassicW256<16>>},
+    {sgn_t1_classic<ClassicW1024>, sgn_t1_classic<ClassicW1024>, sgn_t1_classic<ClassicW1024>}};
 int classic_shape(uint32_t w) {
-    return w <= uint32_t(ClassicW32::kMaxW) ? 0
-           : w <= uint32_t(ClassicW64::kMaxW) ? 1
-           : w <= uint32_t(ClassicW256::kMaxW) ? 2
-                                                : 3;
+    return w <= uint32_t(ClassicW32<64>::kMaxW) ? 0
+           : w <= uint32_t(ClassicW64<64>::kMaxW) ? 1
+           : w <= uint32_t(ClassicW256<64>::kMaxW) ? 2
+                                                    : 3;
 }
-uint32_t classic_lanes(int v) { return v == 3 ? uint32_t(ClassicW1024::kLanes) : 64u; }
+uint32_t classic_lanes(int v, int opt) {
+    return v == 3 ? uint32_t(ClassicW1024::kLanes) : kClassicLaneCount[opt];
+}
 // Lane-list size up to which plans start with the cooperative kernel.  On
 // 256x256 RGB tiles (75 code-blocks each) it took 0.28 against 1.41 ms for
 // 16 tiles and 1.21 against 1.61 ms for 256 tiles (19,200 blocks); the lane
@@ -364,6 +374,7 @@ struct sgn_j2k_plan {
     std::vector<ClassicJob> classic;
     uint32_t classic_begin[5] = {0, 0, 0, 0, 0};
     uint32_t classic_grid[4] = {0, 0, 0, 0};  // work-groups launched per shape
+    int classic_opt[4] = {0, 0, 0, 0};        // lane option per shape (kClassicLaneCount)
     uint32_t *d_crows = nullptr;              // the lanes' row stores
     std::vector<ClassicSeg> cseg;
     std::vector<uint8_t> joined;
@@ -502,6 +513,8 @@ Status build_plan(const sgn_tile_req *reqs, size_t n, sgn_j2k_plan &P, int32_t *
     // payload); only the codestream travels to the device
     std::vector<sgn_tile_req> cs(reqs, reqs + n);
     const sgn_tile_req *jobs = cs.data();
+    // per tile: code-blocks for the lane list, the wave list and refinement
+    std::vector<uint32_t> n_lane(n, 0), n_wave(n, 0), n_ref(n, 0);
     WorkerPool::get().run(n, [&](size_t i) {
         Container c;
         st[i] = locate_codestream(reqs[i].bytes, reqs[i].len, c);
@@ -519,7 +532,18 @@ Status build_plan(const sgn_tile_req *reqs, size_t n, sgn_j2k_plan &P, int32_t *
                                      reinterpret_cast<uint64_t>(jobs[i].dst), jobs[i].pitch,
                                      jobs[i].capacity, lps[i], roi);
         }
-        if (st[i]) errs[i] = last_error();
+        if (st[i]) {
+            errs[i] = last_error();
+            return;
+        }
+        for (const CbJob &j : lps[i].cbs) {
+            if (lane_eligible(j)) {
+                ++n_lane[i];
+                n_ref[i] += lane_refines(j) ? 1u : 0u;
+            } else {
+                ++n_wave[i];
+            }
+        }
     });
     int32_t first_bad = 0;
     size_t bad_index = 0;
@@ -603,8 +627,38 @@ Status build_plan(const sgn_tile_req *reqs, size_t n, sgn_j2k_plan &P, int32_t *
         // classic blocks OR magnitude bits into int32 samples
         P.narrow = allow && !classic && kmax <= 15 && 2 * coef < (1ull << 32);
     }
-    P.lane_plain_begin.assign(1, 0);
-    P.wave_begin.assign(1, 0);
+    // code-block jobs: each tile's slice of the three lists from prefix sums
+    // of its counts, then every tile's jobs rebased into place in parallel
+    P.lane_plain_begin.assign(n + 1, 0);
+    P.wave_begin.assign(n + 1, 0);
+    std::vector<uint32_t> ref_begin(n + 1, 0);
+    for (size_t i = 0; i < n; ++i) {
+        P.lane_plain_begin[i + 1] = P.lane_plain_begin[i] + n_lane[i];
+        P.wave_begin[i + 1] = P.wave_begin[i] + n_wave[i];
+        ref_begin[i + 1] = ref_begin[i] + n_ref[i];
+    }
+    P.cb_lane_plain.resize(P.lane_plain_begin[n]);
+    P.cb_wave.resize(P.wave_begin[n]);
+    P.cb_refine.resize(ref_begin[n]);
+    WorkerPool::get().run(n, [&](size_t i) {
+        const LocalPlan &lp = lps[i];
+        const uint32_t cbase = static_cast<uint32_t>(coef_base[i]);
+        uint32_t a = P.lane_plain_begin[i], w = P.wave_begin[i], r = ref_begin[i];
+        for (size_t k = 0; k < lp.cbs.size(); ++k) {
+            CbJob j = lp.cbs[k];
+            j.src += static_cast<uint32_t>(arena_base[i]);
+            // int16 index of a narrow sample: twice its band's int32 offset
+            // plus the offset inside the band
+            j.dst = P.narrow ? j.dst + lp.cb_band[k] + 2 * cbase : j.dst + cbase;
+            j.tile = static_cast<uint32_t>(i);
+            if (lane_eligible(j)) {
+                P.cb_lane_plain[a++] = j;
+                if (lane_refines(j)) P.cb_refine[r++] = j;
+            } else {
+                P.cb_wave[w++] = j;
+            }
+        }
+    });
     P.tile_levels.assign(n, {});
     P.tile_final_first.assign(n, {0, 0});
     P.tile_final_count.assign(n, 0);
@@ -620,20 +674,6 @@ Status build_plan(const sgn_tile_req *reqs, size_t n, sgn_j2k_plan &P, int32_t *
         P.out_pixels += uint64_t(lp.width) * lp.height;
         P.pruned_cbs += lp.pruned_cbs;
         const uint32_t cbase = static_cast<uint32_t>(coef_base[i]);
-        for (size_t k = 0; k < lp.cbs.size(); ++k) {
-            CbJob j = lp.cbs[k];
-            j.src += static_cast<uint32_t>(arena_base[i]);
-            // int16 index of a narrow sample: twice its band's int32 offset
-            // plus the offset inside the band
-            j.dst = P.narrow ? j.dst + lp.cb_band[k] + 2 * cbase : j.dst + cbase;
-            j.tile = static_cast<uint32_t>(i);
-            if (lane_eligible(j)) {
-                P.cb_lane_plain.push_back(j);
-                if (lane_refines(j)) P.cb_refine.push_back(j);
-            } else {
-                P.cb_wave.push_back(j);
-            }
-        }
         P.tile_store.push_back({static_cast<uint32_t>(P.store.size()),
                                 static_cast<uint32_t>(lp.stores.size())});
         for (StoreJob sj : lp.stores) {
@@ -652,8 +692,6 @@ Status build_plan(const sgn_tile_req *reqs, size_t n, sgn_j2k_plan &P, int32_t *
             j.tile = static_cast<uint32_t>(i);
             P.classic.push_back(j);
         }
-        P.lane_plain_begin.push_back(static_cast<uint32_t>(P.cb_lane_plain.size()));
-        P.wave_begin.push_back(static_cast<uint32_t>(P.cb_wave.size()));
         auto rebase = [&](LevelJob &l) {
             if (!P.narrow) l.narrow = 0;
             for (int b = 0; b < 4; ++b)
@@ -799,7 +837,8 @@ Status slot_reserve(sgn_hip_session *s, int k, size_t bytes, uint8_t **out) {
 // allocated: the plan is carved from it, every run and table is copied
 // straight from host memory (no pinned staging), and the call returns after
 // the copies completed.
-uint32_t classic_resident(sgn_hip_session *s, int v);
+uint32_t classic_resident(sgn_hip_session *s, int v, int opt);
+uint32_t session_cus(sgn_hip_session *s);
 
 Status upload_plan(sgn_hip_session *s, const sgn_tile_req *jobs, sgn_j2k_plan &P,
                    hipStream_t copy, int slot, bool sync, void *fixed = nullptr,
@@ -828,10 +867,25 @@ Status upload_plan(sgn_hip_session *s, const sgn_tile_req *jobs, sgn_j2k_plan &P
     const uint64_t o_stl = take(P.cb_lane.size()), o_stw = take(P.cb_wave.size());
     const uint64_t o_ff = take(P.fin_block_job[1].size() * sizeof(uint32_t));
     uint64_t crows_bytes = 0;
+    // active lanes per classic wave: the most that still give every SIMD two
+    // waves, else the fewest (SGN_CLASSIC_LANES=64, 32 or 16 forces a count)
+    const char *lanes_env = std::getenv("SGN_CLASSIC_LANES");
+    const uint32_t forced = lanes_env ? uint32_t(std::atoi(lanes_env)) : 0u;
     for (int v = 0; v < 4; ++v) {
         const uint32_t n = P.classic_begin[v + 1] - P.classic_begin[v];
-        const uint32_t lanes = classic_lanes(v);
-        P.classic_grid[v] = n ? std::min((n + lanes - 1) / lanes, classic_resident(s, v)) : 0;
+        int opt = kClassicLaneOpts - 1;
+        for (int o = 0; o < kClassicLaneOpts; ++o) {
+            const bool fits = forced ? kClassicLaneCount[o] == forced
+                                     : uint64_t(n) >= uint64_t(kClassicLaneCount[o]) * 8u *
+                                                          uint64_t(session_cus(s));
+            if (fits) {
+                opt = o;
+                break;
+            }
+        }
+        P.classic_opt[v] = opt;
+        const uint32_t lanes = classic_lanes(v, opt);
+        P.classic_grid[v] = n ? std::min((n + lanes - 1) / lanes, classic_resident(s, v, opt)) : 0;
         crows_bytes = std::max<uint64_t>(crows_bytes, uint64_t(P.classic_grid[v]) * lanes *
                                                           kClassicStateDwords * sizeof(uint32_t));
     }
@@ -1033,20 +1087,25 @@ uint32_t resident_groups(sgn_hip_session *s, int v, bool narrow) {
     return s->resident[slot];
 }
 
-uint32_t classic_resident(sgn_hip_session *s, int v) {
-    if (!s->classic_resident[v]) {
+uint32_t session_cus(sgn_hip_session *s) {
+    if (!s->cus && (hipDeviceGetAttribute(&s->cus, hipDeviceAttributeMultiprocessorCount,
+                                          s->device) != hipSuccess ||
+                    s->cus < 1))
+        s->cus = 1;
+    return uint32_t(s->cus);
+}
+
+uint32_t classic_resident(sgn_hip_session *s, int v, int opt) {
+    if (!s->classic_resident[v][opt]) {
         int per_cu = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &per_cu, reinterpret_cast<const void *>(kClassicKernels[v]), 64, 0) != hipSuccess ||
+                &per_cu, reinterpret_cast<const void *>(kClassicKernels[v][opt]), 64, 0) !=
+                hipSuccess ||
             per_cu < 1)
             per_cu = 1;
-        if (!s->cus && (hipDeviceGetAttribute(&s->cus, hipDeviceAttributeMultiprocessorCount,
-                                              s->device) != hipSuccess ||
-                        s->cus < 1))
-            s->cus = 1;
-        s->classic_resident[v] = uint32_t(per_cu) * uint32_t(s->cus);
+        s->classic_resident[v][opt] = uint32_t(per_cu) * session_cus(s);
     }
-    return s->classic_resident[v];
+    return s->classic_resident[v][opt];
 }
 
 // Clears the plan's device counters (word 0: lane work counter, word 1:
@@ -1117,7 +1176,8 @@ uint32_t launch_classic(sgn_hip_session *s, sgn_j2k_plan &P) {
         const uint32_t n = b - a;
         const ClassicJob *jobs = P.d_classic + a;
         int32_t *coef = reinterpret_cast<int32_t *>(P.d_coef);
-        hipLaunchKernelGGL(kClassicKernels[v], dim3(P.classic_grid[v]), dim3(64), 0, s->stream,
+        hipLaunchKernelGGL(kClassicKernels[v][P.classic_opt[v]], dim3(P.classic_grid[v]), dim3(64),
+                           0, s->stream,
                            jobs, n, P.d_arena, P.d_cseg, coef, P.d_crows);
         ++k;
     }

@@ -55,12 +55,21 @@ __host__ __device__ inline uint32_t classic_state_dwords(uint32_t w, uint32_t h)
 // (4 x 1024: 3 x 256 dwords).
 constexpr uint32_t kClassicStateDwords = 768;
 
-// Kernel shapes by block width: LDS ring of three rows per lane, active
-// lanes per 64-thread work-group.
-struct ClassicW32 { static constexpr int kRowDw = 17, kLanes = 64, kMaxW = 32; };
-struct ClassicW64 { static constexpr int kRowDw = 33, kLanes = 64, kMaxW = 64; };
-struct ClassicW256 { static constexpr int kRowDw = 129, kLanes = 64, kMaxW = 256; };
-struct ClassicW1024 { static constexpr int kRowDw = 513, kLanes = 16, kMaxW = 1024; };
+// Kernel shapes: dwords of one LDS ring row (blocks up to kMaxW wide) and
+// the active lanes of the 64-thread work-group.  64 lanes fill a wave; with
+// fewer, a batch with few blocks spreads them over more waves (every SIMD
+// gets some, and a SIMD's waves hide each other's LDS and memory waits).
+template <int ROWDW, int LANES, int MAXW>
+struct ClassicShape {
+    static constexpr int kRowDw = ROWDW, kLanes = LANES, kMaxW = MAXW;
+};
+template <int LANES>
+using ClassicW32 = ClassicShape<17, LANES, 32>;
+template <int LANES>
+using ClassicW64 = ClassicShape<33, LANES, 64>;
+template <int LANES>
+using ClassicW256 = ClassicShape<129, LANES, 256>;
+using ClassicW1024 = ClassicShape<513, 16, 1024>;
 
 // Reads of one codeword segment: absolute arena positions [p, end), bytes
 // past `end` read as 0xFF (the MQ decoder's fill, arithmetic_decoder.rs:

@@ -92,3 +92,27 @@ def test_classic_styles_on_device(session, oracle):
         d = np.abs(plan.surfaces[k].to_numpy().astype(np.int64) - ref).max()
         assert d <= (0 if rev else 1), (STYLES[k], d)  # 9/7: 1 LSB of 8-bit
     plan.close()
+
+
+@pytest.mark.parametrize("lanes", ["64", "32", "16"])
+def test_classic_lanes_per_wave(session, oracle, monkeypatch, lanes):
+    """Every active-lane count of the classic kernels (64, 32 or 16 blocks
+    per wave) on blocks of every width class, against the oracle."""
+    monkeypatch.setenv("SGN_CLASSIC_LANES", lanes)
+    rng = np.random.default_rng(36)
+    streams, want = [], []
+    for k, kw in enumerate([dict(), dict(codeblock_size=(32, 32)), dict(codeblock_size=(256, 16)),
+                            dict(codeblock_size=(1024, 4), num_resolutions=2)]):
+        img = synth.tissue_rgb(rng, 140 + 8 * k, 100)
+        cs = pil_encode(img, **kw)
+        streams.append(cs)
+        want.append(img)
+    for _ in range(3):
+        streams += streams[:4]
+        want += want[:4]
+    plan = S.TilePlan(session, streams, S.PixelFormat.RGB8)
+    plan.decode(S.MODE_GRAPH)
+    session.synchronize()
+    for k, s in enumerate(plan.surfaces):
+        assert np.array_equal(s.to_numpy(), want[k]), (lanes, k)
+    plan.close()
