@@ -496,11 +496,29 @@ void sort_by_cost(const std::vector<CbJob> &in, std::vector<CbJob> &out) {
                          [](const CbJob &a, const CbJob &b) { return a.cost > b.cost; });
         return;
     }
-    std::vector<uint32_t> at(size_t(top) + 2, 0);
-    for (const CbJob &j : in) at[top - j.cost + 1]++;
-    for (size_t k = 1; k < at.size(); ++k) at[k] += at[k - 1];
-    out.resize(in.size());
-    for (const CbJob &j : in) out[at[top - j.cost]++] = j;
+    // counting and scattering over slices of the input on the worker pool;
+    // bucket k (cost top - k) of slice t starts after every bucket below k
+    // and after slice t - 1's part of bucket k, which keeps the order stable
+    const size_t n = in.size(), nb = size_t(top) + 1;
+    size_t slices = std::min<size_t>(16, std::max<size_t>(1, n / 4096));
+    slices = std::max<size_t>(1, std::min(slices, (size_t(4) << 20) / nb));
+    const size_t per = (n + slices - 1) / slices;
+    std::vector<uint32_t> at(nb * slices, 0);  // [bucket][slice]
+    WorkerPool::get().run(slices, [&](size_t t) {
+        const size_t a = t * per, b = std::min(n, a + per);
+        for (size_t i = a; i < b; ++i) at[(top - in[i].cost) * slices + t]++;
+    });
+    uint32_t run = 0;
+    for (uint32_t &c : at) {
+        const uint32_t v = c;
+        c = run;
+        run += v;
+    }
+    out.resize(n);
+    WorkerPool::get().run(slices, [&](size_t t) {
+        const size_t a = t * per, b = std::min(n, a + per);
+        for (size_t i = a; i < b; ++i) out[at[(top - in[i].cost) * slices + t]++] = in[i];
+    });
 }
 
 Status build_plan(const sgn_tile_req *reqs, size_t n, sgn_j2k_plan &P, int32_t *tile_status) {
@@ -731,26 +749,38 @@ Status build_plan(const sgn_tile_req *reqs, size_t n, sgn_j2k_plan &P, int32_t *
     // lane kernel cannot fill the device (each wave takes 64 blocks) and its
     // time is one block's serial decode, so one block per wave finishes first
     P.lane_variant = P.cb_lane.size() <= kCoopMaxBlocks ? kCoopVariant : kDefaultLaneVariant;
-    // synthesis levels grouped by (transform, depth), tiles in order
+    // synthesis levels grouped by (transform, depth), tiles in order: the
+    // levels of each depth counted, then placed tile by tile
+    const uint32_t D = P.depth_max;
     for (int rev = 0; rev < 2; ++rev) {
-        P.lv_group[rev].assign(1, 0);
-        uint32_t blocks = 0;
-        for (uint32_t d = 1; d <= P.depth_max; ++d) {
-            for (size_t i = 0; i < n; ++i) {
-                const LocalPlan &lp = lps[i];
-                LevelRange rg{static_cast<uint8_t>(rev), static_cast<uint8_t>(d),
-                              static_cast<uint32_t>(P.lv[rev].size()), 0};
-                for (size_t k = 0; k < lp.levels.size(); ++k) {
-                    if (lp.level_depth[k] != d || lp.levels[k].reversible != rev) continue;
-                    LevelJob l = lp.levels[k];
-                    l.block_base = blocks;
-                    blocks += uint32_t(l.nbx) * l.nby;
-                    P.lv[rev].push_back(l);
-                    rg.count++;
-                }
-                if (rg.count) P.tile_levels[i].push_back(rg);
+        std::vector<uint32_t> start(D + 2, 0);  // start[d]: first index of depth d
+        for (const LocalPlan &lp : lps)
+            for (size_t k = 0; k < lp.levels.size(); ++k)
+                if (lp.levels[k].reversible == rev) start[lp.level_depth[k] + 1]++;
+        for (uint32_t d = 1; d <= D + 1; ++d) start[d] += start[d - 1];
+        P.lv_group[rev].assign(start.begin() + 1, start.end());
+        P.lv[rev].resize(start[D + 1]);
+        std::vector<uint32_t> pos(start), first(D + 1, 0), count(D + 1, 0);
+        for (size_t i = 0; i < n; ++i) {
+            const LocalPlan &lp = lps[i];
+            for (size_t k = 0; k < lp.levels.size(); ++k) {
+                if (lp.levels[k].reversible != rev) continue;
+                const uint32_t d = lp.level_depth[k];
+                if (!count[d]) first[d] = pos[d];
+                P.lv[rev][pos[d]++] = lp.levels[k];
+                count[d]++;
             }
-            P.lv_group[rev].push_back(static_cast<uint32_t>(P.lv[rev].size()));
+            for (uint32_t d = 1; d <= D; ++d) {
+                if (!count[d]) continue;
+                P.tile_levels[i].push_back(
+                    {static_cast<uint8_t>(rev), static_cast<uint8_t>(d), first[d], count[d]});
+                count[d] = 0;
+            }
+        }
+        uint32_t blocks = 0;  // work-groups in launch order
+        for (LevelJob &l : P.lv[rev]) {
+            l.block_base = blocks;
+            blocks += uint32_t(l.nbx) * l.nby;
         }
     }
     // finals
