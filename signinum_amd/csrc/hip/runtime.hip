@@ -496,29 +496,11 @@ void sort_by_cost(const std::vector<CbJob> &in, std::vector<CbJob> &out) {
                          [](const CbJob &a, const CbJob &b) { return a.cost > b.cost; });
         return;
     }
-    // counting and scattering over slices of the input on the worker pool;
-    // bucket k (cost top - k) of slice t starts after every bucket below k
-    // and after slice t - 1's part of bucket k, which keeps the order stable
-    const size_t n = in.size(), nb = size_t(top) + 1;
-    size_t slices = std::min<size_t>(16, std::max<size_t>(1, n / 4096));
-    slices = std::max<size_t>(1, std::min(slices, (size_t(4) << 20) / nb));
-    const size_t per = (n + slices - 1) / slices;
-    std::vector<uint32_t> at(nb * slices, 0);  // [bucket][slice]
-    WorkerPool::get().run(slices, [&](size_t t) {
-        const size_t a = t * per, b = std::min(n, a + per);
-        for (size_t i = a; i < b; ++i) at[(top - in[i].cost) * slices + t]++;
-    });
-    uint32_t run = 0;
-    for (uint32_t &c : at) {
-        const uint32_t v = c;
-        c = run;
-        run += v;
-    }
-    out.resize(n);
-    WorkerPool::get().run(slices, [&](size_t t) {
-        const size_t a = t * per, b = std::min(n, a + per);
-        for (size_t i = a; i < b; ++i) out[at[(top - in[i].cost) * slices + t]++] = in[i];
-    });
+    std::vector<uint32_t> at(size_t(top) + 2, 0);
+    for (const CbJob &j : in) at[top - j.cost + 1]++;
+    for (size_t k = 1; k < at.size(); ++k) at[k] += at[k - 1];
+    out.resize(in.size());
+    for (const CbJob &j : in) out[at[top - j.cost]++] = j;
 }
 
 Status build_plan(const sgn_tile_req *reqs, size_t n, sgn_j2k_plan &P, int32_t *tile_status) {
