@@ -503,18 +503,37 @@ void sort_by_cost(const std::vector<CbJob> &in, std::vector<CbJob> &out) {
     for (const CbJob &j : in) out[at[top - j.cost]++] = j;
 }
 
-Status build_plan(const sgn_tile_req *reqs, size_t n, sgn_j2k_plan &P, int32_t *tile_status) {
-    const auto t0 = std::chrono::steady_clock::now();
-    P.ntiles = n;
-    std::vector<LocalPlan> lps(n);
-    std::vector<int32_t> st(n, 0);
-    std::vector<std::string> errs(n);
+// Tiles parsed into their local plans: the parallel half of a plan build.
+struct ParsedBatch {
+    size_t n = 0;
+    std::vector<LocalPlan> lps;
     // each request's bytes narrowed to its codestream (a JP2 file's jp2c
     // payload); only the codestream travels to the device
-    std::vector<sgn_tile_req> cs(reqs, reqs + n);
-    const sgn_tile_req *jobs = cs.data();
+    std::vector<sgn_tile_req> cs;
     // per tile: code-blocks for the lane list, the wave list and refinement
-    std::vector<uint32_t> n_lane(n, 0), n_wave(n, 0), n_ref(n, 0);
+    std::vector<uint32_t> n_lane, n_wave, n_ref;
+    double t_parse = 0;
+    Status st = kOk;
+    std::string msg;  // error of the first failing tile
+};
+
+// Parses every tile on the worker pool.  Failures are reported in B.st /
+// B.msg (and per tile in tile_status), not in the calling thread's error.
+void parse_batch(const sgn_tile_req *reqs, size_t n, ParsedBatch &B, int32_t *tile_status) {
+    const auto t0 = std::chrono::steady_clock::now();
+    B.n = n;
+    B.lps.assign(n, LocalPlan());
+    B.cs.assign(reqs, reqs + n);
+    B.n_lane.assign(n, 0);
+    B.n_wave.assign(n, 0);
+    B.n_ref.assign(n, 0);
+    B.st = kOk;
+    B.msg.clear();
+    std::vector<int32_t> st(n, 0);
+    std::vector<std::string> errs(n);
+    std::vector<LocalPlan> &lps = B.lps;
+    std::vector<sgn_tile_req> &cs = B.cs;
+    const sgn_tile_req *jobs = cs.data();
     WorkerPool::get().run(n, [&](size_t i) {
         Container c;
         st[i] = locate_codestream(reqs[i].bytes, reqs[i].len, c);
@@ -538,28 +557,34 @@ Status build_plan(const sgn_tile_req *reqs, size_t n, sgn_j2k_plan &P, int32_t *
         }
         for (const CbJob &j : lps[i].cbs) {
             if (lane_eligible(j)) {
-                ++n_lane[i];
-                n_ref[i] += lane_refines(j) ? 1u : 0u;
+                ++B.n_lane[i];
+                B.n_ref[i] += lane_refines(j) ? 1u : 0u;
             } else {
-                ++n_wave[i];
+                ++B.n_wave[i];
             }
         }
     });
-    int32_t first_bad = 0;
-    size_t bad_index = 0;
     for (size_t i = 0; i < n; ++i) {
         if (tile_status) tile_status[i] = st[i];
-        if (st[i] && !first_bad) {
-            first_bad = st[i];
-            bad_index = i;
+        if (st[i] && !B.st) {
+            B.st = static_cast<Status>(st[i]);
+            B.msg = "tile " + std::to_string(i) + ": " + errs[i];
         }
     }
-    if (first_bad) {
-        set_last_error("tile " + std::to_string(bad_index) + ": " + errs[bad_index]);
-        return static_cast<Status>(first_bad);
-    }
+    B.t_parse = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                    .count();
+}
+
+// Merges the parsed tiles into the batch tables.  pool: fill the code-block
+// lists on the worker pool (false when another thread is parsing on it).
+Status merge_batch(ParsedBatch &B, sgn_j2k_plan &P, bool pool) {
+    const size_t n = B.n;
+    std::vector<LocalPlan> &lps = B.lps;
+    const sgn_tile_req *jobs = B.cs.data();
+    const std::vector<uint32_t> &n_lane = B.n_lane, &n_wave = B.n_wave, &n_ref = B.n_ref;
+    P.ntiles = n;
+    P.t_parse = B.t_parse;
     const auto t1 = std::chrono::steady_clock::now();
-    P.t_parse = std::chrono::duration<double, std::milli>(t1 - t0).count();
     // ---- merge with rebasing ----
     uint64_t arena = 0, coef = 0;
     std::vector<uint64_t> arena_base(n), coef_base(n);
@@ -640,7 +665,7 @@ Status build_plan(const sgn_tile_req *reqs, size_t n, sgn_j2k_plan &P, int32_t *
     P.cb_lane_plain.resize(P.lane_plain_begin[n]);
     P.cb_wave.resize(P.wave_begin[n]);
     P.cb_refine.resize(ref_begin[n]);
-    WorkerPool::get().run(n, [&](size_t i) {
+    auto fill = [&](size_t i) {
         const LocalPlan &lp = lps[i];
         const uint32_t cbase = static_cast<uint32_t>(coef_base[i]);
         uint32_t a = P.lane_plain_begin[i], w = P.wave_begin[i], r = ref_begin[i];
@@ -658,7 +683,11 @@ Status build_plan(const sgn_tile_req *reqs, size_t n, sgn_j2k_plan &P, int32_t *
                 P.cb_wave[w++] = j;
             }
         }
-    });
+    };
+    if (pool)
+        WorkerPool::get().run(n, fill);
+    else
+        for (size_t i = 0; i < n; ++i) fill(i);
     P.tile_levels.assign(n, {});
     P.tile_final_first.assign(n, {0, 0});
     P.tile_final_count.assign(n, 0);
@@ -803,6 +832,16 @@ Status build_plan(const sgn_tile_req *reqs, size_t n, sgn_j2k_plan &P, int32_t *
     P.t_merge = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1)
                     .count();
     return kOk;
+}
+
+Status build_plan(const sgn_tile_req *reqs, size_t n, sgn_j2k_plan &P, int32_t *tile_status) {
+    ParsedBatch B;
+    parse_batch(reqs, n, B, tile_status);
+    if (B.st) {
+        set_last_error(B.msg);
+        return B.st;
+    }
+    return merge_batch(B, P, true);
 }
 
 // True when p lies in page-locked host memory (hipHostMalloc or
@@ -1452,14 +1491,41 @@ Status decode_pipelined(sgn_hip_session *s, const sgn_tile_req *reqs, size_t n, 
     HIP_TRY(hipEventCreateWithFlags(&join, hipEventDisableTiming));
     std::unique_ptr<std::remove_pointer_t<hipEvent_t>, hipError_t (*)(hipEvent_t)> join_guard(
         join, hipEventDestroy);
+    auto chunk_len = [&](size_t a) { return std::min(chunk, n - a); };
+    auto status_at = [&](size_t a) { return tile_status ? tile_status + a : nullptr; };
+    // chunk k + 1 is parsed (on a helper thread and the worker pool) while
+    // chunk k is merged, uploaded and enqueued on this thread
+    auto parse_into = [&](size_t a, ParsedBatch &B) {
+        try {
+            parse_batch(reqs + a, chunk_len(a), B, status_at(a));
+        } catch (const std::bad_alloc &) {
+            B.st = kErrNoMem;
+            B.msg = "out of host memory";
+        }
+    };
+    ParsedBatch cur, nxt;
+    if (n) parse_into(0, cur);
     for (size_t a = 0, c = 0; a < n; a += chunk, ++c) {
-        const size_t m = std::min(chunk, n - a);
+        const size_t m = chunk_len(a);
+        std::thread helper;
+        if (a + chunk < n) helper = std::thread(parse_into, a + chunk, std::ref(nxt));
+        struct Join {
+            std::thread &t;
+            ~Join() {
+                if (t.joinable()) t.join();
+            }
+        } join_helper{helper};
         auto P = std::make_unique<sgn_j2k_plan>();
         P->blocks = s->blocks;
-        Status st = build_plan(reqs + a, m, *P, tile_status ? tile_status + a : nullptr);
+        Status st = cur.st;
+        std::string msg = cur.msg;
+        if (!st) {
+            st = merge_batch(cur, *P, false);
+            if (st) msg = last_error();
+        }
         if (st) {
             parse_st = st;
-            parse_msg = last_error();
+            parse_msg = msg;
             for (size_t i = 0; i < m; ++i)
                 if (!tile_status || tile_status[a + i]) {
                     parse_bad = a + i;
@@ -1486,6 +1552,8 @@ Status decode_pipelined(sgn_hip_session *s, const sgn_tile_req *reqs, size_t n, 
         }
         plans.push_back(std::move(P));
         first.push_back(a);
+        if (helper.joinable()) helper.join();
+        std::swap(cur, nxt);
     }
     HIP_TRY(hipEventRecord(join, s->aux_stream));
     HIP_TRY(hipStreamWaitEvent(s->stream, join, 0));
