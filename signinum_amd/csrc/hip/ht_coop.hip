@@ -122,7 +122,7 @@ __device__ __forceinline__ void ms_ring_fill(MsRing &r, uint32_t need, uint32_t 
         uint64_t mask;
         const uint32_t off = 8u * lane - count_below(pff, mask);
         const uint32_t tot = 512u - (uint32_t)__builtin_popcountll(mask);
-        // words past the partly written one that this round reaches start at zero
+        // words past the partly written one that this 64-byte step reaches start at zero
         const uint32_t w0 = r.fill >> 5, wl = (r.fill + tot) >> 5;
         if (lane >= 1 && w0 + lane <= wl) ring[(w0 + lane) & (kCoopRingWords - 1)] = 0;
         __syncthreads();
