@@ -386,7 +386,9 @@ int sgn_ht_decode_sub_band(sgn_hip_session *session, const uint8_t *data, size_t
  * lib.rs:767-780): one 2D synthesis level over rect [x0,x1) x [y0,y1);
  * band arrays are row-major with the Annex B band sizes.  For 5/3 the f32
  * inputs are integers; each is truncated toward zero (NaN to 0, saturating
- * at the int32 range) before integer lifting. */
+ * at the int32 range) before integer lifting, except on a rect one sample
+ * wide or high at an odd origin, which halves that sample: there the 5/3
+ * steps run in f32 on the inputs as given, as the reference computes them. */
 int sgn_idwt_level(sgn_hip_session *session, int reversible, uint32_t x0, uint32_t y0,
                    uint32_t x1, uint32_t y1, const float *ll, const float *hl, const float *lh,
                    const float *hh, float *out);

@@ -86,6 +86,7 @@ struct LevelView {
     uint32_t x0, y0, fin, narrow;
     int w, h, levels;
     bool aligned;  // every used slot and row pitch 16-byte aligned
+    bool l53;      // f32 samples lifted with the 5/3 steps (LevelJob::lift53)
 };
 
 // Stages the band table of `lj` into `sb`; ends with a barrier.
@@ -123,6 +124,7 @@ __device__ __forceinline__ LevelView level_view(const LevelJob *__restrict__ lj,
     v.w = (int)(lj->x1 - lj->x0);
     v.h = (int)(lj->y1 - lj->y0);
     v.levels = lj->levels;
+    v.l53 = lj->lift53 != 0;
     __syncthreads();
     return v;
 }
@@ -222,7 +224,7 @@ struct Win {
 // it, so segment results equal whole-line lifting (idwt.rs:741-764 for 5/3,
 // :767-835 for 9/7, same operation order).
 template <bool REV, int Q, typename T>
-__device__ __forceinline__ void lift_segment(T (&v)[Win<REV>::R]) {
+__device__ __forceinline__ void lift_segment(T (&v)[Win<REV>::R], bool l53) {
     // keep every rounding of the line-wise lifting: no contraction of the
     // scaling products into the neighbour sums
 #pragma clang fp contract(off)
@@ -234,6 +236,15 @@ __device__ __forceinline__ void lift_segment(T (&v)[Win<REV>::R]) {
 #pragma unroll
         for (int i = 2; i < R - 2; ++i)
             if (((Q + i) & 1) == 1) v[i] = v[i] + ((v[i - 1] + v[i + 1]) >> 1);
+    } else if (l53) {
+        // the 5/3 steps on f32 samples, as the reference computes them
+        // (idwt.rs:741-764): floor((l + r) * 0.25 + 0.5), floor((l + r) * 0.5)
+#pragma unroll
+        for (int i = 1; i < R - 1; ++i)
+            if (((Q + i) & 1) == 0) v[i] = v[i] - floorf(fmaf(v[i - 1] + v[i + 1], 0.25f, 0.5f));
+#pragma unroll
+        for (int i = 2; i < R - 2; ++i)
+            if (((Q + i) & 1) == 1) v[i] = v[i] + floorf((v[i - 1] + v[i + 1]) * 0.5f);
     } else {
         const float inv_k = 1.0f / kKappa;
 #pragma unroll
@@ -310,7 +321,7 @@ struct SinkGlobal {  // the level's output in the coefficient arena
 // direction.  All segments are read before any in-place write.
 template <bool REV, bool HORIZ, int Q, class Sink, class G = Big>
 __device__ __forceinline__ void lift_window_q(typename Sample<REV>::T *win, int nh, int bw, int bh,
-                                              const Sink &sink) {
+                                              const Sink &sink, bool l53) {
     using T = typename Sample<REV>::T;
     constexpr int kBlk = G::kBlk, kIdwtThreads = G::kThreads;
     constexpr int H = Win<REV, G>::H, N = Win<REV, G>::N, P = Win<REV, G>::P, R = Win<REV, G>::R;
@@ -331,7 +342,7 @@ __device__ __forceinline__ void lift_window_q(typename Sample<REV>::T *win, int 
 #pragma unroll
         for (int j = 0; j < R; ++j)
             v[j] = HORIZ ? win[line * P + 8 * seg + j] : win[(8 * seg + j) * P + H + line];
-        lift_segment<REV, Q>(v);
+        lift_segment<REV, Q>(v, l53);
 #pragma unroll
         for (int j = 0; j < 8; ++j) out[it][j] = v[H + j];
     }
@@ -357,11 +368,11 @@ __device__ __forceinline__ void lift_window_q(typename Sample<REV>::T *win, int 
 // form would index the segment registers at run time).
 template <bool REV, bool HORIZ, class Sink, class G = Big>
 __device__ __forceinline__ void lift_window(typename Sample<REV>::T *win, int nh, int bw, int bh,
-                                            uint32_t q, const Sink &sink) {
+                                            uint32_t q, const Sink &sink, bool l53) {
     if (__builtin_amdgcn_readfirstlane(q))
-        lift_window_q<REV, HORIZ, 1, Sink, G>(win, nh, bw, bh, sink);
+        lift_window_q<REV, HORIZ, 1, Sink, G>(win, nh, bw, bh, sink, l53);
     else
-        lift_window_q<REV, HORIZ, 0, Sink, G>(win, nh, bw, bh, sink);
+        lift_window_q<REV, HORIZ, 0, Sink, G>(win, nh, bw, bh, sink, l53);
 }
 
 // Window-row table of block row `by`: for every window row after extension
@@ -690,11 +701,11 @@ __device__ __forceinline__ void lift_block(const LevelView &L, int bx, int by,
         if (L.w == 1)
             lift<REV, P, true, T, G>(win, nh, nw, L.x0 + (uint32_t)(ox - H), true, (L.x0 & 1u) != 0);
         else
-            lift_window<REV, true, Sink, G>(win, nh, bw, bh, qx, sink);
+            lift_window<REV, true, Sink, G>(win, nh, bw, bh, qx, sink, L.l53);
         if (L.h == 1) {
             lift<REV, P, false, T, G>(win, bw, nh, L.y0 + (uint32_t)(oy - H), true, (L.y0 & 1u) != 0);
         } else {
-            lift_window<REV, false, Sink, G>(win, nh, bw, bh, qy, sink);
+            lift_window<REV, false, Sink, G>(win, nh, bw, bh, qy, sink, L.l53);
             in_window = false;
         }
     }
@@ -843,6 +854,7 @@ struct FinalView {
     uint64_t pitch;
     int width, height, nc, fmt, ch, bps, mct;
     uint32_t prec0, prec1, prec2, same8;
+    bool rct_f32;  // f32 planes under the reversible colour transform
 };
 
 // Converts one pixel's component values (after MCT and sign shift) into
@@ -903,6 +915,7 @@ __device__ __forceinline__ void final_body(const FinalJob *jobs, uint32_t njobs,
     F.prec1 = fj->precision[1];
     F.prec2 = fj->precision[2];
     F.same8 = fj->same8;
+    F.rct_f32 = fj->rct_f32 != 0;
     const uint32_t lvl0 = fj->level[0], lvl1 = fj->level[1], lvl2 = fj->level[2];
     int bw = 0, bh = 0;
     __shared__ BandLds sb;
@@ -1043,7 +1056,14 @@ __device__ __forceinline__ void final_body(const FinalJob *jobs, uint32_t njobs,
 #pragma unroll
                 for (int c = 0; c < 3; ++c)
                     v[c] = c < F.nc ? (c + 1 < F.nc ? pl[c < 2 ? c : 1][q] : win[(y + H) * P + x + H]) : 0.0f;
-                if (F.mct) {
+                if (F.mct && F.rct_f32) {
+                    // the reversible transform on f32 samples (mct.rs:71-156)
+                    const float y0 = v[0], y1 = v[1], y2 = v[2];
+                    const float i1 = y0 - floorf((y2 + y1) * 0.25f);
+                    v[0] = y2 + i1;
+                    v[1] = i1;
+                    v[2] = y1 + i1;
+                } else if (F.mct) {
                     const float y0 = v[0], y1 = v[1], y2 = v[2];
                     v[0] = fmaf(y2, 1.402f, y0);
                     v[1] = fmaf(y2, -0.71414f, fmaf(y1, -0.34413f, y0));

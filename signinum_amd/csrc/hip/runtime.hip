@@ -2319,10 +2319,11 @@ int sgn_idwt_level(sgn_hip_session *s, int reversible, uint32_t x0, uint32_t y0,
                    const float *hh, float *out) {
     API_GUARD_BEGIN
     if (!s || x1 < x0 || y1 < y0) return SGN_E_ARG;
-    if (reversible && ((x1 - x0 == 1 && (x0 & 1)) || (y1 - y0 == 1 && (y0 & 1)))) {
-        set_last_error("odd-origin single-sample level in a reversible transform");
-        return SGN_E_UNSUPPORTED;
-    }
+    // a 5/3 level one sample wide or high at an odd origin halves that
+    // sample (idwt.rs:726-731): such a level is lifted in f32 with the 5/3
+    // steps, on the f32 inputs as given
+    const bool f53 = reversible && ((x1 - x0 == 1 && (x0 & 1)) || (y1 - y0 == 1 && (y0 & 1)));
+    if (f53) reversible = 0;
     const uint32_t w = x1 - x0, h = y1 - y0;
     if (!w || !h) return SGN_OK;
     HIP_TRY(hipSetDevice(s->device));
@@ -2364,6 +2365,7 @@ int sgn_idwt_level(sgn_hip_session *s, int reversible, uint32_t x0, uint32_t y0,
     lj.nbx = static_cast<uint16_t>(ceil_div(w, 64));
     lj.nby = static_cast<uint16_t>(ceil_div(h, 64));
     lj.reversible = reversible ? 1 : 0;
+    lj.lift53 = f53 ? 1 : 0;
     lj.levels = 1;
     uint32_t *d_coef = nullptr;
     LevelJob *d_job = nullptr;

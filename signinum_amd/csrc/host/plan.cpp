@@ -320,13 +320,27 @@ Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denomin
             set_last_error("inverse MCT over components with different transforms");
             return kErrMalformed;
         }
+        // A retained level one sample wide (or high) at an odd origin halves
+        // that sample in f32 (idwt.rs:726-731), which later levels carry as a
+        // half: every reversible component of such a tile is then lifted in
+        // f32 with the 5/3 steps, as the reference computes it throughout.
+        bool lone53 = false;
+        for (int c = 0; c < nc; ++c) {
+            if (t.comp[c].transform != 1) continue;
+            const uint32_t R = tcs[c].nres - sk;
+            for (uint32_t r = 1; r < R; ++r) {
+                const Rect &rr = tcs[c].res[r].rect;
+                lone53 |= (rr.w() == 1 && (rr.x0 & 1)) || (rr.h() == 1 && (rr.y0 & 1));
+            }
+        }
         FinalJob fj{};
         Rect out_rect;
         for (int c = 0; c < nc; ++c) {
             const TileCompGeom &g = tcs[c];
             const CompParams &cp = t.comp[c];
             const uint32_t R = g.nres - sk;
-            const bool rev = cp.transform == 1;
+            const bool rev = cp.transform == 1 && !lone53;
+            const bool f53 = cp.transform == 1 && lone53;
             const uint64_t comp_start = lp.coef_elems;
             // coefficient layout: bands of the retained resolutions
             std::vector<uint32_t> band_off(g.bands.size(), 0);
@@ -344,7 +358,7 @@ Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denomin
                     set_last_error("invalid quantisation parameters");
                     return st;
                 }
-                if (rev && d != 1.0f) {
+                if (cp.transform == 1 && d != 1.0f) {
                     set_last_error("quantised reversible transform");
                     return kErrUnsupported;
                 }
@@ -438,6 +452,7 @@ Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denomin
                 lj.x1 = rr.x1;
                 lj.y1 = rr.y1;
                 lj.reversible = rev;
+                lj.lift53 = f53 ? 1 : 0;
                 lj.levels = 0;
                 lj.narrow = 1;  // slot 0 is the code-block band itself
                 if (smp.mixed) {
@@ -449,14 +464,8 @@ Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denomin
             }
             for (uint32_t r = 1; r < R; ++r) {
                 const ResGeom &rs = g.res[r];
-                // a single-sample row or column with odd origin halves the
-                // sample (idwt.rs:726-731); not integer-exact for 5/3
-                if (rev && ((rs.rect.w() == 1 && (rs.rect.x0 & 1)) ||
-                            (rs.rect.h() == 1 && (rs.rect.y0 & 1)))) {
-                    set_last_error("odd-origin single-sample level in a reversible tile");
-                    return kErrUnsupported;
-                }
                 LevelJob lj{};
+                lj.lift53 = f53 ? 1 : 0;
                 lj.band[0] = prev_out;
                 lj.bw[0] = prev_w;
                 lj.delta[0] = prev_delta;
@@ -588,7 +597,8 @@ Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denomin
         fj.ncomp = static_cast<uint8_t>(nc);
         fj.fmt = static_cast<uint8_t>(fmt);
         fj.mct = mct ? 1 : 0;
-        fj.reversible = t.comp[0].transform == 1;
+        fj.reversible = t.comp[0].transform == 1 && !lone53;
+        fj.rct_f32 = mct && lone53 && t.comp[0].transform == 1 ? 1 : 0;
         fj.same8 = same8 ? 1 : 0;
         if (fj.cx1 > fj.cx0 && fj.cy1 > fj.cy0) lp.finals.push_back(fj);
     }

@@ -93,6 +93,11 @@ struct LevelJob {
     uint8_t levels;           // 1: one synthesis level, 0: copy LL only
     uint8_t narrow;           // bit b set: band b holds int16 code-block samples
     uint16_t bx0, by0;        // first 64x64 block of the nbx x nby computed (region decodes)
+    // with reversible == 0: lift with the 5/3 steps in f32 (a reversible
+    // tile-component with a level one sample wide or high at an odd origin,
+    // whose halved sample the integer 5/3 path cannot carry)
+    uint8_t lift53;
+    uint8_t pad[3];
 };
 
 // Final stage of one tile: last synthesis level of every component fused
@@ -107,7 +112,8 @@ struct FinalJob {
     uint8_t ncomp, fmt, mct, reversible;
     uint8_t precision[4];
     uint8_t same8;            // all components 8-bit
-    uint8_t pad[3];
+    uint8_t rct_f32;          // with reversible == 0 and mct: the reversible transform in f32
+    uint8_t pad[2];
     uint32_t out_x, out_y;    // position of this J2K tile inside the output image
     uint16_t bx0, by0;        // first 64x64 block of the nbx x nby decoded
     // Pixels written, relative to the tile's rectangle: [cx0, cx1) x [cy0,

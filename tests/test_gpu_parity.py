@@ -401,9 +401,9 @@ def test_ht_hook_lane_kernel_dequantised(session, oracle):
 @pytest.mark.parametrize("rev", [1, 0])
 @pytest.mark.parametrize("rect", [(0, 0, 70, 45), (3, 5, 100, 38), (1, 2, 2, 9), (0, 0, 1, 1)])
 def test_idwt_level_hook(session, oracle, rev, rect):
-    # A lone sample at an odd origin is halved in f32 (idwt.rs:726-731); the
-    # integer 5/3 path cannot carry the half, so it rejects that level
-    # (UnsupportedError) rather than round it.
+    # A lone sample at an odd origin is halved in f32 (idwt.rs:726-731): such
+    # a 5/3 level is lifted in f32 with the 5/3 steps; its inputs here are
+    # odd integers and halves, so the half survives into the output.
     lone_odd = (rect[2] - rect[0] == 1 and rect[0] % 2) or (rect[3] - rect[1] == 1 and rect[1] % 2)
     x0, y0, x1, y1 = rect
     wl, wh = (x1 + 1) // 2 - (x0 + 1) // 2, x1 // 2 - x0 // 2
@@ -412,12 +412,12 @@ def test_idwt_level_hook(session, oracle, rev, rect):
     mk = (lambda n: rng.integers(-200, 200, n).astype(np.float32)) if rev else \
         (lambda n: rng.normal(0, 50, n).astype(np.float32))
     bands = [mk(wl * hl_), mk(wh * hl_), mk(wl * hh_), mk(wh * hh_)]
-    hook = S.HtCodeBlockDecoder(session)
     if rev and lone_odd:
-        with pytest.raises(S.UnsupportedError):
-            hook.decode_single_decomposition_idwt(rev, rect, *bands)
-        return
+        bands = [b + np.float32(0.5) * (np.arange(b.size) % 2) for b in bands]
+    hook = S.HtCodeBlockDecoder(session)
     got = hook.decode_single_decomposition_idwt(rev, rect, *bands)
+    if rev and lone_odd:
+        assert (got != np.floor(got)).any()
     exp = O.idwt_level(rev, rect, *bands)
     assert np.array_equal(got, exp)
 
@@ -559,3 +559,16 @@ def test_widths_just_past_a_block_column(session, oracle, reversible):
             assert d == 0 if reversible else d <= 1, (w, h, levels, d)  # 9/7: 1 LSB of 8-bit
             if reversible:
                 assert np.array_equal(got, img), (w, h, levels)
+
+
+def test_lone_odd_origin_sample(session, oracle):
+    """Tiles whose coarsest 5/3 level is one sample wide at an odd origin:
+    the device lifts those tiles in f32 with the 5/3 steps and equals the
+    source and the oracle, at full and half resolution."""
+    img = synth.tissue_rgb(np.random.default_rng(193), 193, 70)
+    for kw in (dict(), dict(use_mct=0)):
+        cs = S.encode_j2k_ht(img, tile_width=64, tile_height=64, levels=7, **kw)
+        d = S.J2kDecoder(cs, session)
+        assert np.array_equal(d.decode_to_numpy(S.PixelFormat.RGB8), img)
+        assert np.array_equal(d.decode_to_numpy(S.PixelFormat.RGB8, S.Downscale(2)),
+                              O.decode(cs, 1, 2))

@@ -191,3 +191,16 @@ def test_irreversible_multitile_odd_origins_close_to_openjpeg(oracle):
     p64 = O.decode_planes(cs, use_f64=True)
     assert np.abs(p32 - p64).max() < 0.05
     assert np.abs(out - img.astype(np.int64)).max() < 40
+
+
+def test_lone_odd_origin_sample_round_trip(oracle):
+    """A 193-pixel-wide image in 64-pixel tiles with 7 levels: the last tile
+    column is one sample wide at an odd coordinate (x0 = 3) on the first
+    synthesis level, where the 5/3 synthesis halves the sample
+    (idwt.rs:726-731) that the encoder doubled.  The oracle and OpenJPEG
+    both give the source."""
+    img = synth.tissue_rgb(np.random.default_rng(193), 193, 70)
+    for kw in (dict(), dict(use_mct=0)):
+        cs = S.encode_j2k_ht(img, tile_width=64, tile_height=64, levels=7, **kw)
+        assert np.array_equal(O.decode(cs, 1), img)
+        assert np.array_equal(_pillow(cs), img)
