@@ -33,6 +33,7 @@
 #include "ht_rows.hip"
 #include "idwt.hip"
 #include "t1_classic.hip"
+#include "packets.hip"
 #include "cost_model.hip"
 
 using namespace sgn;
@@ -368,6 +369,24 @@ struct sgn_j2k_plan {
     uint32_t *d_fin_redo = nullptr;  // 5/3 final blocks left to the int32-plane pass
     uint8_t *d_status_lane = nullptr, *d_status_wave = nullptr;
     uint32_t *d_queue = nullptr;  // work counter of the persistent lane kernel
+    // packet headers parsed on the device (tiles planned with device
+    // packets): tables, tag-tree and length scratch, per-tile status; the
+    // lane list then keeps the per-tile order (its jobs are filled on the
+    // device) and every lane block goes through the refinement kernel
+    bool device_packets = false;
+    std::vector<PktTile> pk_tiles;
+    std::vector<PktPart> pk_parts;
+    std::vector<PktPacket> pk_pkts;
+    std::vector<PktBand> pk_bands;
+    std::vector<PktCb> pk_cbs;
+    uint64_t pk_nodes = 0;
+    PktTile *d_pk_tiles = nullptr;
+    PktPart *d_pk_parts = nullptr;
+    PktPacket *d_pk_pkts = nullptr;
+    PktBand *d_pk_bands = nullptr;
+    PktCb *d_pk_cbs = nullptr;
+    uint32_t *d_pk_nodes = nullptr, *d_pk_lens = nullptr;
+    uint8_t *d_pk_status = nullptr;
     // classic (EBCOT) code-blocks, by kernel shape (small, big, wide:
     // [classic_begin[v], classic_begin[v + 1])), costliest first; their
     // segment table; bytes of blocks joined from several packets
@@ -513,13 +532,15 @@ struct ParsedBatch {
     // per tile: code-blocks for the lane list, the wave list and refinement
     std::vector<uint32_t> n_lane, n_wave, n_ref;
     double t_parse = 0;
+    bool device_packets = false;
     Status st = kOk;
     std::string msg;  // error of the first failing tile
 };
 
 // Parses every tile on the worker pool.  Failures are reported in B.st /
 // B.msg (and per tile in tile_status), not in the calling thread's error.
-void parse_batch(const sgn_tile_req *reqs, size_t n, ParsedBatch &B, int32_t *tile_status) {
+void parse_batch(const sgn_tile_req *reqs, size_t n, ParsedBatch &B, int32_t *tile_status,
+                 bool device_packets = false) {
     const auto t0 = std::chrono::steady_clock::now();
     B.n = n;
     B.lps.assign(n, LocalPlan());
@@ -529,6 +550,7 @@ void parse_batch(const sgn_tile_req *reqs, size_t n, ParsedBatch &B, int32_t *ti
     B.n_ref.assign(n, 0);
     B.st = kOk;
     B.msg.clear();
+    B.device_packets = device_packets;
     std::vector<int32_t> st(n, 0);
     std::vector<std::string> errs(n);
     std::vector<LocalPlan> &lps = B.lps;
@@ -549,7 +571,7 @@ void parse_batch(const sgn_tile_req *reqs, size_t n, ParsedBatch &B, int32_t *ti
             roi.h = jobs[i].roi_h;
             st[i] = build_local_plan(jobs[i].bytes, jobs[i].len, jobs[i].fmt, jobs[i].denominator,
                                      reinterpret_cast<uint64_t>(jobs[i].dst), jobs[i].pitch,
-                                     jobs[i].capacity, lps[i], roi);
+                                     jobs[i].capacity, lps[i], roi, device_packets);
         }
         if (st[i]) {
             errs[i] = last_error();
@@ -665,11 +687,40 @@ Status merge_batch(ParsedBatch &B, sgn_j2k_plan &P, bool pool) {
     P.cb_lane_plain.resize(P.lane_plain_begin[n]);
     P.cb_wave.resize(P.wave_begin[n]);
     P.cb_refine.resize(ref_begin[n]);
+    // device packet tables: each tile's slice, rebased with its jobs below
+    struct PkBase {
+        uint32_t tile, part, pkt, band, cb;
+        uint64_t node;
+    };
+    std::vector<PkBase> pkb(n + 1, PkBase{0, 0, 0, 0, 0, 0});
+    for (size_t i = 0; i < n; ++i) {
+        const LocalPlan &lp = lps[i];
+        pkb[i + 1] = {pkb[i].tile + uint32_t(lp.pk_tiles.size()),
+                      pkb[i].part + uint32_t(lp.pk_parts.size()),
+                      pkb[i].pkt + uint32_t(lp.pk_pkts.size()),
+                      pkb[i].band + uint32_t(lp.pk_bands.size()),
+                      pkb[i].cb + uint32_t(lp.pk_cbs.size()), pkb[i].node + lp.pk_nodes};
+    }
+    P.device_packets = B.device_packets && pkb[n].tile > 0;
+    if (pkb[n].node * 2 >= (uint64_t(1) << 32)) {
+        set_last_error("batch too large for the device packet tables; split it");
+        return kErrArg;
+    }
+    P.pk_tiles.resize(pkb[n].tile);
+    P.pk_parts.resize(pkb[n].part);
+    P.pk_pkts.resize(pkb[n].pkt);
+    P.pk_bands.resize(pkb[n].band);
+    P.pk_cbs.resize(pkb[n].cb);
+    P.pk_nodes = pkb[n].node;
     auto fill = [&](size_t i) {
         const LocalPlan &lp = lps[i];
         const uint32_t cbase = static_cast<uint32_t>(coef_base[i]);
         uint32_t a = P.lane_plain_begin[i], w = P.wave_begin[i], r = ref_begin[i];
+        // list position of each local job (device packet tables)
+        std::vector<uint32_t> pos(lp.pk_cbs.empty() ? 0 : lp.cbs.size());
         for (size_t k = 0; k < lp.cbs.size(); ++k) {
+            if (!pos.empty())
+                pos[k] = lane_eligible(lp.cbs[k]) ? a : (kPktWave | w);
             CbJob j = lp.cbs[k];
             j.src += static_cast<uint32_t>(arena_base[i]);
             // int16 index of a narrow sample: twice its band's int32 offset
@@ -682,6 +733,37 @@ Status merge_batch(ParsedBatch &B, sgn_j2k_plan &P, bool pool) {
             } else {
                 P.cb_wave[w++] = j;
             }
+        }
+        const PkBase &pb = pkb[i];
+        for (size_t k = 0; k < lp.pk_tiles.size(); ++k) {
+            PktTile t = lp.pk_tiles[k];
+            t.part0 += pb.part;
+            t.pkt0 += pb.pkt;
+            t.cb0 += pb.cb;
+            t.node0 += static_cast<uint32_t>(pb.node);
+            P.pk_tiles[pb.tile + k] = t;
+        }
+        for (size_t k = 0; k < lp.pk_parts.size(); ++k) {
+            PktPart q = lp.pk_parts[k];
+            q.off += static_cast<uint32_t>(arena_base[i]);
+            P.pk_parts[pb.part + k] = q;
+        }
+        for (size_t k = 0; k < lp.pk_pkts.size(); ++k) {
+            PktPacket q = lp.pk_pkts[k];
+            q.band0 += pb.band;
+            P.pk_pkts[pb.pkt + k] = q;
+        }
+        for (size_t k = 0; k < lp.pk_bands.size(); ++k) {
+            PktBand q = lp.pk_bands[k];
+            q.cb0 += pb.cb;
+            q.incl += static_cast<uint32_t>(pb.node);
+            q.zbp += static_cast<uint32_t>(pb.node);
+            P.pk_bands[pb.band + k] = q;
+        }
+        for (size_t k = 0; k < lp.pk_cbs.size(); ++k) {
+            PktCb q = lp.pk_cbs[k];
+            if (q.job != kPktNoJob) q.job = pos[q.job];
+            P.pk_cbs[pb.cb + k] = q;
         }
     };
     if (pool)
@@ -736,8 +818,15 @@ Status merge_batch(ParsedBatch &B, sgn_j2k_plan &P, bool pool) {
     }
     // code-blocks for the batched lane kernel, ordered by predicted cost so
     // that a wave's lanes finish together
-    sort_by_cost(P.cb_lane_plain, P.cb_lane);
-    P.lane_refined = !P.cb_refine.empty();
+    if (P.device_packets) {
+        // jobs filled on the device at their per-tile positions
+        P.cb_lane = P.cb_lane_plain;
+        P.cb_refine.clear();
+        P.lane_refined = true;
+    } else {
+        sort_by_cost(P.cb_lane_plain, P.cb_lane);
+        P.lane_refined = !P.cb_refine.empty();
+    }
     std::stable_sort(P.cb_refine.begin(), P.cb_refine.end(), [](const CbJob &a, const CbJob &b) {
         return uint32_t(a.lref) + a.lcup > uint32_t(b.lref) + b.lcup;
     });
@@ -834,9 +923,15 @@ Status merge_batch(ParsedBatch &B, sgn_j2k_plan &P, bool pool) {
     return kOk;
 }
 
+// Packet headers of HT tiles parsed on the device (SGN_DEVICE_PACKETS=1).
+bool device_packets_enabled() {
+    const char *e = std::getenv("SGN_DEVICE_PACKETS");
+    return e && e[0] == '1';
+}
+
 Status build_plan(const sgn_tile_req *reqs, size_t n, sgn_j2k_plan &P, int32_t *tile_status) {
     ParsedBatch B;
-    parse_batch(reqs, n, B, tile_status);
+    parse_batch(reqs, n, B, tile_status, device_packets_enabled());
     if (B.st) {
         set_last_error(B.msg);
         return B.st;
@@ -914,7 +1009,12 @@ Status upload_plan(sgn_hip_session *s, const sgn_tile_req *jobs, sgn_j2k_plan &P
     const uint64_t o_fbj0 = take(bytes_of(P.fin_block_job[0])), o_fbj1 = take(bytes_of(P.fin_block_job[1]));
     const uint64_t o_cls = take(bytes_of(P.classic)), o_cseg = take(bytes_of(P.cseg));
     const uint64_t o_sto = take(bytes_of(P.store)), o_sbj = take(bytes_of(P.store_block_job));
+    const uint64_t o_pkt = take(bytes_of(P.pk_tiles)), o_pkp = take(bytes_of(P.pk_parts));
+    const uint64_t o_pkk = take(bytes_of(P.pk_pkts)), o_pkb = take(bytes_of(P.pk_bands));
+    const uint64_t o_pkc = take(bytes_of(P.pk_cbs));
     const uint64_t host_bytes = off;
+    const uint64_t o_pkn = take(P.pk_nodes * 2 * sizeof(uint32_t));
+    const uint64_t o_pkl = take(P.pk_cbs.size() * sizeof(uint32_t)), o_pks = take(P.pk_tiles.size());
     const uint64_t o_stl = take(P.cb_lane.size()), o_stw = take(P.cb_wave.size());
     const uint64_t o_ff = take(P.fin_block_job[1].size() * sizeof(uint32_t));
     uint64_t crows_bytes = 0;
@@ -977,6 +1077,11 @@ Status upload_plan(sgn_hip_session *s, const sgn_tile_req *jobs, sgn_j2k_plan &P
         HIP_TRY(put_dev(o_cseg, P.cseg));
         HIP_TRY(put_dev(o_sto, P.store));
         HIP_TRY(put_dev(o_sbj, P.store_block_job));
+        HIP_TRY(put_dev(o_pkt, P.pk_tiles));
+        HIP_TRY(put_dev(o_pkp, P.pk_parts));
+        HIP_TRY(put_dev(o_pkk, P.pk_pkts));
+        HIP_TRY(put_dev(o_pkb, P.pk_bands));
+        HIP_TRY(put_dev(o_pkc, P.pk_cbs));
         HIP_TRY(hipStreamSynchronize(copy));
         P.t_stage = 0;
         P.t_copy = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() -
@@ -1036,6 +1141,11 @@ Status upload_plan(sgn_hip_session *s, const sgn_tile_req *jobs, sgn_j2k_plan &P
         put(o_cseg, P.cseg);
         put(o_sto, P.store);
         put(o_sbj, P.store_block_job);
+        put(o_pkt, P.pk_tiles);
+        put(o_pkp, P.pk_parts);
+        put(o_pkk, P.pk_pkts);
+        put(o_pkb, P.pk_bands);
+        put(o_pkc, P.pk_cbs);
         const auto t1 = std::chrono::steady_clock::now();
         uint8_t *d = static_cast<uint8_t *>(P.d_block);
         // direct runs first, then the staged range: consecutive staged runs and
@@ -1098,6 +1208,14 @@ Status upload_plan(sgn_hip_session *s, const sgn_tile_req *jobs, sgn_j2k_plan &P
     at(o_cseg, P.d_cseg, !P.cseg.empty());
     at(o_sto, P.d_store, !P.store.empty());
     at(o_sbj, P.d_store_block_job, !P.store_block_job.empty());
+    at(o_pkt, P.d_pk_tiles, !P.pk_tiles.empty());
+    at(o_pkp, P.d_pk_parts, !P.pk_parts.empty());
+    at(o_pkk, P.d_pk_pkts, !P.pk_pkts.empty());
+    at(o_pkb, P.d_pk_bands, !P.pk_bands.empty());
+    at(o_pkc, P.d_pk_cbs, !P.pk_cbs.empty());
+    at(o_pkn, P.d_pk_nodes, P.pk_nodes != 0);
+    at(o_pkl, P.d_pk_lens, !P.pk_cbs.empty());
+    at(o_pks, P.d_pk_status, !P.pk_tiles.empty());
     at(o_stl, P.d_status_lane, !P.cb_lane.empty());
     at(o_stw, P.d_status_wave, !P.cb_wave.empty());
     at(o_ff, P.d_fin_redo, !P.fin_block_job[1].empty());
@@ -1181,17 +1299,19 @@ uint32_t launch_lane_variant(sgn_hip_session *s, sgn_j2k_plan &P, int v, const C
     }
     rv.launch[P.narrow ? 1 : 0](s->stream, grid, d_jobs, count, P.d_arena, P.d_coef, s->d_tables,
                                 status, P.d_queue, P.lane_refined);
-    if (!rv.coop && d_jobs == P.d_cb_lane && !P.cb_refine.empty()) {
+    if (!rv.coop && d_jobs == P.d_cb_lane && (!P.cb_refine.empty() || P.device_packets)) {
         // refinement passes of the batch's lane blocks, after their cleanup
-        const uint32_t nr = static_cast<uint32_t>(P.cb_refine.size());
+        // (with device packets the pass counts are known on the device only:
+        // the kernel walks the whole lane list and skips the other blocks)
+        const CbJob *rj = P.device_packets ? P.d_cb_lane : P.d_cb_refine;
+        const uint32_t nr = static_cast<uint32_t>(P.device_packets ? P.cb_lane.size()
+                                                                   : P.cb_refine.size());
         if (P.narrow)
             hipLaunchKernelGGL((sgn_ht_refine<int16_t>), dim3((nr + 255) / 256), dim3(256), 0,
-                               s->stream, P.d_cb_refine, nr, P.d_arena,
-                               reinterpret_cast<int16_t *>(P.d_coef));
+                               s->stream, rj, nr, P.d_arena, reinterpret_cast<int16_t *>(P.d_coef));
         else
             hipLaunchKernelGGL((sgn_ht_refine<int32_t>), dim3((nr + 255) / 256), dim3(256), 0,
-                               s->stream, P.d_cb_refine, nr, P.d_arena,
-                               reinterpret_cast<int32_t *>(P.d_coef));
+                               s->stream, rj, nr, P.d_arena, reinterpret_cast<int32_t *>(P.d_coef));
         ++k;
     }
     return k;
@@ -1305,8 +1425,21 @@ uint32_t launch_stores(sgn_hip_session *s, sgn_j2k_plan &P, uint32_t a, uint32_t
     return 1;
 }
 
+// Packet headers of the tiles planned with device packets (fills their
+// code-block jobs); the first launch of every decode of such a plan.
+uint32_t launch_packets(sgn_hip_session *s, sgn_j2k_plan &P) {
+    if (P.pk_tiles.empty()) return 0;
+    const uint32_t n = static_cast<uint32_t>(P.pk_tiles.size());
+    hipLaunchKernelGGL(sgn_packet_headers, dim3((n + 63) / 64), dim3(64), 0, s->stream,
+                       P.d_pk_tiles, n, P.d_pk_parts, P.d_pk_pkts, P.d_pk_bands, P.d_pk_cbs,
+                       P.d_pk_nodes, P.d_pk_lens, P.d_arena, P.d_cb_lane, P.d_cb_lane_plain,
+                       P.d_cb_wave, P.d_pk_status);
+    return 1;
+}
+
 uint32_t enqueue_batched(sgn_hip_session *s, sgn_j2k_plan &P) {
     uint32_t k = clear_counters(s, P, 2);  // one node for both counters
+    k += launch_packets(s, P);
     k += launch_classic(s, P);
     k += launch_lane(s, P, P.d_cb_lane, static_cast<uint32_t>(P.cb_lane.size()), P.d_status_lane,
                      false);
@@ -1321,8 +1454,10 @@ uint32_t enqueue_batched(sgn_hip_session *s, sgn_j2k_plan &P) {
 }
 
 uint32_t enqueue_plain(sgn_hip_session *s, sgn_j2k_plan &P) {
-    // classic code-blocks of every tile first (one launch per kernel shape)
-    uint32_t k = launch_classic(s, P);
+    // packet headers, then the classic code-blocks of every tile (one
+    // launch per kernel shape)
+    uint32_t k = launch_packets(s, P);
+    k += launch_classic(s, P);
     for (size_t t = 0; t < P.ntiles; ++t) {
         const uint32_t a0 = P.lane_plain_begin[t], a1 = P.lane_plain_begin[t + 1];
         // one tile's code-blocks: the cooperative kernel (a tile has far
@@ -1369,6 +1504,23 @@ Status decode_plan(sgn_hip_session *s, sgn_j2k_plan &P, int mode) {
 // First failing tile from the per-block status words (after a sync).
 Status check_status(sgn_j2k_plan &P, int mode, size_t *first_fail) {
     size_t bad = SIZE_MAX;
+    if (!P.pk_tiles.empty()) {
+        std::vector<uint8_t> h(P.pk_tiles.size());
+        HIP_TRY(hipMemcpy(h.data(), P.d_pk_status, h.size(), hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < h.size(); ++k)
+            if (h[k]) {
+                // the tile of this packet table: its first job's tile
+                const PktTile &t = P.pk_tiles[k];
+                for (uint32_t c = 0; c < t.ncbs; ++c) {
+                    const uint32_t job = P.pk_cbs[t.cb0 + c].job;
+                    if (job == kPktNoJob) continue;
+                    const CbJob &j = (job & kPktWave) ? P.cb_wave[job & ~kPktWave]
+                                                      : P.cb_lane_plain[job];
+                    bad = std::min<size_t>(bad, j.tile);
+                    break;
+                }
+            }
+    }
     if (!P.cb_lane.empty()) {
         std::vector<uint8_t> h(P.cb_lane.size());
         HIP_TRY(hipMemcpy(h.data(), P.d_status_lane, h.size(), hipMemcpyDeviceToHost));
@@ -1495,9 +1647,10 @@ Status decode_pipelined(sgn_hip_session *s, const sgn_tile_req *reqs, size_t n, 
     auto status_at = [&](size_t a) { return tile_status ? tile_status + a : nullptr; };
     // chunk k + 1 is parsed (on a helper thread and the worker pool) while
     // chunk k is merged, uploaded and enqueued on this thread
+    const bool dev_packets = device_packets_enabled();
     auto parse_into = [&](size_t a, ParsedBatch &B) {
         try {
-            parse_batch(reqs + a, chunk_len(a), B, status_at(a));
+            parse_batch(reqs + a, chunk_len(a), B, status_at(a), dev_packets);
         } catch (const std::bad_alloc &) {
             B.st = kErrNoMem;
             B.msg = "out of host memory";
@@ -1846,6 +1999,7 @@ int sgn_j2k_plan_autotune(sgn_hip_session *s, sgn_j2k_plan *P, int reps, float *
     if (!s || !P || reps < 1) return SGN_E_ARG;
     HIP_TRY(hipSetDevice(s->device));
     const uint32_t n = static_cast<uint32_t>(P->cb_lane.size());
+    s->dispatches += launch_packets(s, *P);  // the jobs the variants decode
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIP_TRY(hipEventCreate(&e0));
     if (hipEventCreate(&e1) != hipSuccess) {
@@ -1968,6 +2122,7 @@ int sgn_j2k_plan_decode_stage(sgn_hip_session *s, sgn_j2k_plan *P, int stage) {
     HIP_TRY(hipSetDevice(s->device));
     uint32_t k = 0;
     if (stage == SGN_STAGE_BLOCKS) {
+        k += launch_packets(s, *P);
         k += launch_classic(s, *P);
         k += launch_lane(s, *P, P->d_cb_lane, static_cast<uint32_t>(P->cb_lane.size()),
                          P->d_status_lane);
@@ -2706,6 +2861,10 @@ int sgn_j2k_plan_time_code_blocks(sgn_hip_session *s, sgn_j2k_plan *P, size_t st
                                   size_t *count) {
     API_GUARD_BEGIN
     if (!s || !P || !stride || !count) return SGN_E_ARG;
+    if (P->device_packets) {
+        set_last_error("block timing needs the host's code-block lengths (no device packets)");
+        return SGN_E_UNSUPPORTED;
+    }
     HIP_TRY(hipSetDevice(s->device));
     std::vector<size_t> pick;
     for (size_t i = 0; i < P->cb_lane.size() && pick.size() < max_samples; i += stride)
@@ -2755,6 +2914,10 @@ int sgn_j2k_plan_order_code_blocks(sgn_hip_session *s, sgn_j2k_plan *P, int orde
     const bool model = order == SGN_ORDER_MODEL || order == SGN_ORDER_GEOMETRY_MODEL;
     if (!s || !P || order < 0 || order > SGN_ORDER_GEOMETRY_MODEL || (model && !m))
         return SGN_E_ARG;
+    if (P->device_packets) {
+        set_last_error("the code-block order of a plan with device packet parsing is fixed");
+        return SGN_E_UNSUPPORTED;
+    }
     HIP_TRY(hipSetDevice(s->device));
     std::vector<CbJob> v = P->cb_lane_plain;
     // per-block cost key: the model's prediction or the length heuristic

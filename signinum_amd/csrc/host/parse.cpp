@@ -707,3 +707,77 @@ Status parse_packets(const uint8_t *cs, const TileBody &t, int ncomp, const Tile
 }
 
 }  // namespace sgn
+
+namespace sgn {
+
+// Packet tables of one tile (HT, one layer, headers in the bodies) for the
+// device header parser: the same walk parse_packets makes, laid out flat.
+Status packet_walk(const TileBody &t, int ncomp, const TileCompGeom *tcs, LocalPlan &lp,
+                   std::vector<std::vector<uint32_t>> &cb_index) {
+    std::vector<PacketId> order;
+    Status st = progression(t.prog, t.layers, t.rect, tcs, ncomp, order);
+    if (st) return st;
+    PktTile T{};
+    T.part0 = static_cast<uint32_t>(lp.pk_parts.size());
+    T.nparts = static_cast<uint32_t>(t.parts.size());
+    for (const TilePart &tp : t.parts)
+        lp.pk_parts.push_back({static_cast<uint32_t>(tp.off), static_cast<uint32_t>(tp.len)});
+    auto tree_nodes = [](uint32_t w, uint32_t h) {
+        uint32_t n = 0;
+        if (!w || !h) return n;
+        for (;;) {
+            n += w * h;
+            if (w == 1 && h == 1) return n;
+            w = (w + 1) / 2;
+            h = (h + 1) / 2;
+        }
+    };
+    // per (component, precinct-band): its first PktCb and its two trees
+    T.cb0 = static_cast<uint32_t>(lp.pk_cbs.size());
+    T.node0 = lp.pk_nodes;
+    uint32_t nodes = 0;
+    cb_index.assign(static_cast<size_t>(ncomp), {});
+    std::vector<std::vector<PktBand>> pbs(static_cast<size_t>(ncomp));
+    for (int c = 0; c < ncomp; ++c) {
+        const TileCompGeom &g = tcs[c];
+        cb_index[c].assign(g.cbs.size(), kPktNoJob);
+        pbs[c].resize(g.precs.size());
+        for (size_t p = 0; p < g.precs.size(); ++p) {
+            const PrecBand &pb = g.precs[p];
+            PktBand &b = pbs[c][p];
+            b.cb0 = static_cast<uint32_t>(lp.pk_cbs.size());
+            b.ncbx = static_cast<uint16_t>(pb.ncbx);
+            b.ncby = static_cast<uint16_t>(pb.ncby);
+            for (uint32_t k = 0; k < pb.ncbx * pb.ncby; ++k) {
+                cb_index[c][pb.cb_base + k] = static_cast<uint32_t>(lp.pk_cbs.size());
+                lp.pk_cbs.push_back({kPktNoJob, 0});
+            }
+            const uint32_t tn = tree_nodes(pb.ncbx, pb.ncby);
+            b.incl = T.node0 + nodes;
+            b.zbp = b.incl + tn;
+            nodes += 2 * tn;
+        }
+    }
+    T.pkt0 = static_cast<uint32_t>(lp.pk_pkts.size());
+    for (const PacketId &pk : order) {
+        const TileCompGeom &g = tcs[pk.comp];
+        const ResGeom &rs = g.res[pk.res];
+        const CompParams &cp = t.comp[pk.comp];
+        PktPacket P{};
+        P.band0 = static_cast<uint32_t>(lp.pk_bands.size());
+        P.nbands = rs.nbands;
+        P.sop = (cp.scod & 2) ? 1 : 0;
+        P.eph = (cp.scod & 4) ? 1 : 0;
+        for (uint32_t bi = 0; bi < rs.nbands; ++bi)
+            lp.pk_bands.push_back(pbs[pk.comp][rs.prec_base + pk.prec * rs.nbands + bi]);
+        lp.pk_pkts.push_back(P);
+    }
+    T.npkts = static_cast<uint32_t>(lp.pk_pkts.size()) - T.pkt0;
+    T.nnodes = nodes;
+    T.ncbs = static_cast<uint32_t>(lp.pk_cbs.size()) - T.cb0;
+    lp.pk_nodes += nodes;
+    lp.pk_tiles.push_back(T);
+    return kOk;
+}
+
+}  // namespace sgn

@@ -129,6 +129,14 @@ struct LocalPlan {
     // components on different sampling grids: per tile-component output
     // jobs in place of the fused final stage
     std::vector<StoreJob> stores;
+    // packet headers left to the device (build_local_plan with
+    // device_packets): tables with codestream offsets and local job indices
+    std::vector<PktTile> pk_tiles;
+    std::vector<PktPart> pk_parts;
+    std::vector<PktPacket> pk_pkts;
+    std::vector<PktBand> pk_bands;
+    std::vector<PktCb> pk_cbs;
+    uint32_t pk_nodes = 0;
 };
 
 // Region of interest in full-resolution image coordinates (Rect of
@@ -149,8 +157,16 @@ Status region_output(const MainHeader &h, const OutputGeom &og, uint32_t denomin
 // dst + y0 * pitch + x0 * bytes_per_pixel.  With a region, only the region
 // is decoded and written (from dst): tiles outside it are dropped and the
 // code-blocks and synthesis blocks its pixels do not depend on are pruned.
+// device_packets: tiles whose packet headers the device can parse (HT, one
+// layer, headers in the bodies) get packet tables instead of parsed
+// code-block lengths; their jobs start as zero blocks.
 Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denominator,
                         uint64_t dst, uint64_t pitch, uint64_t capacity, LocalPlan &lp,
-                        const Region &roi = Region());
+                        const Region &roi = Region(), bool device_packets = false);
+
+// The packet tables of one tile for the device parser; cb_index[c][k]: the
+// PktCb entry of code-block k of component c.
+Status packet_walk(const TileBody &t, int ncomp, const TileCompGeom *tcs, LocalPlan &lp,
+                   std::vector<std::vector<uint32_t>> &cb_index);
 
 }  // namespace sgn

@@ -220,7 +220,7 @@ static Status classic_job(const uint8_t *cs, const CompParams &cp, const BandGeo
 
 Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denominator,
                         uint64_t dst, uint64_t pitch, uint64_t capacity, LocalPlan &lp,
-                        const Region &roi) {
+                        const Region &roi, bool device_packets) {
     lp = LocalPlan();
     MainHeader h;
     Status st = read_main_header(cs, len, h);
@@ -308,7 +308,19 @@ Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denomin
                     need[c][r - 1] = need_of(need[c][r], g.res[r - 1].rect);
             }
         }
-        st = parse_packets(cs, t, nc, tcs.data(), segs);
+        // packet headers on the device: HT code-blocks, one layer, headers
+        // in the tile-part bodies (no PPM / PPT)
+        bool dev = device_packets && t.layers == 1;
+        for (int c = 0; c < nc && dev; ++c) dev = (t.comp[c].cbstyle & kCbsHt) != 0;
+        for (const TilePart &tp : t.parts) dev = dev && tp.hdr.empty();
+        std::vector<std::vector<uint32_t>> pk_index;
+        if (dev) {
+            st = packet_walk(t, nc, tcs.data(), lp, pk_index);
+            segs.assign(static_cast<size_t>(nc), {});
+            for (int c = 0; c < nc; ++c) segs[c].resize(tcs[c].cbs.size());
+        } else {
+            st = parse_packets(cs, t, nc, tcs.data(), segs);
+        }
         if (st) return st;
         const bool mct = t.mct && nc >= 3;
         if (mct && smp.mixed) {
@@ -400,7 +412,11 @@ Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denomin
                     set_last_error("too many bit-planes");
                     return kErrMalformed;
                 }
-                if (K == 0 || sg.passes == 0 || actual == 0 || sg.offset < 0) {
+                if (dev) {
+                    // a zero block until the device reads its packet header
+                    j.flags |= kCbZero;
+                    lp.pk_cbs[pk_index[c][k]].kmax = K;
+                } else if (K == 0 || sg.passes == 0 || actual == 0 || sg.offset < 0) {
                     j.flags |= kCbZero;
                 } else {
                     j.src = static_cast<uint32_t>(sg.offset);
@@ -419,7 +435,9 @@ Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denomin
                         continue;
                     }
                 }
-                j.cost = (j.flags & kCbZero) ? 0 : sg.lcup + sg.lref + (uint32_t(j.w) * j.h) / 4;
+                j.cost = dev ? (uint32_t(j.w) * j.h) / 4
+                       : (j.flags & kCbZero) ? 0 : sg.lcup + sg.lref + (uint32_t(j.w) * j.h) / 4;
+                if (dev) lp.pk_cbs[pk_index[c][k]].job = static_cast<uint32_t>(lp.cbs.size());
                 lp.cbs.push_back(j);
                 lp.cb_band.push_back(band_off[cg.band]);
             }

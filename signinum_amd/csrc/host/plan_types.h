@@ -72,6 +72,38 @@ constexpr inline CbFeatures cb_features(const CbJob &j) {
                        float(uint32_t(j.w) * j.h) * (1.0f / 4096.0f)}};
 }
 
+// Packet headers parsed on the device (HT, one quality layer, headers in
+// the tile-part bodies).  The host lays out what the headers refer to:
+// per tile its tile-parts and packets in progression order; per packet the
+// precinct-bands it carries; per precinct-band its code-block grid (the
+// two tag trees) and, per code-block in raster order, the plan job it
+// fills.  Offsets into the next table are absolute within the plan.
+struct PktTile {
+    uint32_t part0, nparts;  // PktPart range
+    uint32_t pkt0, npkts;    // PktPacket range, progression order
+    uint32_t node0, nnodes;  // tag-tree nodes (value, low) pairs in the node scratch
+    uint32_t cb0, ncbs;      // PktCb range (every code-block of the tile's packets)
+};
+struct PktPart {
+    uint32_t off, len;  // tile-part body in the arena
+};
+struct PktPacket {
+    uint32_t band0;  // PktBand range
+    uint16_t nbands;
+    uint8_t sop, eph;  // SOP markers may precede the packet; EPH ends its header
+};
+struct PktBand {
+    uint32_t cb0;           // PktCb of the band's first code-block
+    uint16_t ncbx, ncby;    // code-block grid of the precinct-band
+    uint32_t incl, zbp;     // first node of the inclusion / zero-bit-plane tag tree
+};
+constexpr uint32_t kPktNoJob = 0xFFFFFFFFu;
+constexpr uint32_t kPktWave = 0x80000000u;  // job index in the wave list
+struct PktCb {
+    uint32_t job;  // lane-list index, kPktWave | wave-list index, or kPktNoJob (pruned)
+    uint32_t kmax; // magnitude bit-planes of its band
+};
+
 // Row pitch (elements) of every band and intermediate level in the
 // coefficient arena: a multiple of 16, so that rows of int16 samples start
 // 32-byte aligned (the HT kernels store aligned 32-byte row chunks) and rows

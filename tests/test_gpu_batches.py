@@ -150,3 +150,73 @@ def test_large_tiles_reduced_resolution(session, oracle):
     for j, ((cs, fmt, den), surf) in enumerate(zip(cases, plan.surfaces)):
         assert np.array_equal(surf.to_numpy(), O.decode(cs, int(fmt), int(den))), j
     plan.close()
+
+
+def _packet_cases():
+    """HT codestreams over the packet-header features: five progressions,
+    precincts, SOP/EPH, tiles, origins, refinement passes, 9/7, 12-bit gray,
+    elongated code-blocks, and the OpenHTJ2K fixtures."""
+    rng = np.random.default_rng(4242)
+    cases = [dict(), dict(progression=1), dict(progression=2, precinct_log2=5),
+             dict(progression=3, precinct_log2=6), dict(progression=4, precinct_log2=5),
+             dict(sop=1, eph=1, progression=2, precinct_log2=5), dict(tile_width=64, tile_height=48),
+             dict(ht_refine=2), dict(ht_refine=1, cb_width_log2=5, cb_height_log2=4),
+             dict(reversible=0, base_step=1 / 128), dict(levels=7, tile_width=64, tile_height=64),
+             dict(cb_width_log2=4, cb_height_log2=6, levels=3)]
+    out = []
+    for k, kw in enumerate(cases):
+        img = synth.tissue_rgb(rng, 150 + 7 * k, 100 + 3 * k)
+        out.append((S.encode_j2k_ht(img, **kw), S.PixelFormat.RGB8))
+    g = synth.spots_gray12(rng, 130, 90)
+    out.append((S.encode_j2k_ht(g, bit_depth=12, use_mct=0, progression=2, precinct_log2=5),
+                S.PixelFormat.GRAY16))
+    here = os.path.dirname(os.path.abspath(__file__))
+    for name in ("openhtj2k_ds0_ht_09_b11.j2k", "openhtj2k_ds0_ht_12_b11.j2k"):
+        with open(os.path.join(here, "golden", name), "rb") as f:
+            out.append((f.read(), S.PixelFormat.GRAY8))
+    return out
+
+
+def test_device_packet_headers(session, oracle, monkeypatch):
+    """Packet headers parsed on the device (SGN_DEVICE_PACKETS=1) give the
+    oracle's output on every case, in one batch per format, graph and
+    per-tile modes, full and half resolution; the plan's code-block order is
+    fixed, and its lane variants still autotune."""
+    monkeypatch.setenv("SGN_DEVICE_PACKETS", "1")
+    cases = _packet_cases()
+    for fmt in (S.PixelFormat.RGB8, S.PixelFormat.GRAY16, S.PixelFormat.GRAY8):
+        streams = [cs for cs, f in cases if f == fmt]
+        for den in (1, 2):
+            plan = S.TilePlan(session, streams, fmt, S.Downscale(den))
+            for mode in (S.MODE_GRAPH, S.MODE_PLAIN):
+                for s in plan.surfaces:
+                    s.fill(0)
+                plan.decode(mode)
+                session.synchronize()
+                for k, cs in enumerate(streams):
+                    want = O.decode(cs, int(fmt), den)
+                    got = plan.surfaces[k].to_numpy()
+                    assert np.array_equal(got.reshape(want.shape), want), (fmt, den, mode, k)
+            if fmt == S.PixelFormat.RGB8 and den == 1:
+                plan.autotune(reps=1)
+                with pytest.raises(S.UnsupportedError):
+                    plan.autotune_order(reps=1)
+            plan.close()
+
+
+def test_device_packet_headers_pipelined_and_region(session, oracle, small_chunks, monkeypatch):
+    """The pipelined host-buffer decode and a region decode with the packet
+    headers parsed on the device."""
+    monkeypatch.setenv("SGN_DEVICE_PACKETS", "1")
+    streams, pix, _cls = synth.lossless_batch_with_pixels(40, size=128)
+    surfs = S._alloc_surfaces(session, [(128, 128)] * 40, [S.PixelFormat.RGB8] * 40)
+    keep = [np.frombuffer(s, np.uint8) for s in streams]
+    rc, _ = _decode_tiles(session, _jobs([k.ctypes.data for k in keep],
+                                         [len(s) for s in streams], surfs), 40)
+    assert rc == 0, _lib.last_error()
+    for i, (s, img) in enumerate(zip(surfs, pix)):
+        assert np.array_equal(s.to_numpy(), img), i
+    img = synth.tissue_rgb(np.random.default_rng(77), 300, 200)
+    cs = S.encode_j2k_ht(img, tile_width=128, tile_height=96, progression=2, precinct_log2=5)
+    got = S.J2kDecoder(cs, session).decode_region_to_numpy(S.PixelFormat.RGB8, (70, 33, 150, 90))
+    assert np.array_equal(got, img[33:123, 70:220])
