@@ -13,7 +13,11 @@
 // tile-part and the next tile-part continues (non-strict, segment.rs:
 // 36-50).  The results land in the plan's code-block jobs, which the host
 // created as zero blocks with their geometry.
-#include <hip/hip_runtime.h>
+//
+// The including file provides the HIP definitions (__global__, __device__,
+// blockIdx, threadIdx, min).
+#pragma once
+#include <stdint.h>
 
 #include "../host/plan_types.h"
 

@@ -33,7 +33,7 @@
 #include "ht_rows.hip"
 #include "idwt.hip"
 #include "t1_classic.hip"
-#include "packets.hip"
+#include "packets.hpp"
 #include "cost_model.hip"
 
 using namespace sgn;

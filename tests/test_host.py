@@ -134,3 +134,47 @@ def test_packet_parse_matches_oracle_on_fixtures(oracle):
         ref = [(c.x1 - c.x0, c.y1 - c.y0, c.lcup if c.passes else 0, c.lref if c.passes else 0,
                 c.passes) for c in O.code_blocks(cs) if c.x1 > c.x0 and c.y1 > c.y0]
         assert ours == ref
+
+
+def test_device_packet_parser_matches_host_parser(tmp_path):
+    """The device packet-header parser (packets.hpp, compiled for the host by
+    packets_check.cpp with the host planner) against the host parser on HT
+    codestreams over progressions, precincts, SOP/EPH, tiles, origins,
+    refinement passes, 9/7, 12-bit gray and the OpenHTJ2K fixtures."""
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("needs g++")
+    here = os.path.dirname(os.path.abspath(__file__))
+    root = os.path.dirname(here)
+    host = os.path.join(root, "signinum_amd", "csrc", "host")
+    exe = tmp_path / "packets_check"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-w", "-o", str(exe),
+                    os.path.join(here, "packets_check.cpp")] +
+                   [os.path.join(host, f) for f in ("parse.cpp", "plan.cpp", "geometry.cpp",
+                                                    "container.cpp")] + ["-lpthread"],
+                   check=True)
+    rng = np.random.default_rng(3)
+    cases = [dict(), dict(progression=1), dict(progression=2, precinct_log2=5),
+             dict(progression=3, precinct_log2=6), dict(progression=4, precinct_log2=5),
+             dict(sop=1, eph=1), dict(sop=1, eph=1, progression=2, precinct_log2=5),
+             dict(tile_width=64, tile_height=48), dict(ht_refine=2),
+             dict(ht_refine=1, cb_width_log2=5, cb_height_log2=4),
+             dict(reversible=0, base_step=1 / 128),
+             dict(image_x0=5, image_y0=3, tile_width=64, tile_height=64),
+             dict(levels=7, tile_width=64, tile_height=64),
+             dict(cb_width_log2=4, cb_height_log2=6, levels=3)]
+    files = []
+    for k, kw in enumerate(cases):
+        img = synth.tissue_rgb(rng, 150 + 7 * k, 100 + 3 * k)
+        p = tmp_path / f"c{k}.j2k"
+        p.write_bytes(S.encode_j2k_ht(img, **kw))
+        files.append(str(p))
+    g = synth.spots_gray12(rng, 130, 90)
+    p = tmp_path / "g.j2k"
+    p.write_bytes(S.encode_j2k_ht(g, bit_depth=12, use_mct=0, progression=2, precinct_log2=5))
+    files.append(str(p))
+    for name in ("openhtj2k_ds0_ht_09_b11.j2k", "openhtj2k_ds0_ht_12_b11.j2k"):
+        files.append(os.path.join(here, "golden", name))
+    out = subprocess.run([str(exe)] + files, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0 and out.stdout.startswith("ok"), out.stdout + out.stderr
