@@ -1,8 +1,10 @@
-// HT packet headers parsed on the device: one tile per lane.
+// HT packet headers parsed on the device: one tile per wavefront.
 //
 // A tile's packet headers are one serial bit stream per tile-part (each
 // header's tag-tree and length fields depend on everything read before
-// it), so the parallelism is across tiles.  The walk follows the host
+// it), so the parallelism is across tiles: one work-group (one wave) per
+// tile, the wave clearing the tile's trees and jobs, lane 0 walking the
+// headers (64 tiles on the lanes of one wave diverged on every branch).  The walk follows the host
 // parser (host/parse.cpp parse_packets, which restates segment.rs:36-133
 // and 508-581): per packet an optional SOP marker, the non-empty bit, per
 // precinct-band and code-block the inclusion tag tree (first layer), the
@@ -95,14 +97,14 @@ __device__ int pkt_tag_decode(uint32_t *nodes, uint32_t w, uint32_t h, uint32_t 
     return value < threshold ? 1 : 0;
 }
 
-__device__ __forceinline__ CbJob *pkt_job(uint32_t job, CbJob *lane, CbJob *wave) {
-    if (job == kPktNoJob) return nullptr;
-    return (job & kPktWave) ? wave + (job & ~kPktWave) : lane + job;
+__device__ __forceinline__ CbJob *pkt_job(const PktCb &cb, CbJob *lanes, CbJob *wave) {
+    if (cb.job == kPktNoJob) return nullptr;
+    return (cb.job & kPktWave) ? wave + (cb.job & ~kPktWave) : lanes + cb.lane_pos;
 }
 
-// Tiles tiles[0 .. n), one per lane.  lane / wave: the plan's code-block
-// lists; lane2: a second copy of the lane list (the per-tile list of the
-// plain mode) or null.  nodes: tag-tree scratch (two words per node);
+// Tiles tiles[0 .. n), one per 64-thread work-group.  lanes / wave: the
+// plan's code-block lists (a block's place in `lanes` is PktCb::lane_pos);
+// lane2: the per-tile lane list of the plain mode (at PktCb::job) or null.  nodes: tag-tree scratch (two words per node);
 // lens: one word per PktCb (Lcup | Lref << 16 of an included block, 0
 // otherwise: blocks without a job still have bytes in the body).
 // status[t] = 1 for a tile whose header asks for more than 31 bit-planes.
@@ -112,20 +114,20 @@ sgn_packet_headers(const PktTile *__restrict__ tiles, uint32_t n,
                    const PktBand *__restrict__ bands, const PktCb *__restrict__ cbs,
                    uint32_t *__restrict__ nodes, uint32_t *__restrict__ lens,
                    const uint8_t *__restrict__ arena,
-                   CbJob *__restrict__ lane, CbJob *__restrict__ lane2, CbJob *__restrict__ wave,
+                   CbJob *__restrict__ lanes, CbJob *__restrict__ lane2, CbJob *__restrict__ wave,
                    uint8_t *__restrict__ status) {
-    const uint32_t i = blockIdx.x * 64 + threadIdx.x;
+    const uint32_t i = blockIdx.x, lane = threadIdx.x;
     if (i >= n) return;
     const PktTile T = tiles[i];
     uint8_t bad = 0;
     // fresh trees and zero blocks on every run (plans are replayed)
-    for (uint32_t k = 0; k < T.nnodes; ++k) {
+    for (uint32_t k = lane; k < T.nnodes; k += 64) {
         nodes[2 * (T.node0 + k)] = 0xFFFFFFFFu;
         nodes[2 * (T.node0 + k) + 1] = 0;
     }
-    for (uint32_t k = 0; k < T.ncbs; ++k) {
+    for (uint32_t k = lane; k < T.ncbs; k += 64) {
         lens[T.cb0 + k] = 0;
-        CbJob *j = pkt_job(cbs[T.cb0 + k].job, lane, wave);
+        CbJob *j = pkt_job(cbs[T.cb0 + k], lanes, wave);
         if (!j) continue;
         j->src = 0;
         j->lcup = j->lref = 0;
@@ -133,8 +135,9 @@ sgn_packet_headers(const PktTile *__restrict__ tiles, uint32_t n,
         j->passes = 0;
         j->flags |= kCbZero;
     }
+    __syncthreads();
     uint32_t next = 0;  // next packet of the progression
-    for (uint32_t pi = 0; pi < T.nparts; ++pi) {
+    for (uint32_t pi = 0; pi < T.nparts && lane == 0; ++pi) {
         const PktPart part = parts[T.part0 + pi];
         const uint8_t *body = arena + part.off;
         const uint32_t blen = part.len;
@@ -243,7 +246,7 @@ sgn_packet_headers(const PktTile *__restrict__ tiles, uint32_t n,
                                 break;
                             }
                         }
-                        CbJob *j = pkt_job(cb.job, lane, wave);
+                        CbJob *j = pkt_job(cb, lanes, wave);
                         if (cb.kmax > 31) bad = 1;
                         lens[B.cb0 + k] = lc | (lr << 16);
                         if (j) {
@@ -275,7 +278,7 @@ sgn_packet_headers(const PktTile *__restrict__ tiles, uint32_t n,
                             ok = false;
                             break;
                         }
-                        CbJob *j = pkt_job(cbs[B.cb0 + k].job, lane, wave);
+                        CbJob *j = pkt_job(cbs[B.cb0 + k], lanes, wave);
                         if (!j) {
                             bpos += need;  // a block of a skipped resolution or outside the region
                             continue;
@@ -292,13 +295,14 @@ sgn_packet_headers(const PktTile *__restrict__ tiles, uint32_t n,
             r.bit = bpos * 8;
         }
     }
+    __syncthreads();
     // the per-tile copy of the lane list for the plain mode
     if (lane2)
-        for (uint32_t k = 0; k < T.ncbs; ++k) {
-            const uint32_t job = cbs[T.cb0 + k].job;
-            if (job != kPktNoJob && !(job & kPktWave)) lane2[job] = lane[job];
+        for (uint32_t k = lane; k < T.ncbs; k += 64) {
+            const PktCb cb = cbs[T.cb0 + k];
+            if (cb.job != kPktNoJob && !(cb.job & kPktWave)) lane2[cb.job] = lanes[cb.lane_pos];
         }
-    status[i] = bad;
+    if (lane == 0) status[i] = bad;
 }
 
 }  // namespace sgn

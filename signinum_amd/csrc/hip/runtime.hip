@@ -762,7 +762,7 @@ Status merge_batch(ParsedBatch &B, sgn_j2k_plan &P, bool pool) {
         }
         for (size_t k = 0; k < lp.pk_cbs.size(); ++k) {
             PktCb q = lp.pk_cbs[k];
-            if (q.job != kPktNoJob) q.job = pos[q.job];
+            if (q.job != kPktNoJob) q.job = q.lane_pos = pos[q.job];
             P.pk_cbs[pb.cb + k] = q;
         }
     };
@@ -819,8 +819,29 @@ Status merge_batch(ParsedBatch &B, sgn_j2k_plan &P, bool pool) {
     // code-blocks for the batched lane kernel, ordered by predicted cost so
     // that a wave's lanes finish together
     if (P.device_packets) {
-        // jobs filled on the device at their per-tile positions
-        P.cb_lane = P.cb_lane_plain;
+        // lengths are known on the device only: the batched lane list is
+        // ordered by block shape (largest first, per-tile order inside a
+        // shape), and each packet-table entry learns its block's place
+        const size_t nl = P.cb_lane_plain.size();
+        std::vector<uint32_t> idx(nl), at(nl);
+        {
+            // counting sort by area (lane blocks are at most 64 x 64)
+            constexpr uint32_t kTop = kLaneMaxDim * kLaneMaxDim;
+            std::vector<uint32_t> start(kTop + 2, 0);
+            auto area = [&](size_t k) {
+                return uint32_t(P.cb_lane_plain[k].w) * P.cb_lane_plain[k].h;
+            };
+            for (size_t k = 0; k < nl; ++k) start[kTop - area(k) + 1]++;
+            for (size_t b = 1; b < start.size(); ++b) start[b] += start[b - 1];
+            for (size_t k = 0; k < nl; ++k) idx[start[kTop - area(k)]++] = static_cast<uint32_t>(k);
+        }
+        P.cb_lane.resize(idx.size());
+        for (size_t k = 0; k < idx.size(); ++k) {
+            P.cb_lane[k] = P.cb_lane_plain[idx[k]];
+            at[idx[k]] = static_cast<uint32_t>(k);
+        }
+        for (PktCb &q : P.pk_cbs)
+            if (q.job != kPktNoJob && !(q.job & kPktWave)) q.lane_pos = at[q.job];
         P.cb_refine.clear();
         P.lane_refined = true;
     } else {
@@ -1430,7 +1451,7 @@ uint32_t launch_stores(sgn_hip_session *s, sgn_j2k_plan &P, uint32_t a, uint32_t
 uint32_t launch_packets(sgn_hip_session *s, sgn_j2k_plan &P) {
     if (P.pk_tiles.empty()) return 0;
     const uint32_t n = static_cast<uint32_t>(P.pk_tiles.size());
-    hipLaunchKernelGGL(sgn_packet_headers, dim3((n + 63) / 64), dim3(64), 0, s->stream,
+    hipLaunchKernelGGL(sgn_packet_headers, dim3(n), dim3(64), 0, s->stream,
                        P.d_pk_tiles, n, P.d_pk_parts, P.d_pk_pkts, P.d_pk_bands, P.d_pk_cbs,
                        P.d_pk_nodes, P.d_pk_lens, P.d_arena, P.d_cb_lane, P.d_cb_lane_plain,
                        P.d_cb_wave, P.d_pk_status);

@@ -750,7 +750,7 @@ Status packet_walk(const TileBody &t, int ncomp, const TileCompGeom *tcs, LocalP
             b.ncby = static_cast<uint16_t>(pb.ncby);
             for (uint32_t k = 0; k < pb.ncbx * pb.ncby; ++k) {
                 cb_index[c][pb.cb_base + k] = static_cast<uint32_t>(lp.pk_cbs.size());
-                lp.pk_cbs.push_back({kPktNoJob, 0});
+                lp.pk_cbs.push_back({kPktNoJob, kPktNoJob, 0});
             }
             const uint32_t tn = tree_nodes(pb.ncbx, pb.ncby);
             b.incl = T.node0 + nodes;

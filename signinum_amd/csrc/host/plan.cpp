@@ -437,7 +437,9 @@ Status build_local_plan(const uint8_t *cs, size_t len, int fmt, uint32_t denomin
                 }
                 j.cost = dev ? (uint32_t(j.w) * j.h) / 4
                        : (j.flags & kCbZero) ? 0 : sg.lcup + sg.lref + (uint32_t(j.w) * j.h) / 4;
-                if (dev) lp.pk_cbs[pk_index[c][k]].job = static_cast<uint32_t>(lp.cbs.size());
+                if (dev)
+                    lp.pk_cbs[pk_index[c][k]].job = lp.pk_cbs[pk_index[c][k]].lane_pos =
+                        static_cast<uint32_t>(lp.cbs.size());
                 lp.cbs.push_back(j);
                 lp.cb_band.push_back(band_off[cg.band]);
             }

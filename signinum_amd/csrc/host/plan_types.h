@@ -100,8 +100,9 @@ struct PktBand {
 constexpr uint32_t kPktNoJob = 0xFFFFFFFFu;
 constexpr uint32_t kPktWave = 0x80000000u;  // job index in the wave list
 struct PktCb {
-    uint32_t job;  // lane-list index, kPktWave | wave-list index, or kPktNoJob (pruned)
-    uint32_t kmax; // magnitude bit-planes of its band
+    uint32_t job;       // per-tile lane-list index, kPktWave | wave-list index, or kPktNoJob
+    uint32_t lane_pos;  // its index in the batched (ordered) lane list
+    uint32_t kmax;      // magnitude bit-planes of its band
 };
 
 // Row pitch (elements) of every band and intermediate level in the

@@ -23,6 +23,7 @@ struct Dim3 {
     uint32_t x = 0, y = 0, z = 0;
 };
 static Dim3 threadIdx, blockIdx;
+static void __syncthreads() {}
 using std::max;
 using std::min;
 #include "../signinum_amd/csrc/hip/packets.hpp"
@@ -58,7 +59,14 @@ int main(int argc, char **argv) {
             std::vector<uint8_t> st(dev.pk_tiles.size() + 1);
             std::vector<CbJob> jobs = dev.cbs;
             for (size_t t = 0; t < dev.pk_tiles.size(); ++t) {
-                blockIdx.x = 0; threadIdx.x = (uint32_t)t;
+                // one work-group per tile; lane 0 (which walks the headers
+                // after the barrier) runs last
+                blockIdx.x = (uint32_t)t;
+                for (int l = 63; l > 0; --l) {
+                    threadIdx.x = (uint32_t)l;
+                    sgn_packet_headers(dev.pk_tiles.data(), (uint32_t)dev.pk_tiles.size(), dev.pk_parts.data(), dev.pk_pkts.data(), dev.pk_bands.data(), dev.pk_cbs.data(), nodes.data(), lens.data(), cs.data(), jobs.data(), nullptr, nullptr, st.data());
+                }
+                threadIdx.x = 0;
                 sgn_packet_headers(dev.pk_tiles.data(), (uint32_t)dev.pk_tiles.size(), dev.pk_parts.data(), dev.pk_pkts.data(), dev.pk_bands.data(), dev.pk_cbs.data(), nodes.data(), lens.data(), cs.data(), jobs.data(), nullptr, nullptr, st.data());
                 ++tiles;
             }
