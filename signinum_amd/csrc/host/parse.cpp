@@ -64,6 +64,9 @@ bool read_coding(ByteReader &b, uint8_t scod, CompParams &c) {
     c.xcb = uint8_t(v + 2);
     if (!b.u8(v)) return false;
     c.ycb = uint8_t(v + 2);
+    // A.6.1: each exponent at most 10, a code-block at most 4096 samples
+    // (the block kernels size their per-lane state by that limit)
+    if (c.xcb > 10 || c.ycb > 10 || c.xcb + c.ycb > 12) return false;
     if (!b.u8(v)) return false;
     c.cbstyle = uint8_t(v);
     if (!b.u8(v) || v > 1) return false;
@@ -393,6 +396,8 @@ uint32_t skipped_levels(const MainHeader &h, uint32_t denominator) {
 Status parse_tiles(const uint8_t *data, size_t len, const MainHeader &h,
                    std::vector<TileBody> &tiles) {
     const uint32_t ntx = h.tiles_x(), nty = h.tiles_y();
+    // SOT's Isot is 16 bits (A.4.2): a grid of more tiles cannot be coded
+    if (uint64_t(ntx) * nty > 65535u) return fail(kErrMalformed, "more than 65535 tiles");
     tiles.assign(size_t(ntx) * nty, TileBody());
     for (uint32_t i = 0; i < ntx * nty; ++i) {
         TileBody &t = tiles[i];

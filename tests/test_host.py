@@ -90,6 +90,24 @@ def test_malformed_inputs_raise():
         S.scaled_dims(cs, 3)
 
 
+def test_code_block_exponents_beyond_a61_are_malformed():
+    """COD code-block exponents (A.6.1): each at most 10 and a block at
+    most 4096 samples; 256x64 and 2048x2 blocks are refused, 1024x4 is
+    accepted."""
+    rng = np.random.default_rng(5)
+    cs = S.encode_j2k_ht(synth.texture_rgb(rng, 40, 24))
+    at = cs.index(b"\xff\x52") + 10  # FF52, Lcod, Scod, SGcod (4 bytes), levels: xcb
+    assert S.J2kDecoder.inspect(cs)["width"] == 40
+    for xcb, ycb, ok in ((6, 4, False), (9, 0, False), (8, 0, True)):
+        bad = bytearray(cs)
+        bad[at], bad[at + 1] = xcb, ycb
+        if ok:
+            S.J2kDecoder.inspect(bytes(bad))
+        else:
+            with pytest.raises(S.J2kError):
+                S.J2kDecoder.inspect(bytes(bad))
+
+
 def test_encoder_rejects_bad_options():
     img = np.zeros((8, 8, 3), np.uint8)
     with pytest.raises(S.J2kError):
@@ -178,3 +196,57 @@ def test_device_packet_parser_matches_host_parser(tmp_path):
         files.append(os.path.join(here, "golden", name))
     out = subprocess.run([str(exe)] + files, capture_output=True, text=True, timeout=300)
     assert out.returncode == 0 and out.stdout.startswith("ok"), out.stdout + out.stderr
+
+
+def test_planner_survives_damaged_codestreams(tmp_path):
+    """plan_fuzz.cpp (the host planner under AddressSanitizer) on damaged
+    HT, classic, multi-tile, PPM/PPT, region and OpenHTJ2K codestreams:
+    every damaged stream is refused or planned with every job inside the
+    buffers the kernels index."""
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("needs g++")
+    here = os.path.dirname(os.path.abspath(__file__))
+    host = os.path.join(os.path.dirname(here), "signinum_amd", "csrc", "host")
+    exe = tmp_path / "plan_fuzz"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-g", "-w", "-fsanitize=address",
+                    "-fno-omit-frame-pointer", "-o", str(exe), os.path.join(here, "plan_fuzz.cpp")] +
+                   [os.path.join(host, f) for f in ("parse.cpp", "plan.cpp", "geometry.cpp",
+                                                    "container.cpp")] + ["-lpthread"],
+                   check=True)
+    rng = np.random.default_rng(11)
+    img = synth.tissue_rgb(rng, 150, 110)
+    streams = [
+        S.encode_j2k_ht(img),
+        S.encode_j2k_ht(img, progression=2, precinct_log2=5, sop=1, eph=1, ht_refine=2),
+        S.encode_j2k_ht(img, tile_width=64, tile_height=48, image_x0=5, image_y0=3),
+        S.encode_j2k_ht(img, reversible=0, base_step=1 / 128, cb_width_log2=5, cb_height_log2=4),
+        S.encode_j2k_classic(img, cb_style=0x3F & ~0x10),
+        S.encode_j2k_classic(img, progression=1, precinct_log2=5, tile_width=64, tile_height=64),
+    ]
+    small = synth.tissue_rgb(rng, 120, 90)
+    streams += [S.encode_j2k_ht(small, packed_headers=1, tile_width=64, tile_height=64),
+                S.encode_j2k_classic(small, packed_headers=2, tile_width=64, tile_height=64),
+                S.encode_j2k_ht(synth.spots_gray12(rng, 100, 70), bit_depth=12, use_mct=0)]
+    # 300 x 300 in 1 x 1 tiles: 90000 tiles, more than SOT can number (A.4.2)
+    grid = bytearray(S.encode_j2k_ht(synth.texture_rgb(rng, 300, 300)))
+    siz = grid.index(b"\xff\x51")
+    grid[siz + 22:siz + 30] = (1).to_bytes(4, "big") * 2  # XTsiz, YTsiz
+    streams.append(bytes(grid))
+    files = []
+    for k, cs in enumerate(streams):
+        p = tmp_path / f"s{k}.j2k"
+        p.write_bytes(cs)
+        files.append(str(p))
+    files += [os.path.join(here, "golden", n)
+              for n in ("openhtj2k_ds0_ht_09_b11.j2k", "openhtj2k_ds0_ht_12_b11.j2k")]
+    # no allocation above 2 GiB: a header must not size host tables by itself
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:max_allocation_size_mb=2048")
+    out = subprocess.run([str(exe)] + files, capture_output=True, text=True, timeout=600, env=env)
+    lines = out.stdout.splitlines()
+    assert out.returncode == 0 and lines and lines[-1].startswith("ok"), \
+        (out.stdout + out.stderr)[-3000:]
+    status = {ln.split()[1]: int(ln.split()[2]) for ln in lines if ln.startswith("plan ")}
+    assert status[files[len(streams) - 1]] == -3  # malformed
+    assert all(status[f] == 0 for f in files if f != files[len(streams) - 1]), status
