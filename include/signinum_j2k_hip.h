@@ -185,7 +185,7 @@ typedef struct {
  * (crates/signinum-j2k/src/batch.rs:110-147; crates/signinum-j2k-cuda/src/
  * codec.rs:190-210).  On failure *first_fail holds the first failing tile
  * index (TileBatchError, batch.rs:47-67).  Synchronous.
- * SGN_MODE_BATCHED pipelines batches of more than 1024 tiles in chunks
+ * SGN_MODE_BATCHED pipelines batches of more than 768 tiles in chunks
  * (SGN_PIPELINE_TILES in the environment sets the chunk): the host parses
  * chunk k+1 while chunk k's bytes travel and chunk k-1 decodes.  Codestreams
  * that lie back to back (gaps under 4 KB) in page-locked host memory (see

@@ -1639,11 +1639,14 @@ class OnStream {
 };
 
 // Tiles per chunk of a pipelined batch decode (SGN_PIPELINE_TILES
-// overrides; 0 or a value >= n decodes the batch as one plan).
+// overrides; 0 or a value >= n decodes the batch as one plan).  768 tiles
+// of 256x256 RGB: chunks large enough for the lane kernel (57600
+// code-blocks), small enough that the first parse and the last decode,
+// which nothing overlaps, stay short.
 size_t pipeline_tiles() {
     const char *e = std::getenv("SGN_PIPELINE_TILES");
     if (e && *e) return size_t(std::strtoull(e, nullptr, 10));
-    return 1024;
+    return 768;
 }
 
 // Batch decode from host codestreams, pipelined in chunks: while chunk k's
