@@ -1,5 +1,5 @@
 // Mutation check of the host planner (build under AddressSanitizer): each
-// codestream given is damaged many times over (bytes flipped, a run set to
+// codestream or JP2 file given is damaged many times over (bytes flipped, a run set to
 // 0xFF, marker bytes planted, truncation, bytes dropped), and every damaged
 // stream goes through build_local_plan at full and reduced resolution, with
 // and without a region and with the device packet tables.  The planner must
@@ -129,6 +129,22 @@ static int check(const LocalPlan &lp, size_t len, uint64_t capacity, const char 
     return bad;
 }
 
+// What the decode entry points do: find the codestream (raw, or a JP2
+// file's jp2c box), then plan it.  `cs_len`: the codestream's length, the
+// bound of the plan's data offsets.
+static Status plan(const std::vector<uint8_t> &in, int fmt, uint32_t den, uint64_t capacity,
+                   LocalPlan &lp, const Region &roi, bool dev, size_t &cs_len) {
+    Container c;
+    const Status st = locate_codestream(in.data(), in.size(), c);
+    if (st != kOk) return st;
+    if (c.cs < in.data() || c.cs + c.cs_len > in.data() + in.size()) {
+        printf("codestream outside the input\n");
+        return kErrArg;
+    }
+    cs_len = c.cs_len;
+    return build_local_plan(c.cs, c.cs_len, fmt, den, 0, 1 << 16, capacity, lp, roi, dev);
+}
+
 int main(int argc, char **argv) {
     // SGN_FUZZ_ROUNDS / SGN_FUZZ_SEED: a longer or different run
     const char *er = getenv("SGN_FUZZ_ROUNDS"), *es = getenv("SGN_FUZZ_SEED");
@@ -150,12 +166,13 @@ int main(int argc, char **argv) {
             std::vector<uint8_t> cs(src);
             LocalPlan lp;
             Status st = kErrArg;
+            size_t cs_len = 0;
             for (int fmt = 0; fmt < 5 && st != kOk; ++fmt) {
                 lp = LocalPlan();
-                st = build_local_plan(cs.data(), cs.size(), fmt, 1, 0, 1 << 16, capacity, lp);
+                st = plan(cs, fmt, 1, capacity, lp, Region(), false, cs_len);
             }
             printf("plan %s %d\n", argv[a], int(st));
-            if (st == kOk) bad += check(lp, cs.size(), capacity, argv[a]);
+            if (st == kOk) bad += check(lp, cs_len, capacity, argv[a]);
         }
         for (int r = 0; r < rounds; ++r) {
             const std::vector<uint8_t> m = mutate(src);
@@ -173,8 +190,8 @@ int main(int argc, char **argv) {
             }
             const bool dev = rnd() % 2;
             LocalPlan lp;
-            const Status st = build_local_plan(cs.data(), cs.size(), fmt, den, 0, 1 << 16, capacity,
-                                               lp, roi, dev);
+            size_t cs_len = 0;
+            const Status st = plan(cs, fmt, den, capacity, lp, roi, dev, cs_len);
             if (st != kOk) {
                 ++refused;
                 continue;
@@ -183,7 +200,7 @@ int main(int argc, char **argv) {
             char what[512];
             snprintf(what, sizeof what, "%s round %d fmt %d den %u roi %d dev %d", argv[a], r, fmt,
                      den, int(roi.on), int(dev));
-            bad += check(lp, cs.size(), capacity, what);
+            bad += check(lp, cs_len, capacity, what);
         }
     }
     if (bad) {

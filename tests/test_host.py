@@ -200,7 +200,7 @@ def test_device_packet_parser_matches_host_parser(tmp_path):
 
 def test_planner_survives_damaged_codestreams(tmp_path):
     """plan_fuzz.cpp (the host planner under AddressSanitizer) on damaged
-    HT, classic, multi-tile, PPM/PPT, region and OpenHTJ2K codestreams:
+    HT, classic, multi-tile, PPM/PPT, JP2, region and OpenHTJ2K codestreams:
     every damaged stream is refused or planned with every job inside the
     buffers the kernels index."""
     import shutil
@@ -229,6 +229,8 @@ def test_planner_survives_damaged_codestreams(tmp_path):
     streams += [S.encode_j2k_ht(small, packed_headers=1, tile_width=64, tile_height=64),
                 S.encode_j2k_classic(small, packed_headers=2, tile_width=64, tile_height=64),
                 S.encode_j2k_ht(synth.spots_gray12(rng, 100, 70), bit_depth=12, use_mct=0)]
+    from tests.test_containers import jp2
+    streams.append(jp2(S.encode_j2k_ht(small), 120, 90, 3))  # JP2 boxes around it
     # 300 x 300 in 1 x 1 tiles: 90000 tiles, more than SOT can number (A.4.2)
     grid = bytearray(S.encode_j2k_ht(synth.texture_rgb(rng, 300, 300)))
     siz = grid.index(b"\xff\x51")
@@ -249,4 +251,5 @@ def test_planner_survives_damaged_codestreams(tmp_path):
         (out.stdout + out.stderr)[-3000:]
     status = {ln.split()[1]: int(ln.split()[2]) for ln in lines if ln.startswith("plan ")}
     assert status[files[len(streams) - 1]] == -3  # malformed
-    assert all(status[f] == 0 for f in files if f != files[len(streams) - 1]), status
+    assert all(status[f] == 0 for f in files if f != files[len(streams) - 1]), \
+        {f: v for f, v in status.items() if v}
