@@ -215,3 +215,39 @@ def test_dicom_errors():
         S.extract_dicom_encapsulated_frames(ok[:-12])
     with pytest.raises(S.MalformedError, match="different entry counts"):
         S.extract_dicom_encapsulated_frames(with_extended([0, 12], [3], b"", [b"abc\0", b"de"]))
+
+
+def test_damaged_dicom_and_jp2_inputs_return_or_raise():
+    """DICOM encapsulated pixel data (basic and extended offset tables) and
+    a JP2 file, each damaged 1500 times (flipped bytes, truncation, 0xFF
+    runs): extraction and inspect return or raise J2kError, and extracted
+    frames never hold more bytes than the input."""
+    rng = np.random.default_rng(19)
+    second = 8 + 2 + 8 + 2
+    seeds = [
+        explicit_ob_pixel_data(struct.pack("<II", 0, second), [b"aa", b"bb", b"cc"]),
+        with_extended([0, 10], [2, 2], b"", [b"ab", b"cd"]),
+        jp2(_rgb()[1], 70, 45, 3),
+    ]
+    for src in seeds:
+        for _ in range(1500):
+            d = bytearray(src)
+            kind = rng.integers(3)
+            if kind == 0:
+                for _ in range(int(rng.integers(1, 5))):
+                    d[int(rng.integers(len(d)))] ^= 1 << int(rng.integers(8))
+            elif kind == 1:
+                d = d[:int(rng.integers(1, len(d)))]
+            else:
+                at = int(rng.integers(len(d)))
+                d[at:at + 8] = b"\xff" * len(d[at:at + 8])
+            d = bytes(d)
+            try:
+                frames = S.extract_dicom_encapsulated_frames(d)
+                assert sum(len(f) for f in frames) <= len(d)
+            except S.J2kError:
+                pass
+            try:
+                S.J2kDecoder.inspect(d)
+            except S.J2kError:
+                pass
